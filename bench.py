@@ -1,0 +1,260 @@
+"""Benchmark: solves/s and factor seconds of the HPS Helmholtz solver on one
+MI355X, 2D 1024^2 (L=12, 16x16 leaves), kb = 80, seeded random potential,
+256 Gaussian-source right-hand sides; mixed precision (complex64 solve passes)
+against the plain FP64 GPU path.
+
+One "step" is one multi-RHS solve of the whole batch (up pass + down pass over
+all 12 levels) with every input already resident in HBM.  `value` is the
+mixed-path throughput in solves/s.  Beside it: factor seconds (FP64 factor;
+the mixed path adds the complex64 conversion), the plain FP64 path (one RHS
+per call, the reference's semantics), the batched FP64 path, the rel. l2
+error of every mixed RHS against the FP64 solution, the roofline of the
+dominant kernel (HIP events inside the library, on the stream the kernels run
+on) and the CPU oracle timed on a bounded sample on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+PEAK = {"hbm_GBs": 8000.0, "fp64_TF": 78.6, "fp32_TF": 157.3}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def potential(pts_int, kb, seed, nmodes=16, d=2):
+    """kb^2 (1 + 0.3 q) + i 0.01 kb^2, q = sum_m a_m cos(2 pi p_m.x + phi_m) / sum|a_m|,
+    a_m ~ U[-1,1], p_m in {1..8}^d, phi_m ~ U[0, 2 pi)."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-1, 1, nmodes)
+    p = rng.integers(1, 9, size=(nmodes, d))
+    ph = rng.uniform(0, 2 * np.pi, nmodes)
+    q = np.zeros(pts_int.shape[:-1])
+    for j in range(nmodes):
+        q += a[j] * np.cos(2 * np.pi * (pts_int[..., :d] @ p[j]) + ph[j])
+    q /= np.abs(a).sum()
+    return kb ** 2 * (1 + 0.3 * q) + 1j * 0.01 * kb ** 2
+
+
+def gaussian_sources(torch, pts_int_dev, nrhs, seed, sigma=0.02, d=2):
+    rng = np.random.default_rng(seed + 1000)
+    c = torch.tensor(rng.uniform(0.1, 0.9, size=(nrhs, d)), device=pts_int_dev.device, dtype=torch.float64)
+    x = pts_int_dev[..., :d]  # (n_leaves, n_int, d)
+    f = torch.empty((nrhs,) + tuple(x.shape[:2]), dtype=torch.complex128, device=x.device)
+    for r in range(nrhs):
+        r2 = ((x - c[r]) ** 2).sum(-1)
+        f[r] = torch.exp(-r2 / (2 * sigma * sigma)).to(torch.complex128)
+    return f
+
+
+def cpu_baseline(args, kb_phys, kb_opt):
+    """The oracle (reference algorithm restated in C, OpenMP over leaves and
+    nodes) on a bounded sample of the workload: the [0,1/4]^2 corner at the
+    same grid spacing (256^2, L=8, same 16x16 leaves, kb, potential family),
+    factor + solves timed, scaled to the full problem by the ratio of the
+    reference algorithm's solve/factor flop counts."""
+    from oracle.pyoracle import OracleTree
+    import robinsolver_amd as rsa
+    threads = min(os.cpu_count() or 1, 16)
+    L = args.levels - 4
+    dom = [[0, 0.25], [0, 0.25], [0, 1]]
+    o = OracleTree(2, L, dom, 1.0 / args.grid, kb_opt, threads=threads)
+    pts, _ = o.all_points()
+    k = potential(pts[:, o.nb:], kb_phys, args.seed)
+    t0 = time.perf_counter()
+    o.factor(k)
+    tf = time.perf_counter() - t0
+    rng = np.random.default_rng(7)
+    f = np.exp(-((pts[:, o.nb:, 0] - 0.12) ** 2 + (pts[:, o.nb:, 1] - 0.13) ** 2) / (2 * 0.02 ** 2)) + 0j
+    g = np.zeros((o.n_leaves, o.nb), np.complex128)
+    ns = 0
+    t0 = time.perf_counter()
+    while ns < 3 or time.perf_counter() - t0 < 2.0:
+        o.solve(f, g)
+        ns += 1
+    ts = (time.perf_counter() - t0) / ns
+    # flop ratio sample -> full, reference-algorithm counts
+    full = rsa.GpuTree(2, args.levels, [[0, 1], [0, 1], [0, 1]], 1.0 / args.grid, kb_opt)
+    samp = rsa.GpuTree(2, L, dom, 1.0 / args.grid, kb_opt)
+    ff_full, fs_full = full.algorithmic_counts()
+    ff_s, fs_s = samp.algorithmic_counts()
+    return {
+        "value": (1.0 / ts) * fs_s / fs_full,
+        "unit": "solves/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": ("oracle on the [0,1/4]^2 corner at h=1/%d (%d^2, L=%d): factor %.2fs, solve %.3fs; "
+                   "scaled by the reference-algorithm flop ratio (solve x%.1f, factor x%.1f)"
+                   % (args.grid, args.grid // 4, L, tf, ts, fs_full / fs_s, ff_full / ff_s)),
+        "factor_s_est": tf * ff_full / ff_s,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=1024)
+    ap.add_argument("--levels", type=int, default=12)
+    ap.add_argument("--kb", type=float, default=80.0)
+    ap.add_argument("--nrhs", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--r1-solves", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-run", action="store_true", help="short run for rocprofv3 (mixed steps only)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is design-only)." % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import robinsolver_amd as rsa
+
+    dev = torch.device("cuda:0")
+    kb_phys = args.kb
+    kb_opt = -args.kb          # Robin a1 = -i kb realised as opts%kb = -kb
+    unit = [[0, 1], [0, 1], [0, 1]]
+    log("plan: 2D %d^2, L=%d, kb=%g, nrhs=%d" % (args.grid, args.levels, kb_phys, args.nrhs))
+    t = rsa.GpuTree(2, args.levels, unit, 1.0 / args.grid, kb_opt, precision=rsa.RS_MIXED)
+    pts, _ = t.all_points()
+    pint = pts[:, t.nb:, :]
+    k = potential(pint, kb_phys, args.seed)
+    k_dev = torch.from_numpy(k).to(dev)
+    pint_dev = torch.from_numpy(np.ascontiguousarray(pint)).to(dev)
+    f = gaussian_sources(torch, pint_dev, args.nrhs, args.seed)
+    g = torch.zeros((args.nrhs, t.n_leaves, t.nb), dtype=torch.complex128, device=dev)
+    u64 = torch.zeros((args.nrhs, t.n_leaves, t.m), dtype=torch.complex128, device=dev)
+    u32 = torch.zeros_like(u64)
+    torch.cuda.synchronize()
+
+    # factor (FP64 interface inverses; the mixed tree also converts the solve data)
+    log("factor ...")
+    t.factor(k)
+    f_time_mixed = t.timings()[0]
+    t.factor(k)
+    f_time_mixed = min(f_time_mixed, t.timings()[0])
+    t64 = rsa.GpuTree(2, args.levels, unit, 1.0 / args.grid, kb_opt, precision=rsa.RS_FP64)
+    t64.factor(k)
+    f_time_64 = t64.timings()[0]
+    t64.factor(k)
+    f_time_64 = min(f_time_64, t64.timings()[0])
+    log("factor s: fp64 %.4f, mixed %.4f" % (f_time_64, f_time_mixed))
+
+    def solve(tree, nrhs, fp, gp, up, prec):
+        tree.solve_device(nrhs, fp, gp, up, precision=prec)
+
+    sz_f = t.n_leaves * t.n_int
+    sz_g = t.n_leaves * t.nb
+    sz_u = t.n_leaves * t.m
+
+    # plain FP64 path: one RHS per call
+    nr1 = min(args.r1_solves, args.nrhs)
+    for r in range(2):
+        solve(t64, 1, f.data_ptr(), g.data_ptr(), u64.data_ptr(), rsa.RS_FP64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(nr1):
+        solve(t64, 1, f.data_ptr() + r * sz_f * 16, g.data_ptr() + r * sz_g * 16, u64.data_ptr() + r * sz_u * 16,
+              rsa.RS_FP64)
+    torch.cuda.synchronize()
+    r1_rate = nr1 / (time.perf_counter() - t0)
+    log("fp64 R=1: %.1f solves/s" % r1_rate)
+
+    # batched FP64
+    solve(t64, args.nrhs, f.data_ptr(), g.data_ptr(), u64.data_ptr(), rsa.RS_FP64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nb64 = max(1, args.steps // 2)
+    for _ in range(nb64):
+        solve(t64, args.nrhs, f.data_ptr(), g.data_ptr(), u64.data_ptr(), rsa.RS_FP64)
+    torch.cuda.synchronize()
+    b64_rate = nb64 * args.nrhs / (time.perf_counter() - t0)
+    log("fp64 batched: %.1f solves/s" % b64_rate)
+
+    # mixed path, timed steps
+    for _ in range(args.warmup):
+        solve(t, args.nrhs, f.data_ptr(), g.data_ptr(), u32.data_ptr(), rsa.RS_MIXED)
+    torch.cuda.synchronize()
+    t.profile(True)
+    t.profile_read()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solve(t, args.nrhs, f.data_ptr(), g.data_ptr(), u32.data_ptr(), rsa.RS_MIXED)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = t.profile_read()
+    t.profile(False)
+    mixed_rate = args.steps * args.nrhs / elapsed
+    log("mixed: %.1f solves/s (%.2f ms/step)" % (mixed_rate, 1e3 * elapsed / args.steps))
+
+    # accuracy of every mixed RHS against the FP64 GPU solution
+    diff = (u32 - u64).reshape(args.nrhs, -1)
+    rel = (torch.linalg.vector_norm(diff, dim=1) / torch.linalg.vector_norm(u64.reshape(args.nrhs, -1), dim=1))
+    rel = rel.cpu().numpy()
+
+    # roofline of the dominant kernel category
+    cat = max(prof, key=lambda c: prof[c][0])
+    ms, nl, byt, fl = prof[cat]
+    dur = ms / 1e3 / max(nl, 1)
+    bpl, fpl = byt / max(nl, 1), fl / max(nl, 1)
+    peak_f = PEAK["fp32_TF"]
+    if fpl / max(bpl, 1.0) > peak_f * 1e12 / (PEAK["hbm_GBs"] * 1e9):
+        roof = {"bound": "mfma", "achieved": fpl / dur / 1e12, "peak": peak_f, "unit": "TFLOP/s"}
+    else:
+        roof = {"bound": "hbm", "achieved": bpl / dur / 1e9, "peak": PEAK["hbm_GBs"], "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = cat
+    roof["avg_launch_ms"] = dur * 1e3
+    roof["launches"] = nl
+
+    ff, fs = t.algorithmic_counts()
+    out = {
+        "metric": "solves/s + factor s, 2D 1024^2 Helmholtz; mixed vs FP64 GPU at rel.l2<=1e-8",
+        "value": mixed_rate,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": "2D %d^2 Helmholtz, L=%d (16x16 leaves), kb=%g, seeded 16-mode potential, "
+                               "%d Gaussian-source RHS per step" % (args.grid, args.levels, kb_phys, args.nrhs),
+                   "grid": args.grid, "levels": args.levels, "kb": kb_phys, "nrhs": args.nrhs, "seed": args.seed},
+        "factor_s_fp64": f_time_64,
+        "factor_s_mixed": f_time_mixed,
+        "fp64_r1_solves_per_s": r1_rate,
+        "fp64_batched_solves_per_s": b64_rate,
+        "mixed_over_fp64_r1": mixed_rate / r1_rate,
+        "mixed_rel_l2_vs_fp64": {"max": float(rel.max()), "median": float(np.median(rel))},
+        "refinement_steps": 0,
+        "reference_flops": {"factor_GF": ff / 1e9, "solve_GF_per_rhs": fs / 1e9},
+        "roofline": roof,
+        "kernel_ms_per_step": {c: v[0] / args.steps for c, v in prof.items()},
+    }
+    if not args.no_cpu_baseline and not args.profile_run:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(args, kb_phys, kb_opt)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,128 @@
+/*
+ * robinsolver.h -- C ABI of the MI355X (gfx950) HPS factor and solve.
+ *
+ * Drop-in boundary for robinsolver's Fortran API.  Every entry point below is
+ * what the Fortran shim (robinsolver_amd/fortran/robinsolver_f.f90) binds with
+ * ISO_C_BINDING to keep the reference's module and procedure names; the
+ * reference interface each one replaces is cited.  Plain pointers and sizes
+ * only.  Complex arrays are interleaved (re, im) doubles, the memory layout of
+ * Fortran complex(dp).  Every call returns 0 on success or a nonzero status;
+ * rs_last_error() then holds the message (the reference's `stop` text where
+ * the reference has one).
+ *
+ * Layouts:
+ *   leaf order        DFS order of get_solution (robin_output_module.f90:282-322)
+ *   point order       fgrid order of a leaf: boundary points face by face
+ *                     (x1,x2,y1,y2,z1,z2), then interior points x-fastest
+ *                     (fd_module.f90:867-939)
+ *   k_int, f          [n_leaves][n_int]     samples at interior points
+ *   g                 [n_leaves][n_bnd]     samples at leaf boundary points;
+ *                                           only physical-boundary points are read
+ *   u                 [n_leaves][m]         leaf solution vectors, m = n_bnd + n_int
+ *   multi-RHS         an outer [nrhs] dimension in front of each of the above
+ */
+#ifndef ROBINSOLVER_H
+#define ROBINSOLVER_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rs_tree rs_tree;
+
+enum {
+    RS_FP64 = 0,   /* complex128 everywhere: the plain FP64 GPU path          */
+    RS_MIXED = 1   /* complex64 solve passes on the matrix cores, FP64 factor
+                      interface inverses, FP64 residual + iterative refinement */
+};
+
+typedef struct rs_opts {
+    int precision;        /* RS_FP64 or RS_MIXED (default solve precision)      */
+    int skip_root_rtr;    /* 1: do not form the root RtR, which no solve reads  */
+    int n_refine;         /* refinement steps of the mixed path                 */
+    int verbose;          /* 1: progress lines as the reference prints them     */
+    long long hbm_budget; /* bytes this tree may hold on the device (0 = no cap) */
+} rs_opts;
+
+/* Defaults: precision RS_FP64, skip_root_rtr 1, n_refine 2, verbose 0, no cap. */
+void rs_opts_default(rs_opts *o);
+
+/* Create / destroy a tree handle (type robin_tree, robin_tree_module.f90:55-77;
+ * destroy has no reference counterpart: the reference never frees a tree). */
+int rs_tree_create(rs_tree **t, const rs_opts *o);
+int rs_tree_destroy(rs_tree *t);
+
+/* Geometry of robin_tree_factor (robin_tree_module.f90:96-156: validation,
+ * bin_tree, node_divide, get_h).  domain = elliptic_operator%domain(1:2,1:3)
+ * column-major {x_lo,x_hi,y_lo,y_hi,z_lo,z_hi}; kb = solve_opts%kb;
+ * bndry_params = robin_tree%bndry_params(2,2) complex column-major.  Outputs the
+ * sizes the caller needs to sample k, f and g. */
+int rs_plan(rs_tree *t, int d, int L, const double *domain, double h_tgt, double kb,
+            const double *bndry_params, int *n_leaves, int *m, int *n_int, int *n_bnd);
+
+/* Coordinates (x,y,z) of every unknown of a leaf in point order (grid2real,
+ * fd_module.f90:669-700) and the face id (1..6, 0 = interior) of each point.
+ * face may be NULL. */
+int rs_leaf_points(const rs_tree *t, int leaf, double *xyz, int *face);
+/* Leaf box (2,3) column-major (leaf_solution%box, robin_output_module.f90:32-37). */
+int rs_leaf_box(const rs_tree *t, int leaf, double *box);
+
+/* robin_tree%factor (robin_tree_module.f90:96-185) with the potential sampled
+ * at the interior points: k_int [n_leaves][n_int] complex, host memory. */
+int rs_factor(rs_tree *t, const double *k_int);
+
+/* robin_tree%solve (robin_tree_module.f90:198-233) for nrhs right-hand sides
+ * at once, host memory: f [nrhs][n_leaves][n_int], g [nrhs][n_leaves][n_bnd].
+ * Repeatable (the reference allows one solve per process).  precision < 0
+ * uses rs_opts.precision. */
+int rs_solve(rs_tree *t, int nrhs, const double *f, const double *g, int precision);
+
+/* get_solution (robin_output_module.f90:48-75) for RHS number rhs of the last
+ * rs_solve: u [n_leaves][m] complex, host memory. */
+int rs_get_solution(const rs_tree *t, int rhs, double *u);
+
+/* Device-resident variant of rs_solve (no reference counterpart: the caller's
+ * inputs already live in HBM).  f_dev, g_dev, u_dev are device pointers with
+ * the multi-RHS layouts above; stream is a hipStream_t (NULL = the tree's own
+ * stream).  Asynchronous with respect to the host. */
+int rs_solve_device(rs_tree *t, int nrhs, const void *f_dev, const void *g_dev, void *u_dev,
+                    int precision, void *stream);
+
+/* T%f_time and T%s_time (robin_tree_module.f90:63-64, time_module.f90:26-36):
+ * wall seconds of the last factor and solve. */
+int rs_timings(const rs_tree *t, double *f_time, double *s_time);
+
+/* Message of the last failed call on this handle (or of the last failed
+ * rs_tree_create when t is NULL). */
+const char *rs_last_error(const rs_tree *t);
+
+/* Inspection (robin_tree%inspect, robin_tree_module.f90:263-274): factored flag,
+ * number of levels, device bytes held. */
+int rs_inspect(const rs_tree *t, int *factored, int *nlvls, long long *device_bytes);
+
+/* Per-level shape of the plan (no reference counterpart; host-side, needs no
+ * GPU): nodes of level lvl, boundary points P of each, and for lvl < L the
+ * interface size and the children's boundary size of its merge (0 at the
+ * leaf level). */
+int rs_level_info(const rs_tree *t, int lvl, int *nodes, int *P, int *isz, int *child_P);
+
+/* Test hooks (no reference counterpart).  Node RtR of node idx (left to right)
+ * of level lvl, P x P column-major, in face order (RtR(6,6) blocks of
+ * robin_tree_node assembled, derived_type_module.f90:239-269).  P via *P; out
+ * may be NULL to query P. */
+int rs_node_rtr(const rs_tree *t, int lvl, int idx, double *out, int *P);
+
+/* Kernel timing (no reference counterpart; the reference's only timer is
+ * time_module).  While enabled, every solve-pass launch is bracketed by HIP
+ * events on the stream it runs on, and its algorithmic bytes and flops are
+ * recorded.  rs_profile_read() synchronises and returns, for one category,
+ * the summed milliseconds, launch count, bytes and flops since the last read
+ * (and clears that category).  Categories are numbered 0..rs_profile_ncat()-1. */
+int rs_profile(rs_tree *t, int enable);
+int rs_profile_ncat(void);
+const char *rs_profile_name(int cat);
+int rs_profile_read(rs_tree *t, int cat, double *ms, int *launches, double *bytes, double *flops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

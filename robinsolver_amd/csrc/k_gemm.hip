@@ -1,0 +1,147 @@
+// k_gemm.hip -- batched complex GEMM on the gfx950 matrix cores.
+//
+// Replaces the zgemm/zgemv call sites of the reference merge and solve passes
+// (robin_tree_module.f90:550-605, 898-959, 1094-1109) and the zgetrs-based
+// block products of the merge (robin_tree_module.f90:482-534), which this
+// build expresses as products with explicit interface inverses.
+//
+// Complex products run as four real MFMA products (4M) on interleaved storage:
+//   Cr += Ar*Br - Ai*Bi,  Ci += Ar*Bi + Ai*Br,
+// with v_mfma_f64_16x16x4f64 (complex128) or v_mfma_f32_16x16x4f32
+// (complex64, FP32 accumulation).  A 256-thread workgroup computes a 64 x BN
+// output tile from 64 x 16 and 16 x BN operand tiles staged in LDS; BN = 64 for
+// wide products (merge), BN = 16 for the few-RHS solve passes, which then stay
+// HBM-bound rather than MFMA-bound.  The K reduction of every output column is
+// a fixed sequence that does not depend on N or on the other columns, so a
+// right-hand side gets the same bits alone or batched with others.
+//
+// Each problem of a batch may scatter its output rows/columns through index
+// maps (the face permutation between a child's and its parent's boundary
+// ordering), read B from two row ranges (the two children's interface
+// vectors), and add beta*Din in the epilogue.
+#include "rs_common.hpp"
+#include "rs_kernels.hpp"
+
+namespace rs {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename R> struct acc_t;
+template <> struct acc_t<double> { using T = d4; };
+template <> struct acc_t<float> { using T = f4; };
+
+__device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ inline f4 mfma(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+constexpr int BM = 64;
+constexpr int BK = 16;
+
+template <typename R, int BN>
+__global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ descs, int M, int N, int K,
+                                                  int tilesM, int tilesN) {
+    using V = c_t<R>;
+    using A4 = typename acc_t<R>::T;
+    constexpr int WAVES_N = (BN >= 32) ? 2 : 1;
+    constexpr int WAVES_M = 4 / WAVES_N;
+    constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
+    constexpr int TM = WM / 16, TN = WN / 16;
+    __shared__ V As[BK][BM];
+    __shared__ V Bs[BK][BN];
+
+    const int tiles = tilesM * tilesN;
+    const int p = blockIdx.x / tiles;
+    const int t = blockIdx.x - p * tiles;
+    const int tm = t % tilesM, tn = t / tilesM;
+    const GemmDesc& D = descs[p];
+    const V* __restrict__ A = (const V*)D.A;
+    const V* __restrict__ B = (const V*)D.B;
+    const V* __restrict__ B2 = (const V*)D.B2;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave % WAVES_M) * WM, wn = (wave / WAVES_M) * WN;
+
+    A4 accR[TM][TN], accI[TM][TN];
+    for (int i = 0; i < TM; ++i)
+        for (int j = 0; j < TN; ++j)
+            for (int v = 0; v < 4; ++v) { accR[i][j][v] = R(0); accI[i][j][v] = R(0); }
+
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        for (int e = tid; e < BM * BK; e += 256) {
+            int r = e % BM, k = e / BM;
+            int gr = m0 + r, gk = k0 + k;
+            As[k][r] = (gr < M && gk < K) ? A[(size_t)gr + (size_t)gk * D.lda] : cz<R>();
+        }
+        for (int e = tid; e < BK * BN; e += 256) {
+            int k = e % BK, c = e / BK;
+            int gk = k0 + k, gc = n0 + c;
+            V v = cz<R>();
+            if (gk < K && gc < N)
+                v = (gk < D.ksplit) ? B[(size_t)gk + (size_t)gc * D.ldb]
+                                    : B2[(size_t)(gk - D.ksplit) + (size_t)gc * D.ldb];
+            Bs[k][c] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            V af[TM], bf[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) af[i] = As[kk + (lane >> 4)][wm + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) bf[j] = Bs[kk + (lane >> 4)][wn + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    accR[i][j] = mfma(af[i].x, bf[j].x, accR[i][j]);
+                    accR[i][j] = mfma(-af[i].y, bf[j].y, accR[i][j]);
+                    accI[i][j] = mfma(af[i].x, bf[j].y, accI[i][j]);
+                    accI[i][j] = mfma(af[i].y, bf[j].x, accI[i][j]);
+                }
+        }
+        __syncthreads();
+    }
+
+    V* Out = (V*)D.Out;
+    const V* Din = (const V*)D.Din;
+    const V alpha = mk<R>((R)D.ar, (R)D.ai), beta = mk<R>((R)D.br, (R)D.bi);
+    for (int i = 0; i < TM; ++i)
+        for (int j = 0; j < TN; ++j)
+            for (int v = 0; v < 4; ++v) {
+                int row = m0 + wm + i * 16 + 4 * (lane >> 4) + v;
+                int col = n0 + wn + j * 16 + (lane & 15);
+                if (row >= M || col >= N) continue;
+                V val = cmul(alpha, mk<R>(accR[i][j][v], accI[i][j][v]));
+                int orow = D.rmap ? D.rmap[row] : row;
+                int ocol = D.cmap ? D.cmap[col] : col;
+                if (Din) {
+                    size_t di = D.dmap ? (size_t)orow + (size_t)ocol * D.ldd : (size_t)row + (size_t)col * D.ldd;
+                    val = cfma(beta, Din[di], val);
+                }
+                Out[(size_t)orow + (size_t)ocol * D.ldo] = val;
+            }
+}
+
+}  // namespace
+
+template <typename R>
+void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st) {
+    if (nprob <= 0 || M <= 0 || N <= 0) return;
+    int tilesM = (M + BM - 1) / BM;
+    if (N <= 16) {
+        int tilesN = (N + 15) / 16;
+        hipLaunchKernelGGL((gemm_kernel<R, 16>), dim3(nprob * tilesM * tilesN), dim3(256), 0, st, d_descs, M, N,
+                           K, tilesM, tilesN);
+    } else {
+        int tilesN = (N + 63) / 64;
+        hipLaunchKernelGGL((gemm_kernel<R, 64>), dim3(nprob * tilesM * tilesN), dim3(256), 0, st, d_descs, M, N,
+                           K, tilesM, tilesN);
+    }
+}
+
+template void launch_gemm<double>(const GemmDesc*, int, int, int, int, hipStream_t);
+template void launch_gemm<float>(const GemmDesc*, int, int, int, int, hipStream_t);
+
+}  // namespace rs

@@ -1,0 +1,210 @@
+// k_inv.hip -- batched FP64 complex matrix inversion with partial pivoting
+// for the interface systems of the merge.
+//
+// Reference: D = (zeta I - T2_II)^-1 by zgetrf + zgetri and the LU of
+// S^-1 = zeta I - T1_II - nu^2 D by zgetrf (robin_tree_module.f90:440-467;
+// factorization_module.f90:49-141).  Here both D and S are formed explicitly
+// (Gauss-Jordan, the pivot being the first entry of maximal |re|+|im| in the
+// column, as zgetrf chooses it), so that the merge and the solve passes are
+// plain matrix products on the matrix cores.
+//
+// Small interfaces (the whole matrix fits in LDS) use one workgroup per matrix.
+// Larger ones use a blocked right-looking Gauss-Jordan: an n x nb panel is
+// eliminated in LDS, the row interchanges are applied to the other columns and
+// the pivot rows moved out (W), and the rest of the matrix gets the rank-nb
+// update A[:, R] += A[:, panel] W as a batched GEMM.  After the last panel the
+// interchanges are undone as column interchanges.
+#include "rs_common.hpp"
+#include "rs_kernels.hpp"
+
+namespace rs {
+
+namespace {
+
+// block-wide argmax of |re|+|im|; ties go to the smallest index
+__device__ void block_argmax(double v, int i, double* rv, int* ri, double& best, int& bi) {
+    for (int off = 32; off > 0; off >>= 1) {
+        double ov = __shfl_xor(v, off);
+        int oi = __shfl_xor(i, off);
+        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) { rv[wave] = v; ri[wave] = i; }
+    __syncthreads();
+    best = rv[0]; bi = ri[0];
+    for (int w = 1; w < nw; ++w)
+        if (rv[w] > best || (rv[w] == best && ri[w] < bi)) { best = rv[w]; bi = ri[w]; }
+    __syncthreads();
+}
+
+// Whole-matrix Gauss-Jordan in LDS, one workgroup per matrix.
+__global__ void __launch_bounds__(256) inv_small_kernel(double2* __restrict__ mats, int n, size_t stride,
+                                                       int* __restrict__ info) {
+    extern __shared__ double2 sm[];
+    double2* M = sm;
+    double2* colj = sm + (size_t)n * n;
+    int* piv = (int*)(colj + n);
+    __shared__ double rv[4];
+    __shared__ int ri[4];
+    __shared__ int bad;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double2* G = mats + (size_t)blockIdx.x * stride;
+    if (tid == 0) bad = 0;
+    for (int e = tid; e < n * n; e += nt) M[e] = G[e];
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        double v = -1.0;
+        int vi = n;
+        for (int r = j + tid; r < n; r += nt) {
+            double a = cabs1(M[r + j * n]);
+            if (a > v) { v = a; vi = r; }
+        }
+        double best;
+        int p;
+        block_argmax(v, vi, rv, ri, best, p);
+        if (tid == 0) { piv[j] = p; if (!(best > 0.0)) bad = 1; }
+        if (p != j)
+            for (int c = tid; c < n; c += nt) {
+                double2 t = M[j + c * n]; M[j + c * n] = M[p + c * n]; M[p + c * n] = t;
+            }
+        __syncthreads();
+        for (int r = tid; r < n; r += nt) colj[r] = M[r + j * n];
+        __syncthreads();
+        const double2 inv = cinv(colj[j]);
+        for (int c = tid; c < n; c += nt) M[j + c * n] = (c == j) ? inv : cmul(M[j + c * n], inv);
+        __syncthreads();
+        for (int e = tid; e < n * n; e += nt) {
+            int r = e % n, c = e / n;
+            if (r == j) continue;
+            double2 base = (c == j) ? make_double2(0.0, 0.0) : M[e];
+            M[e] = csub(base, cmul(colj[r], M[j + c * n]));
+        }
+        __syncthreads();
+    }
+    for (int r = tid; r < n; r += nt)
+        for (int j = n - 1; j >= 0; --j) {
+            int p = piv[j];
+            if (p != j) { double2 t = M[r + j * n]; M[r + j * n] = M[r + p * n]; M[r + p * n] = t; }
+        }
+    __syncthreads();
+    for (int e = tid; e < n * n; e += nt) G[e] = M[e];
+    if (tid == 0 && bad) atomicOr(info, 2);
+}
+
+// Panel elimination of columns [j0, j0+nb) over all n rows, in LDS.
+__global__ void __launch_bounds__(256) gj_panel_kernel(double2* __restrict__ mats, int n, size_t stride, int j0,
+                                                      int nb, int* __restrict__ pivs, int* __restrict__ info) {
+    extern __shared__ double2 sm[];
+    double2* P = sm;              // [nb][n] column-major panel
+    double2* colj = sm + (size_t)n * nb;
+    __shared__ double rv[4];
+    __shared__ int ri[4];
+    __shared__ int bad;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double2* G = mats + (size_t)blockIdx.x * stride + (size_t)j0 * n;
+    int* piv = pivs + (size_t)blockIdx.x * n;
+    if (tid == 0) bad = 0;
+    for (int e = tid; e < n * nb; e += nt) P[e] = G[e];
+    __syncthreads();
+    for (int jj = 0; jj < nb; ++jj) {
+        const int j = j0 + jj;
+        double v = -1.0;
+        int vi = n;
+        for (int r = j + tid; r < n; r += nt) {
+            double a = cabs1(P[r + jj * n]);
+            if (a > v) { v = a; vi = r; }
+        }
+        double best;
+        int p;
+        block_argmax(v, vi, rv, ri, best, p);
+        if (tid == 0) { piv[j] = p; if (!(best > 0.0)) bad = 1; }
+        if (p != j)
+            for (int c = tid; c < nb; c += nt) {
+                double2 t = P[j + c * n]; P[j + c * n] = P[p + c * n]; P[p + c * n] = t;
+            }
+        __syncthreads();
+        for (int r = tid; r < n; r += nt) colj[r] = P[r + jj * n];
+        __syncthreads();
+        const double2 inv = cinv(colj[j]);
+        for (int c = tid; c < nb; c += nt) P[j + c * n] = (c == jj) ? inv : cmul(P[j + c * n], inv);
+        __syncthreads();
+        for (int e = tid; e < n * nb; e += nt) {
+            int r = e % n, c = e / n;
+            if (r == j) continue;
+            double2 base = (c == jj) ? make_double2(0.0, 0.0) : P[e];
+            P[e] = csub(base, cmul(colj[r], P[j + c * n]));
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < n * nb; e += nt) G[e] = P[e];
+    if (tid == 0 && bad) atomicOr(info, 2);
+}
+
+// Apply the panel's row interchanges to the columns outside the panel, move the
+// pivot rows of those columns to W (nb x (n-nb), compact) and zero them.
+__global__ void gj_swap_extract_kernel(double2* __restrict__ mats, int n, size_t stride, int j0, int nb,
+                                       const int* __restrict__ pivs, double2* __restrict__ W, size_t wstride) {
+    const int c2 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c2 >= n - nb) return;
+    const int c = c2 < j0 ? c2 : c2 + nb;
+    double2* col = mats + (size_t)blockIdx.y * stride + (size_t)c * n;
+    const int* piv = pivs + (size_t)blockIdx.y * n;
+    double2* w = W + (size_t)blockIdx.y * wstride + (size_t)c2 * nb;
+    for (int jj = 0; jj < nb; ++jj) {
+        int j = j0 + jj, p = piv[j];
+        if (p != j) { double2 t = col[j]; col[j] = col[p]; col[p] = t; }
+    }
+    for (int jj = 0; jj < nb; ++jj) { w[jj] = col[j0 + jj]; col[j0 + jj] = make_double2(0.0, 0.0); }
+}
+
+__global__ void gj_unpermute_kernel(double2* __restrict__ mats, int n, size_t stride, const int* __restrict__ pivs) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double2* M = mats + (size_t)blockIdx.y * stride;
+    const int* piv = pivs + (size_t)blockIdx.y * n;
+    for (int j = n - 1; j >= 0; --j) {
+        int p = piv[j];
+        if (p != j) { double2 t = M[r + (size_t)j * n]; M[r + (size_t)j * n] = M[r + (size_t)p * n]; M[r + (size_t)p * n] = t; }
+    }
+}
+
+}  // namespace
+
+size_t inv_small_lds(int n) { return (size_t)n * n * sizeof(double2) + (size_t)n * (sizeof(double2) + sizeof(int)); }
+
+int gj_panel_width(int n, size_t lds_limit) {
+    int nb = 16;
+    while (nb > 1 && ((size_t)n * nb + n) * sizeof(double2) + 1024 > lds_limit) nb >>= 1;
+    while (nb > 1 && n % nb) nb >>= 1;
+    return nb;
+}
+
+void InvPlan::build(int n_, int batch_, size_t stride_, size_t lds_limit_) {
+    n = n_; batch = batch_; stride = stride_; lds_limit = lds_limit_;
+    small = inv_small_lds(n) + 512 <= lds_limit;
+    nb = small ? n : gj_panel_width(n, lds_limit);
+}
+
+void launch_inverse(const InvPlan& ip, double2* mats, int* pivs, double2* W, const GemmDesc* d_descs,
+                    int* info, hipStream_t st) {
+    if (ip.batch <= 0) return;
+    if (ip.small) {
+        hipLaunchKernelGGL(inv_small_kernel, dim3(ip.batch), dim3(256), inv_small_lds(ip.n), st, mats, ip.n,
+                           ip.stride, info);
+        return;
+    }
+    const int n = ip.n, nb = ip.nb;
+    const size_t plds = ((size_t)n * nb + n) * sizeof(double2);
+    const size_t wstride = (size_t)nb * (n - nb);
+    for (int j0 = 0, pi = 0; j0 < n; j0 += nb, ++pi) {
+        hipLaunchKernelGGL(gj_panel_kernel, dim3(ip.batch), dim3(256), plds, st, mats, n, ip.stride, j0, nb, pivs,
+                           info);
+        hipLaunchKernelGGL(gj_swap_extract_kernel, dim3((n - nb + 255) / 256, ip.batch), dim3(256), 0, st, mats, n,
+                           ip.stride, j0, nb, pivs, W, wstride);
+        launch_gemm<double>(d_descs + (size_t)pi * ip.batch, ip.batch, n, n - nb, nb, st);
+    }
+    hipLaunchKernelGGL(gj_unpermute_kernel, dim3((n + 255) / 256, ip.batch), dim3(256), 0, st, mats, n, ip.stride,
+                       pivs);
+}
+
+}  // namespace rs

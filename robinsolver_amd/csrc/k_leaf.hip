@@ -1,0 +1,410 @@
+// k_leaf.hip -- leaf stage on gfx950: batched factorization of every leaf FD
+// system and the leaf solves of the factor (RtR maps) and of both solve passes.
+//
+// Reference: the dense m x m leaf matrix of fd_module.f90:49-224, factored by
+// zgetrf (fd_module.f90:273) and applied by zgetrs (fd_module.f90:275, 434, 562),
+// with the outgoing-data stencil of robin_tree_module.f90:667-718 and
+// fd_module.f90:449-513.
+//
+// MI355X design.  The leaf matrix A = [A_BB A_BI; A_IB A_II] has a diagonal A_BB
+// (each boundary row couples its point to the two interior points along the
+// normal), so the boundary unknowns are eliminated exactly:
+//   u_I = S^-1 (f - A_IB A_BB^-1 g),  u_B = A_BB^-1 (g - A_BI u_I),
+//   S   = A_II - A_IB A_BB^-1 A_BI    (same 5/7-point pattern as A_II).
+// Ordered slow-axis last, S is block tridiagonal with dense s x s diagonal
+// blocks (one x-row in 2D, one xy-layer in 3D) and off-diagonal blocks that are
+// scalar multiples of the identity (the slow-axis stencil weight only depends
+// on the layer index).  Block LU therefore needs only the K inverses
+//   Shat_k^-1,  Shat_k = S_kk - lambda_k mu_{k-1} Shat_{k-1}^-1,
+// each computed in LDS by Gauss-Jordan with partial pivoting.  Those K blocks
+// (K s^2 entries, 4096 for a 16x16 leaf instead of 320^2 for the dense LU) are
+// all that the solve streams per leaf.  The outgoing data needs no stencil
+// pass: the boundary row gives (alpha + D_n) u = g, so
+//   h = (sigma + D_n) u = g + (sigma - alpha) u_B.
+#include "rs_common.hpp"
+#include "rs_kernels.hpp"
+
+namespace rs {
+
+namespace {
+
+// Per-axis constants of the eliminated system.
+struct AxisCoef {
+    double2 dd;     // A_BB entry: alpha + 8/(3h)
+    double2 bndd;   // diagonal of S gained from eliminating the adjacent boundary point: 8/(h^3 dd)
+    double2 eoff;   // S coupling of an edge point to its inward neighbour: (4/3)/h^2 - (8/9)/(h^3 dd)
+    double cib;     // A_IB entry: (8/3)/h^2
+    double mid;     // 1/h^2
+    double w0e;     // centre weight at an edge point: -4/h^2
+    double w0m;     // centre weight elsewhere: -2/h^2
+    double a1, a2;  // A_BI entries: -3/h at the first interior point, 1/(3h) at the second
+};
+
+struct LeafCoef {
+    AxisCoef ax[3];
+};
+
+__host__ LeafCoef make_coef(const LeafGeom& G) {
+    LeafCoef C{};
+    for (int j = 0; j < G.d; ++j) {
+        double h = G.h[j];
+        double2 dd = make_double2(8.0 / (3.0 * h), G.kb);
+        double2 idd = cinv(dd);
+        AxisCoef& a = C.ax[j];
+        a.dd = dd;
+        a.bndd = cscale(idd, 8.0 / (h * h * h));
+        a.eoff = csub(make_double2(4.0 / 3.0 / (h * h), 0.0), cscale(idd, 8.0 / 9.0 / (h * h * h)));
+        a.cib = 8.0 / 3.0 / (h * h);
+        a.mid = 1.0 / (h * h);
+        a.w0e = -4.0 / (h * h);
+        a.w0m = -2.0 / (h * h);
+        a.a1 = -3.0 / h;
+        a.a2 = 1.0 / (3.0 * h);
+    }
+    return C;
+}
+
+__device__ inline void interior_coords(const LeafGeom& G, int q, int g[3]) {
+    g[0] = q % G.n[0] + 1;
+    int r = q / G.n[0];
+    if (G.d == 2) { g[1] = r + 1; g[2] = 1; }
+    else { g[1] = r % G.n[1] + 1; g[2] = r / G.n[1] + 1; }
+}
+
+__device__ inline int stride_of(const LeafGeom& G, int j) {
+    return j == 0 ? 1 : (j == 1 ? G.n[0] : G.n[0] * G.n[1]);
+}
+
+// Local boundary index (fgrid order) of the face point of face f adjacent to
+// interior point with grid coords g.
+__device__ inline int face_point(const LeafGeom& G, int f, const int g[3]) {
+    int j = (f - 1) >> 1;
+    int t;
+    if (G.d == 2) {
+        t = g[1 - j] - 1;
+    } else {
+        int a0 = (j == 0) ? 1 : 0;
+        int a1 = (j == 2) ? 1 : 2;
+        t = (g[a0] - 1) + G.n[a0] * (g[a1] - 1);
+    }
+    return G.foff[f] + t;
+}
+
+// Decompose a local boundary index into (face, first interior point p1, axis j).
+__device__ inline void boundary_info(const LeafGeom& G, int b, int& f, int& j, int& q1, int& step) {
+    f = 1;
+    while (f < 2 * G.d && b >= G.foff[f] + G.pf[f]) ++f;
+    int t = b - G.foff[f];
+    j = (f - 1) >> 1;
+    int g[3] = {1, 1, 1};
+    if (G.d == 2) {
+        g[1 - j] = t + 1;
+    } else {
+        int a0 = (j == 0) ? 1 : 0;
+        int a1 = (j == 2) ? 1 : 2;
+        g[a0] = t % G.n[a0] + 1;
+        g[a1] = t / G.n[a0] + 1;
+    }
+    g[j] = (f & 1) ? 1 : G.n[j];
+    q1 = (g[0] - 1) + G.n[0] * ((g[1] - 1) + (G.d == 3 ? G.n[1] * (g[2] - 1) : 0));
+    step = ((f & 1) ? 1 : -1) * stride_of(G, j);
+}
+
+__device__ inline int local_to_node(const LeafGeom& G, int parity, int b) {
+    int f = 1;
+    while (f < 2 * G.d && b >= G.foff[f] + G.pf[f]) ++f;
+    return G.noff[parity][f] + (b - G.foff[f]);
+}
+
+__device__ inline int node_to_local(const LeafGeom& G, int parity, int nc) {
+    for (int f = 1; f <= 2 * G.d; ++f)
+        if (nc >= G.noff[parity][f] && nc < G.noff[parity][f] + G.pf[f]) return G.foff[f] + (nc - G.noff[parity][f]);
+    return -1;
+}
+
+// Slow-axis couplings of S: lambda_k (layer k to k-1), mu_k (layer k to k+1).
+__device__ inline double2 slow_lambda(const LeafGeom& G, const LeafCoef& C, int k) {
+    const AxisCoef& a = C.ax[G.d - 1];
+    return (k + 1 == G.n[G.d - 1]) ? a.eoff : make_double2(a.mid, 0.0);
+}
+__device__ inline double2 slow_mu(const LeafGeom& G, const LeafCoef& C, int k) {
+    const AxisCoef& a = C.ax[G.d - 1];
+    return (k == 0) ? a.eoff : make_double2(a.mid, 0.0);
+}
+
+// Cooperative in-place Gauss-Jordan inversion with partial pivoting of an
+// s x s column-major matrix in LDS.  Pivot = first entry of maximal |re|+|im|
+// in the column (the zgetrf rule).  Sets *bad on an exact zero pivot.
+__device__ void gj_invert_lds(double2* M, int s, int* piv, double2* colj, int* bad) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int j = 0; j < s; ++j) {
+        if (tid < 64) {
+            double best = -1.0;
+            int bi = s;
+            for (int i = j + tid; i < s; i += 64) {
+                double v = cabs1(M[i + j * s]);
+                if (v > best) { best = v; bi = i; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                double ob = __shfl_xor(best, off);
+                int oi = __shfl_xor(bi, off);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            if (tid == 0) {
+                piv[j] = bi;
+                if (!(best > 0.0)) *bad = 1;
+            }
+        }
+        __syncthreads();
+        const int p = piv[j];
+        if (p != j)
+            for (int c = tid; c < s; c += nt) {
+                double2 t = M[j + c * s]; M[j + c * s] = M[p + c * s]; M[p + c * s] = t;
+            }
+        __syncthreads();
+        for (int i = tid; i < s; i += nt) colj[i] = M[i + j * s];
+        __syncthreads();
+        const double2 inv = cinv(colj[j]);
+        for (int c = tid; c < s; c += nt) M[j + c * s] = (c == j) ? inv : cmul(M[j + c * s], inv);
+        __syncthreads();
+        for (int e = tid; e < s * s; e += nt) {
+            int i = e % s, c = e / s;
+            if (i == j) continue;
+            double2 base = (c == j) ? make_double2(0.0, 0.0) : M[e];
+            M[e] = csub(base, cmul(colj[i], M[j + c * s]));
+        }
+        __syncthreads();
+    }
+    // undo the row interchanges as column interchanges, last first
+    for (int i = tid; i < s; i += nt)
+        for (int j = s - 1; j >= 0; --j) {
+            int p = piv[j];
+            if (p != j) { double2 t = M[i + j * s]; M[i + j * s] = M[i + p * s]; M[i + p * s] = t; }
+        }
+    __syncthreads();
+}
+
+// One workgroup per leaf: Shat_k^-1 for k = 0..K-1, FP64.
+__global__ void __launch_bounds__(256) leaf_blocks_kernel(LeafGeom G, LeafCoef C, const double2* __restrict__ kint,
+                                                         double2* __restrict__ sinv, int* __restrict__ info) {
+    extern __shared__ double2 sm[];
+    const int s = G.s, K = G.K, ss = s * s;
+    double2* M = sm;
+    double2* Pv = sm + ss;
+    double2* colj = sm + 2 * ss;
+    int* piv = (int*)(colj + s);
+    __shared__ int bad;
+    const int leaf = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    if (tid == 0) bad = 0;
+    const double2* kl = kint + (size_t)leaf * G.n_int;
+    for (int k = 0; k < K; ++k) {
+        double2 lm = make_double2(0.0, 0.0);
+        if (k > 0) lm = cmul(slow_lambda(G, C, k), slow_mu(G, C, k - 1));
+        for (int e = tid; e < ss; e += nt) {
+            int r = e % s, c = e / s;
+            int q = k * s + r;
+            int g[3];
+            interior_coords(G, q, g);
+            double2 v = make_double2(0.0, 0.0);
+            if (r == c) {
+                v = kl[q];
+                for (int j = 0; j < G.d; ++j) {
+                    const AxisCoef& a = C.ax[j];
+                    bool edge = (g[j] == 1 || g[j] == G.n[j]);
+                    v.x += edge ? a.w0e : a.w0m;
+                    if (g[j] == 1) v = cadd(v, a.bndd);
+                    if (g[j] == G.n[j]) v = cadd(v, a.bndd);
+                }
+            } else {
+                for (int j = 0; j < G.d - 1; ++j) {
+                    const AxisCoef& a = C.ax[j];
+                    int st = stride_of(G, j);
+                    if (c == r + st && g[j] < G.n[j]) v = (g[j] == 1) ? a.eoff : make_double2(a.mid, 0.0);
+                    if (c == r - st && g[j] > 1) v = (g[j] == G.n[j]) ? a.eoff : make_double2(a.mid, 0.0);
+                }
+            }
+            if (k > 0) v = csub(v, cmul(lm, Pv[e]));
+            M[e] = v;
+        }
+        __syncthreads();
+        gj_invert_lds(M, s, piv, colj, &bad);
+        double2* out = sinv + ((size_t)leaf * K + k) * ss;
+        for (int e = tid; e < ss; e += nt) { out[e] = M[e]; Pv[e] = M[e]; }
+        __syncthreads();
+    }
+    if (tid == 0 && bad) atomicOr(info, 1);
+}
+
+template <typename R> struct LeafIO {
+    int mode;                 // LEAF_FACTOR_T, LEAF_UP, LEAF_DOWN
+    int ncols;                // columns: nb (factor) or nrhs
+    int CH;                   // columns per workgroup
+    const c_t<R>* sinv;       // [leaf][K][s*s]
+    const double2* f;         // UP: [col][leaf][n_int] or null
+    const double2* jump;      // UP: [col][leaf][nb] local order, subtracted from h; or null
+    const c_t<R>* gin;        // DOWN: [col][leaf][nb] node order
+    c_t<R>* hout;             // UP: [col][leaf][nb] node order
+    double2* u;               // UP (overwrite) / DOWN (accumulate): [col][leaf][m]
+    double2* T;               // FACTOR: [leaf][nb][nb] node order, column-major
+    int n_leaves;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C, LeafIO<R> io) {
+    using V = c_t<R>;
+    extern __shared__ unsigned char smraw[];
+    const int s = G.s, K = G.K, ss = s * s, CH = io.CH, ni = G.n_int, nb = G.nb;
+    V* Y = (V*)smraw;                // [ni][CH]
+    V* B = Y + (size_t)ni * CH;      // [s*s]
+    V* Vt = B + ss;                  // [s][CH]
+    const int leaf = blockIdx.x, c0 = blockIdx.y * CH, tid = threadIdx.x, nt = blockDim.x;
+    const int ncol = min(CH, io.ncols - c0);
+    const int parity = leaf & 1;
+    const V* sl = io.sinv + (size_t)leaf * K * ss;
+
+    // incoming data g for boundary point b, column c (chunk-local)
+    auto gval = [&](int b, int c) -> V {
+        int col = c0 + c;
+        if (io.mode == LEAF_FACTOR_T) {
+            return (node_to_local(G, parity, col) == b) ? mk<R>(R(1), R(0)) : cz<R>();
+        } else if (io.mode == LEAF_DOWN) {
+            return io.gin[((size_t)col * io.n_leaves + leaf) * nb + local_to_node(G, parity, b)];
+        }
+        return cz<R>();
+    };
+
+    // 1. right-hand side r = f - A_IB A_BB^-1 g
+    for (int e = tid; e < ni * CH; e += nt) {
+        int q = e / CH, c = e % CH;
+        V v = cz<R>();
+        if (c < ncol) {
+            if (io.mode == LEAF_UP && io.f) v = ccast<V>(io.f[((size_t)(c0 + c) * io.n_leaves + leaf) * ni + q]);
+            if (io.mode != LEAF_UP) {
+                int g[3];
+                interior_coords(G, q, g);
+                for (int j = 0; j < G.d; ++j) {
+                    const AxisCoef& a = C.ax[j];
+                    double2 w = cscale(cinv(a.dd), -a.cib);
+                    V wr = ccast<V>(w);
+                    if (g[j] == 1) v = cfma(wr, gval(face_point(G, 2 * j + 1, g), c), v);
+                    if (g[j] == G.n[j]) v = cfma(wr, gval(face_point(G, 2 * j + 2, g), c), v);
+                }
+            }
+        }
+        Y[e] = v;
+    }
+    __syncthreads();
+
+    // 2. forward sweep: y_k = Shat_k^-1 (r_k - lambda_k y_{k-1})
+    for (int k = 0; k < K; ++k) {
+        for (int e = tid; e < ss; e += nt) B[e] = sl[(size_t)k * ss + e];
+        V lam = ccast<V>(slow_lambda(G, C, k));
+        for (int e = tid; e < s * CH; e += nt) {
+            V v = Y[(size_t)k * s * CH + e];
+            if (k > 0) v = csub(v, cmul(lam, Y[(size_t)(k - 1) * s * CH + e]));
+            Vt[e] = v;
+        }
+        __syncthreads();
+        for (int e = tid; e < s * CH; e += nt) {
+            int w = e / CH, c = e % CH;
+            V acc = cz<R>();
+            for (int w2 = 0; w2 < s; ++w2) acc = cfma(B[w + w2 * s], Vt[w2 * CH + c], acc);
+            Y[(size_t)k * s * CH + e] = acc;
+        }
+        __syncthreads();
+    }
+    // 3. backward sweep: x_k = y_k - mu_k Shat_k^-1 x_{k+1}
+    for (int k = K - 2; k >= 0; --k) {
+        for (int e = tid; e < ss; e += nt) B[e] = sl[(size_t)k * ss + e];
+        V mu = ccast<V>(slow_mu(G, C, k));
+        for (int e = tid; e < s * CH; e += nt) Vt[e] = cmul(mu, Y[(size_t)(k + 1) * s * CH + e]);
+        __syncthreads();
+        for (int e = tid; e < s * CH; e += nt) {
+            int w = e / CH, c = e % CH;
+            V acc = cz<R>();
+            for (int w2 = 0; w2 < s; ++w2) acc = cfma(B[w + w2 * s], Vt[w2 * CH + c], acc);
+            Y[(size_t)k * s * CH + e] = csub(Y[(size_t)k * s * CH + e], acc);
+        }
+        __syncthreads();
+    }
+    // 4. boundary values and outputs
+    const double2 sma = make_double2(0.0, -2.0 * G.kb);  // sigma - alpha
+    for (int e = tid; e < nb * CH; e += nt) {
+        int b = e / CH, c = e % CH;
+        if (c >= ncol) continue;
+        int f, j, q1, step;
+        boundary_info(G, b, f, j, q1, step);
+        const AxisCoef& a = C.ax[j];
+        V gb = gval(b, c);
+        V x1 = Y[(size_t)q1 * CH + c], x2 = Y[(size_t)(q1 + step) * CH + c];
+        // u_B = (g - a1 x1 - a2 x2) / dd
+        V num = csub(csub(gb, cscale(x1, (R)a.a1)), cscale(x2, (R)a.a2));
+        V uB = cmul(num, ccast<V>(cinv(a.dd)));
+        V hb = cfma(ccast<V>(sma), uB, gb);
+        int col = c0 + c;
+        if (io.mode == LEAF_FACTOR_T) {
+            io.T[(size_t)leaf * nb * nb + (size_t)col * nb + local_to_node(G, parity, b)] = ccast<double2>(hb);
+        } else if (io.mode == LEAF_UP) {
+            if (io.jump) hb = csub(hb, ccast<V>(io.jump[((size_t)col * io.n_leaves + leaf) * nb + b]));
+            io.hout[((size_t)col * io.n_leaves + leaf) * nb + local_to_node(G, parity, b)] = hb;
+            io.u[((size_t)col * io.n_leaves + leaf) * G.m + b] = ccast<double2>(uB);
+        } else {
+            double2* up = io.u + ((size_t)col * io.n_leaves + leaf) * G.m + b;
+            *up = cadd(*up, ccast<double2>(uB));
+        }
+    }
+    if (io.mode != LEAF_FACTOR_T) {
+        for (int e = tid; e < ni * CH; e += nt) {
+            int q = e / CH, c = e % CH;
+            if (c >= ncol) continue;
+            double2* up = io.u + ((size_t)(c0 + c) * io.n_leaves + leaf) * G.m + nb + q;
+            double2 v = ccast<double2>(Y[e]);
+            *up = (io.mode == LEAF_UP) ? v : cadd(*up, v);
+        }
+    }
+}
+
+}  // namespace
+
+size_t leaf_blocks_lds(const LeafGeom& G) {
+    return (size_t)(2 * G.s * G.s + G.s) * sizeof(double2) + (size_t)G.s * sizeof(int) + 64;
+}
+
+void launch_leaf_blocks(const LeafGeom& G, const double2* kint, double2* sinv, int* info, int n_leaves,
+                        hipStream_t st) {
+    LeafCoef C = make_coef(G);
+    size_t lds = leaf_blocks_lds(G);
+    hipLaunchKernelGGL(leaf_blocks_kernel, dim3(n_leaves), dim3(256), lds, st, G, C, kint, sinv, info);
+}
+
+template <typename R> int leaf_apply_chunk(const LeafGeom& G, int ncols, size_t lds_limit) {
+    // largest CH (<= 16) whose LDS footprint fits the device limit
+    size_t e = sizeof(c_t<R>);
+    int ch = 16;
+    while (ch > 1 && ((size_t)G.n_int * ch + (size_t)G.s * G.s + (size_t)G.s * ch) * e + 256 > lds_limit) ch >>= 1;
+    if (ncols < ch) ch = ncols < 1 ? 1 : ncols;
+    return ch;
+}
+
+template <typename R>
+void launch_leaf_apply(const LeafGeom& G, int mode, int ncols, int n_leaves, const c_t<R>* sinv, const double2* f,
+                       const double2* jump, const c_t<R>* gin, c_t<R>* hout, double2* u, double2* T,
+                       size_t lds_limit, hipStream_t st) {
+    LeafCoef C = make_coef(G);
+    LeafIO<R> io;
+    io.mode = mode; io.ncols = ncols; io.CH = leaf_apply_chunk<R>(G, ncols, lds_limit);
+    io.sinv = sinv; io.f = f; io.jump = jump; io.gin = gin; io.hout = hout; io.u = u; io.T = T;
+    io.n_leaves = n_leaves;
+    size_t lds = ((size_t)G.n_int * io.CH + (size_t)G.s * G.s + (size_t)G.s * io.CH) * sizeof(c_t<R>);
+    dim3 grid(n_leaves, (ncols + io.CH - 1) / io.CH);
+    hipLaunchKernelGGL(leaf_apply_kernel<R>, grid, dim3(256), lds, st, G, C, io);
+}
+
+template int leaf_apply_chunk<double>(const LeafGeom&, int, size_t);
+template int leaf_apply_chunk<float>(const LeafGeom&, int, size_t);
+template void launch_leaf_apply<double>(const LeafGeom&, int, int, int, const double2*, const double2*, const double2*,
+                                        const double2*, double2*, double2*, double2*, size_t, hipStream_t);
+template void launch_leaf_apply<float>(const LeafGeom&, int, int, int, const float2*, const double2*, const double2*,
+                                       const float2*, float2*, double2*, double2*, size_t, hipStream_t);
+
+}  // namespace rs

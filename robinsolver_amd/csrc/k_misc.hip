@@ -1,0 +1,127 @@
+// k_misc.hip -- data-movement kernels of the merge and the solve passes: the
+// interface blocks of the merge (robin_tree_module.f90:440-467), the face
+// scatter/gather between a node and its children (robin_tree_module.f90:896-959,
+// 1045-1075, 1205-1230), and precision conversion of the stored factors.
+#include "rs_common.hpp"
+#include "rs_kernels.hpp"
+
+namespace rs {
+
+namespace {
+
+// out[n] = zeta I - T_child(n)[off:off+i, off:off+i] + coef * D[n]
+__global__ void form_iface_kernel(double2* __restrict__ out, const double2* __restrict__ T, size_t tstride,
+                                  int child_mul, int child_add, int b, int off, int isz, double2 zeta,
+                                  double2 coef, const double2* __restrict__ D) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t ii = (size_t)isz * isz;
+    if (e >= ii) return;
+    const int node = blockIdx.y;
+    const int r = (int)(e % isz), c = (int)(e / isz);
+    const double2* Tc = T + (size_t)(child_mul * node + child_add) * tstride;
+    double2 v = make_double2(-Tc[(size_t)(off + r) + (size_t)(off + c) * b].x, -Tc[(size_t)(off + r) + (size_t)(off + c) * b].y);
+    if (r == c) v = cadd(v, zeta);
+    if (D) v = cfma(coef, D[(size_t)node * ii + e], v);
+    out[(size_t)node * ii + e] = v;
+}
+
+// dst[bat] (rows x cols, ldd) = scale * src[bat] (lds)
+__global__ void copy_blocks_kernel(double2* __restrict__ dst, size_t dstride, int ldd, const double2* __restrict__ src,
+                                   size_t sstride, int lds, int rows, int cols, double2 scale) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)rows * cols) return;
+    const int r = (int)(e % rows), c = (int)(e / rows);
+    const size_t bat = blockIdx.y;
+    dst[bat * dstride + r + (size_t)c * ldd] = cmul(scale, src[bat * sstride + r + (size_t)c * lds]);
+}
+
+template <typename V>
+__global__ void root_gather_kernel(V* __restrict__ out, const double2* __restrict__ gin, const int* __restrict__ leaf_of,
+                                   const int* __restrict__ loc_of, int P, int n_leaves, int nbl) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int col = blockIdx.y;
+    if (p >= P) return;
+    out[(size_t)col * P + p] = ccast<V>(gin[((size_t)col * n_leaves + leaf_of[p]) * nbl + loc_of[p]]);
+}
+
+// child N-part gather: child (2n+c) rows [0, bn) <- parent n rows prow[parity(n)][c][j]
+template <typename V>
+__global__ void child_gather_kernel(V* __restrict__ child, const V* __restrict__ parent, const int* __restrict__ prow,
+                                    int bn, int bc, int P, int nodes, int root) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= bn) return;
+    const int node = blockIdx.y >> 1, c = blockIdx.y & 1;
+    const int col = blockIdx.z;
+    const int par = root ? 0 : (node & 1);
+    const int* map = prow + (size_t)(par * 2 + c) * bn;
+    child[(size_t)col * (2 * nodes) * bc + (size_t)(2 * node + c) * bc + j] =
+        parent[(size_t)col * nodes * P + (size_t)node * P + map[j]];
+}
+
+__global__ void to_c64_kernel(float2* __restrict__ dst, const double2* __restrict__ src, size_t n) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) dst[e] = make_float2((float)src[e].x, (float)src[e].y);
+}
+
+__global__ void zero_kernel(double2* __restrict__ p, size_t n) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[e] = make_double2(0.0, 0.0);
+}
+
+__global__ void axpy_kernel(double2* __restrict__ y, const double2* __restrict__ x, size_t n) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) y[e] = cadd(y[e], x[e]);
+}
+
+inline unsigned nblk(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+void launch_form_iface(double2* out, const double2* T, size_t tstride, int child_mul, int child_add, int b, int off,
+                       int isz, int nodes, double2 zeta, double2 coef, const double2* D, hipStream_t st) {
+    hipLaunchKernelGGL(form_iface_kernel, dim3(nblk((size_t)isz * isz), nodes), dim3(256), 0, st, out, T, tstride,
+                       child_mul, child_add, b, off, isz, zeta, coef, D);
+}
+
+void launch_copy_blocks(double2* dst, size_t dstride, int ldd, const double2* src, size_t sstride, int lds, int rows,
+                        int cols, int batch, double2 scale, hipStream_t st) {
+    hipLaunchKernelGGL(copy_blocks_kernel, dim3(nblk((size_t)rows * cols), batch), dim3(256), 0, st, dst, dstride,
+                       ldd, src, sstride, lds, rows, cols, scale);
+}
+
+template <typename V>
+void launch_root_gather(V* out, const double2* gin, const int* leaf_of, const int* loc_of, int P, int n_leaves,
+                        int nbl, int ncols, hipStream_t st) {
+    hipLaunchKernelGGL(root_gather_kernel<V>, dim3(nblk(P), ncols), dim3(256), 0, st, out, gin, leaf_of, loc_of, P,
+                       n_leaves, nbl);
+}
+
+template <typename V>
+void launch_child_gather(V* child, const V* parent, const int* prow, int bn, int bc, int P, int nodes, int root,
+                         int ncols, hipStream_t st) {
+    hipLaunchKernelGGL(child_gather_kernel<V>, dim3(nblk(bn), 2 * nodes, ncols), dim3(256), 0, st, child, parent,
+                       prow, bn, bc, P, nodes, root);
+}
+
+void launch_to_c64(float2* dst, const double2* src, size_t n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(to_c64_kernel, dim3(nblk(n)), dim3(256), 0, st, dst, src, n);
+}
+
+void launch_zero(double2* p, size_t n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(zero_kernel, dim3(nblk(n)), dim3(256), 0, st, p, n);
+}
+
+void launch_axpy(double2* y, const double2* x, size_t n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(axpy_kernel, dim3(nblk(n)), dim3(256), 0, st, y, x, n);
+}
+
+template void launch_root_gather<double2>(double2*, const double2*, const int*, const int*, int, int, int, int,
+                                          hipStream_t);
+template void launch_root_gather<float2>(float2*, const double2*, const int*, const int*, int, int, int, int,
+                                         hipStream_t);
+template void launch_child_gather<double2>(double2*, const double2*, const int*, int, int, int, int, int, int,
+                                           hipStream_t);
+template void launch_child_gather<float2>(float2*, const float2*, const int*, int, int, int, int, int, int,
+                                          hipStream_t);
+
+}  // namespace rs

@@ -1,0 +1,58 @@
+// rs_common.hpp -- complex helpers and shared device-side types for the
+// MI355X (gfx950) HPS factor/solve library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace rs {
+
+// Complex scalar stored as an interleaved (re, im) pair, matching Fortran
+// complex(dp) / C99 double complex memory layout.
+template <typename R> struct cpx;
+template <> struct cpx<double> { using T = double2; };
+template <> struct cpx<float> { using T = float2; };
+template <typename R> using c_t = typename cpx<R>::T;
+
+template <typename R> __host__ __device__ inline c_t<R> mk(R re, R im) { c_t<R> z; z.x = re; z.y = im; return z; }
+template <typename R> __host__ __device__ inline c_t<R> cz() { return mk<R>(R(0), R(0)); }
+template <typename V> __host__ __device__ inline V cadd(V a, V b) { V z; z.x = a.x + b.x; z.y = a.y + b.y; return z; }
+template <typename V> __host__ __device__ inline V csub(V a, V b) { V z; z.x = a.x - b.x; z.y = a.y - b.y; return z; }
+template <typename V> __host__ __device__ inline V cmul(V a, V b) {
+    V z; z.x = a.x * b.x - a.y * b.y; z.y = a.x * b.y + a.y * b.x; return z;
+}
+// a + b*c
+template <typename V> __host__ __device__ inline V cfma(V b, V c, V a) {
+    V z; z.x = a.x + b.x * c.x - b.y * c.y; z.y = a.y + b.x * c.y + b.y * c.x; return z;
+}
+template <typename V, typename S> __host__ __device__ inline V cscale(V a, S s) { V z; z.x = a.x * s; z.y = a.y * s; return z; }
+template <typename V> __host__ __device__ inline V cinv(V a) {
+    // 1/a with scaling to avoid overflow of |a|^2
+    auto s = fabs(a.x) > fabs(a.y) ? fabs(a.x) : fabs(a.y);
+    V b; b.x = a.x / s; b.y = a.y / s;
+    auto den = (b.x * b.x + b.y * b.y) * s;
+    V z; z.x = b.x / den; z.y = -b.y / den; return z;
+}
+template <typename V> __host__ __device__ inline auto cabs1(V a) { return fabs(a.x) + fabs(a.y); }
+
+template <typename Dst, typename Src> __host__ __device__ inline Dst ccast(Src a) {
+    Dst z; z.x = (decltype(z.x))a.x; z.y = (decltype(z.y))a.y; return z;
+}
+
+// Geometry shared by every leaf of one tree (all leaves of a tree have the
+// same grid; the host refuses trees where they do not).
+struct LeafGeom {
+    int d;            // 2 or 3
+    int n[3];         // interior points per axis (ptbox)
+    double h[3];      // grid spacing per axis
+    int n_int;        // product of n
+    int nb;           // boundary points
+    int m;            // nb + n_int
+    int s;            // block size: interior points per slow-axis layer
+    int K;            // number of blocks = n[d-1]
+    int pf[7];        // points per face, 1..6
+    int foff[7];      // start of face f in the leaf's local boundary order (fgrid order)
+    int noff[2][7];   // start of face f in the node order of an even / odd leaf
+    double kb;        // opts%kb: alpha = +i kb (incoming), sigma = -i kb (outgoing)
+};
+
+}  // namespace rs

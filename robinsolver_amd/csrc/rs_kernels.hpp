@@ -1,0 +1,70 @@
+// rs_kernels.hpp -- host-side launchers of the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+#include "rs_common.hpp"
+
+namespace rs {
+
+enum LeafMode { LEAF_FACTOR_T = 0, LEAF_UP = 1, LEAF_DOWN = 2 };
+
+// One problem of a batched complex GEMM:
+//   Out[rmap(r), cmap(c)] = alpha * sum_k A[r,k] B[k,c] + beta * Din[...]
+// All matrices column-major; rows k >= ksplit of B come from B2 (row k-ksplit).
+// Din is indexed (r, c) with ldd, or through the output maps when dmap != 0.
+struct GemmDesc {
+    const void* A;
+    const void* B;
+    const void* B2;
+    void* Out;
+    const void* Din;
+    const int* rmap;
+    const int* cmap;
+    int lda, ldb, ldo, ldd;
+    int ksplit, dmap;
+    double ar, ai, br, bi;
+};
+
+template <typename R>
+void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st);
+
+// leaf stage
+size_t leaf_blocks_lds(const LeafGeom& G);
+void launch_leaf_blocks(const LeafGeom& G, const double2* kint, double2* sinv, int* info, int n_leaves,
+                        hipStream_t st);
+template <typename R> int leaf_apply_chunk(const LeafGeom& G, int ncols, size_t lds_limit);
+template <typename R>
+void launch_leaf_apply(const LeafGeom& G, int mode, int ncols, int n_leaves, const c_t<R>* sinv, const double2* f,
+                       const double2* jump, const c_t<R>* gin, c_t<R>* hout, double2* u, double2* T,
+                       size_t lds_limit, hipStream_t st);
+
+// batched inverse
+struct InvPlan {
+    int n = 0, batch = 0, nb = 0;
+    size_t stride = 0, lds_limit = 0;
+    bool small = true;
+    void build(int n, int batch, size_t stride, size_t lds_limit);
+    int panels() const { return small ? 0 : n / nb; }
+    size_t w_elems() const { return small ? 0 : (size_t)nb * (n - nb) * batch; }
+};
+size_t inv_small_lds(int n);
+void launch_inverse(const InvPlan& ip, double2* mats, int* pivs, double2* W, const GemmDesc* d_descs, int* info,
+                    hipStream_t st);
+
+// data movement
+void launch_form_iface(double2* out, const double2* T, size_t tstride, int child_mul, int child_add, int b, int off,
+                       int isz, int nodes, double2 zeta, double2 coef, const double2* D, hipStream_t st);
+void launch_copy_blocks(double2* dst, size_t dstride, int ldd, const double2* src, size_t sstride, int lds, int rows,
+                        int cols, int batch, double2 scale, hipStream_t st);
+template <typename V>
+void launch_root_gather(V* out, const double2* gin, const int* leaf_of, const int* loc_of, int P, int n_leaves,
+                        int nbl, int ncols, hipStream_t st);
+template <typename V>
+void launch_child_gather(V* child, const V* parent, const int* prow, int bn, int bc, int P, int nodes, int root,
+                         int ncols, hipStream_t st);
+void launch_to_c64(float2* dst, const double2* src, size_t n, hipStream_t st);
+void launch_zero(double2* p, size_t n, hipStream_t st);
+void launch_axpy(double2* y, const double2* x, size_t n, hipStream_t st);
+
+}  // namespace rs

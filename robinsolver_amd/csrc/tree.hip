@@ -1,0 +1,594 @@
+// tree.hip -- factor and solve orchestration of the HPS tree on one MI355X.
+//
+// Factor (robin_tree_factor / compute_RTR, robin_tree_module.f90:96-185, 375-650):
+//   1. one launch factors every leaf (block-tridiagonal LU, k_leaf.hip) and one
+//      launch builds every leaf RtR map;
+//   2. per level, bottom-up, all nodes of the level at once:
+//        D = (zeta I - T2_II)^-1, S = (zeta I - T1_II - nu^2 D)^-1   (FP64, pivoted)
+//        invM = [S, -nu S D; -nu D S, D + nu^2 D S D]                 (3 GEMMs + copies)
+//        Y    = invM diag(T1_IN, T2_IN)                               (GEMM)
+//        T    = diag(T1_NN, T2_NN) + diag(T1_NI, T2_NI) Y, scattered into the
+//               parent's node order                                  (GEMM)
+//      which is the reference's C11..C22 / zgemm block algebra written with
+//      explicit inverses.  The root RtR, which no solve reads, is skipped by
+//      default.
+// Solve (forward_sub / backward_sub, robin_tree_module.f90:813-1146):
+//   up:   leaf particular solves -> per level g_I = invM [h1_I; h2_I],
+//         h_parent = P (h_N + T_NI g_I)
+//   down: root g from the physical data -> per level children's g_N by gather,
+//         g_I += invM [T1_IN g1_N; T2_IN g2_N] -> leaf updates u += A^-1 [g; 0]
+//   every step batched over the nodes of a level and over all right-hand sides.
+#include "tree.hpp"
+
+#include <chrono>
+#include <cstring>
+
+namespace rs {
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) return fail(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+    } while (0)
+
+GemmDesc desc0() {
+    GemmDesc d;
+    std::memset(&d, 0, sizeof d);
+    d.ar = 1.0;
+    d.ksplit = 1 << 30;
+    return d;
+}
+
+}  // namespace
+
+template <typename R> void SolveWork<R>::clear() {
+    for (auto* v : {&hV, &gF, &G, &Z})
+        for (DevBuf* b : *v) delete b;
+    hV.clear(); gF.clear(); G.clear(); Z.clear();
+    for (DevBuf* b : descs) delete b;
+    descs.clear();
+    d_up_g.clear(); d_up_h.clear(); d_dn_t.clear(); d_dn_g.clear();
+    f.release(); g.release(); u.release(); jump.release();
+    cap = 0;
+}
+
+Tree::Tree() { rs_opts_default(&opts); }
+
+int Tree::ensure_device() {
+    if (device_ready) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail("Error: no HIP device (the MI355X library needs a GPU).");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    lds_limit = prop.sharedMemPerBlock;
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    device_ready = true;
+    if (leaf_blocks_lds(G) > lds_limit) return fail("Error: leaf layer too large for LDS (reduce the leaf size).");
+    const Plan& P = plan;
+    for (DevBuf* b : d_prow) delete b;
+    for (DevBuf* b : d_imap) delete b;
+    d_prow.assign(P.L, nullptr);
+    d_imap.assign(P.L, nullptr);
+    for (int l = 0; l < P.L; ++l) {
+        d_prow[l] = new DevBuf; d_imap[l] = new DevBuf;
+        if (upload(*d_prow[l], P.lv[l].prow) || upload(*d_imap[l], P.lv[l].imap)) return 1;
+    }
+    if (upload(d_root_leaf, P.root_leaf) || upload(d_root_loc, P.root_loc)) return 1;
+    return 0;
+}
+
+Tree::~Tree() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    w64.clear();
+    w32.clear();
+    free_factor();
+    for (DevBuf* b : d_prow) delete b;
+    for (DevBuf* b : d_imap) delete b;
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Tree::alloc(DevBuf& b, size_t bytes) {
+    b.release();
+    if (bytes == 0) bytes = 16;
+    if (opts.hbm_budget > 0 && dev_bytes + (long long)bytes > opts.hbm_budget)
+        return fail("Error: device memory budget of this tree exceeded.");
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) { b.p = nullptr; return fail(std::string("Error: hipMalloc failed: ") + hipGetErrorString(e)); }
+    b.bytes = bytes;
+    b.counter = &dev_bytes;
+    dev_bytes += (long long)bytes;
+    return 0;
+}
+
+template <typename T_> int Tree::upload(DevBuf& b, const std::vector<T_>& v) {
+    if (alloc(b, v.size() * sizeof(T_))) return 1;
+    HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T_), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+void Tree::free_factor() {
+    auto drop = [&](std::vector<DevBuf*>& v) {
+        for (DevBuf* b : v) delete b;
+        v.clear();
+    };
+    drop(T); drop(invM); drop(T32); drop(invM32);
+    for (DevBuf* b : {&sinv, &sinv32, &info}) b->release();
+    have32 = false;
+    factored = false;
+}
+
+int Tree::plan_tree(int d, int L, const double* domain, double h_tgt, double kb, const double* bp) {
+    std::string e = plan.build(d, L, domain, h_tgt, kb, bp);
+    if (!e.empty()) return fail(e);
+    const Plan& P = plan;
+    const Level& LF = P.lv[P.L];
+    G = LeafGeom{};
+    G.d = d;
+    for (int j = 0; j < 3; ++j) { G.n[j] = j < d ? LF.ptbox[j] : 1; G.h[j] = P.h[j]; }
+    G.n_int = P.n_int; G.nb = P.nb; G.m = P.m;
+    G.K = LF.ptbox[d - 1];
+    G.s = P.n_int / G.K;
+    for (int f = 0; f < 7; ++f) {
+        G.pf[f] = LF.pf[f];
+        G.foff[f] = P.leaf_foff[f];
+        G.noff[0][f] = LF.noff[0][f];
+        G.noff[1][f] = LF.noff[1][f];
+    }
+    G.kb = kb;
+    if (device_ready) {
+        // a re-plan on a live handle: drop everything sized for the old tree
+        if (stream) (void)hipStreamSynchronize(stream);
+        free_factor();
+        w64.clear();
+        w32.clear();
+        device_ready = false;
+        if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
+    }
+    planned = true;
+    return 0;
+}
+
+int Tree::factor(const double* k_host) {
+    if (!planned) return fail("Error: rs_plan must be called before rs_factor.");
+    if (ensure_device()) return 1;
+    size_t n = (size_t)plan.n_leaves * plan.n_int;
+    DevBuf tmp;
+    if (alloc(tmp, n * sizeof(double2))) return 1;
+    HIPCHK(hipMemcpy(tmp.p, k_host, n * sizeof(double2), hipMemcpyHostToDevice));
+    return factor_device(tmp.as<double2>());
+}
+
+int Tree::factor_device(const double2* k) {
+    if (!planned) return fail("Error: rs_plan must be called before rs_factor.");
+    if (ensure_device()) return 1;
+    const double t0 = now_s();
+    free_factor();
+    w64.clear();
+    w32.clear();
+    const Plan& P = plan;
+    const int L = P.L, nl = P.n_leaves;
+    // keep the potential resident (the FP64 residual of the mixed path reads it)
+    if (alloc(kint, (size_t)nl * P.n_int * sizeof(double2))) return 1;
+    HIPCHK(hipMemcpyAsync(kint.p, k, (size_t)nl * P.n_int * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+    if (alloc(sinv, (size_t)nl * G.K * G.s * G.s * sizeof(double2)) || alloc(info, sizeof(int))) return 1;
+    HIPCHK(hipMemsetAsync(info.p, 0, sizeof(int), stream));
+    launch_leaf_blocks(G, kint.as<double2>(), sinv.as<double2>(), info.as<int>(), nl, stream);
+    T.assign(L + 1, nullptr);
+    invM.assign(L, nullptr);
+    T[L] = new DevBuf;
+    if (alloc(*T[L], (size_t)nl * P.nb * P.nb * sizeof(double2))) return 1;
+    launch_leaf_apply<double>(G, LEAF_FACTOR_T, P.nb, nl, sinv.as<double2>(), nullptr, nullptr, nullptr, nullptr,
+                              nullptr, T[L]->as<double2>(), lds_limit, stream);
+    HIPCHK(hipGetLastError());
+    const double2 zeta = make_double2(P.zeta[0], P.zeta[1]);
+    const double2 nu = make_double2(P.nu[0], P.nu[1]);
+    const double2 nu2 = cmul(nu, nu);
+    for (int l = L - 1; l >= 0; --l) {
+        const Level& V = P.lv[l];
+        const int nodes = V.nodes, i = V.isz, b = V.bc, bn = V.bn, Pp = V.P;
+        const size_t ii = (size_t)i * i, tstride = (size_t)b * b;
+        const double2* Tc = T[l + 1]->as<double2>();
+        DevBuf Dm, Sm, SD, Y, W, piv, dd_inv, dd_sd, dd_y, dd_t;
+        if (alloc(Dm, nodes * ii * 16) || alloc(Sm, nodes * ii * 16) || alloc(SD, nodes * ii * 16) ||
+            alloc(Y, (size_t)nodes * 2 * i * 2 * bn * 16))
+            return 1;
+        InvPlan ip;
+        ip.build(i, nodes, ii, lds_limit);
+        std::vector<GemmDesc> inv_descs;
+        std::vector<int> colmaps;
+        if (!ip.small) {
+            if (alloc(W, ip.w_elems() * 16) || alloc(piv, (size_t)nodes * i * sizeof(int))) return 1;
+            colmaps.resize((size_t)ip.panels() * (i - ip.nb));
+        }
+        // the GJ panel descriptors differ between the D and S passes only in
+        // the matrix base pointer: build for Dm, patch for Sm
+        auto build_inv = [&](double2* mats, DevBuf& dbuf, DevBuf& cmbuf) -> int {
+            if (ip.small) return 0;
+            inv_descs.clear();
+            const int n = i, nb_ = ip.nb;
+            for (int pi = 0; pi < ip.panels(); ++pi) {
+                int j0 = pi * nb_;
+                for (int c2 = 0; c2 < n - nb_; ++c2) colmaps[(size_t)pi * (n - nb_) + c2] = c2 < j0 ? c2 : c2 + nb_;
+            }
+            if (!cmbuf.p && upload(cmbuf, colmaps)) return 1;
+            for (int pi = 0; pi < ip.panels(); ++pi)
+                for (int bt = 0; bt < nodes; ++bt) {
+                    GemmDesc g = desc0();
+                    double2* Mb = mats + (size_t)bt * ii;
+                    g.A = Mb + (size_t)pi * nb_ * n; g.lda = n;
+                    g.B = W.as<double2>() + (size_t)bt * nb_ * (n - nb_); g.ldb = nb_;
+                    g.Out = Mb; g.ldo = n;
+                    g.cmap = cmbuf.as<int>() + (size_t)pi * (n - nb_);
+                    g.Din = Mb; g.ldd = n; g.dmap = 1; g.br = 1.0;
+                    inv_descs.push_back(g);
+                }
+            return upload(dbuf, inv_descs);
+        };
+        DevBuf cmaps, dd_inv2;
+        // D
+        launch_form_iface(Dm.as<double2>(), Tc, tstride, 2, 1, b, bn, i, nodes, zeta, make_double2(0, 0), nullptr,
+                          stream);
+        if (build_inv(Dm.as<double2>(), dd_inv, cmaps)) return 1;
+        launch_inverse(ip, Dm.as<double2>(), piv.as<int>(), W.as<double2>(), dd_inv.as<GemmDesc>(), info.as<int>(),
+                       stream);
+        // S
+        launch_form_iface(Sm.as<double2>(), Tc, tstride, 2, 0, b, bn, i, nodes, zeta,
+                          make_double2(-nu2.x, -nu2.y), Dm.as<double2>(), stream);
+        if (build_inv(Sm.as<double2>(), dd_inv2, cmaps)) return 1;
+        launch_inverse(ip, Sm.as<double2>(), piv.as<int>(), W.as<double2>(), dd_inv2.as<GemmDesc>(), info.as<int>(),
+                       stream);
+        // invM
+        invM[l] = new DevBuf;
+        if (alloc(*invM[l], nodes * 4 * ii * 16)) return 1;
+        double2* IM = invM[l]->as<double2>();
+        {
+            std::vector<GemmDesc> v;
+            for (int n = 0; n < nodes; ++n) {  // SD = S D
+                GemmDesc g = desc0();
+                g.A = Sm.as<double2>() + n * ii; g.lda = i;
+                g.B = Dm.as<double2>() + n * ii; g.ldb = i;
+                g.Out = SD.as<double2>() + n * ii; g.ldo = i;
+                v.push_back(g);
+            }
+            for (int n = 0; n < nodes; ++n) {  // B21 = -nu D S
+                GemmDesc g = desc0();
+                g.A = Dm.as<double2>() + n * ii; g.lda = i;
+                g.B = Sm.as<double2>() + n * ii; g.ldb = i;
+                g.Out = IM + n * 4 * ii + i; g.ldo = 2 * i;
+                g.ar = -nu.x; g.ai = -nu.y;
+                v.push_back(g);
+            }
+            for (int n = 0; n < nodes; ++n) {  // B22 = D + nu^2 D (S D)
+                GemmDesc g = desc0();
+                g.A = Dm.as<double2>() + n * ii; g.lda = i;
+                g.B = SD.as<double2>() + n * ii; g.ldb = i;
+                g.Out = IM + n * 4 * ii + i + (size_t)i * 2 * i; g.ldo = 2 * i;
+                g.ar = nu2.x; g.ai = nu2.y;
+                g.Din = Dm.as<double2>() + n * ii; g.ldd = i; g.br = 1.0;
+                v.push_back(g);
+            }
+            if (upload(dd_sd, v)) return 1;
+            launch_gemm<double>(dd_sd.as<GemmDesc>(), nodes, i, i, i, stream);
+            launch_gemm<double>(dd_sd.as<GemmDesc>() + nodes, 2 * nodes, i, i, i, stream);
+            launch_copy_blocks(IM, 4 * ii, 2 * i, Sm.as<double2>(), ii, i, i, i, nodes, make_double2(1, 0), stream);
+            launch_copy_blocks(IM + (size_t)i * 2 * i, 4 * ii, 2 * i, SD.as<double2>(), ii, i, i, i, nodes,
+                               make_double2(-nu.x, -nu.y), stream);
+        }
+        // Y = invM diag(T1_IN, T2_IN)
+        {
+            std::vector<GemmDesc> v;
+            for (int n = 0; n < nodes; ++n)
+                for (int c = 0; c < 2; ++c) {
+                    GemmDesc g = desc0();
+                    g.A = IM + n * 4 * ii + (size_t)c * i * 2 * i; g.lda = 2 * i;
+                    g.B = Tc + (size_t)(2 * n + c) * tstride + bn; g.ldb = b;
+                    g.Out = Y.as<double2>() + (size_t)n * 2 * i * 2 * bn + (size_t)c * bn * 2 * i; g.ldo = 2 * i;
+                    v.push_back(g);
+                }
+            if (upload(dd_y, v)) return 1;
+            launch_gemm<double>(dd_y.as<GemmDesc>(), 2 * nodes, 2 * i, bn, i, stream);
+        }
+        // parent RtR
+        if (l > 0 || !opts.skip_root_rtr) {
+            T[l] = new DevBuf;
+            if (alloc(*T[l], (size_t)nodes * Pp * Pp * 16)) return 1;
+            std::vector<GemmDesc> v;
+            const int* prow = d_prow[l]->as<int>();
+            for (int n = 0; n < nodes; ++n) {
+                int par = (l == 0) ? 0 : (n & 1);
+                for (int c = 0; c < 2; ++c)
+                    for (int c2 = 0; c2 < 2; ++c2) {
+                        GemmDesc g = desc0();
+                        const double2* Tcn = Tc + (size_t)(2 * n + c) * tstride;
+                        g.A = Tcn + (size_t)bn * b; g.lda = b;
+                        g.B = Y.as<double2>() + (size_t)n * 2 * i * 2 * bn + (size_t)c * i + (size_t)c2 * bn * 2 * i;
+                        g.ldb = 2 * i;
+                        g.Out = T[l]->as<double2>() + (size_t)n * Pp * Pp; g.ldo = Pp;
+                        g.rmap = prow + (size_t)(par * 2 + c) * bn;
+                        g.cmap = prow + (size_t)(par * 2 + c2) * bn;
+                        if (c == c2) { g.Din = Tcn; g.ldd = b; g.br = 1.0; }
+                        v.push_back(g);
+                    }
+            }
+            if (upload(dd_t, v)) return 1;
+            launch_gemm<double>(dd_t.as<GemmDesc>(), 4 * nodes, bn, bn, i, stream);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    int hinfo = 0;
+    HIPCHK(hipMemcpy(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (hinfo & 1) return fail("Error: singular leaf block in the leaf factorization.");
+    if (hinfo & 2) return fail("Error: singular interface matrix in the merge.");
+    factored = true;
+    if (opts.precision == RS_MIXED && make_c64()) return 1;
+    f_time = now_s() - t0;
+    return 0;
+}
+
+int Tree::make_c64() {
+    const Plan& P = plan;
+    auto conv = [&](DevBuf& dst, const DevBuf& src) -> int {
+        size_t n = src.bytes / sizeof(double2);
+        if (alloc(dst, n * sizeof(float2))) return 1;
+        launch_to_c64(dst.as<float2>(), src.as<double2>(), n, stream);
+        return 0;
+    };
+    for (DevBuf* b : T32) delete b;
+    for (DevBuf* b : invM32) delete b;
+    T32.assign(P.L + 1, nullptr);
+    invM32.assign(P.L, nullptr);
+    if (conv(sinv32, sinv)) return 1;
+    for (int l = 1; l <= P.L; ++l) { T32[l] = new DevBuf; if (conv(*T32[l], *T[l])) return 1; }
+    for (int l = 0; l < P.L; ++l) { invM32[l] = new DevBuf; if (conv(*invM32[l], *invM[l])) return 1; }
+    HIPCHK(hipStreamSynchronize(stream));
+    have32 = true;
+    return 0;
+}
+
+template <typename R> int Tree::ensure_work(SolveWork<R>& w, int nrhs) {
+    if (w.cap >= nrhs) return 0;
+    w.clear();
+    using V = c_t<R>;
+    const Plan& P = plan;
+    const int L = P.L;
+    const std::vector<DevBuf*>& Tt = std::is_same<R, double>::value ? T : T32;
+    const std::vector<DevBuf*>& IMt = std::is_same<R, double>::value ? invM : invM32;
+    const int cap = nrhs < 1 ? 1 : nrhs;
+    w.hV.assign(L + 1, nullptr); w.gF.assign(L + 1, nullptr);
+    w.G.assign(L, nullptr); w.Z.assign(L, nullptr);
+    for (int l = 0; l <= L; ++l) {
+        size_t n = (size_t)cap * P.lv[l].nodes * P.lv[l].P * sizeof(V);
+        w.gF[l] = new DevBuf;
+        if (alloc(*w.gF[l], n)) return 1;
+        if (l > 0 || L == 0) { w.hV[l] = new DevBuf; if (alloc(*w.hV[l], n)) return 1; }
+    }
+    for (int l = 0; l < L; ++l) {
+        size_t n = (size_t)cap * P.lv[l].nodes * 2 * P.lv[l].isz * sizeof(V);
+        w.G[l] = new DevBuf; w.Z[l] = new DevBuf;
+        if (alloc(*w.G[l], n) || alloc(*w.Z[l], n)) return 1;
+    }
+    w.d_up_g.assign(L, nullptr); w.d_up_h.assign(L, nullptr); w.d_dn_t.assign(L, nullptr); w.d_dn_g.assign(L, nullptr);
+    for (int l = 0; l < L; ++l) {
+        const Level& Vl = P.lv[l];
+        const Level& C = P.lv[l + 1];
+        const int nodes = Vl.nodes, i = Vl.isz, b = Vl.bc, bn = Vl.bn, Pp = Vl.P;
+        const size_t ii = (size_t)i * i, tstride = (size_t)b * b;
+        const V* IM = IMt[l]->template as<V>();
+        const V* Tc = Tt[l + 1]->template as<V>();
+        V* hC = w.hV[l + 1]->template as<V>();
+        V* hP = (l > 0) ? w.hV[l]->template as<V>() : nullptr;
+        V* Gl = w.G[l]->template as<V>();
+        V* Zl = w.Z[l]->template as<V>();
+        V* gC = w.gF[l + 1]->template as<V>();
+        const int ldc = C.nodes * b, ldg = nodes * 2 * i, ldp = nodes * Pp;
+        std::vector<GemmDesc> ug, uh, dt, dg;
+        for (int n = 0; n < nodes; ++n) {
+            int par = (l == 0) ? 0 : (n & 1);
+            GemmDesc g = desc0();
+            g.A = IM + n * 4 * ii; g.lda = 2 * i;
+            g.B = hC + (size_t)(2 * n) * b + bn; g.B2 = hC + (size_t)(2 * n + 1) * b + bn; g.ksplit = i; g.ldb = ldc;
+            g.Out = Gl + (size_t)n * 2 * i; g.ldo = ldg;
+            ug.push_back(g);
+            GemmDesc q = desc0();
+            q.A = IM + n * 4 * ii; q.lda = 2 * i;
+            q.B = Zl + (size_t)n * 2 * i; q.ldb = ldg;
+            q.Din = Gl + (size_t)n * 2 * i; q.ldd = ldg; q.br = 1.0;
+            q.Out = gC + (size_t)(2 * n) * b; q.ldo = ldc;
+            q.rmap = d_imap[l]->as<int>();
+            dg.push_back(q);
+            for (int c = 0; c < 2; ++c) {
+                const V* Tcn = Tc + (size_t)(2 * n + c) * tstride;
+                if (l > 0) {
+                    GemmDesc u = desc0();
+                    u.A = Tcn + (size_t)bn * b; u.lda = b;
+                    u.B = Gl + (size_t)n * 2 * i + c * i; u.ldb = ldg;
+                    u.Out = hP + (size_t)n * Pp; u.ldo = ldp;
+                    u.rmap = d_prow[l]->as<int>() + (size_t)(par * 2 + c) * bn;
+                    u.Din = hC + (size_t)(2 * n + c) * b; u.ldd = ldc; u.br = 1.0;
+                    uh.push_back(u);
+                }
+                GemmDesc t = desc0();
+                t.A = Tcn + bn; t.lda = b;
+                t.B = gC + (size_t)(2 * n + c) * b; t.ldb = ldc;
+                t.Out = Zl + (size_t)n * 2 * i + c * i; t.ldo = ldg;
+                dt.push_back(t);
+            }
+        }
+        for (auto* pr : {&ug, &uh, &dt, &dg}) {
+            DevBuf* db = new DevBuf;
+            w.descs.push_back(db);
+            if (!pr->empty() && upload(*db, *pr)) return 1;
+        }
+        size_t k0 = w.descs.size() - 4;
+        w.d_up_g[l] = w.descs[k0]->template as<GemmDesc>();
+        w.d_up_h[l] = w.descs[k0 + 1]->template as<GemmDesc>();
+        w.d_dn_t[l] = w.descs[k0 + 2]->template as<GemmDesc>();
+        w.d_dn_g[l] = w.descs[k0 + 3]->template as<GemmDesc>();
+    }
+    w.cap = cap;
+    return 0;
+}
+
+void Tree::pbeg(int cat, hipStream_t st) {
+    if (!prof_on) return;
+    ProfRec r;
+    r.cat = cat;
+    for (hipEvent_t* e : {&r.a, &r.b}) {
+        if (ev_pool.empty()) { (void)hipEventCreate(e); }
+        else { *e = ev_pool.back(); ev_pool.pop_back(); }
+    }
+    r.bytes = r.flops = 0;
+    (void)hipEventRecord(r.a, st);
+    prof.push_back(r);
+}
+
+void Tree::pend(double bytes, double flops, hipStream_t st) {
+    if (!prof_on || prof.empty()) return;
+    ProfRec& r = prof.back();
+    r.bytes = bytes;
+    r.flops = flops;
+    (void)hipEventRecord(r.b, st);
+}
+
+int Tree::prof_collect() {
+    for (ProfRec& r : prof) {
+        HIPCHK(hipEventSynchronize(r.b));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+        prof_ms[r.cat] += ms;
+        prof_n[r.cat] += 1;
+        prof_bytes[r.cat] += r.bytes;
+        prof_flops[r.cat] += r.flops;
+        ev_pool.push_back(r.a);
+        ev_pool.push_back(r.b);
+    }
+    prof.clear();
+    return 0;
+}
+
+template <typename R>
+int Tree::solve_pass(SolveWork<R>& w, int nrhs, const double2* f, const double2* jump, const double2* g, double2* u,
+                     hipStream_t st) {
+    using V = c_t<R>;
+    const Plan& P = plan;
+    const int L = P.L, nl = P.n_leaves;
+    const V* sv = std::is_same<R, double>::value ? (const V*)sinv.p : (const V*)sinv32.p;
+    const double e = sizeof(V), nr = nrhs;
+    // algorithmic traffic of a leaf pass: the block inverses once, plus the
+    // per-RHS vectors (f in, u out, h out / g in, u in+out)
+    const double sinv_b = (double)nl * G.K * G.s * G.s * e;
+    const double leaf_fl = (double)nl * nr * (2.0 * G.K * G.s * G.s * 8.0);
+    pbeg(P_LEAF_UP, st);
+    launch_leaf_apply<R>(G, LEAF_UP, nrhs, nl, sv, f, jump, nullptr, w.hV[L]->template as<V>(), u, nullptr,
+                         lds_limit, st);
+    pend(sinv_b + nr * nl * (16.0 * (P.n_int + P.m) + e * P.nb), leaf_fl, st);
+    auto gemm = [&](int cat, const GemmDesc* d, int np, int M, int N, int K, bool din) {
+        pbeg(cat, st);
+        launch_gemm<R>(d, np, M, N, K, st);
+        pend((double)np * e * ((double)M * K + (double)K * N + (double)M * N * (din ? 2 : 1)),
+             8.0 * np * (double)M * N * K, st);
+    };
+    for (int l = L - 1; l >= 0; --l) {
+        const Level& Vl = P.lv[l];
+        gemm(P_GEMM_UP_G, w.d_up_g[l], Vl.nodes, 2 * Vl.isz, nrhs, 2 * Vl.isz, false);
+        if (l > 0) gemm(P_GEMM_UP_H, w.d_up_h[l], 2 * Vl.nodes, Vl.bn, nrhs, Vl.isz, true);
+    }
+    pbeg(P_GATHER, st);
+    launch_root_gather<V>(w.gF[0]->template as<V>(), g, d_root_leaf.as<int>(), d_root_loc.as<int>(), P.lv[0].P, nl,
+                          P.nb, nrhs, st);
+    pend(nr * P.lv[0].P * (16.0 + e), 0, st);
+    for (int l = 0; l < L; ++l) {
+        const Level& Vl = P.lv[l];
+        pbeg(P_GATHER, st);
+        launch_child_gather<V>(w.gF[l + 1]->template as<V>(), w.gF[l]->template as<V>(), d_prow[l]->as<int>(),
+                               Vl.bn, Vl.bc, Vl.P, Vl.nodes, l == 0, nrhs, st);
+        pend(nr * 2.0 * Vl.nodes * Vl.bn * 2 * e, 0, st);
+        gemm(P_GEMM_DN_T, w.d_dn_t[l], 2 * Vl.nodes, Vl.isz, nrhs, Vl.bn, false);
+        gemm(P_GEMM_DN_G, w.d_dn_g[l], Vl.nodes, 2 * Vl.isz, nrhs, 2 * Vl.isz, true);
+    }
+    pbeg(P_LEAF_DOWN, st);
+    launch_leaf_apply<R>(G, LEAF_DOWN, nrhs, nl, sv, nullptr, nullptr, w.gF[L]->template as<V>(), nullptr, u, nullptr,
+                         lds_limit, st);
+    pend(sinv_b + nr * nl * (32.0 * P.m + e * P.nb), leaf_fl, st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int Tree::solve_device(int nrhs, const double2* f, const double2* g, double2* u, int prec, hipStream_t st) {
+    if (!factored) return fail("Error: Robin Tree must be factored first.");
+    if (nrhs < 1) return fail("Error: nrhs must be positive.");
+    if (prec < 0) prec = opts.precision;
+    if (!st) st = stream;
+    if (prec == RS_MIXED) {
+        if (!have32 && make_c64()) return 1;
+        if (ensure_work(w32, nrhs)) return 1;
+        return solve_pass(w32, nrhs, f, nullptr, g, u, st);
+    }
+    if (ensure_work(w64, nrhs)) return 1;
+    return solve_pass(w64, nrhs, f, nullptr, g, u, st);
+}
+
+int Tree::solve_host(int nrhs, const double* f, const double* g, int prec) {
+    if (!factored) return fail("Error: Robin Tree must be factored first.");
+    if (nrhs < 1) return fail("Error: nrhs must be positive.");
+    const double t0 = now_s();
+    const Plan& P = plan;
+    size_t nf = (size_t)nrhs * P.n_leaves * P.n_int, ng = (size_t)nrhs * P.n_leaves * P.nb,
+           nu = (size_t)nrhs * P.n_leaves * P.m;
+    SolveWork<double>& w = w64;  // host staging lives in the FP64 workspace
+    if (w.f.bytes < nf * 16 && alloc(w.f, nf * 16)) return 1;
+    if (w.g.bytes < ng * 16 && alloc(w.g, ng * 16)) return 1;
+    if (w.u.bytes < nu * 16 && alloc(w.u, nu * 16)) return 1;
+    HIPCHK(hipMemcpyAsync(w.f.p, f, nf * 16, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(w.g.p, g, ng * 16, hipMemcpyHostToDevice, stream));
+    if (solve_device(nrhs, w.f.as<double2>(), w.g.as<double2>(), w.u.as<double2>(), prec, stream)) return 1;
+    HIPCHK(hipStreamSynchronize(stream));
+    last_nrhs = nrhs;
+    s_time = now_s() - t0;
+    return 0;
+}
+
+int Tree::get_solution(int rhs, double* u) const {
+    if (!factored) return const_cast<Tree*>(this)->fail("Error in GET_SOLUTION: Robin Tree not factored.");
+    if (rhs < 0 || rhs >= last_nrhs) return const_cast<Tree*>(this)->fail("Error: no such right-hand side.");
+    size_t n = (size_t)plan.n_leaves * plan.m;
+    hipError_t e = hipMemcpy(u, w64.u.as<double2>() + (size_t)rhs * n, n * 16, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return const_cast<Tree*>(this)->fail(hipGetErrorString(e));
+    return 0;
+}
+
+int Tree::node_rtr(int lvl, int idx, double* out, int* Pout) const {
+    Tree* self = const_cast<Tree*>(this);
+    if (!factored) return self->fail("Error: not factored.");
+    if (lvl < 0 || lvl > plan.L || idx < 0 || idx >= plan.lv[lvl].nodes) return self->fail("Error: no such node.");
+    if (!T[lvl] || !T[lvl]->p) return self->fail("Error: RtR of this node is not formed.");
+    const int Pp = plan.lv[lvl].P;
+    if (Pout) *Pout = Pp;
+    if (!out) return 0;
+    std::vector<double2> h((size_t)Pp * Pp);
+    hipError_t e = hipMemcpy(h.data(), T[lvl]->as<double2>() + (size_t)idx * Pp * Pp, h.size() * 16,
+                             hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return self->fail(hipGetErrorString(e));
+    std::vector<int> perm = plan.node_to_face_order(lvl, lvl == 0 ? 0 : (idx & 1));
+    double2* o = (double2*)out;
+    for (int c = 0; c < Pp; ++c)
+        for (int r = 0; r < Pp; ++r) o[(size_t)perm[r] + (size_t)perm[c] * Pp] = h[(size_t)r + (size_t)c * Pp];
+    return 0;
+}
+
+template int Tree::ensure_work<double>(SolveWork<double>&, int);
+template int Tree::ensure_work<float>(SolveWork<float>&, int);
+
+}  // namespace rs

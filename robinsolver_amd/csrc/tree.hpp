@@ -1,0 +1,112 @@
+// tree.hpp -- device-resident HPS tree: factor and solve orchestration.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/robinsolver.h"
+#include "plan.hpp"
+#include "rs_kernels.hpp"
+
+namespace rs {
+
+// Device allocation owned by a tree; frees itself.
+// The owning tree's byte counter is charged on allocation and credited on release.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    long long* counter = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        if (counter) *counter -= (long long)bytes;
+        p = nullptr; bytes = 0; counter = nullptr;
+    }
+    template <typename T> T* as() const { return (T*)p; }
+};
+
+// Solve workspace for one precision and a capacity of RHS columns.
+template <typename R> struct SolveWork {
+    using V = c_t<R>;
+    int cap = 0;
+    std::vector<DevBuf*> hV, gF, G, Z;   // per level
+    DevBuf f, g, u, jump;                // host-API staging (double2)
+    std::vector<DevBuf*> descs;          // per (level, kind)
+    std::vector<GemmDesc*> d_up_g, d_up_h, d_dn_t, d_dn_g;
+    ~SolveWork() { clear(); }
+    void clear();
+};
+
+enum ProfCat { P_LEAF_UP = 0, P_LEAF_DOWN, P_GEMM_UP_G, P_GEMM_UP_H, P_GEMM_DN_T, P_GEMM_DN_G, P_GATHER,
+               P_RESIDUAL, P_NCAT };
+
+struct ProfRec {
+    int cat;
+    hipEvent_t a, b;
+    double bytes, flops;
+};
+
+struct Tree {
+    rs_opts opts;
+    Plan plan;
+    bool planned = false, factored = false, device_ready = false;
+    double f_time = 0, s_time = 0;
+    std::string err;
+    hipStream_t stream = nullptr;
+    size_t lds_limit = 65536;
+    LeafGeom G{};
+    long long dev_bytes = 0;
+
+    // factor data (FP64)
+    DevBuf kint, sinv, info;
+    std::vector<DevBuf*> T;      // per level: [nodes][P][P] node order
+    std::vector<DevBuf*> invM;   // per level < L: [nodes][2i][2i]
+    // complex64 copies for the mixed solve
+    DevBuf sinv32;
+    std::vector<DevBuf*> T32, invM32;
+    bool have32 = false;
+    // maps
+    std::vector<DevBuf*> d_prow, d_imap;
+    DevBuf d_root_leaf, d_root_loc;
+
+    // kernel timing
+    bool prof_on = false;
+    std::vector<ProfRec> prof;
+    std::vector<hipEvent_t> ev_pool;
+    double prof_ms[P_NCAT] = {0}, prof_bytes[P_NCAT] = {0}, prof_flops[P_NCAT] = {0};
+    int prof_n[P_NCAT] = {0};
+    void pbeg(int cat, hipStream_t st);
+    void pend(double bytes, double flops, hipStream_t st);
+    int prof_collect();
+
+    SolveWork<double> w64;
+    SolveWork<float> w32;
+    int last_nrhs = 0;
+    int last_prec = RS_FP64;
+
+    Tree();
+    ~Tree();
+    int fail(const std::string& m) { err = m; return 1; }
+    int alloc(DevBuf& b, size_t bytes);
+    template <typename T_> int upload(DevBuf& b, const std::vector<T_>& v);
+    void free_factor();
+
+    int ensure_device();
+    int plan_tree(int d, int L, const double* domain, double h_tgt, double kb, const double* bp);
+    int factor(const double* k_int_host);
+    int factor_device(const double2* k_int_dev);
+    int make_c64();
+    template <typename R> int ensure_work(SolveWork<R>& w, int nrhs);
+    template <typename R> int solve_pass(SolveWork<R>& w, int nrhs, const double2* f, const double2* jump,
+                                         const double2* g, double2* u, hipStream_t st);
+    int solve_device(int nrhs, const double2* f, const double2* g, double2* u, int prec, hipStream_t st);
+    int solve_host(int nrhs, const double* f, const double* g, int prec);
+    int get_solution(int rhs, double* u) const;
+    int node_rtr(int lvl, int idx, double* out, int* P) const;
+};
+
+}  // namespace rs
