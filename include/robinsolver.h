@@ -63,6 +63,10 @@ int rs_plan(rs_tree *t, int d, int L, const double *domain, double h_tgt, double
  * fd_module.f90:669-700) and the face id (1..6, 0 = interior) of each point.
  * face may be NULL. */
 int rs_leaf_points(const rs_tree *t, int leaf, double *xyz, int *face);
+/* Leaf grid: points per axis (ptbox) and spacing h, the values the reference
+ * stores in node%ptbox and T%opts%h (fd_module.f90:1060-1084,
+ * robin_tree_module.f90:392).  h(3) = 0 and ptbox(3) = -1 in 2D. */
+int rs_leaf_grid(const rs_tree *t, int *ptbox, double *h);
 /* Leaf box (2,3) column-major (leaf_solution%box, robin_output_module.f90:32-37). */
 int rs_leaf_box(const rs_tree *t, int leaf, double *box);
 

@@ -64,6 +64,16 @@ int rs_leaf_points(const rs_tree* t, int leaf, double* xyz, int* face) {
     return 0;
 }
 
+int rs_leaf_grid(const rs_tree* t, int* ptbox, double* h) {
+    if (!t || !t->t.planned) return 1;
+    const rs::Plan& P = t->t.plan;
+    for (int j = 0; j < 3; ++j) {
+        if (ptbox) ptbox[j] = j < P.d ? P.lv[P.L].ptbox[j] : -1;
+        if (h) h[j] = P.h[j];
+    }
+    return 0;
+}
+
 int rs_leaf_box(const rs_tree* t, int leaf, double* box) {
     if (!t || !t->t.planned || !box) return 1;
     if (leaf < 0 || leaf >= t->t.plan.n_leaves) return const_cast<rs::Tree&>(t->t).fail("Error: no such leaf.");

@@ -247,6 +247,7 @@ template <typename R> struct LeafIO {
     double2* u;               // UP (overwrite) / DOWN (accumulate): [col][leaf][m]
     double2* T;               // FACTOR: [leaf][nb][nb] node order, column-major
     int n_leaves;
+    int accumulate;           // UP: add to u instead of overwriting it
 };
 
 template <typename R>
@@ -347,7 +348,8 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
         } else if (io.mode == LEAF_UP) {
             if (io.jump) hb = csub(hb, ccast<V>(io.jump[((size_t)col * io.n_leaves + leaf) * nb + b]));
             io.hout[((size_t)col * io.n_leaves + leaf) * nb + local_to_node(G, parity, b)] = hb;
-            io.u[((size_t)col * io.n_leaves + leaf) * G.m + b] = ccast<double2>(uB);
+            double2* up = io.u + ((size_t)col * io.n_leaves + leaf) * G.m + b;
+            *up = io.accumulate ? cadd(*up, ccast<double2>(uB)) : ccast<double2>(uB);
         } else {
             double2* up = io.u + ((size_t)col * io.n_leaves + leaf) * G.m + b;
             *up = cadd(*up, ccast<double2>(uB));
@@ -359,12 +361,88 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
             if (c >= ncol) continue;
             double2* up = io.u + ((size_t)(c0 + c) * io.n_leaves + leaf) * G.m + nb + q;
             double2 v = ccast<double2>(Y[e]);
-            *up = (io.mode == LEAF_UP) ? v : cadd(*up, v);
+            *up = (io.mode == LEAF_UP && !io.accumulate) ? v : cadd(*up, v);
         }
     }
 }
 
+// FP64 residual of the global FD system at the current leaf solutions u:
+//   interior rows       rf = f - (Delta_h + k) u                     (fd_module.f90:140-181)
+//   physical rows       rg = g - (alpha + D_n) u                     (fd_module.f90:187-216)
+//   transmission rows   rj = zeta g_l + nu g_nb - h_l,  g = (alpha + D_n) u, h = (sigma + D_n) u
+// The transmission condition is the pointwise form of the merge system
+// M [g1; g2] = [h1; h2] (robin_tree_module.f90:425-467): it holds at every pair
+// of coincident leaf boundary points once the tree has solved.  A correction
+// solve takes (rf, rg) as (f, g) and subtracts rj from the leaf outgoing data.
+__global__ void residual_kernel(LeafGeom G, LeafCoef C, const double2* __restrict__ u, const double2* __restrict__ kint,
+                                const double2* __restrict__ f, const double2* __restrict__ g,
+                                const int* __restrict__ nbr, double2 nu, double2 zeta, double2* __restrict__ rf,
+                                double2* __restrict__ rg, double2* __restrict__ rj, int n_leaves) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int leaf = blockIdx.y, col = blockIdx.z;
+    if (p >= G.m) return;
+    const int nb = G.nb, ni = G.n_int;
+    const size_t lc = (size_t)col * n_leaves + leaf;
+    const double2* ul = u + lc * G.m;
+    const double2 alpha = make_double2(0.0, G.kb), sma = make_double2(0.0, -2.0 * G.kb);
+    auto incoming = [&](const double2* uu, int b) -> double2 {
+        int f_, j_, q1, st;
+        boundary_info(G, b, f_, j_, q1, st);
+        const AxisCoef& a = C.ax[j_];
+        double2 v = cmul(a.dd, uu[b]);
+        v = cadd(v, cscale(uu[nb + q1], a.a1));
+        return cadd(v, cscale(uu[nb + q1 + st], a.a2));
+    };
+    if (p < nb) {
+        int fc = 1;
+        while (fc < 2 * G.d && p >= G.foff[fc] + G.pf[fc]) ++fc;
+        const double2 gA = incoming(ul, p);
+        const int nl2 = nbr[(size_t)leaf * 7 + fc];
+        if (nl2 < 0) {
+            rg[lc * nb + p] = csub(g[lc * nb + p], gA);
+            rj[lc * nb + p] = make_double2(0.0, 0.0);
+        } else {
+            const int f2 = (fc & 1) ? fc + 1 : fc - 1;
+            const int b2 = G.foff[f2] + (p - G.foff[fc]);
+            const double2 gN = incoming(u + ((size_t)col * n_leaves + nl2) * G.m, b2);
+            const double2 hA = cfma(sma, ul[p], gA);
+            rj[lc * nb + p] = csub(cadd(cmul(zeta, gA), cmul(nu, gN)), hA);
+            rg[lc * nb + p] = make_double2(0.0, 0.0);
+        }
+        (void)alpha;
+        return;
+    }
+    const int q = p - nb;
+    int gq[3];
+    interior_coords(G, q, gq);
+    double2 Au = cmul(kint[(size_t)leaf * ni + q], ul[p]);
+    for (int j = 0; j < G.d; ++j) {
+        const AxisCoef& a = C.ax[j];
+        const double ih2 = a.mid;
+        const int st = stride_of(G, j);
+        double wm, w0, wp;
+        if (gq[j] == 1) { wm = 8.0 / 3.0; w0 = -4.0; wp = 4.0 / 3.0; }
+        else if (gq[j] == G.n[j]) { wm = 4.0 / 3.0; w0 = -4.0; wp = 8.0 / 3.0; }
+        else { wm = 1.0; w0 = -2.0; wp = 1.0; }
+        const double2 um = (gq[j] == 1) ? ul[face_point(G, 2 * j + 1, gq)] : ul[nb + q - st];
+        const double2 upp = (gq[j] == G.n[j]) ? ul[face_point(G, 2 * j + 2, gq)] : ul[nb + q + st];
+        Au = cadd(Au, cscale(ul[p], w0 * ih2));
+        Au = cadd(Au, cscale(um, wm * ih2));
+        Au = cadd(Au, cscale(upp, wp * ih2));
+    }
+    rf[lc * ni + q] = csub(f[lc * ni + q], Au);
+}
+
 }  // namespace
+
+void launch_residual(const LeafGeom& G, const double2* u, const double2* kint, const double2* f, const double2* g,
+                     const int* nbr, double2 nu, double2 zeta, double2* rf, double2* rg, double2* rj, int n_leaves,
+                     int ncols, hipStream_t st) {
+    LeafCoef C = make_coef(G);
+    dim3 grid((G.m + 127) / 128, n_leaves, ncols);
+    hipLaunchKernelGGL(residual_kernel, grid, dim3(128), 0, st, G, C, u, kint, f, g, nbr, nu, zeta, rf, rg, rj,
+                       n_leaves);
+}
 
 size_t leaf_blocks_lds(const LeafGeom& G) {
     return (size_t)(2 * G.s * G.s + G.s) * sizeof(double2) + (size_t)G.s * sizeof(int) + 64;
@@ -389,9 +467,10 @@ template <typename R> int leaf_apply_chunk(const LeafGeom& G, int ncols, size_t 
 template <typename R>
 void launch_leaf_apply(const LeafGeom& G, int mode, int ncols, int n_leaves, const c_t<R>* sinv, const double2* f,
                        const double2* jump, const c_t<R>* gin, c_t<R>* hout, double2* u, double2* T,
-                       size_t lds_limit, hipStream_t st) {
+                       size_t lds_limit, hipStream_t st, int accumulate) {
     LeafCoef C = make_coef(G);
     LeafIO<R> io;
+    io.accumulate = accumulate;
     io.mode = mode; io.ncols = ncols; io.CH = leaf_apply_chunk<R>(G, ncols, lds_limit);
     io.sinv = sinv; io.f = f; io.jump = jump; io.gin = gin; io.hout = hout; io.u = u; io.T = T;
     io.n_leaves = n_leaves;
@@ -403,8 +482,8 @@ void launch_leaf_apply(const LeafGeom& G, int mode, int ncols, int n_leaves, con
 template int leaf_apply_chunk<double>(const LeafGeom&, int, size_t);
 template int leaf_apply_chunk<float>(const LeafGeom&, int, size_t);
 template void launch_leaf_apply<double>(const LeafGeom&, int, int, int, const double2*, const double2*, const double2*,
-                                        const double2*, double2*, double2*, double2*, size_t, hipStream_t);
+                                        const double2*, double2*, double2*, double2*, size_t, hipStream_t, int);
 template void launch_leaf_apply<float>(const LeafGeom&, int, int, int, const float2*, const double2*, const double2*,
-                                       const float2*, float2*, double2*, double2*, size_t, hipStream_t);
+                                       const float2*, float2*, double2*, double2*, size_t, hipStream_t, int);
 
 }  // namespace rs

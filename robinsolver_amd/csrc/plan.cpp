@@ -196,6 +196,37 @@ std::string Plan::build(int d_, int L_, const double* domain_, double h_tgt_, do
         rec.go(0, 0);
     }
     if ((int)root_leaf.size() != lv[0].P) return "Error: root boundary size mismatch.";
+    // leaf neighbours across every face (leaves tile the domain on a regular grid)
+    {
+        double w[3];
+        for (int j = 0; j < 3; ++j) w[j] = nodes[L][0].box[1][j] - nodes[L][0].box[0][j];
+        int cnt[3] = {1, 1, 1};
+        std::vector<int> pos((size_t)3 * n_leaves, 0);
+        for (int q = 0; q < n_leaves; ++q)
+            for (int j = 0; j < d; ++j) {
+                int ix = (int)std::llround((nodes[L][q].box[0][j] - domain[0][j]) / w[j]);
+                pos[(size_t)3 * q + j] = ix;
+                if (ix + 1 > cnt[j]) cnt[j] = ix + 1;
+            }
+        std::vector<int> at((size_t)cnt[0] * cnt[1] * cnt[2], -1);
+        auto key = [&](int x, int y, int z) { return ((size_t)z * cnt[1] + y) * cnt[0] + x; };
+        for (int q = 0; q < n_leaves; ++q) {
+            size_t kq = key(pos[3 * q], pos[3 * q + 1], pos[3 * q + 2]);
+            if (at[kq] >= 0) return "Error: leaves overlap (internal geometry error).";
+            at[kq] = q;
+        }
+        leaf_nbr.assign((size_t)7 * n_leaves, -1);
+        for (int q = 0; q < n_leaves; ++q)
+            for (int f = 1; f <= 2 * d; ++f) {
+                int j = (f - 1) / 2, s = (f % 2 == 1) ? -1 : 1;
+                int p3[3] = {pos[3 * q], pos[3 * q + 1], pos[3 * q + 2]};
+                p3[j] += s;
+                int nb_ = -1;
+                if (p3[j] >= 0 && p3[j] < cnt[j]) nb_ = at[key(p3[0], p3[1], p3[2])];
+                if ((nb_ < 0) != (nodes[L][q].isbndry[f] != 0)) return "Error: leaf adjacency inconsistent.";
+                leaf_nbr[(size_t)7 * q + f] = nb_;
+            }
+    }
     return "";
 }
 

@@ -51,6 +51,7 @@ struct Plan {
     std::vector<Level> lv;
     std::vector<int> root_leaf, root_loc;      // root boundary point -> (leaf, local boundary index)
     int leaf_foff[7];                          // face starts in leaf point order
+    std::vector<int> leaf_nbr;                 // [leaf][7]: leaf across face f (1..6), -1 on the physical boundary
     // returns empty string on success, else the error message
     std::string build(int d, int L, const double* domain, double h_tgt, double kb, const double* bp);
     void leaf_points(int leaf, double* xyz, int* face) const;

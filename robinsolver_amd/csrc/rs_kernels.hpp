@@ -37,7 +37,10 @@ template <typename R> int leaf_apply_chunk(const LeafGeom& G, int ncols, size_t 
 template <typename R>
 void launch_leaf_apply(const LeafGeom& G, int mode, int ncols, int n_leaves, const c_t<R>* sinv, const double2* f,
                        const double2* jump, const c_t<R>* gin, c_t<R>* hout, double2* u, double2* T,
-                       size_t lds_limit, hipStream_t st);
+                       size_t lds_limit, hipStream_t st, int accumulate = 0);
+void launch_residual(const LeafGeom& G, const double2* u, const double2* kint, const double2* f, const double2* g,
+                     const int* nbr, double2 nu, double2 zeta, double2* rf, double2* rg, double2* rj, int n_leaves,
+                     int ncols, hipStream_t st);
 
 // batched inverse
 struct InvPlan {

@@ -54,7 +54,7 @@ template <typename R> void SolveWork<R>::clear() {
     for (DevBuf* b : descs) delete b;
     descs.clear();
     d_up_g.clear(); d_up_h.clear(); d_dn_t.clear(); d_dn_g.clear();
-    f.release(); g.release(); u.release(); jump.release();
+    rf.release(); rg.release(); rj.release();
     cap = 0;
 }
 
@@ -82,7 +82,8 @@ int Tree::ensure_device() {
         d_prow[l] = new DevBuf; d_imap[l] = new DevBuf;
         if (upload(*d_prow[l], P.lv[l].prow) || upload(*d_imap[l], P.lv[l].imap)) return 1;
     }
-    if (upload(d_root_leaf, P.root_leaf) || upload(d_root_loc, P.root_loc)) return 1;
+    if (upload(d_root_leaf, P.root_leaf) || upload(d_root_loc, P.root_loc) || upload(d_leaf_nbr, P.leaf_nbr))
+        return 1;
     return 0;
 }
 
@@ -480,7 +481,7 @@ int Tree::prof_collect() {
 
 template <typename R>
 int Tree::solve_pass(SolveWork<R>& w, int nrhs, const double2* f, const double2* jump, const double2* g, double2* u,
-                     hipStream_t st) {
+                     hipStream_t st, int accumulate) {
     using V = c_t<R>;
     const Plan& P = plan;
     const int L = P.L, nl = P.n_leaves;
@@ -492,7 +493,7 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const double2* f, const double2*
     const double leaf_fl = (double)nl * nr * (2.0 * G.K * G.s * G.s * 8.0);
     pbeg(P_LEAF_UP, st);
     launch_leaf_apply<R>(G, LEAF_UP, nrhs, nl, sv, f, jump, nullptr, w.hV[L]->template as<V>(), u, nullptr,
-                         lds_limit, st);
+                         lds_limit, st, accumulate);
     pend(sinv_b + nr * nl * (16.0 * (P.n_int + P.m) + e * P.nb), leaf_fl, st);
     auto gemm = [&](int cat, const GemmDesc* d, int np, int M, int N, int K, bool din) {
         pbeg(cat, st);
@@ -534,10 +535,36 @@ int Tree::solve_device(int nrhs, const double2* f, const double2* g, double2* u,
     if (prec == RS_MIXED) {
         if (!have32 && make_c64()) return 1;
         if (ensure_work(w32, nrhs)) return 1;
-        return solve_pass(w32, nrhs, f, nullptr, g, u, st);
+        if (solve_pass(w32, nrhs, f, nullptr, g, u, st, 0)) return 1;
+        // FP64 iterative refinement: u += K32^-1 (b - K u), a fixed number of
+        // steps so that a right-hand side gets the same bits alone or batched
+        const size_t nl = plan.n_leaves;
+        const size_t need_f = (size_t)w32.cap * nl * plan.n_int * 16, need_g = (size_t)w32.cap * nl * plan.nb * 16;
+        if (opts.n_refine > 0 && w32.rf.bytes < need_f) {
+            if (alloc(w32.rf, need_f) || alloc(w32.rg, need_g) || alloc(w32.rj, need_g)) return 1;
+        }
+        for (int it = 0; it < opts.n_refine; ++it) {
+            if (residual(nrhs, u, f, g, w32.rf.as<double2>(), w32.rg.as<double2>(), w32.rj.as<double2>(), st))
+                return 1;
+            if (solve_pass(w32, nrhs, w32.rf.as<double2>(), w32.rj.as<double2>(), w32.rg.as<double2>(), u, st, 1))
+                return 1;
+        }
+        return 0;
     }
     if (ensure_work(w64, nrhs)) return 1;
-    return solve_pass(w64, nrhs, f, nullptr, g, u, st);
+    return solve_pass(w64, nrhs, f, nullptr, g, u, st, 0);
+}
+
+int Tree::residual(int nrhs, const double2* u, const double2* f, const double2* g, double2* rf, double2* rg,
+                   double2* rj, hipStream_t st) {
+    const Plan& P = plan;
+    pbeg(P_RESIDUAL, st);
+    launch_residual(G, u, kint.as<double2>(), f, g, d_leaf_nbr.as<int>(), make_double2(P.nu[0], P.nu[1]),
+                    make_double2(P.zeta[0], P.zeta[1]), rf, rg, rj, P.n_leaves, nrhs, st);
+    pend((double)nrhs * P.n_leaves * 16.0 * (2.0 * P.m + 2.0 * P.n_int + 3.0 * P.nb) + 16.0 * P.n_leaves * P.n_int,
+         (double)nrhs * P.n_leaves * P.m * 8.0 * (2 * G.d + 1), st);
+    HIPCHK(hipGetLastError());
+    return 0;
 }
 
 int Tree::solve_host(int nrhs, const double* f, const double* g, int prec) {
@@ -547,13 +574,12 @@ int Tree::solve_host(int nrhs, const double* f, const double* g, int prec) {
     const Plan& P = plan;
     size_t nf = (size_t)nrhs * P.n_leaves * P.n_int, ng = (size_t)nrhs * P.n_leaves * P.nb,
            nu = (size_t)nrhs * P.n_leaves * P.m;
-    SolveWork<double>& w = w64;  // host staging lives in the FP64 workspace
-    if (w.f.bytes < nf * 16 && alloc(w.f, nf * 16)) return 1;
-    if (w.g.bytes < ng * 16 && alloc(w.g, ng * 16)) return 1;
-    if (w.u.bytes < nu * 16 && alloc(w.u, nu * 16)) return 1;
-    HIPCHK(hipMemcpyAsync(w.f.p, f, nf * 16, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(w.g.p, g, ng * 16, hipMemcpyHostToDevice, stream));
-    if (solve_device(nrhs, w.f.as<double2>(), w.g.as<double2>(), w.u.as<double2>(), prec, stream)) return 1;
+    if (st_f.bytes < nf * 16 && alloc(st_f, nf * 16)) return 1;
+    if (st_g.bytes < ng * 16 && alloc(st_g, ng * 16)) return 1;
+    if (st_u.bytes < nu * 16 && alloc(st_u, nu * 16)) return 1;
+    HIPCHK(hipMemcpyAsync(st_f.p, f, nf * 16, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(st_g.p, g, ng * 16, hipMemcpyHostToDevice, stream));
+    if (solve_device(nrhs, st_f.as<double2>(), st_g.as<double2>(), st_u.as<double2>(), prec, stream)) return 1;
     HIPCHK(hipStreamSynchronize(stream));
     last_nrhs = nrhs;
     s_time = now_s() - t0;
@@ -564,7 +590,7 @@ int Tree::get_solution(int rhs, double* u) const {
     if (!factored) return const_cast<Tree*>(this)->fail("Error in GET_SOLUTION: Robin Tree not factored.");
     if (rhs < 0 || rhs >= last_nrhs) return const_cast<Tree*>(this)->fail("Error: no such right-hand side.");
     size_t n = (size_t)plan.n_leaves * plan.m;
-    hipError_t e = hipMemcpy(u, w64.u.as<double2>() + (size_t)rhs * n, n * 16, hipMemcpyDeviceToHost);
+    hipError_t e = hipMemcpy(u, st_u.as<double2>() + (size_t)rhs * n, n * 16, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return const_cast<Tree*>(this)->fail(hipGetErrorString(e));
     return 0;
 }

@@ -34,7 +34,7 @@ template <typename R> struct SolveWork {
     using V = c_t<R>;
     int cap = 0;
     std::vector<DevBuf*> hV, gF, G, Z;   // per level
-    DevBuf f, g, u, jump;                // host-API staging (double2)
+    DevBuf rf, rg, rj;                   // refinement residuals (double2)
     std::vector<DevBuf*> descs;          // per (level, kind)
     std::vector<GemmDesc*> d_up_g, d_up_h, d_dn_t, d_dn_g;
     ~SolveWork() { clear(); }
@@ -71,7 +71,7 @@ struct Tree {
     bool have32 = false;
     // maps
     std::vector<DevBuf*> d_prow, d_imap;
-    DevBuf d_root_leaf, d_root_loc;
+    DevBuf d_root_leaf, d_root_loc, d_leaf_nbr;
 
     // kernel timing
     bool prof_on = false;
@@ -82,6 +82,8 @@ struct Tree {
     void pbeg(int cat, hipStream_t st);
     void pend(double bytes, double flops, hipStream_t st);
     int prof_collect();
+
+    DevBuf st_f, st_g, st_u;     // staging of the host-memory API (double2)
 
     SolveWork<double> w64;
     SolveWork<float> w32;
@@ -102,7 +104,9 @@ struct Tree {
     int make_c64();
     template <typename R> int ensure_work(SolveWork<R>& w, int nrhs);
     template <typename R> int solve_pass(SolveWork<R>& w, int nrhs, const double2* f, const double2* jump,
-                                         const double2* g, double2* u, hipStream_t st);
+                                         const double2* g, double2* u, hipStream_t st, int accumulate);
+    int residual(int nrhs, const double2* u, const double2* f, const double2* g, double2* rf, double2* rg,
+                 double2* rj, hipStream_t st);
     int solve_device(int nrhs, const double2* f, const double2* g, double2* u, int prec, hipStream_t st);
     int solve_host(int nrhs, const double* f, const double* g, int prec);
     int get_solution(int rhs, double* u) const;

@@ -1,0 +1,67 @@
+"""GPU: the mixed-precision path (complex64 solve passes on the matrix cores,
+FP64 residual and iterative refinement) against the FP64 oracle.
+Tolerance: rel. l2 <= 1e-8 per RHS after refinement (the mixed-path bound)."""
+import numpy as np
+import pytest
+
+import robinsolver_amd as rsa
+from oracle.pyoracle import OracleTree
+
+pytestmark = pytest.mark.gpu
+
+UNIT = [[0, 1], [0, 1], [0, 1]]
+TOL_MIXED = 1e-8
+
+
+def _rand_c(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _k(x, kb, seed=0):
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-1, 1, 8)
+    p = rng.integers(1, 9, size=(8, 2))
+    ph = rng.uniform(0, 2 * np.pi, 8)
+    q = sum(a[j] * np.cos(2 * np.pi * (x[..., :2] @ p[j]) + ph[j]) for j in range(8)) / np.abs(a).sum()
+    return kb ** 2 * (1 + 0.3 * q) + 0.01j * kb ** 2
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("d,n,L,kb,n_refine", [(2, 64, 4, 10.0, 2), (2, 128, 6, 30.0, 2), (3, 16, 6, 4.0, 2),
+                                                 (2, 64, 4, 10.0, 0)])
+def test_mixed_matches_oracle(d, n, L, kb, n_refine, gpu):
+    print("case: mixed", d, n, L, kb, n_refine, flush=True)
+    o = OracleTree(d, L, UNIT, 1.0 / n, -kb)
+    t = rsa.GpuTree(d, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=n_refine)
+    pts, face = o.all_points()
+    k = _k(pts[:, o.nb:], kb)
+    o.factor(k)
+    t.factor(k)
+    rng = np.random.default_rng(3)
+    nrhs = 4
+    f = _rand_c(rng, (nrhs, o.n_leaves, o.n_int))
+    g = _rand_c(rng, (nrhs, o.n_leaves, o.nb))
+    u = t.solve(f, g)
+    rels = [_rel(u[r], o.solve(f[r], g[r])) for r in range(nrhs)]
+    print("rel l2:", rels, flush=True)
+    if n_refine == 0:
+        assert max(rels) < 1e-3      # a complex64 solve alone
+    else:
+        assert max(rels) <= TOL_MIXED, rels
+
+
+def test_mixed_batch_invariance(gpu):
+    print("case: mixed batch invariance", flush=True)
+    t = rsa.GpuTree(2, 4, UNIT, 1.0 / 64, -10.0, precision=rsa.RS_MIXED)
+    pts, _ = t.all_points()
+    t.factor(_k(pts[:, t.nb:], 10.0))
+    rng = np.random.default_rng(4)
+    nrhs = 20
+    f = _rand_c(rng, (nrhs, t.n_leaves, t.n_int))
+    g = _rand_c(rng, (nrhs, t.n_leaves, t.nb))
+    u = t.solve(f, g)
+    for r in (0, 11, 19):
+        assert np.array_equal(t.solve(f[r], g[r]), u[r])
