@@ -115,6 +115,19 @@ int rs_level_info(const rs_tree *t, int lvl, int *nodes, int *P, int *isz, int *
  * may be NULL to query P. */
 int rs_node_rtr(const rs_tree *t, int lvl, int idx, double *out, int *P);
 
+/* Kernel checks (test hooks, no reference counterpart), host memory in and out:
+ *   rs_check_gemm: C = alpha A B + beta C for one column-major complex problem
+ *     (M x K times K x N) through the batched MFMA GEMM, precision RS_FP64
+ *     (complex128) or RS_MIXED (complex64 arithmetic on complex128 data);
+ *     rows k >= ksplit of B are taken from B2 (ksplit >= K: unused); rmap, when
+ *     not NULL, scatters output row r to row rmap[r] of C (ldc rows).
+ *   rs_check_inverse: in-place inverse of `batch` n x n complex128 matrices,
+ *     the interface inversion of the merge. */
+int rs_check_gemm(int precision, int M, int N, int K, const double *A, int lda, const double *B,
+                  const double *B2, int ldb, int ksplit, double *C, int ldc, const int *rmap,
+                  const double *alpha, const double *beta);
+int rs_check_inverse(int n, int batch, double *mats);
+
 /* Kernel timing (no reference counterpart; the reference's only timer is
  * time_module).  While enabled, every solve-pass launch is bracketed by HIP
  * events on the stream it runs on, and its algorithmic bytes and flops are

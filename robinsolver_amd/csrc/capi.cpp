@@ -2,6 +2,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "tree.hpp"
 
@@ -137,6 +138,125 @@ int rs_level_info(const rs_tree* t, int lvl, int* nodes, int* P, int* isz, int* 
 int rs_node_rtr(const rs_tree* t, int lvl, int idx, double* out, int* P) {
     if (!t) return 1;
     return t->t.node_rtr(lvl, idx, out, P);
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename V>
+std::vector<V> to_prec(const double* p, size_t n) {
+    std::vector<V> v(n);
+    for (size_t q = 0; q < n; ++q) { v[q].x = (decltype(v[q].x))p[2 * q]; v[q].y = (decltype(v[q].y))p[2 * q + 1]; }
+    return v;
+}
+
+template <typename R>
+int check_gemm_t(int M, int N, int K, const double* A, int lda, const double* B, const double* B2, int ldb,
+                 int ksplit, double* C, int ldc, const int* rmap, const double* alpha, const double* beta) {
+    using V = rs::c_t<R>;
+    size_t na = (size_t)lda * K, nb = (size_t)ldb * N, nc = (size_t)ldc * N;
+    std::vector<V> hA = to_prec<V>(A, na), hB = to_prec<V>(B, nb), hC = to_prec<V>(C, nc);
+    std::vector<V> hB2 = B2 ? to_prec<V>(B2, nb) : std::vector<V>();
+    V *dA = nullptr, *dB = nullptr, *dB2 = nullptr, *dC = nullptr, *dD = nullptr;
+    int* dmap = nullptr;
+    rs::GemmDesc* dd = nullptr;
+    auto cleanup = [&]() {
+        for (void* p : {(void*)dA, (void*)dB, (void*)dB2, (void*)dC, (void*)dD, (void*)dmap, (void*)dd})
+            if (p) (void)hipFree(p);
+    };
+    bool ok = hipMalloc(&dA, na * sizeof(V)) == hipSuccess && hipMalloc(&dB, nb * sizeof(V)) == hipSuccess &&
+              hipMalloc(&dC, nc * sizeof(V)) == hipSuccess && hipMalloc(&dD, nc * sizeof(V)) == hipSuccess &&
+              hipMalloc(&dd, sizeof(rs::GemmDesc)) == hipSuccess;
+    if (ok && B2) ok = hipMalloc(&dB2, nb * sizeof(V)) == hipSuccess;
+    if (ok && rmap) ok = hipMalloc(&dmap, (size_t)M * sizeof(int)) == hipSuccess;
+    if (!ok) { cleanup(); g_err = "Error: hipMalloc failed in rs_check_gemm."; return 1; }
+    (void)hipMemcpy(dA, hA.data(), na * sizeof(V), hipMemcpyHostToDevice);
+    (void)hipMemcpy(dB, hB.data(), nb * sizeof(V), hipMemcpyHostToDevice);
+    if (B2) (void)hipMemcpy(dB2, hB2.data(), nb * sizeof(V), hipMemcpyHostToDevice);
+    (void)hipMemcpy(dC, hC.data(), nc * sizeof(V), hipMemcpyHostToDevice);
+    (void)hipMemcpy(dD, hC.data(), nc * sizeof(V), hipMemcpyHostToDevice);
+    if (rmap) (void)hipMemcpy(dmap, rmap, (size_t)M * sizeof(int), hipMemcpyHostToDevice);
+    rs::GemmDesc d;
+    std::memset(&d, 0, sizeof d);
+    d.A = dA; d.lda = lda; d.B = dB; d.B2 = dB2; d.ldb = ldb; d.ksplit = B2 ? ksplit : (1 << 30);
+    d.Out = dC; d.ldo = ldc; d.rmap = dmap; d.dmap = rmap ? 1 : 0;
+    d.Din = dD; d.ldd = ldc;
+    d.ar = alpha[0]; d.ai = alpha[1]; d.br = beta[0]; d.bi = beta[1];
+    (void)hipMemcpy(dd, &d, sizeof d, hipMemcpyHostToDevice);
+    rs::launch_gemm<R>(dd, 1, M, N, K, nullptr);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(hC.data(), dC, nc * sizeof(V), hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) { g_err = hipGetErrorString(e); return 1; }
+    for (size_t q = 0; q < nc; ++q) { C[2 * q] = hC[q].x; C[2 * q + 1] = hC[q].y; }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rs_check_gemm(int precision, int M, int N, int K, const double* A, int lda, const double* B, const double* B2,
+                  int ldb, int ksplit, double* C, int ldc, const int* rmap, const double* alpha, const double* beta) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_err = "Error: no HIP device."; return 1; }
+    if (precision == RS_MIXED)
+        return check_gemm_t<float>(M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta);
+    return check_gemm_t<double>(M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta);
+}
+
+int rs_check_inverse(int n, int batch, double* mats) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_err = "Error: no HIP device."; return 1; }
+    hipDeviceProp_t prop;
+    (void)hipGetDeviceProperties(&prop, 0);
+    rs::InvPlan ip;
+    size_t nn = (size_t)n * n;
+    ip.build(n, batch, nn, prop.sharedMemPerBlock);
+    double2 *dm = nullptr, *W = nullptr;
+    int *piv = nullptr, *info = nullptr, *cm = nullptr;
+    rs::GemmDesc* dd = nullptr;
+    std::vector<rs::GemmDesc> descs;
+    std::vector<int> cmaps;
+    (void)hipMalloc(&dm, nn * batch * 16);
+    (void)hipMalloc(&info, sizeof(int));
+    (void)hipMemset(info, 0, sizeof(int));
+    (void)hipMemcpy(dm, mats, nn * batch * 16, hipMemcpyHostToDevice);
+    if (!ip.small) {
+        (void)hipMalloc(&W, ip.w_elems() * 16 + 16);
+        (void)hipMalloc(&piv, (size_t)n * batch * sizeof(int));
+        cmaps.resize((size_t)ip.panels() * (n - ip.nb));
+        for (int pi = 0; pi < ip.panels(); ++pi) {
+            int j0 = pi * ip.nb;
+            for (int c2 = 0; c2 < n - ip.nb; ++c2) cmaps[(size_t)pi * (n - ip.nb) + c2] = c2 < j0 ? c2 : c2 + ip.nb;
+        }
+        (void)hipMalloc(&cm, cmaps.size() * sizeof(int) + 4);
+        (void)hipMemcpy(cm, cmaps.data(), cmaps.size() * sizeof(int), hipMemcpyHostToDevice);
+        for (int pi = 0; pi < ip.panels(); ++pi)
+            for (int b = 0; b < batch; ++b) {
+                rs::GemmDesc g;
+                std::memset(&g, 0, sizeof g);
+                double2* Mb = dm + (size_t)b * nn;
+                g.A = Mb + (size_t)pi * ip.nb * n; g.lda = n;
+                g.B = W + (size_t)b * ip.nb * (n - ip.nb); g.ldb = ip.nb;
+                g.Out = Mb; g.ldo = n; g.cmap = cm + (size_t)pi * (n - ip.nb);
+                g.Din = Mb; g.ldd = n; g.dmap = 1; g.ar = 1.0; g.br = 1.0; g.ksplit = 1 << 30;
+                descs.push_back(g);
+            }
+        (void)hipMalloc(&dd, descs.size() * sizeof(rs::GemmDesc));
+        (void)hipMemcpy(dd, descs.data(), descs.size() * sizeof(rs::GemmDesc), hipMemcpyHostToDevice);
+    }
+    rs::launch_inverse(ip, dm, piv, W, dd, info, nullptr);
+    hipError_t e = hipDeviceSynchronize();
+    int hinfo = 0;
+    if (e == hipSuccess) e = hipMemcpy(mats, dm, nn * batch * 16, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost);
+    for (void* p : {(void*)dm, (void*)W, (void*)piv, (void*)info, (void*)cm, (void*)dd})
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) { g_err = hipGetErrorString(e); return 1; }
+    if (hinfo) { g_err = "Error: singular matrix."; return 2; }
+    return 0;
 }
 
 int rs_profile(rs_tree* t, int enable) {

@@ -36,6 +36,15 @@ template <> struct acc_t<float> { using T = f4; };
 __device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 __device__ inline f4 mfma(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
+// Row of accumulator element v held by `lane` in a 16x16 output block.  Both
+// instructions read A[i][k] from lane i + 16k and B[k][j] from lane j + 16k and
+// put column j = lane % 16 in the lane, but they order the rows differently:
+// the f32 form keeps four consecutive rows per lane group, the f64 form
+// interleaves the four lane groups.
+template <typename R> __device__ inline int out_row(int lane, int v);
+template <> __device__ inline int out_row<float>(int lane, int v) { return 4 * (lane >> 4) + v; }
+template <> __device__ inline int out_row<double>(int lane, int v) { return (lane >> 4) + 4 * v; }
+
 constexpr int BM = 64;
 constexpr int BK = 16;
 
@@ -110,7 +119,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
     for (int i = 0; i < TM; ++i)
         for (int j = 0; j < TN; ++j)
             for (int v = 0; v < 4; ++v) {
-                int row = m0 + wm + i * 16 + 4 * (lane >> 4) + v;
+                int row = m0 + wm + i * 16 + out_row<R>(lane, v);
                 int col = n0 + wn + j * 16 + (lane & 15);
                 if (row >= M || col >= N) continue;
                 V val = cmul(alpha, mk<R>(accR[i][j][v], accI[i][j][v]));

@@ -1,0 +1,103 @@
+"""GPU: the kernels in isolation against numpy -- the batched MFMA GEMM
+(complex128 and complex64 paths, tile edges, B split, row scatter, beta) and
+the pivoted batched inverse (whole-matrix LDS path and blocked panel path)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import robinsolver_amd as rsa
+
+pytestmark = pytest.mark.gpu
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _lib():
+    L = rsa.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.rs_check_gemm.argtypes = [ctypes.c_int] * 4 + [dp, ctypes.c_int, dp, dp, ctypes.c_int, ctypes.c_int, dp,
+                                                     ctypes.c_int, ip, dp, dp]
+    L.rs_check_inverse.argtypes = [ctypes.c_int, ctypes.c_int, dp]
+    L.rs_last_error.restype = ctypes.c_char_p
+    return L
+
+
+def _rc(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _run_gemm(prec, A, B, C, alpha, beta, B2=None, ksplit=None, rmap=None, ldc=None):
+    L = _lib()
+    M, K = A.shape
+    N = B.shape[1]
+    Af = np.asfortranarray(A).astype(np.complex128)
+    Bf = np.asfortranarray(B).astype(np.complex128)
+    B2f = None if B2 is None else np.asfortranarray(B2).astype(np.complex128)
+    Cf = np.asfortranarray(C).astype(np.complex128)
+    al = np.array([alpha.real, alpha.imag])
+    be = np.array([beta.real, beta.imag])
+    rm = None
+    if rmap is not None:
+        rm = np.ascontiguousarray(rmap, dtype=np.int32)
+    a_flat = Af.reshape(-1, order="F").copy()
+    b_flat = Bf.reshape(-1, order="F").copy()
+    b2_flat = None if B2f is None else B2f.reshape(-1, order="F").copy()
+    c_flat = Cf.reshape(-1, order="F").copy()
+    rc = L.rs_check_gemm(prec, M, N, K, _dp(a_flat.view(np.float64)), M, _dp(b_flat.view(np.float64)),
+                         None if b2_flat is None else _dp(b2_flat.view(np.float64)), Bf.shape[0],
+                         ksplit if ksplit is not None else K, _dp(c_flat.view(np.float64)), Cf.shape[0],
+                         None if rm is None else rm.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(al), _dp(be))
+    assert rc == 0, L.rs_last_error(None)
+    return c_flat.reshape(Cf.shape, order="F")
+
+
+@pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED])
+@pytest.mark.parametrize("M,N,K", [(16, 16, 16), (64, 64, 16), (37, 5, 23), (130, 200, 70), (48, 1, 96),
+                                   (7, 65, 3)])
+def test_gemm_matches_numpy(prec, M, N, K, gpu):
+    print("case: gemm", prec, M, N, K, flush=True)
+    rng = np.random.default_rng(M * 1000 + N * 10 + K)
+    A, B, C = _rc(rng, M, K), _rc(rng, K, N), _rc(rng, M, N)
+    alpha, beta = 0.7 - 0.3j, 1.1 + 0.2j
+    out = _run_gemm(prec, A, B, C, alpha, beta)
+    ref = alpha * (A @ B) + beta * C
+    tol = 1e-13 if prec == rsa.RS_FP64 else 2e-6
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < tol
+
+
+@pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED])
+def test_gemm_split_and_scatter(prec, gpu):
+    print("case: gemm split/scatter", prec, flush=True)
+    rng = np.random.default_rng(1)
+    M, N, K, ks = 40, 20, 30, 12
+    A = _rc(rng, M, K)
+    B1, B2 = _rc(rng, K, N), _rc(rng, K, N)
+    Bfull = np.vstack([B1[:ks], B2[:K - ks]])
+    rmap = rng.permutation(M + 9)[:M]
+    C = _rc(rng, M + 9, N)
+    out = _run_gemm(prec, A, B1, C, 1.0 + 0j, 0.5 + 0j, B2=B2, ksplit=ks, rmap=rmap)
+    ref = C.copy()
+    ref[rmap] = A @ Bfull + 0.5 * C[rmap]
+    tol = 1e-13 if prec == rsa.RS_FP64 else 2e-6
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < tol
+
+
+@pytest.mark.parametrize("n,batch", [(16, 5), (48, 3), (64, 2), (128, 2), (200, 1), (512, 2)])
+def test_inverse_matches_numpy(n, batch, gpu):
+    print("case: inverse", n, batch, flush=True)
+    rng = np.random.default_rng(n)
+    mats = _rc(rng, batch, n, n) + 0.1 * np.eye(n)
+    # permute rows so that partial pivoting has work to do
+    for b in range(batch):
+        mats[b] = mats[b][rng.permutation(n)]
+    flat = np.concatenate([m.reshape(-1, order="F") for m in mats]).copy()
+    L = _lib()
+    assert L.rs_check_inverse(n, batch, _dp(flat.view(np.float64))) == 0
+    for b in range(batch):
+        inv = flat[b * n * n:(b + 1) * n * n].reshape(n, n, order="F")
+        err = np.linalg.norm(inv @ mats[b] - np.eye(n)) / np.sqrt(n)
+        assert err < 1e-10, (b, err)
