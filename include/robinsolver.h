@@ -91,6 +91,10 @@ int rs_get_solution(const rs_tree *t, int rhs, double *u);
 int rs_solve_device(rs_tree *t, int nrhs, const void *f_dev, const void *g_dev, void *u_dev,
                     int precision, void *stream);
 
+/* Number of FP64 refinement steps of the mixed path for later solves
+ * (rs_opts.n_refine; no reference counterpart). */
+int rs_set_refine(rs_tree *t, int n_refine);
+
 /* T%f_time and T%s_time (robin_tree_module.f90:63-64, time_module.f90:26-36):
  * wall seconds of the last factor and solve. */
 int rs_timings(const rs_tree *t, double *f_time, double *s_time);

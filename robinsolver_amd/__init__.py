@@ -62,6 +62,7 @@ def lib():
         L.rs_last_error.restype = ctypes.c_char_p
         L.rs_inspect.argtypes = [vp, ip, ip, ctypes.POINTER(ctypes.c_longlong)]
         L.rs_node_rtr.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ip]
+        L.rs_set_refine.argtypes = [vp, ctypes.c_int]
         L.rs_level_info.argtypes = [vp, ctypes.c_int, ip, ip, ip, ip]
         L.rs_profile.argtypes = [vp, ctypes.c_int]
         L.rs_profile_name.restype = ctypes.c_char_p
@@ -159,6 +160,9 @@ class GpuTree:
         """Device-resident solve: raw device pointers (e.g. torch tensor.data_ptr())."""
         self._check(lib().rs_solve_device(self.h, nrhs, ctypes.c_void_p(f_ptr), ctypes.c_void_p(g_ptr),
                                           ctypes.c_void_p(u_ptr), precision, ctypes.c_void_p(stream or 0)))
+
+    def set_refine(self, n):
+        self._check(lib().rs_set_refine(self.h, int(n)))
 
     def timings(self):
         f, s = ctypes.c_double(), ctypes.c_double()

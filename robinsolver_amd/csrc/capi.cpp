@@ -105,6 +105,12 @@ int rs_solve_device(rs_tree* t, int nrhs, const void* f_dev, const void* g_dev, 
                              (hipStream_t)stream);
 }
 
+int rs_set_refine(rs_tree* t, int n_refine) {
+    if (!t || n_refine < 0 || n_refine > 16) return 1;
+    t->t.opts.n_refine = n_refine;
+    return 0;
+}
+
 int rs_timings(const rs_tree* t, double* f_time, double* s_time) {
     if (!t) return 1;
     if (f_time) *f_time = t->t.f_time;
