@@ -21,6 +21,8 @@
 // all that the solve streams per leaf.  The outgoing data needs no stencil
 // pass: the boundary row gives (alpha + D_n) u = g, so
 //   h = (sigma + D_n) u = g + (sigma - alpha) u_B.
+#include <type_traits>
+
 #include "rs_common.hpp"
 #include "rs_kernels.hpp"
 
@@ -238,17 +240,37 @@ __global__ void __launch_bounds__(256) leaf_blocks_kernel(LeafGeom G, LeafCoef C
 template <typename R> struct LeafIO {
     int mode;                 // LEAF_FACTOR_T, LEAF_UP, LEAF_DOWN
     int ncols;                // columns: nb (factor) or nrhs
-    int CH;                   // columns per workgroup
-    const c_t<R>* sinv;       // [leaf][K][s*s]
-    const double2* f;         // UP: [col][leaf][n_int] or null
-    const double2* jump;      // UP: [col][leaf][nb] local order, subtracted from h; or null
-    const c_t<R>* gin;        // DOWN: [col][leaf][nb] node order
-    c_t<R>* hout;             // UP: [col][leaf][nb] node order
-    double2* u;               // UP (overwrite) / DOWN (accumulate): [col][leaf][m]
-    double2* T;               // FACTOR: [leaf][nb][nb] node order, column-major
+    int CH;                   // columns per workgroup (generic kernel)
     int n_leaves;
-    int accumulate;           // UP: add to u instead of overwriting it
+    const c_t<R>* sinv;       // [leaf][K][s*s]
+    const double2* f64;       // interior data f: [col][leaf][n_int], complex128 ...
+    const float2* f32;        //   ... or complex64 (at most one set; none = 0)
+    const double2* jump;      // UP: subtracted from the outgoing data h: [col][leaf][nb] local order
+    const c_t<R>* gin;        // DOWN: incoming data g: [col][leaf][nb] node order
+    c_t<R>* hout;             // UP: outgoing data h: [col][leaf][nb] node order
+    double2* u64;             // DOWN: solution [col][leaf][m], complex128 ...
+    float2* u32;              //   ... or complex64 (one set)
+    const double2* ub64;      // DOWN: base added to the solution (complex128 ...
+    const float2* ub32;       //   ... or complex64; optional, may alias the output)
+    double2* T;               // FACTOR: [leaf][nb][nb] node order, column-major
 };
+
+template <typename V, typename R>
+__device__ inline V load_f(const LeafIO<R>& io, size_t idx) {
+    if (io.f64) return ccast<V>(io.f64[idx]);
+    if (io.f32) return ccast<V>(io.f32[idx]);
+    return V{};
+}
+
+template <typename V, typename R>
+__device__ inline void store_u(const LeafIO<R>& io, size_t idx, V x) {
+    double2 v = ccast<double2>(x);
+    if (io.ub64) v = cadd(v, io.ub64[idx]);
+    else if (io.ub32) v = cadd(v, ccast<double2>(io.ub32[idx]));
+    if (io.u64) io.u64[idx] = v;
+    else io.u32[idx] = ccast<float2>(v);
+}
+
 
 template <typename R>
 __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C, LeafIO<R> io) {
@@ -279,7 +301,7 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
         int q = e / CH, c = e % CH;
         V v = cz<R>();
         if (c < ncol) {
-            if (io.mode == LEAF_UP && io.f) v = ccast<V>(io.f[((size_t)(c0 + c) * io.n_leaves + leaf) * ni + q]);
+            if (io.mode != LEAF_FACTOR_T) v = load_f<V>(io, ((size_t)(c0 + c) * io.n_leaves + leaf) * ni + q);
             if (io.mode != LEAF_UP) {
                 int g[3];
                 interior_coords(G, q, g);
@@ -348,20 +370,15 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
         } else if (io.mode == LEAF_UP) {
             if (io.jump) hb = csub(hb, ccast<V>(io.jump[((size_t)col * io.n_leaves + leaf) * nb + b]));
             io.hout[((size_t)col * io.n_leaves + leaf) * nb + local_to_node(G, parity, b)] = hb;
-            double2* up = io.u + ((size_t)col * io.n_leaves + leaf) * G.m + b;
-            *up = io.accumulate ? cadd(*up, ccast<double2>(uB)) : ccast<double2>(uB);
         } else {
-            double2* up = io.u + ((size_t)col * io.n_leaves + leaf) * G.m + b;
-            *up = cadd(*up, ccast<double2>(uB));
+            store_u(io, ((size_t)col * io.n_leaves + leaf) * G.m + b, uB);
         }
     }
-    if (io.mode != LEAF_FACTOR_T) {
+    if (io.mode == LEAF_DOWN) {
         for (int e = tid; e < ni * CH; e += nt) {
             int q = e / CH, c = e % CH;
             if (c >= ncol) continue;
-            double2* up = io.u + ((size_t)(c0 + c) * io.n_leaves + leaf) * G.m + nb + q;
-            double2 v = ccast<double2>(Y[e]);
-            *up = (io.mode == LEAF_UP && !io.accumulate) ? v : cadd(*up, v);
+            store_u(io, ((size_t)(c0 + c) * io.n_leaves + leaf) * G.m + nb + q, Y[e]);
         }
     }
 }
@@ -374,24 +391,26 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
 // M [g1; g2] = [h1; h2] (robin_tree_module.f90:425-467): it holds at every pair
 // of coincident leaf boundary points once the tree has solved.  A correction
 // solve takes (rf, rg) as (f, g) and subtracts rj from the leaf outgoing data.
-__global__ void residual_kernel(LeafGeom G, LeafCoef C, const double2* __restrict__ u, const double2* __restrict__ kint,
+template <typename U>
+__global__ void residual_kernel(LeafGeom G, LeafCoef C, const U* __restrict__ u, const double2* __restrict__ kint,
                                 const double2* __restrict__ f, const double2* __restrict__ g,
-                                const int* __restrict__ nbr, double2 nu, double2 zeta, double2* __restrict__ rf,
+                                const int* __restrict__ nbr, double2 nu, double2 zeta, float2* __restrict__ rf,
                                 double2* __restrict__ rg, double2* __restrict__ rj, int n_leaves) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int leaf = blockIdx.y, col = blockIdx.z;
     if (p >= G.m) return;
     const int nb = G.nb, ni = G.n_int;
     const size_t lc = (size_t)col * n_leaves + leaf;
-    const double2* ul = u + lc * G.m;
-    const double2 alpha = make_double2(0.0, G.kb), sma = make_double2(0.0, -2.0 * G.kb);
-    auto incoming = [&](const double2* uu, int b) -> double2 {
+    const U* ul = u + lc * G.m;
+    const double2 sma = make_double2(0.0, -2.0 * G.kb);
+    auto uval = [&](const U* uu, int i) -> double2 { return ccast<double2>(uu[i]); };
+    auto incoming = [&](const U* uu, int b) -> double2 {
         int f_, j_, q1, st;
         boundary_info(G, b, f_, j_, q1, st);
         const AxisCoef& a = C.ax[j_];
-        double2 v = cmul(a.dd, uu[b]);
-        v = cadd(v, cscale(uu[nb + q1], a.a1));
-        return cadd(v, cscale(uu[nb + q1 + st], a.a2));
+        double2 v = cmul(a.dd, uval(uu, b));
+        v = cadd(v, cscale(uval(uu, nb + q1), a.a1));
+        return cadd(v, cscale(uval(uu, nb + q1 + st), a.a2));
     };
     if (p < nb) {
         int fc = 1;
@@ -405,17 +424,16 @@ __global__ void residual_kernel(LeafGeom G, LeafCoef C, const double2* __restric
             const int f2 = (fc & 1) ? fc + 1 : fc - 1;
             const int b2 = G.foff[f2] + (p - G.foff[fc]);
             const double2 gN = incoming(u + ((size_t)col * n_leaves + nl2) * G.m, b2);
-            const double2 hA = cfma(sma, ul[p], gA);
+            const double2 hA = cfma(sma, uval(ul, p), gA);
             rj[lc * nb + p] = csub(cadd(cmul(zeta, gA), cmul(nu, gN)), hA);
             rg[lc * nb + p] = make_double2(0.0, 0.0);
         }
-        (void)alpha;
         return;
     }
     const int q = p - nb;
     int gq[3];
     interior_coords(G, q, gq);
-    double2 Au = cmul(kint[(size_t)leaf * ni + q], ul[p]);
+    double2 Au = cmul(kint[(size_t)leaf * ni + q], uval(ul, p));
     for (int j = 0; j < G.d; ++j) {
         const AxisCoef& a = C.ax[j];
         const double ih2 = a.mid;
@@ -424,24 +442,190 @@ __global__ void residual_kernel(LeafGeom G, LeafCoef C, const double2* __restric
         if (gq[j] == 1) { wm = 8.0 / 3.0; w0 = -4.0; wp = 4.0 / 3.0; }
         else if (gq[j] == G.n[j]) { wm = 4.0 / 3.0; w0 = -4.0; wp = 8.0 / 3.0; }
         else { wm = 1.0; w0 = -2.0; wp = 1.0; }
-        const double2 um = (gq[j] == 1) ? ul[face_point(G, 2 * j + 1, gq)] : ul[nb + q - st];
-        const double2 upp = (gq[j] == G.n[j]) ? ul[face_point(G, 2 * j + 2, gq)] : ul[nb + q + st];
-        Au = cadd(Au, cscale(ul[p], w0 * ih2));
+        const double2 um = (gq[j] == 1) ? uval(ul, face_point(G, 2 * j + 1, gq)) : uval(ul, nb + q - st);
+        const double2 upp = (gq[j] == G.n[j]) ? uval(ul, face_point(G, 2 * j + 2, gq)) : uval(ul, nb + q + st);
+        Au = cadd(Au, cscale(uval(ul, p), w0 * ih2));
         Au = cadd(Au, cscale(um, wm * ih2));
         Au = cadd(Au, cscale(upp, wp * ih2));
     }
-    rf[lc * ni + q] = csub(f[lc * ni + q], Au);
+    rf[lc * ni + q] = ccast<float2>(csub(f[lc * ni + q], Au));
+}
+
+// ---------------------------------------------------------------------------
+// Multi-RHS leaf solve on the matrix cores (complex64, 2D leaves with an even
+// number s <= 16 of points per layer and K layers).  Each wave owns 16 RHS
+// columns; a workgroup covers 64 columns of one leaf.  The layer recursion
+//   forward   y_k = Shat_k^-1 (r_k - lambda_k y_{k-1})
+//   backward  x_k = y_k - mu_k Shat_k^-1 x_{k+1}
+// is one 16x16x16 complex product per layer (v_mfma_f32_16x16x4f32, four real
+// products per complex k-chunk).  The leaf's K blocks are staged once in LDS;
+// y_k stays in registers in the MFMA output layout (lane l holds rows
+// 4(l/16)..4(l/16)+3 of column l%16), and between layers the 16x16 operand
+// goes through a small per-wave LDS tile to reach the B-operand layout.  The
+// boundary values need x at the first two interior points along the normal:
+// for x-faces those rows sit in one lane (s even), for y-faces they are
+// layers 0,1 / K-2,K-1 of the same lane, so no further exchange is needed.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// a[i] for a runtime i without indexing the register array (which would move it to scratch)
+__device__ inline float2 pick4(float2 a0, float2 a1, float2 a2, float2 a3, int i) {
+    return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
+}
+
+template <int K>
+__global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, LeafIO<float> io, int nchunks) {
+    extern __shared__ float2 lds2[];
+    const int s = G.s, ni = G.n_int, nb = G.nb, m = G.m, nl = io.n_leaves;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float2* Bl = lds2;                                   // [K][16 x 16], column-major
+    float2* scr = lds2 + K * 256 + wave * 512;           // two 16x16 tiles, row-major
+    const int leaf = blockIdx.x / nchunks, chunk = blockIdx.x - leaf * nchunks;
+    const int ccol = lane & 15, grp = lane >> 4;
+    const int mycol = chunk * 64 + wave * 16 + ccol;
+    const bool colok = mycol < io.ncols;
+    const int parity = leaf & 1;
+    const size_t lc = (size_t)(colok ? mycol : 0) * nl + leaf;
+
+    const float2* sl = io.sinv + (size_t)leaf * K * s * s;
+    for (int e = tid; e < K * 256; e += 256) {
+        int k = e >> 8, r = e & 15, c = (e >> 4) & 15;
+        Bl[e] = (r < s && c < s) ? sl[(size_t)k * s * s + r + c * s] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+
+    const AxisCoef& ax0 = C.ax[0];
+    const AxisCoef& ax1 = C.ax[1];
+    const float2 cbx = ccast<float2>(cscale(cinv(ax0.dd), -ax0.cib));
+    const float2 cby = ccast<float2>(cscale(cinv(ax1.dd), -ax1.cib));
+    auto gval = [&](int face, int t) -> float2 {
+        return colok ? io.gin[lc * nb + G.noff[parity][face] + t] : make_float2(0.f, 0.f);
+    };
+
+    float2 y[K][4];
+    float2 prev[4];
+    int buf = 0;
+    auto product = [&](int k, const float2 v[4], f4v& aR, f4v& aI) {
+        float2* sb = scr + buf * 256;
+        for (int vv = 0; vv < 4; ++vv) sb[(4 * grp + vv) * 16 + ccol] = v[vv];
+        __syncthreads();
+        aR = f4v{0.f, 0.f, 0.f, 0.f};
+        aI = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float2 a = Bl[k * 256 + ccol + 16 * (4 * q + grp)];
+            const float2 b = sb[(4 * q + grp) * 16 + ccol];
+            aR = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, aR, 0, 0, 0);
+            aR = __builtin_amdgcn_mfma_f32_16x16x4f32(-a.y, b.y, aR, 0, 0, 0);
+            aI = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.y, aI, 0, 0, 0);
+            aI = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.x, aI, 0, 0, 0);
+        }
+        buf ^= 1;
+    };
+
+    // forward sweep; the interior data of layer k+1 is fetched while layer k is
+    // being processed
+    auto fetch = [&](int k, double2 (&raw)[4]) {
+#pragma unroll
+        for (int vv = 0; vv < 4; ++vv) {
+            const int row = 4 * grp + vv;
+            raw[vv] = (row < s && colok) ? load_f<double2>(io, lc * ni + k * s + row) : make_double2(0.0, 0.0);
+        }
+    };
+    double2 fcur[4];
+    fetch(0, fcur);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double2 fnext[4];
+        if (k + 1 < K) fetch(k + 1, fnext);
+        float2 v[4];
+#pragma unroll
+        for (int vv = 0; vv < 4; ++vv) {
+            const int row = 4 * grp + vv;
+            float2 r = ccast<float2>(fcur[vv]);
+            if (row < s && colok && io.mode == LEAF_DOWN) {
+                if (row == 0) r = cfma(cbx, gval(1, k), r);
+                if (row == s - 1) r = cfma(cbx, gval(2, k), r);
+                if (k == 0) r = cfma(cby, gval(3, row), r);
+                if (k == K - 1) r = cfma(cby, gval(4, row), r);
+            }
+            if (k > 0) r = csub(r, cmul(ccast<float2>(slow_lambda(G, C, k)), prev[vv]));
+            v[vv] = r;
+        }
+        f4v aR, aI;
+        product(k, v, aR, aI);
+#pragma unroll
+        for (int vv = 0; vv < 4; ++vv) { y[k][vv] = make_float2(aR[vv], aI[vv]); prev[vv] = y[k][vv]; }
+        if (k + 1 < K)
+#pragma unroll
+            for (int vv = 0; vv < 4; ++vv) fcur[vv] = fnext[vv];
+    }
+    // backward sweep
+#pragma unroll
+    for (int k = K - 2; k >= 0; --k) {
+        float2 w[4];
+        const float2 mu = ccast<float2>(slow_mu(G, C, k));
+#pragma unroll
+        for (int vv = 0; vv < 4; ++vv) w[vv] = cmul(mu, y[k + 1][vv]);
+        f4v aR, aI;
+        product(k, w, aR, aI);
+#pragma unroll
+        for (int vv = 0; vv < 4; ++vv) y[k][vv] = csub(y[k][vv], make_float2(aR[vv], aI[vv]));
+    }
+    if (!colok) return;
+
+    const size_t ub = lc * m;
+    if (io.mode == LEAF_DOWN) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int vv = 0; vv < 4; ++vv) {
+                const int row = 4 * grp + vv;
+                if (row < s) store_u(io, ub + nb + k * s + row, y[k][vv]);
+            }
+    }
+    // boundary values: u_B = (g - a1 x1 - a2 x2) / dd, h = g + (sigma - alpha) u_B
+    const float2 sma = make_float2(0.f, (float)(-2.0 * G.kb));
+    auto emit = [&](int face, int t, float2 x1, float2 x2, const AxisCoef& a) {
+        const int b = G.foff[face] + t;
+        const float2 gb = (io.mode == LEAF_DOWN) ? gval(face, t) : make_float2(0.f, 0.f);
+        float2 num = csub(csub(gb, cscale(x1, (float)a.a1)), cscale(x2, (float)a.a2));
+        float2 uB = cmul(num, ccast<float2>(cinv(a.dd)));
+        if (io.mode == LEAF_UP) {
+            float2 hb = cfma(sma, uB, gb);
+            if (io.jump) hb = csub(hb, ccast<float2>(io.jump[lc * nb + b]));
+            io.hout[lc * nb + G.noff[parity][face] + t] = hb;
+        } else {
+            store_u(io, ub + b, uB);
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (grp == 0) emit(1, k, y[k][0], y[k][1], ax0);
+        if (grp == (s - 1) / 4) emit(2, k, pick4(y[k][0], y[k][1], y[k][2], y[k][3], (s - 1) & 3),
+                                        pick4(y[k][0], y[k][1], y[k][2], y[k][3], (s - 2) & 3), ax0);
+    }
+#pragma unroll
+    for (int vv = 0; vv < 4; ++vv) {
+        const int row = 4 * grp + vv;
+        if (row >= s) continue;
+        emit(3, row, y[0][vv], y[1][vv], ax1);
+        emit(4, row, y[K - 1][vv], y[K - 2][vv], ax1);
+    }
 }
 
 }  // namespace
 
-void launch_residual(const LeafGeom& G, const double2* u, const double2* kint, const double2* f, const double2* g,
-                     const int* nbr, double2 nu, double2 zeta, double2* rf, double2* rg, double2* rj, int n_leaves,
-                     int ncols, hipStream_t st) {
+void launch_residual(const LeafGeom& G, const double2* u64, const float2* u32, const double2* kint, const double2* f,
+                     const double2* g, const int* nbr, double2 nu, double2 zeta, float2* rf, double2* rg,
+                     double2* rj, int n_leaves, int ncols, hipStream_t st) {
     LeafCoef C = make_coef(G);
     dim3 grid((G.m + 127) / 128, n_leaves, ncols);
-    hipLaunchKernelGGL(residual_kernel, grid, dim3(128), 0, st, G, C, u, kint, f, g, nbr, nu, zeta, rf, rg, rj,
-                       n_leaves);
+    if (u64)
+        hipLaunchKernelGGL(residual_kernel<double2>, grid, dim3(128), 0, st, G, C, u64, kint, f, g, nbr, nu, zeta,
+                           rf, rg, rj, n_leaves);
+    else
+        hipLaunchKernelGGL(residual_kernel<float2>, grid, dim3(128), 0, st, G, C, u32, kint, f, g, nbr, nu, zeta,
+                           rf, rg, rj, n_leaves);
 }
 
 size_t leaf_blocks_lds(const LeafGeom& G) {
@@ -464,16 +648,35 @@ template <typename R> int leaf_apply_chunk(const LeafGeom& G, int ncols, size_t 
     return ch;
 }
 
+// The matrix-core path takes 2D leaves of 12x12, 14x14 or 16x16 interior points.
+static bool mfma_leaf_ok(const LeafGeom& G, int mode) {
+    return G.d == 2 && mode != LEAF_FACTOR_T && G.s == G.n[0] && G.s <= 16 && (G.s % 2) == 0 &&
+           (G.K == 12 || G.K == 14 || G.K == 16);
+}
+
 template <typename R>
-void launch_leaf_apply(const LeafGeom& G, int mode, int ncols, int n_leaves, const c_t<R>* sinv, const double2* f,
-                       const double2* jump, const c_t<R>* gin, c_t<R>* hout, double2* u, double2* T,
-                       size_t lds_limit, hipStream_t st, int accumulate) {
+void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit, hipStream_t st) {
     LeafCoef C = make_coef(G);
     LeafIO<R> io;
-    io.accumulate = accumulate;
-    io.mode = mode; io.ncols = ncols; io.CH = leaf_apply_chunk<R>(G, ncols, lds_limit);
-    io.sinv = sinv; io.f = f; io.jump = jump; io.gin = gin; io.hout = hout; io.u = u; io.T = T;
-    io.n_leaves = n_leaves;
+    io.mode = a.mode; io.ncols = a.ncols; io.n_leaves = a.n_leaves;
+    io.CH = leaf_apply_chunk<R>(G, a.ncols, lds_limit);
+    io.sinv = a.sinv; io.f64 = a.f64; io.f32 = a.f32; io.jump = a.jump; io.gin = a.gin; io.hout = a.hout;
+    io.u64 = a.u64; io.u32 = a.u32; io.ub64 = a.ub64; io.ub32 = a.ub32; io.T = a.T;
+    const int mode = a.mode, ncols = a.ncols, n_leaves = a.n_leaves;
+    if constexpr (std::is_same<R, float>::value) {
+        if (mfma_leaf_ok(G, mode)) {
+            const int nchunks = (ncols + 63) / 64;
+            const size_t lds = (size_t)G.K * 256 * sizeof(float2) + 4 * 512 * sizeof(float2);
+            dim3 grid((unsigned)n_leaves * nchunks);
+            if (G.K == 16)
+                hipLaunchKernelGGL(leaf_mfma_kernel<16>, grid, dim3(256), lds, st, G, C, io, nchunks);
+            else if (G.K == 14)
+                hipLaunchKernelGGL(leaf_mfma_kernel<14>, grid, dim3(256), lds, st, G, C, io, nchunks);
+            else
+                hipLaunchKernelGGL(leaf_mfma_kernel<12>, grid, dim3(256), lds, st, G, C, io, nchunks);
+            return;
+        }
+    }
     size_t lds = ((size_t)G.n_int * io.CH + (size_t)G.s * G.s + (size_t)G.s * io.CH) * sizeof(c_t<R>);
     dim3 grid(n_leaves, (ncols + io.CH - 1) / io.CH);
     hipLaunchKernelGGL(leaf_apply_kernel<R>, grid, dim3(256), lds, st, G, C, io);
@@ -481,9 +684,7 @@ void launch_leaf_apply(const LeafGeom& G, int mode, int ncols, int n_leaves, con
 
 template int leaf_apply_chunk<double>(const LeafGeom&, int, size_t);
 template int leaf_apply_chunk<float>(const LeafGeom&, int, size_t);
-template void launch_leaf_apply<double>(const LeafGeom&, int, int, int, const double2*, const double2*, const double2*,
-                                        const double2*, double2*, double2*, double2*, size_t, hipStream_t, int);
-template void launch_leaf_apply<float>(const LeafGeom&, int, int, int, const float2*, const double2*, const double2*,
-                                       const float2*, float2*, double2*, double2*, size_t, hipStream_t, int);
+template void launch_leaf_apply<double>(const LeafGeom&, const LeafArgs<double>&, size_t, hipStream_t);
+template void launch_leaf_apply<float>(const LeafGeom&, const LeafArgs<float>&, size_t, hipStream_t);
 
 }  // namespace rs

@@ -34,13 +34,27 @@ size_t leaf_blocks_lds(const LeafGeom& G);
 void launch_leaf_blocks(const LeafGeom& G, const double2* kint, double2* sinv, int* info, int n_leaves,
                         hipStream_t st);
 template <typename R> int leaf_apply_chunk(const LeafGeom& G, int ncols, size_t lds_limit);
+// Arguments of a leaf pass (see LeafIO in k_leaf.hip).
+template <typename R> struct LeafArgs {
+    int mode = LEAF_UP, ncols = 0, n_leaves = 0;
+    const c_t<R>* sinv = nullptr;
+    const double2* f64 = nullptr;
+    const float2* f32 = nullptr;
+    const double2* jump = nullptr;
+    const c_t<R>* gin = nullptr;
+    c_t<R>* hout = nullptr;
+    double2* u64 = nullptr;
+    float2* u32 = nullptr;
+    const double2* ub64 = nullptr;
+    const float2* ub32 = nullptr;
+    double2* T = nullptr;
+};
 template <typename R>
-void launch_leaf_apply(const LeafGeom& G, int mode, int ncols, int n_leaves, const c_t<R>* sinv, const double2* f,
-                       const double2* jump, const c_t<R>* gin, c_t<R>* hout, double2* u, double2* T,
-                       size_t lds_limit, hipStream_t st, int accumulate = 0);
-void launch_residual(const LeafGeom& G, const double2* u, const double2* kint, const double2* f, const double2* g,
-                     const int* nbr, double2 nu, double2 zeta, double2* rf, double2* rg, double2* rj, int n_leaves,
-                     int ncols, hipStream_t st);
+void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit, hipStream_t st);
+// u is complex128 (u64) or complex64 (u32); rf is written in complex64.
+void launch_residual(const LeafGeom& G, const double2* u64, const float2* u32, const double2* kint, const double2* f,
+                     const double2* g, const int* nbr, double2 nu, double2 zeta, float2* rf, double2* rg,
+                     double2* rj, int n_leaves, int ncols, hipStream_t st);
 
 // batched inverse
 struct InvPlan {

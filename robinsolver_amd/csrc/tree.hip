@@ -54,7 +54,7 @@ template <typename R> void SolveWork<R>::clear() {
     for (DevBuf* b : descs) delete b;
     descs.clear();
     d_up_g.clear(); d_up_h.clear(); d_dn_t.clear(); d_dn_g.clear();
-    rf.release(); rg.release(); rj.release();
+    rf.release(); rg.release(); rj.release(); uA.release();
     cap = 0;
 }
 
@@ -188,8 +188,12 @@ int Tree::factor_device(const double2* k) {
     invM.assign(L, nullptr);
     T[L] = new DevBuf;
     if (alloc(*T[L], (size_t)nl * P.nb * P.nb * sizeof(double2))) return 1;
-    launch_leaf_apply<double>(G, LEAF_FACTOR_T, P.nb, nl, sinv.as<double2>(), nullptr, nullptr, nullptr, nullptr,
-                              nullptr, T[L]->as<double2>(), lds_limit, stream);
+    {
+        LeafArgs<double> a;
+        a.mode = LEAF_FACTOR_T; a.ncols = P.nb; a.n_leaves = nl;
+        a.sinv = sinv.as<double2>(); a.T = T[L]->as<double2>();
+        launch_leaf_apply<double>(G, a, lds_limit, stream);
+    }
     HIPCHK(hipGetLastError());
     const double2 zeta = make_double2(P.zeta[0], P.zeta[1]);
     const double2 nu = make_double2(P.nu[0], P.nu[1]);
@@ -480,21 +484,25 @@ int Tree::prof_collect() {
 }
 
 template <typename R>
-int Tree::solve_pass(SolveWork<R>& w, int nrhs, const double2* f, const double2* jump, const double2* g, double2* u,
-                     hipStream_t st, int accumulate) {
+int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st) {
     using V = c_t<R>;
     const Plan& P = plan;
     const int L = P.L, nl = P.n_leaves;
     const V* sv = std::is_same<R, double>::value ? (const V*)sinv.p : (const V*)sinv32.p;
     const double e = sizeof(V), nr = nrhs;
+    const double ef = io.f64 ? 16.0 : (io.f32 ? 8.0 : 0.0);
+    const double eu = io.u64 ? 16.0 : 8.0;
+    const double eb = io.ub64 ? 16.0 : (io.ub32 ? 8.0 : 0.0);
     // algorithmic traffic of a leaf pass: the block inverses once, plus the
-    // per-RHS vectors (f in, u out, h out / g in, u in+out)
+    // per-RHS vectors (f in; h out / g in, u out, u base in)
     const double sinv_b = (double)nl * G.K * G.s * G.s * e;
     const double leaf_fl = (double)nl * nr * (2.0 * G.K * G.s * G.s * 8.0);
+    LeafArgs<R> up;
+    up.mode = LEAF_UP; up.ncols = nrhs; up.n_leaves = nl; up.sinv = sv;
+    up.f64 = io.f64; up.f32 = io.f32; up.jump = io.jump; up.hout = w.hV[L]->template as<V>();
     pbeg(P_LEAF_UP, st);
-    launch_leaf_apply<R>(G, LEAF_UP, nrhs, nl, sv, f, jump, nullptr, w.hV[L]->template as<V>(), u, nullptr,
-                         lds_limit, st, accumulate);
-    pend(sinv_b + nr * nl * (16.0 * (P.n_int + P.m) + e * P.nb), leaf_fl, st);
+    launch_leaf_apply<R>(G, up, lds_limit, st);
+    pend(sinv_b + nr * nl * (ef * P.n_int + e * P.nb + (io.jump ? 16.0 * P.nb : 0.0)), leaf_fl, st);
     auto gemm = [&](int cat, const GemmDesc* d, int np, int M, int N, int K, bool din) {
         pbeg(cat, st);
         launch_gemm<R>(d, np, M, N, K, st);
@@ -507,8 +515,8 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const double2* f, const double2*
         if (l > 0) gemm(P_GEMM_UP_H, w.d_up_h[l], 2 * Vl.nodes, Vl.bn, nrhs, Vl.isz, true);
     }
     pbeg(P_GATHER, st);
-    launch_root_gather<V>(w.gF[0]->template as<V>(), g, d_root_leaf.as<int>(), d_root_loc.as<int>(), P.lv[0].P, nl,
-                          P.nb, nrhs, st);
+    launch_root_gather<V>(w.gF[0]->template as<V>(), io.g, d_root_leaf.as<int>(), d_root_loc.as<int>(), P.lv[0].P,
+                          nl, P.nb, nrhs, st);
     pend(nr * P.lv[0].P * (16.0 + e), 0, st);
     for (int l = 0; l < L; ++l) {
         const Level& Vl = P.lv[l];
@@ -519,10 +527,13 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const double2* f, const double2*
         gemm(P_GEMM_DN_T, w.d_dn_t[l], 2 * Vl.nodes, Vl.isz, nrhs, Vl.bn, false);
         gemm(P_GEMM_DN_G, w.d_dn_g[l], Vl.nodes, 2 * Vl.isz, nrhs, 2 * Vl.isz, true);
     }
+    LeafArgs<R> dn;
+    dn.mode = LEAF_DOWN; dn.ncols = nrhs; dn.n_leaves = nl; dn.sinv = sv;
+    dn.f64 = io.f64; dn.f32 = io.f32; dn.gin = w.gF[L]->template as<V>();
+    dn.u64 = io.u64; dn.u32 = io.u32; dn.ub64 = io.ub64; dn.ub32 = io.ub32;
     pbeg(P_LEAF_DOWN, st);
-    launch_leaf_apply<R>(G, LEAF_DOWN, nrhs, nl, sv, nullptr, nullptr, w.gF[L]->template as<V>(), nullptr, u, nullptr,
-                         lds_limit, st);
-    pend(sinv_b + nr * nl * (32.0 * P.m + e * P.nb), leaf_fl, st);
+    launch_leaf_apply<R>(G, dn, lds_limit, st);
+    pend(sinv_b + nr * nl * (ef * P.n_int + e * P.nb + (eu + eb) * P.m), leaf_fl, st);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -535,33 +546,52 @@ int Tree::solve_device(int nrhs, const double2* f, const double2* g, double2* u,
     if (prec == RS_MIXED) {
         if (!have32 && make_c64()) return 1;
         if (ensure_work(w32, nrhs)) return 1;
-        if (solve_pass(w32, nrhs, f, nullptr, g, u, st, 0)) return 1;
-        // FP64 iterative refinement: u += K32^-1 (b - K u), a fixed number of
-        // steps so that a right-hand side gets the same bits alone or batched
-        const size_t nl = plan.n_leaves;
-        const size_t need_f = (size_t)w32.cap * nl * plan.n_int * 16, need_g = (size_t)w32.cap * nl * plan.nb * 16;
-        if (opts.n_refine > 0 && w32.rf.bytes < need_f) {
-            if (alloc(w32.rf, need_f) || alloc(w32.rg, need_g) || alloc(w32.rj, need_g)) return 1;
+        PassIO io;
+        io.f64 = f; io.g = g;
+        if (opts.n_refine <= 0) {
+            io.u64 = u;
+            return solve_pass(w32, nrhs, io, st);
         }
+        // The first iterate is kept in complex64: its FP64 residual is exact for
+        // the rounded iterate, and the correction restores full accuracy.  Then
+        // u = u_{i} + K32^-1 (b - K u_{i}) for a fixed number of steps, so that a
+        // right-hand side gets the same bits alone or batched.
+        const size_t nl = plan.n_leaves, cap = w32.cap;
+        if (!w32.rf.p) {
+            if (alloc(w32.rf, cap * nl * plan.n_int * 8) || alloc(w32.rg, cap * nl * plan.nb * 16) ||
+                alloc(w32.rj, cap * nl * plan.nb * 16) || alloc(w32.uA, cap * nl * plan.m * 8))
+                return 1;
+        }
+        float2* uA = w32.uA.as<float2>();
+        io.u32 = uA;
+        if (solve_pass(w32, nrhs, io, st)) return 1;
         for (int it = 0; it < opts.n_refine; ++it) {
-            if (residual(nrhs, u, f, g, w32.rf.as<double2>(), w32.rg.as<double2>(), w32.rj.as<double2>(), st))
+            if (residual(nrhs, it == 0 ? nullptr : u, it == 0 ? uA : nullptr, f, g, w32.rf.as<float2>(),
+                         w32.rg.as<double2>(), w32.rj.as<double2>(), st))
                 return 1;
-            if (solve_pass(w32, nrhs, w32.rf.as<double2>(), w32.rj.as<double2>(), w32.rg.as<double2>(), u, st, 1))
-                return 1;
+            PassIO c;
+            c.f32 = w32.rf.as<float2>(); c.jump = w32.rj.as<double2>(); c.g = w32.rg.as<double2>();
+            c.u64 = u;
+            if (it == 0) c.ub32 = uA; else c.ub64 = u;
+            if (solve_pass(w32, nrhs, c, st)) return 1;
         }
         return 0;
     }
     if (ensure_work(w64, nrhs)) return 1;
-    return solve_pass(w64, nrhs, f, nullptr, g, u, st, 0);
+    PassIO io;
+    io.f64 = f; io.g = g; io.u64 = u;
+    return solve_pass(w64, nrhs, io, st);
 }
 
-int Tree::residual(int nrhs, const double2* u, const double2* f, const double2* g, double2* rf, double2* rg,
-                   double2* rj, hipStream_t st) {
+int Tree::residual(int nrhs, const double2* u64, const float2* u32, const double2* f, const double2* g, float2* rf,
+                   double2* rg, double2* rj, hipStream_t st) {
     const Plan& P = plan;
     pbeg(P_RESIDUAL, st);
-    launch_residual(G, u, kint.as<double2>(), f, g, d_leaf_nbr.as<int>(), make_double2(P.nu[0], P.nu[1]),
+    launch_residual(G, u64, u32, kint.as<double2>(), f, g, d_leaf_nbr.as<int>(), make_double2(P.nu[0], P.nu[1]),
                     make_double2(P.zeta[0], P.zeta[1]), rf, rg, rj, P.n_leaves, nrhs, st);
-    pend((double)nrhs * P.n_leaves * 16.0 * (2.0 * P.m + 2.0 * P.n_int + 3.0 * P.nb) + 16.0 * P.n_leaves * P.n_int,
+    const double eu = u64 ? 16.0 : 8.0;
+    pend((double)nrhs * P.n_leaves * (eu * P.m + 16.0 * P.n_int + 8.0 * P.n_int + 16.0 * 3 * P.nb) +
+             16.0 * P.n_leaves * P.n_int,
          (double)nrhs * P.n_leaves * P.m * 8.0 * (2 * G.d + 1), st);
     HIPCHK(hipGetLastError());
     return 0;

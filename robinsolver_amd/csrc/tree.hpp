@@ -34,7 +34,7 @@ template <typename R> struct SolveWork {
     using V = c_t<R>;
     int cap = 0;
     std::vector<DevBuf*> hV, gF, G, Z;   // per level
-    DevBuf rf, rg, rj;                   // refinement residuals (double2)
+    DevBuf rf, rg, rj, uA;               // refinement: residuals (c64 / c128 / c128), first iterate (c64)
     std::vector<DevBuf*> descs;          // per (level, kind)
     std::vector<GemmDesc*> d_up_g, d_up_h, d_dn_t, d_dn_g;
     ~SolveWork() { clear(); }
@@ -48,6 +48,18 @@ struct ProfRec {
     int cat;
     hipEvent_t a, b;
     double bytes, flops;
+};
+
+// Inputs and outputs of one tree solve pass (up + down) over all right-hand sides.
+struct PassIO {
+    const double2* f64 = nullptr;   // interior data, complex128 ...
+    const float2* f32 = nullptr;    //   ... or complex64
+    const double2* jump = nullptr;  // transmission residual subtracted from the leaf outgoing data
+    const double2* g = nullptr;     // physical boundary data (complex128)
+    double2* u64 = nullptr;         // solution out, complex128 ...
+    float2* u32 = nullptr;          //   ... or complex64
+    const double2* ub64 = nullptr;  // base added to the solution
+    const float2* ub32 = nullptr;
 };
 
 struct Tree {
@@ -103,10 +115,9 @@ struct Tree {
     int factor_device(const double2* k_int_dev);
     int make_c64();
     template <typename R> int ensure_work(SolveWork<R>& w, int nrhs);
-    template <typename R> int solve_pass(SolveWork<R>& w, int nrhs, const double2* f, const double2* jump,
-                                         const double2* g, double2* u, hipStream_t st, int accumulate);
-    int residual(int nrhs, const double2* u, const double2* f, const double2* g, double2* rf, double2* rg,
-                 double2* rj, hipStream_t st);
+    template <typename R> int solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st);
+    int residual(int nrhs, const double2* u64, const float2* u32, const double2* f, const double2* g, float2* rf,
+                 double2* rg, double2* rj, hipStream_t st);
     int solve_device(int nrhs, const double2* f, const double2* g, double2* u, int prec, hipStream_t st);
     int solve_host(int nrhs, const double* f, const double* g, int prec);
     int get_solution(int rhs, double* u) const;

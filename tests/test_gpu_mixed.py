@@ -53,6 +53,28 @@ def test_mixed_matches_oracle(d, n, L, kb, n_refine, gpu):
         assert max(rels) <= TOL_MIXED, rels
 
 
+@pytest.mark.parametrize("n,L", [(64, 4), (56, 4), (48, 4)])   # 16x16, 14x14, 12x12 leaves
+def test_mixed_wide_batch_matches_oracle(n, L, gpu):
+    """More RHS than one 64-column leaf tile, on each leaf size of the
+    matrix-core leaf kernel."""
+    print("case: mixed wide batch", n, L, flush=True)
+    kb = 12.0
+    o = OracleTree(2, L, UNIT, 1.0 / n, -kb)
+    t = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=1)
+    pts, face = o.all_points()
+    k = _k(pts[:, o.nb:], kb)
+    o.factor(k)
+    t.factor(k)
+    rng = np.random.default_rng(8)
+    nrhs = 70
+    f = _rand_c(rng, (nrhs, o.n_leaves, o.n_int))
+    g = _rand_c(rng, (nrhs, o.n_leaves, o.nb))
+    u = t.solve(f, g)
+    for r in (0, 33, 64, 69):
+        rel = _rel(u[r], o.solve(f[r], g[r]))
+        assert rel <= TOL_MIXED, (r, rel)
+
+
 def test_mixed_batch_invariance(gpu):
     print("case: mixed batch invariance", flush=True)
     t = rsa.GpuTree(2, 4, UNIT, 1.0 / 64, -10.0, precision=rsa.RS_MIXED)
