@@ -275,7 +275,7 @@ int rs_profile_ncat(void) { return rs::P_NCAT; }
 
 const char* rs_profile_name(int cat) {
     static const char* names[rs::P_NCAT] = {"leaf_up", "leaf_down", "gemm_up_g", "gemm_up_h",
-                                            "gemm_dn_t", "gemm_dn_g", "gather", "residual"};
+                                            "gemm_dn_t", "gemm_dn_g", "gather", "residual", "combine"};
     return (cat >= 0 && cat < rs::P_NCAT) ? names[cat] : "";
 }
 

@@ -249,9 +249,7 @@ template <typename R> struct LeafIO {
     const c_t<R>* gin;        // DOWN: incoming data g: [col][leaf][nb] node order
     c_t<R>* hout;             // UP: outgoing data h: [col][leaf][nb] node order
     double2* u64;             // DOWN: solution [col][leaf][m], complex128 ...
-    float2* u32;              //   ... or complex64 (one set)
-    const double2* ub64;      // DOWN: base added to the solution (complex128 ...
-    const float2* ub32;       //   ... or complex64; optional, may alias the output)
+    float2* u32;              //   ... or complex64 (one set); written, never read
     double2* T;               // FACTOR: [leaf][nb][nb] node order, column-major
 };
 
@@ -264,11 +262,8 @@ __device__ inline V load_f(const LeafIO<R>& io, size_t idx) {
 
 template <typename V, typename R>
 __device__ inline void store_u(const LeafIO<R>& io, size_t idx, V x) {
-    double2 v = ccast<double2>(x);
-    if (io.ub64) v = cadd(v, io.ub64[idx]);
-    else if (io.ub32) v = cadd(v, ccast<double2>(io.ub32[idx]));
-    if (io.u64) io.u64[idx] = v;
-    else io.u32[idx] = ccast<float2>(v);
+    if (io.u64) io.u64[idx] = ccast<double2>(x);
+    else io.u32[idx] = ccast<float2>(x);
 }
 
 
@@ -479,6 +474,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     float2* Bl = lds2;                                   // [K][16 x 16], column-major
     float2* scr = lds2 + K * 256 + wave * 512;           // two 16x16 tiles, row-major
+    float2* bd = lds2 + K * 256 + 4 * 512 + wave * 16 * nb;  // per-column boundary data (g or jump)
     const int leaf = blockIdx.x / nchunks, chunk = blockIdx.x - leaf * nchunks;
     const int ccol = lane & 15, grp = lane >> 4;
     const int mycol = chunk * 64 + wave * 16 + ccol;
@@ -491,15 +487,29 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
         int k = e >> 8, r = e & 15, c = (e >> 4) & 15;
         Bl[e] = (r < s && c < s) ? sl[(size_t)k * s * s + r + c * s] : make_float2(0.f, 0.f);
     }
+    // boundary data of the wave's 16 columns, read once with coalesced loads:
+    // DOWN: incoming g (node order); UP: the transmission residual (local order)
+    {
+        const int wcol0 = chunk * 64 + wave * 16;
+        for (int e = lane; e < 16 * nb; e += 64) {
+            const int c = e / nb, i = e - c * nb;
+            const int col = wcol0 + c;
+            float2 v = make_float2(0.f, 0.f);
+            if (col < io.ncols) {
+                const size_t gi = ((size_t)col * nl + leaf) * nb + i;
+                if (io.mode == LEAF_DOWN) v = io.gin[gi];
+                else if (io.jump) v = ccast<float2>(io.jump[gi]);
+            }
+            bd[e] = v;
+        }
+    }
     __syncthreads();
 
     const AxisCoef& ax0 = C.ax[0];
     const AxisCoef& ax1 = C.ax[1];
     const float2 cbx = ccast<float2>(cscale(cinv(ax0.dd), -ax0.cib));
     const float2 cby = ccast<float2>(cscale(cinv(ax1.dd), -ax1.cib));
-    auto gval = [&](int face, int t) -> float2 {
-        return colok ? io.gin[lc * nb + G.noff[parity][face] + t] : make_float2(0.f, 0.f);
-    };
+    auto gval = [&](int face, int t) -> float2 { return bd[ccol * nb + G.noff[parity][face] + t]; };
 
     float2 y[K][4];
     float2 prev[4];
@@ -592,7 +602,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
         float2 uB = cmul(num, ccast<float2>(cinv(a.dd)));
         if (io.mode == LEAF_UP) {
             float2 hb = cfma(sma, uB, gb);
-            if (io.jump) hb = csub(hb, ccast<float2>(io.jump[lc * nb + b]));
+            hb = csub(hb, bd[ccol * nb + b]);
             io.hout[lc * nb + G.noff[parity][face] + t] = hb;
         } else {
             store_u(io, ub + b, uB);
@@ -661,12 +671,12 @@ void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit
     io.mode = a.mode; io.ncols = a.ncols; io.n_leaves = a.n_leaves;
     io.CH = leaf_apply_chunk<R>(G, a.ncols, lds_limit);
     io.sinv = a.sinv; io.f64 = a.f64; io.f32 = a.f32; io.jump = a.jump; io.gin = a.gin; io.hout = a.hout;
-    io.u64 = a.u64; io.u32 = a.u32; io.ub64 = a.ub64; io.ub32 = a.ub32; io.T = a.T;
+    io.u64 = a.u64; io.u32 = a.u32; io.T = a.T;
     const int mode = a.mode, ncols = a.ncols, n_leaves = a.n_leaves;
     if constexpr (std::is_same<R, float>::value) {
         if (mfma_leaf_ok(G, mode)) {
             const int nchunks = (ncols + 63) / 64;
-            const size_t lds = (size_t)G.K * 256 * sizeof(float2) + 4 * 512 * sizeof(float2);
+            const size_t lds = ((size_t)G.K * 256 + 4 * 512 + 4 * 16 * (size_t)G.nb) * sizeof(float2);
             dim3 grid((unsigned)n_leaves * nchunks);
             if (G.K == 16)
                 hipLaunchKernelGGL(leaf_mfma_kernel<16>, grid, dim3(256), lds, st, G, C, io, nchunks);

@@ -73,6 +73,16 @@ __global__ void axpy_kernel(double2* __restrict__ y, const double2* __restrict__
     if (e < n) y[e] = cadd(y[e], x[e]);
 }
 
+__global__ void combine_kernel(double2* u, const double2* base64, const float2* __restrict__ base32,
+                               const float2* __restrict__ d, size_t n) {
+    size_t e = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
+    for (int q = 0; q < 2; ++q, ++e) {
+        if (e >= n) return;
+        const double2 b = base64 ? base64[e] : ccast<double2>(base32[e]);
+        u[e] = cadd(b, ccast<double2>(d[e]));
+    }
+}
+
 inline unsigned nblk(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -113,6 +123,11 @@ void launch_zero(double2* p, size_t n, hipStream_t st) {
 
 void launch_axpy(double2* y, const double2* x, size_t n, hipStream_t st) {
     if (n) hipLaunchKernelGGL(axpy_kernel, dim3(nblk(n)), dim3(256), 0, st, y, x, n);
+}
+
+void launch_combine(double2* u, const double2* base64, const float2* base32, const float2* d, size_t n,
+                    hipStream_t st) {
+    if (n) hipLaunchKernelGGL(combine_kernel, dim3(nblk((n + 1) / 2)), dim3(256), 0, st, u, base64, base32, d, n);
 }
 
 template void launch_root_gather<double2>(double2*, const double2*, const int*, const int*, int, int, int, int,

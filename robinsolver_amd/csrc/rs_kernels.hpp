@@ -45,8 +45,6 @@ template <typename R> struct LeafArgs {
     c_t<R>* hout = nullptr;
     double2* u64 = nullptr;
     float2* u32 = nullptr;
-    const double2* ub64 = nullptr;
-    const float2* ub32 = nullptr;
     double2* T = nullptr;
 };
 template <typename R>
@@ -83,5 +81,8 @@ void launch_child_gather(V* child, const V* parent, const int* prow, int bn, int
 void launch_to_c64(float2* dst, const double2* src, size_t n, hipStream_t st);
 void launch_zero(double2* p, size_t n, hipStream_t st);
 void launch_axpy(double2* y, const double2* x, size_t n, hipStream_t st);
+// u = base + d, base complex128 (base64, may alias u) or complex64 (base32), d complex64
+void launch_combine(double2* u, const double2* base64, const float2* base32, const float2* d, size_t n,
+                    hipStream_t st);
 
 }  // namespace rs

@@ -54,7 +54,7 @@ template <typename R> void SolveWork<R>::clear() {
     for (DevBuf* b : descs) delete b;
     descs.clear();
     d_up_g.clear(); d_up_h.clear(); d_dn_t.clear(); d_dn_g.clear();
-    rf.release(); rg.release(); rj.release(); uA.release();
+    rf.release(); rg.release(); rj.release(); uA.release(); du.release();
     cap = 0;
 }
 
@@ -492,7 +492,6 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     const double e = sizeof(V), nr = nrhs;
     const double ef = io.f64 ? 16.0 : (io.f32 ? 8.0 : 0.0);
     const double eu = io.u64 ? 16.0 : 8.0;
-    const double eb = io.ub64 ? 16.0 : (io.ub32 ? 8.0 : 0.0);
     // algorithmic traffic of a leaf pass: the block inverses once, plus the
     // per-RHS vectors (f in; h out / g in, u out, u base in)
     const double sinv_b = (double)nl * G.K * G.s * G.s * e;
@@ -530,10 +529,10 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     LeafArgs<R> dn;
     dn.mode = LEAF_DOWN; dn.ncols = nrhs; dn.n_leaves = nl; dn.sinv = sv;
     dn.f64 = io.f64; dn.f32 = io.f32; dn.gin = w.gF[L]->template as<V>();
-    dn.u64 = io.u64; dn.u32 = io.u32; dn.ub64 = io.ub64; dn.ub32 = io.ub32;
+    dn.u64 = io.u64; dn.u32 = io.u32;
     pbeg(P_LEAF_DOWN, st);
     launch_leaf_apply<R>(G, dn, lds_limit, st);
-    pend(sinv_b + nr * nl * (ef * P.n_int + e * P.nb + (eu + eb) * P.m), leaf_fl, st);
+    pend(sinv_b + nr * nl * (ef * P.n_int + e * P.nb + eu * P.m), leaf_fl, st);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -559,7 +558,8 @@ int Tree::solve_device(int nrhs, const double2* f, const double2* g, double2* u,
         const size_t nl = plan.n_leaves, cap = w32.cap;
         if (!w32.rf.p) {
             if (alloc(w32.rf, cap * nl * plan.n_int * 8) || alloc(w32.rg, cap * nl * plan.nb * 16) ||
-                alloc(w32.rj, cap * nl * plan.nb * 16) || alloc(w32.uA, cap * nl * plan.m * 8))
+                alloc(w32.rj, cap * nl * plan.nb * 16) || alloc(w32.uA, cap * nl * plan.m * 8) ||
+                alloc(w32.du, cap * nl * plan.m * 8))
                 return 1;
         }
         float2* uA = w32.uA.as<float2>();
@@ -571,9 +571,12 @@ int Tree::solve_device(int nrhs, const double2* f, const double2* g, double2* u,
                 return 1;
             PassIO c;
             c.f32 = w32.rf.as<float2>(); c.jump = w32.rj.as<double2>(); c.g = w32.rg.as<double2>();
-            c.u64 = u;
-            if (it == 0) c.ub32 = uA; else c.ub64 = u;
+            c.u32 = w32.du.as<float2>();
             if (solve_pass(w32, nrhs, c, st)) return 1;
+            const size_t n = (size_t)nrhs * nl * plan.m;
+            pbeg(P_COMBINE, st);
+            launch_combine(u, it == 0 ? nullptr : u, it == 0 ? uA : nullptr, w32.du.as<float2>(), n, st);
+            pend((double)n * (8.0 + (it == 0 ? 8.0 : 16.0) + 16.0), 0, st);
         }
         return 0;
     }

@@ -34,7 +34,7 @@ template <typename R> struct SolveWork {
     using V = c_t<R>;
     int cap = 0;
     std::vector<DevBuf*> hV, gF, G, Z;   // per level
-    DevBuf rf, rg, rj, uA;               // refinement: residuals (c64 / c128 / c128), first iterate (c64)
+    DevBuf rf, rg, rj, uA, du;           // refinement: residuals (c64 / c128 / c128), first iterate, correction (c64)
     std::vector<DevBuf*> descs;          // per (level, kind)
     std::vector<GemmDesc*> d_up_g, d_up_h, d_dn_t, d_dn_g;
     ~SolveWork() { clear(); }
@@ -42,7 +42,7 @@ template <typename R> struct SolveWork {
 };
 
 enum ProfCat { P_LEAF_UP = 0, P_LEAF_DOWN, P_GEMM_UP_G, P_GEMM_UP_H, P_GEMM_DN_T, P_GEMM_DN_G, P_GATHER,
-               P_RESIDUAL, P_NCAT };
+               P_RESIDUAL, P_COMBINE, P_NCAT };
 
 struct ProfRec {
     int cat;
@@ -58,8 +58,6 @@ struct PassIO {
     const double2* g = nullptr;     // physical boundary data (complex128)
     double2* u64 = nullptr;         // solution out, complex128 ...
     float2* u32 = nullptr;          //   ... or complex64
-    const double2* ub64 = nullptr;  // base added to the solution
-    const float2* ub32 = nullptr;
 };
 
 struct Tree {
