@@ -190,8 +190,12 @@ int check_gemm_t(int M, int N, int K, const double* A, int lda, const double* B,
     d.Din = dD; d.ldd = ldc;
     d.ar = alpha[0]; d.ai = alpha[1]; d.br = beta[0]; d.bi = beta[1];
     (void)hipMemcpy(dd, &d, sizeof d, hipMemcpyHostToDevice);
-    rs::launch_gemm<R>(dd, 1, M, N, K, nullptr);
+    void* ws = nullptr;
+    size_t wse = rs::gemm_ws_elems(1, M, N, K);
+    if (wse) (void)hipMalloc(&ws, wse * sizeof(V));
+    rs::launch_gemm<R>(dd, 1, M, N, K, nullptr, ws);
     hipError_t e = hipDeviceSynchronize();
+    if (ws) (void)hipFree(ws);
     if (e == hipSuccess) e = hipMemcpy(hC.data(), dC, nc * sizeof(V), hipMemcpyDeviceToHost);
     cleanup();
     if (e != hipSuccess) { g_err = hipGetErrorString(e); return 1; }

@@ -46,29 +46,40 @@ template <> __device__ inline int out_row<float>(int lane, int v) { return 4 * (
 template <> __device__ inline int out_row<double>(int lane, int v) { return (lane >> 4) + 4 * v; }
 
 constexpr int BM = 64;
-constexpr int BK = 16;
+constexpr int BK = 32;
 
+// One workgroup computes a 64 x BN tile over the K range of its slice.  The
+// next 64 x 32 / 32 x BN operand tiles are fetched into registers while the
+// current ones, staged in LDS, feed the matrix cores.  With kslices > 1 the raw
+// partial products go to a workspace and gemm_reduce_kernel sums the slices in
+// a fixed order.
 template <typename R, int BN>
 __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ descs, int M, int N, int K,
-                                                  int tilesM, int tilesN) {
+                                                  int tilesM, int tilesN, int kslices, int kchunk,
+                                                  c_t<R>* __restrict__ ws) {
     using V = c_t<R>;
     using A4 = typename acc_t<R>::T;
     constexpr int WAVES_N = (BN >= 32) ? 2 : 1;
     constexpr int WAVES_M = 4 / WAVES_N;
     constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
     constexpr int TM = WM / 16, TN = WN / 16;
+    constexpr int NA = BM * BK / 256, NB = BK * BN / 256;
     __shared__ V As[BK][BM];
     __shared__ V Bs[BK][BN];
 
     const int tiles = tilesM * tilesN;
-    const int p = blockIdx.x / tiles;
-    const int t = blockIdx.x - p * tiles;
+    const int per = tiles * kslices;
+    const int p = blockIdx.x / per;
+    const int rem = blockIdx.x - p * per;
+    const int ks = rem / tiles;
+    const int t = rem - ks * tiles;
     const int tm = t % tilesM, tn = t / tilesM;
     const GemmDesc& D = descs[p];
     const V* __restrict__ A = (const V*)D.A;
     const V* __restrict__ B = (const V*)D.B;
     const V* __restrict__ B2 = (const V*)D.B2;
     const int m0 = tm * BM, n0 = tn * BN;
+    const int kbeg = ks * kchunk, kend = min(K, kbeg + kchunk);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave % WAVES_M) * WM, wn = (wave / WAVES_M) * WN;
 
@@ -77,22 +88,42 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
         for (int j = 0; j < TN; ++j)
             for (int v = 0; v < 4; ++v) { accR[i][j][v] = R(0); accI[i][j][v] = R(0); }
 
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        for (int e = tid; e < BM * BK; e += 256) {
-            int r = e % BM, k = e / BM;
-            int gr = m0 + r, gk = k0 + k;
-            As[k][r] = (gr < M && gk < K) ? A[(size_t)gr + (size_t)gk * D.lda] : cz<R>();
+    V ra[NA], rb[NB];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e % BM, k = e / BM;
+            const int gr = m0 + r, gk = k0 + k;
+            ra[q] = (gr < M && gk < kend) ? A[(size_t)gr + (size_t)gk * D.lda] : cz<R>();
         }
-        for (int e = tid; e < BK * BN; e += 256) {
-            int k = e % BK, c = e / BK;
-            int gk = k0 + k, gc = n0 + c;
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int e = tid + 256 * q;
+            const int k = e % BK, c = e / BK;
+            const int gk = k0 + k, gc = n0 + c;
             V v = cz<R>();
-            if (gk < K && gc < N)
+            if (gk < kend && gc < N)
                 v = (gk < D.ksplit) ? B[(size_t)gk + (size_t)gc * D.ldb]
                                     : B2[(size_t)(gk - D.ksplit) + (size_t)gc * D.ldb];
-            Bs[k][c] = v;
+            rb[q] = v;
         }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) { const int e = tid + 256 * q; As[e / BM][e % BM] = ra[q]; }
+#pragma unroll
+        for (int q = 0; q < NB; ++q) { const int e = tid + 256 * q; Bs[e % BK][e / BK] = rb[q]; }
+    };
+
+    if (kbeg < kend) {
+        fetch(kbeg);
+        stash();
         __syncthreads();
+    }
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        const bool more = k0 + BK < kend;
+        if (more) fetch(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             V af[TM], bf[TN];
@@ -111,8 +142,23 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
                 }
         }
         __syncthreads();
+        if (more) {
+            stash();
+            __syncthreads();
+        }
     }
 
+    if (kslices > 1) {
+        V* W = ws + ((size_t)p * kslices + ks) * (size_t)M * N;
+        for (int i = 0; i < TM; ++i)
+            for (int j = 0; j < TN; ++j)
+                for (int v = 0; v < 4; ++v) {
+                    const int row = m0 + wm + i * 16 + out_row<R>(lane, v);
+                    const int col = n0 + wn + j * 16 + (lane & 15);
+                    if (row < M && col < N) W[(size_t)row + (size_t)col * M] = mk<R>(accR[i][j][v], accI[i][j][v]);
+                }
+        return;
+    }
     V* Out = (V*)D.Out;
     const V* Din = (const V*)D.Din;
     const V alpha = mk<R>((R)D.ar, (R)D.ai), beta = mk<R>((R)D.br, (R)D.bi);
@@ -133,24 +179,75 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
             }
 }
 
+// Sum of the K slices in slice order, then the epilogue of gemm_kernel.
+template <typename R>
+__global__ void gemm_reduce_kernel(const GemmDesc* __restrict__ descs, int M, int N, int kslices,
+                                   const c_t<R>* __restrict__ ws) {
+    using V = c_t<R>;
+    const size_t mn = (size_t)M * N;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (e >= mn) return;
+    const int row = (int)(e % M), col = (int)(e / M);
+    const GemmDesc& D = descs[p];
+    const V* W = ws + (size_t)p * kslices * mn + e;
+    V sum = W[0];
+    for (int k = 1; k < kslices; ++k) sum = cadd(sum, W[(size_t)k * mn]);
+    V val = cmul(mk<R>((R)D.ar, (R)D.ai), sum);
+    const int orow = D.rmap ? D.rmap[row] : row;
+    const int ocol = D.cmap ? D.cmap[col] : col;
+    const V* Din = (const V*)D.Din;
+    if (Din) {
+        size_t di = D.dmap ? (size_t)orow + (size_t)ocol * D.ldd : (size_t)row + (size_t)col * D.ldd;
+        val = cfma(mk<R>((R)D.br, (R)D.bi), Din[di], val);
+    }
+    ((V*)D.Out)[(size_t)orow + (size_t)ocol * D.ldo] = val;
+}
+
 }  // namespace
 
+int gemm_kslices(int nprob, int M, int K) {
+    // split K only when the row tiles of all problems cannot fill the chip;
+    // the choice never depends on N
+    const long long rowtiles = (long long)nprob * ((M + BM - 1) / BM);
+    if (rowtiles >= 128 || K < 256) return 1;
+    long long ks = (256 + rowtiles - 1) / rowtiles;
+    if (ks > K / 128) ks = K / 128;
+    if (ks > 16) ks = 16;
+    return ks < 1 ? 1 : (int)ks;
+}
+
+size_t gemm_ws_elems(int nprob, int M, int N, int K) {
+    const int ks = gemm_kslices(nprob, M, K);
+    return ks > 1 ? (size_t)nprob * ks * M * N : 0;
+}
+
 template <typename R>
-void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st) {
+void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws) {
     if (nprob <= 0 || M <= 0 || N <= 0) return;
-    int tilesM = (M + BM - 1) / BM;
+    const int tilesM = (M + BM - 1) / BM;
+    int ks = gemm_kslices(nprob, M, K);
+    if (!ws) ks = 1;
+    const int kchunk = ((K + ks - 1) / ks + BK - 1) / BK * BK;
+    ks = (K + kchunk - 1) / kchunk;
+    c_t<R>* W = (c_t<R>*)ws;
     if (N <= 16) {
-        int tilesN = (N + 15) / 16;
-        hipLaunchKernelGGL((gemm_kernel<R, 16>), dim3(nprob * tilesM * tilesN), dim3(256), 0, st, d_descs, M, N,
-                           K, tilesM, tilesN);
+        const int tilesN = (N + 15) / 16;
+        hipLaunchKernelGGL((gemm_kernel<R, 16>), dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs, M,
+                           N, K, tilesM, tilesN, ks, kchunk, W);
     } else {
-        int tilesN = (N + 63) / 64;
-        hipLaunchKernelGGL((gemm_kernel<R, 64>), dim3(nprob * tilesM * tilesN), dim3(256), 0, st, d_descs, M, N,
-                           K, tilesM, tilesN);
+        const int tilesN = (N + 63) / 64;
+        hipLaunchKernelGGL((gemm_kernel<R, 64>), dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs, M,
+                           N, K, tilesM, tilesN, ks, kchunk, W);
+    }
+    if (ks > 1) {
+        const size_t mn = (size_t)M * N;
+        hipLaunchKernelGGL(gemm_reduce_kernel<R>, dim3((unsigned)((mn + 255) / 256), nprob), dim3(256), 0, st,
+                           d_descs, M, N, ks, W);
     }
 }
 
-template void launch_gemm<double>(const GemmDesc*, int, int, int, int, hipStream_t);
-template void launch_gemm<float>(const GemmDesc*, int, int, int, int, hipStream_t);
+template void launch_gemm<double>(const GemmDesc*, int, int, int, int, hipStream_t, void*);
+template void launch_gemm<float>(const GemmDesc*, int, int, int, int, hipStream_t, void*);
 
 }  // namespace rs

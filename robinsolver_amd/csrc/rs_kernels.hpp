@@ -26,8 +26,12 @@ struct GemmDesc {
     double ar, ai, br, bi;
 };
 
+// ws: workspace of gemm_ws_elems(...) complex elements of precision R for the
+// K-split form (NULL: no split).
 template <typename R>
-void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st);
+void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws = nullptr);
+int gemm_kslices(int nprob, int M, int K);
+size_t gemm_ws_elems(int nprob, int M, int N, int K);
 
 // leaf stage
 size_t leaf_blocks_lds(const LeafGeom& G);

@@ -20,6 +20,7 @@
 //   every step batched over the nodes of a level and over all right-hand sides.
 #include "tree.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -54,7 +55,7 @@ template <typename R> void SolveWork<R>::clear() {
     for (DevBuf* b : descs) delete b;
     descs.clear();
     d_up_g.clear(); d_up_h.clear(); d_dn_t.clear(); d_dn_g.clear();
-    rf.release(); rg.release(); rj.release(); uA.release(); du.release();
+    rf.release(); rg.release(); rj.release(); uA.release(); du.release(); ws.release();
     cap = 0;
 }
 
@@ -207,6 +208,14 @@ int Tree::factor_device(const double2* k) {
         if (alloc(Dm, nodes * ii * 16) || alloc(Sm, nodes * ii * 16) || alloc(SD, nodes * ii * 16) ||
             alloc(Y, (size_t)nodes * 2 * i * 2 * bn * 16))
             return 1;
+        // K-split workspace for this level's products
+        size_t wse = 0;
+        wse = std::max(wse, gemm_ws_elems(nodes, i, i, i));
+        wse = std::max(wse, gemm_ws_elems(2 * nodes, i, i, i));
+        wse = std::max(wse, gemm_ws_elems(2 * nodes, 2 * i, bn, i));
+        wse = std::max(wse, gemm_ws_elems(4 * nodes, bn, bn, i));
+        DevBuf gws;
+        if (wse && alloc(gws, wse * sizeof(double2))) return 1;
         InvPlan ip;
         ip.build(i, nodes, ii, lds_limit);
         std::vector<GemmDesc> inv_descs;
@@ -283,8 +292,8 @@ int Tree::factor_device(const double2* k) {
                 v.push_back(g);
             }
             if (upload(dd_sd, v)) return 1;
-            launch_gemm<double>(dd_sd.as<GemmDesc>(), nodes, i, i, i, stream);
-            launch_gemm<double>(dd_sd.as<GemmDesc>() + nodes, 2 * nodes, i, i, i, stream);
+            launch_gemm<double>(dd_sd.as<GemmDesc>(), nodes, i, i, i, stream, gws.p);
+            launch_gemm<double>(dd_sd.as<GemmDesc>() + nodes, 2 * nodes, i, i, i, stream, gws.p);
             launch_copy_blocks(IM, 4 * ii, 2 * i, Sm.as<double2>(), ii, i, i, i, nodes, make_double2(1, 0), stream);
             launch_copy_blocks(IM + (size_t)i * 2 * i, 4 * ii, 2 * i, SD.as<double2>(), ii, i, i, i, nodes,
                                make_double2(-nu.x, -nu.y), stream);
@@ -301,7 +310,7 @@ int Tree::factor_device(const double2* k) {
                     v.push_back(g);
                 }
             if (upload(dd_y, v)) return 1;
-            launch_gemm<double>(dd_y.as<GemmDesc>(), 2 * nodes, 2 * i, bn, i, stream);
+            launch_gemm<double>(dd_y.as<GemmDesc>(), 2 * nodes, 2 * i, bn, i, stream, gws.p);
         }
         // parent RtR
         if (l > 0 || !opts.skip_root_rtr) {
@@ -326,7 +335,7 @@ int Tree::factor_device(const double2* k) {
                     }
             }
             if (upload(dd_t, v)) return 1;
-            launch_gemm<double>(dd_t.as<GemmDesc>(), 4 * nodes, bn, bn, i, stream);
+            launch_gemm<double>(dd_t.as<GemmDesc>(), 4 * nodes, bn, bn, i, stream, gws.p);
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream));
@@ -385,6 +394,14 @@ template <typename R> int Tree::ensure_work(SolveWork<R>& w, int nrhs) {
         if (alloc(*w.G[l], n) || alloc(*w.Z[l], n)) return 1;
     }
     w.d_up_g.assign(L, nullptr); w.d_up_h.assign(L, nullptr); w.d_dn_t.assign(L, nullptr); w.d_dn_g.assign(L, nullptr);
+    size_t wse = 0;
+    for (int l = 0; l < L; ++l) {
+        const Level& Vl = P.lv[l];
+        wse = std::max(wse, gemm_ws_elems(Vl.nodes, 2 * Vl.isz, cap, 2 * Vl.isz));
+        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.bn, cap, Vl.isz));
+        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.isz, cap, Vl.bn));
+    }
+    if (wse && alloc(w.ws, wse * sizeof(V))) return 1;
     for (int l = 0; l < L; ++l) {
         const Level& Vl = P.lv[l];
         const Level& C = P.lv[l + 1];
@@ -504,7 +521,7 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     pend(sinv_b + nr * nl * (ef * P.n_int + e * P.nb + (io.jump ? 16.0 * P.nb : 0.0)), leaf_fl, st);
     auto gemm = [&](int cat, const GemmDesc* d, int np, int M, int N, int K, bool din) {
         pbeg(cat, st);
-        launch_gemm<R>(d, np, M, N, K, st);
+        launch_gemm<R>(d, np, M, N, K, st, w.ws.p);
         pend((double)np * e * ((double)M * K + (double)K * N + (double)M * N * (din ? 2 : 1)),
              8.0 * np * (double)M * N * K, st);
     };

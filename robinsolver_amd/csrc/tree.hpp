@@ -34,6 +34,7 @@ template <typename R> struct SolveWork {
     using V = c_t<R>;
     int cap = 0;
     std::vector<DevBuf*> hV, gF, G, Z;   // per level
+    DevBuf ws;                           // K-split GEMM workspace
     DevBuf rf, rg, rj, uA, du;           // refinement: residuals (c64 / c128 / c128), first iterate, correction (c64)
     std::vector<DevBuf*> descs;          // per (level, kind)
     std::vector<GemmDesc*> d_up_g, d_up_h, d_dn_t, d_dn_g;

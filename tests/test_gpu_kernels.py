@@ -57,7 +57,7 @@ def _run_gemm(prec, A, B, C, alpha, beta, B2=None, ksplit=None, rmap=None, ldc=N
 
 @pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED])
 @pytest.mark.parametrize("M,N,K", [(16, 16, 16), (64, 64, 16), (37, 5, 23), (130, 200, 70), (48, 1, 96),
-                                   (7, 65, 3)])
+                                   (7, 65, 3), (64, 20, 2048), (200, 1, 1000), (100, 70, 777)])
 def test_gemm_matches_numpy(prec, M, N, K, gpu):
     print("case: gemm", prec, M, N, K, flush=True)
     rng = np.random.default_rng(M * 1000 + N * 10 + K)
@@ -67,6 +67,19 @@ def test_gemm_matches_numpy(prec, M, N, K, gpu):
     ref = alpha * (A @ B) + beta * C
     tol = 1e-13 if prec == rsa.RS_FP64 else 2e-6
     assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < tol
+
+
+@pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED])
+def test_gemm_column_results_do_not_depend_on_width(prec, gpu):
+    """Each output column gets the same bits whatever the number of columns
+    (the K split and the tile width are chosen without looking at N)."""
+    rng = np.random.default_rng(3)
+    M, K = 96, 1536
+    A, B = _rc(rng, M, K), _rc(rng, K, 80)
+    wide = _run_gemm(prec, A, B, np.zeros((M, 80), complex), 1.0 + 0j, 0j)
+    for c in (0, 17, 79):
+        one = _run_gemm(prec, A, B[:, c:c + 1], np.zeros((M, 1), complex), 1.0 + 0j, 0j)
+        assert np.array_equal(one[:, 0], wide[:, c])
 
 
 @pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED])
