@@ -45,7 +45,7 @@ template <typename R> __device__ inline int out_row(int lane, int v);
 template <> __device__ inline int out_row<float>(int lane, int v) { return 4 * (lane >> 4) + v; }
 template <> __device__ inline int out_row<double>(int lane, int v) { return (lane >> 4) + 4 * v; }
 
-constexpr int BM = 64;
+constexpr int BMX = 64;   // largest tile height
 constexpr int BK = 32;
 
 // One workgroup computes a 64 x BN tile over the K range of its slice.  The
@@ -53,13 +53,14 @@ constexpr int BK = 32;
 // current ones, staged in LDS, feed the matrix cores.  With kslices > 1 the raw
 // partial products go to a workspace and gemm_reduce_kernel sums the slices in
 // a fixed order.
-template <typename R, int BN>
+template <typename R, int BM, int BN>
 __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ descs, int M, int N, int K,
                                                   int tilesM, int tilesN, int kslices, int kchunk,
                                                   c_t<R>* __restrict__ ws) {
     using V = c_t<R>;
     using A4 = typename acc_t<R>::T;
-    constexpr int WAVES_N = (BN >= 32) ? 2 : 1;
+    // four waves: 2x2 for 64x64 and 32x64 tiles, 4x1 for 64x16, 1x4 for 16x64
+    constexpr int WAVES_N = (BN == 16) ? 1 : (BM == 16 ? 4 : 2);
     constexpr int WAVES_M = 4 / WAVES_N;
     constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
     constexpr int TM = WM / 16, TN = WN / 16;
@@ -209,7 +210,7 @@ __global__ void gemm_reduce_kernel(const GemmDesc* __restrict__ descs, int M, in
 int gemm_kslices(int nprob, int M, int K) {
     // split K only when the row tiles of all problems cannot fill the chip;
     // the choice never depends on N
-    const long long rowtiles = (long long)nprob * ((M + BM - 1) / BM);
+    const long long rowtiles = (long long)nprob * ((M + BMX - 1) / BMX);
     if (rowtiles >= 128 || K < 256) return 1;
     long long ks = (256 + rowtiles - 1) / rowtiles;
     if (ks > K / 128) ks = K / 128;
@@ -225,20 +226,29 @@ size_t gemm_ws_elems(int nprob, int M, int N, int K) {
 template <typename R>
 void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws) {
     if (nprob <= 0 || M <= 0 || N <= 0) return;
-    const int tilesM = (M + BM - 1) / BM;
     int ks = gemm_kslices(nprob, M, K);
     if (!ws) ks = 1;
     const int kchunk = ((K + ks - 1) / ks + BK - 1) / BK * BK;
     ks = (K + kchunk - 1) / kchunk;
     c_t<R>* W = (c_t<R>*)ws;
+    // tile shape from M and N only; every 16x16 output block runs the same
+    // MFMA sequence over K whatever the tile, so results do not depend on it
     if (N <= 16) {
-        const int tilesN = (N + 15) / 16;
-        hipLaunchKernelGGL((gemm_kernel<R, 16>), dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs, M,
-                           N, K, tilesM, tilesN, ks, kchunk, W);
+        const int tilesM = (M + 63) / 64, tilesN = (N + 15) / 16;
+        hipLaunchKernelGGL((gemm_kernel<R, 64, 16>), dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs,
+                           M, N, K, tilesM, tilesN, ks, kchunk, W);
+    } else if (M <= 16) {
+        const int tilesM = (M + 15) / 16, tilesN = (N + 63) / 64;
+        hipLaunchKernelGGL((gemm_kernel<R, 16, 64>), dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs,
+                           M, N, K, tilesM, tilesN, ks, kchunk, W);
+    } else if (M <= 32) {
+        const int tilesM = (M + 31) / 32, tilesN = (N + 63) / 64;
+        hipLaunchKernelGGL((gemm_kernel<R, 32, 64>), dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs,
+                           M, N, K, tilesM, tilesN, ks, kchunk, W);
     } else {
-        const int tilesN = (N + 63) / 64;
-        hipLaunchKernelGGL((gemm_kernel<R, 64>), dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs, M,
-                           N, K, tilesM, tilesN, ks, kchunk, W);
+        const int tilesM = (M + 63) / 64, tilesN = (N + 63) / 64;
+        hipLaunchKernelGGL((gemm_kernel<R, 64, 64>), dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs,
+                           M, N, K, tilesM, tilesN, ks, kchunk, W);
     }
     if (ks > 1) {
         const size_t mn = (size_t)M * N;

@@ -390,11 +390,16 @@ template <typename U>
 __global__ void residual_kernel(LeafGeom G, LeafCoef C, const U* __restrict__ u, const double2* __restrict__ kint,
                                 const double2* __restrict__ f, const double2* __restrict__ g,
                                 const int* __restrict__ nbr, double2 nu, double2 zeta, float2* __restrict__ rf,
-                                double2* __restrict__ rg, double2* __restrict__ rj, int n_leaves) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const int leaf = blockIdx.y, col = blockIdx.z;
-    if (p >= G.m) return;
+                                double2* __restrict__ rg, double2* __restrict__ rj, int n_leaves, int ncols,
+                                int cols_per_block) {
+    // one block: one leaf, cols_per_block columns; threads run over (column, point)
+    const int leaf = blockIdx.x % n_leaves;
+    const int cb = blockIdx.x / n_leaves;
     const int nb = G.nb, ni = G.n_int;
+    for (int e = threadIdx.x; e < cols_per_block * G.m; e += blockDim.x) {
+    const int col = cb * cols_per_block + e / G.m;
+    const int p = e % G.m;
+    if (col >= ncols) break;
     const size_t lc = (size_t)col * n_leaves + leaf;
     const U* ul = u + lc * G.m;
     const double2 sma = make_double2(0.0, -2.0 * G.kb);
@@ -423,7 +428,7 @@ __global__ void residual_kernel(LeafGeom G, LeafCoef C, const U* __restrict__ u,
             rj[lc * nb + p] = csub(cadd(cmul(zeta, gA), cmul(nu, gN)), hA);
             rg[lc * nb + p] = make_double2(0.0, 0.0);
         }
-        return;
+        continue;
     }
     const int q = p - nb;
     int gq[3];
@@ -444,6 +449,7 @@ __global__ void residual_kernel(LeafGeom G, LeafCoef C, const U* __restrict__ u,
         Au = cadd(Au, cscale(upp, wp * ih2));
     }
     rf[lc * ni + q] = ccast<float2>(csub(f[lc * ni + q], Au));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -541,17 +547,19 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
             raw[vv] = (row < s && colok) ? load_f<double2>(io, lc * ni + k * s + row) : make_double2(0.0, 0.0);
         }
     };
-    double2 fcur[4];
-    fetch(0, fcur);
+    // the interior data runs PF layers ahead of the sweep (memory latency is
+    // several layer steps long)
+    constexpr int PF = 4;
+    double2 fbuf[PF][4];
+#pragma unroll
+    for (int k = 0; k < PF && k < K; ++k) fetch(k, fbuf[k]);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        double2 fnext[4];
-        if (k + 1 < K) fetch(k + 1, fnext);
         float2 v[4];
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) {
             const int row = 4 * grp + vv;
-            float2 r = ccast<float2>(fcur[vv]);
+            float2 r = ccast<float2>(fbuf[k % PF][vv]);
             if (row < s && colok && io.mode == LEAF_DOWN) {
                 if (row == 0) r = cfma(cbx, gval(1, k), r);
                 if (row == s - 1) r = cfma(cbx, gval(2, k), r);
@@ -561,13 +569,11 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
             if (k > 0) r = csub(r, cmul(ccast<float2>(slow_lambda(G, C, k)), prev[vv]));
             v[vv] = r;
         }
+        if (k + PF < K) fetch(k + PF, fbuf[k % PF]);
         f4v aR, aI;
         product(k, v, aR, aI);
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) { y[k][vv] = make_float2(aR[vv], aI[vv]); prev[vv] = y[k][vv]; }
-        if (k + 1 < K)
-#pragma unroll
-            for (int vv = 0; vv < 4; ++vv) fcur[vv] = fnext[vv];
     }
     // backward sweep
 #pragma unroll
@@ -629,13 +635,14 @@ void launch_residual(const LeafGeom& G, const double2* u64, const float2* u32, c
                      const double2* g, const int* nbr, double2 nu, double2 zeta, float2* rf, double2* rg,
                      double2* rj, int n_leaves, int ncols, hipStream_t st) {
     LeafCoef C = make_coef(G);
-    dim3 grid((G.m + 127) / 128, n_leaves, ncols);
+    const int cpb = 8;
+    dim3 grid((unsigned)n_leaves * ((ncols + cpb - 1) / cpb));
     if (u64)
-        hipLaunchKernelGGL(residual_kernel<double2>, grid, dim3(128), 0, st, G, C, u64, kint, f, g, nbr, nu, zeta,
-                           rf, rg, rj, n_leaves);
+        hipLaunchKernelGGL(residual_kernel<double2>, grid, dim3(256), 0, st, G, C, u64, kint, f, g, nbr, nu, zeta,
+                           rf, rg, rj, n_leaves, ncols, cpb);
     else
-        hipLaunchKernelGGL(residual_kernel<float2>, grid, dim3(128), 0, st, G, C, u32, kint, f, g, nbr, nu, zeta,
-                           rf, rg, rj, n_leaves);
+        hipLaunchKernelGGL(residual_kernel<float2>, grid, dim3(256), 0, st, G, C, u32, kint, f, g, nbr, nu, zeta,
+                           rf, rg, rj, n_leaves, ncols, cpb);
 }
 
 size_t leaf_blocks_lds(const LeafGeom& G) {

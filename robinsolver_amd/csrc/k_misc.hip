@@ -2,6 +2,8 @@
 // interface blocks of the merge (robin_tree_module.f90:440-467), the face
 // scatter/gather between a node and its children (robin_tree_module.f90:896-959,
 // 1045-1075, 1205-1230), and precision conversion of the stored factors.
+#include <algorithm>
+
 #include "rs_common.hpp"
 #include "rs_kernels.hpp"
 
@@ -75,9 +77,8 @@ __global__ void axpy_kernel(double2* __restrict__ y, const double2* __restrict__
 
 __global__ void combine_kernel(double2* u, const double2* base64, const float2* __restrict__ base32,
                                const float2* __restrict__ d, size_t n) {
-    size_t e = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
-    for (int q = 0; q < 2; ++q, ++e) {
-        if (e >= n) return;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
         const double2 b = base64 ? base64[e] : ccast<double2>(base32[e]);
         u[e] = cadd(b, ccast<double2>(d[e]));
     }
@@ -127,7 +128,8 @@ void launch_axpy(double2* y, const double2* x, size_t n, hipStream_t st) {
 
 void launch_combine(double2* u, const double2* base64, const float2* base32, const float2* d, size_t n,
                     hipStream_t st) {
-    if (n) hipLaunchKernelGGL(combine_kernel, dim3(nblk((n + 1) / 2)), dim3(256), 0, st, u, base64, base32, d, n);
+    const unsigned blocks = (unsigned)std::min<size_t>(nblk(n), 256 * 64);
+    if (n) hipLaunchKernelGGL(combine_kernel, dim3(blocks), dim3(256), 0, st, u, base64, base32, d, n);
 }
 
 template void launch_root_gather<double2>(double2*, const double2*, const int*, const int*, int, int, int, int,

@@ -57,7 +57,7 @@ def _run_gemm(prec, A, B, C, alpha, beta, B2=None, ksplit=None, rmap=None, ldc=N
 
 @pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED])
 @pytest.mark.parametrize("M,N,K", [(16, 16, 16), (64, 64, 16), (37, 5, 23), (130, 200, 70), (48, 1, 96),
-                                   (7, 65, 3), (64, 20, 2048), (200, 1, 1000), (100, 70, 777)])
+                                   (7, 65, 3), (64, 20, 2048), (200, 1, 1000), (100, 70, 777), (16, 40, 64), (30, 100, 50), (12, 300, 2000)])
 def test_gemm_matches_numpy(prec, M, N, K, gpu):
     print("case: gemm", prec, M, N, K, flush=True)
     rng = np.random.default_rng(M * 1000 + N * 10 + K)
