@@ -271,6 +271,13 @@ def main():
         "roofline": roof,
         "kernel_ms_per_step": {c: v[0] / args.steps for c, v in prof.items()},
     }
+    # a plain copy of known size, the calibration point for hardware byte counters
+    cal = torch.empty(1 << 25, dtype=torch.float64, device=dev)   # 256 MiB
+    cal2 = torch.empty_like(cal)
+    cal2.copy_(cal)
+    torch.cuda.synchronize()
+    out["counter_calibration_copy_bytes"] = cal.numel() * 8
+
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args, kb_phys, kb_opt)
