@@ -107,6 +107,15 @@ const char *rs_last_error(const rs_tree *t);
  * number of levels, device bytes held. */
 int rs_inspect(const rs_tree *t, int *factored, int *nlvls, long long *device_bytes);
 
+/* Device memory (no reference counterpart; the reference's memsize is empty,
+ * robin_tree_module.f90:243-250).  rs_memory_estimate: host-side, from the plan:
+ * bytes the factorization keeps resident, its peak while it runs, and one solve
+ * workspace of ncols right-hand sides at the tree's precision.
+ * rs_memory_used: bytes the tree holds now and the most it has held. */
+int rs_memory_estimate(const rs_tree *t, int ncols, long long *factor_resident, long long *factor_peak,
+                       long long *solve_workspace);
+int rs_memory_used(const rs_tree *t, long long *held, long long *peak);
+
 /* Per-level shape of the plan (no reference counterpart; host-side, needs no
  * GPU): nodes of level lvl, boundary points P of each, and for lvl < L the
  * interface size and the children's boundary size of its merge (0 at the
@@ -142,6 +151,83 @@ int rs_profile(rs_tree *t, int enable);
 int rs_profile_ncat(void);
 const char *rs_profile_name(int cat);
 int rs_profile_read(rs_tree *t, int cat, double *ms, int *launches, double *bytes, double *flops);
+
+/* ---------------------------------------------------------------------------
+ * Serving loop (no reference counterpart: the reference solves one right-hand
+ * side per call and process, robin_tree_module.f90:198-233, 1014-1029).  A
+ * server keeps factored trees resident, queues solve and factor requests per
+ * tree in arrival order, and runs them on `lanes` streams: whenever a lane is
+ * free it takes the tree whose oldest queued solve is oldest and coalesces that
+ * tree's queued solves (up to max_cols right-hand sides) into one multi-RHS pass.
+ * Requests that arrive while passes run join the next pass on a free lane.
+ * Passes and factorizations are admitted by free device memory (hipMemGetInfo
+ * less hbm_reserve, or hbm_limit less what the server holds); idle solve
+ * workspaces are evicted to make room.  A factor request waits for the tree's
+ * earlier solves and holds back its later ones.  Results do not depend on how
+ * requests are batched: every kernel computes each right-hand side the same
+ * way whatever the batch, and the refinement count is fixed per tree.
+ * ------------------------------------------------------------------------- */
+typedef struct rs_server rs_server;
+
+typedef struct rs_server_opts {
+    int max_cols;          /* right-hand sides per pass (default 256)                     */
+    int lanes;             /* passes in flight at once, one stream each (default 2)       */
+    long long hbm_reserve; /* device bytes admission leaves free (default 2 GiB)          */
+    long long hbm_limit;   /* > 0: admit against this many bytes held by the server's trees
+                              and workspaces instead of the device's free memory          */
+    int simulate;          /* 1: no device work; a pass takes sim_pass_s + sim_col_s per
+                              right-hand side and a factorization sim_factor_s (tests of
+                              the scheduling policy on a host without a GPU)              */
+    double sim_pass_s, sim_col_s, sim_factor_s;
+} rs_server_opts;
+
+typedef struct rs_server_stats {
+    long long requests_done;   /* solve requests completed (done or failed)        */
+    long long rhs_done;        /* right-hand sides solved                          */
+    long long passes;          /* multi-RHS passes run                             */
+    long long factors_done;    /* factorizations completed                         */
+    long long admission_waits; /* times a request was held back for device memory */
+    long long evictions;       /* solve workspaces freed to admit other work       */
+    long long failed;          /* requests that failed                             */
+    double busy_s;             /* wall seconds with at least one pass in flight    */
+    double lat_p50_s, lat_p99_s, lat_max_s;  /* solve latency, submission to completion */
+    double mean_cols;          /* mean right-hand sides per pass                   */
+    long long held_bytes;      /* device bytes the server accounts to its trees    */
+} rs_server_stats;
+
+enum { RS_REQ_QUEUED = 0, RS_REQ_RUNNING = 1, RS_REQ_DONE = 2, RS_REQ_FAILED = 3 };
+
+void rs_server_opts_default(rs_server_opts *o);
+int rs_server_create(rs_server **s, const rs_server_opts *o);
+/* Completes every queued request, then stops the loop and frees the server's
+ * streams and tables (the trees stay with their owner; destroy them after). */
+int rs_server_destroy(rs_server *s);
+/* Register a planned (and possibly factored) tree; its id is returned.  While
+ * registered, the tree must be used only through the server. */
+int rs_server_add_tree(rs_server *s, rs_tree *t, int *tree_id);
+/* Queue a (re)factorization of tree tree_id with the potential k_dev: device
+ * memory [n_leaves][n_int] complex128, valid until the request completes. */
+int rs_server_factor(rs_server *s, int tree_id, const void *k_dev, long long *req);
+/* Queue a solve of nrhs right-hand sides: device arrays f [nrhs][n_leaves][n_int],
+ * g [nrhs][n_leaves][n_bnd] in, u [nrhs][n_leaves][m] out (complex128), valid
+ * until the request completes.  nrhs <= max_cols. */
+int rs_server_solve(rs_server *s, int tree_id, int nrhs, const void *f_dev, const void *g_dev, void *u_dev,
+                    long long *req);
+/* State of a request (RS_REQ_*), without blocking. */
+int rs_server_test(rs_server *s, long long req, int *state);
+/* Block until the request completes: 0 done, 1 failed (message by rs_server_error). */
+int rs_server_wait(rs_server *s, long long req);
+/* Block until every queued request has completed. */
+int rs_server_drain(rs_server *s);
+/* Times of a request (seconds on one monotonic clock) and the pass it ran in. */
+int rs_server_request_info(rs_server *s, long long req, double *t_submit, double *t_start, double *t_done,
+                           long long *pass);
+/* Forget a completed request's record. */
+int rs_server_release(rs_server *s, long long req);
+int rs_server_get_stats(rs_server *s, rs_server_stats *st);
+int rs_server_reset_stats(rs_server *s);
+/* Message of a failed request, or of the last failed call when req < 0. */
+const char *rs_server_error(rs_server *s, long long req);
 
 #ifdef __cplusplus
 }

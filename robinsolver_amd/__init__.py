@@ -68,6 +68,11 @@ def lib():
         L.rs_profile_name.restype = ctypes.c_char_p
         L.rs_profile_name.argtypes = [ctypes.c_int]
         L.rs_profile_read.argtypes = [vp, ctypes.c_int, dp, ip, dp, dp]
+        llp = ctypes.POINTER(ctypes.c_longlong)
+        L.rs_memory_estimate.argtypes = [vp, ctypes.c_int, llp, llp, llp]
+        L.rs_memory_used.argtypes = [vp, llp, llp]
+        from . import serve as _serve
+        _serve.bind(L)
         _lib = L
     return _lib
 
@@ -168,6 +173,18 @@ class GpuTree:
         f, s = ctypes.c_double(), ctypes.c_double()
         lib().rs_timings(self.h, ctypes.byref(f), ctypes.byref(s))
         return f.value, s.value
+
+    def memory_estimate(self, ncols=1):
+        """(factor resident bytes, factor peak bytes, solve workspace bytes of ncols RHS)."""
+        r, p, w = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        self._check(lib().rs_memory_estimate(self.h, int(ncols), ctypes.byref(r), ctypes.byref(p), ctypes.byref(w)))
+        return r.value, p.value, w.value
+
+    def memory_used(self):
+        """(device bytes held now, most held)."""
+        h, p = ctypes.c_longlong(), ctypes.c_longlong()
+        lib().rs_memory_used(self.h, ctypes.byref(h), ctypes.byref(p))
+        return h.value, p.value
 
     def inspect(self):
         fa, nl, db = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()

@@ -6,10 +6,6 @@
 
 #include "tree.hpp"
 
-struct rs_tree {
-    rs::Tree t;
-};
-
 namespace {
 std::string g_err;
 thread_local std::string g_tls;
@@ -128,6 +124,25 @@ int rs_inspect(const rs_tree* t, int* factored, int* nlvls, long long* device_by
     if (factored) *factored = t->t.factored ? 1 : 0;
     if (nlvls) *nlvls = t->t.plan.L;
     if (device_bytes) *device_bytes = t->t.dev_bytes;
+    return 0;
+}
+
+int rs_memory_estimate(const rs_tree* t, int ncols, long long* factor_resident, long long* factor_peak,
+                       long long* solve_workspace) {
+    if (!t) return 1;
+    if (!t->t.planned) return const_cast<rs_tree*>(t)->t.fail("Error: rs_plan must be called first.");
+    long long r = 0, p = 0;
+    t->t.factor_bytes(&r, &p);
+    if (factor_resident) *factor_resident = r;
+    if (factor_peak) *factor_peak = p;
+    if (solve_workspace) *solve_workspace = t->t.work_bytes(ncols < 1 ? 1 : ncols, t->t.opts.precision);
+    return 0;
+}
+
+int rs_memory_used(const rs_tree* t, long long* held, long long* peak) {
+    if (!t) return 1;
+    if (held) *held = t->t.dev_bytes;
+    if (peak) *peak = t->t.peak_bytes;
     return 0;
 }
 

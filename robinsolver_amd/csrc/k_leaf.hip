@@ -243,27 +243,28 @@ template <typename R> struct LeafIO {
     int CH;                   // columns per workgroup (generic kernel)
     int n_leaves;
     const c_t<R>* sinv;       // [leaf][K][s*s]
-    const double2* f64;       // interior data f: [col][leaf][n_int], complex128 ...
-    const float2* f32;        //   ... or complex64 (at most one set; none = 0)
+    const double2* const* f64;  // interior data f, complex128: column c at f64[c], [leaf][n_int] ...
+    const float2* f32;        //   ... or complex64 [col][leaf][n_int] (at most one set; none = 0)
     const double2* jump;      // UP: subtracted from the outgoing data h: [col][leaf][nb] local order
     const c_t<R>* gin;        // DOWN: incoming data g: [col][leaf][nb] node order
     c_t<R>* hout;             // UP: outgoing data h: [col][leaf][nb] node order
-    double2* u64;             // DOWN: solution [col][leaf][m], complex128 ...
-    float2* u32;              //   ... or complex64 (one set); written, never read
+    double2* const* u64;      // DOWN: solution, complex128: column c at u64[c], [leaf][m] ...
+    float2* u32;              //   ... or complex64 [col][leaf][m] (one set); written, never read
     double2* T;               // FACTOR: [leaf][nb][nb] node order, column-major
 };
 
+// f at interior point q of (col, leaf); u at point p of (col, leaf)
 template <typename V, typename R>
-__device__ inline V load_f(const LeafIO<R>& io, size_t idx) {
-    if (io.f64) return ccast<V>(io.f64[idx]);
-    if (io.f32) return ccast<V>(io.f32[idx]);
+__device__ inline V load_f(const LeafIO<R>& io, int col, int leaf, int ni, int q) {
+    if (io.f64) return ccast<V>(io.f64[col][(size_t)leaf * ni + q]);
+    if (io.f32) return ccast<V>(io.f32[((size_t)col * io.n_leaves + leaf) * ni + q]);
     return V{};
 }
 
 template <typename V, typename R>
-__device__ inline void store_u(const LeafIO<R>& io, size_t idx, V x) {
-    if (io.u64) io.u64[idx] = ccast<double2>(x);
-    else io.u32[idx] = ccast<float2>(x);
+__device__ inline void store_u(const LeafIO<R>& io, int col, int leaf, int m, int p, V x) {
+    if (io.u64) io.u64[col][(size_t)leaf * m + p] = ccast<double2>(x);
+    else io.u32[((size_t)col * io.n_leaves + leaf) * m + p] = ccast<float2>(x);
 }
 
 
@@ -296,7 +297,7 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
         int q = e / CH, c = e % CH;
         V v = cz<R>();
         if (c < ncol) {
-            if (io.mode != LEAF_FACTOR_T) v = load_f<V>(io, ((size_t)(c0 + c) * io.n_leaves + leaf) * ni + q);
+            if (io.mode != LEAF_FACTOR_T) v = load_f<V>(io, c0 + c, leaf, ni, q);
             if (io.mode != LEAF_UP) {
                 int g[3];
                 interior_coords(G, q, g);
@@ -366,14 +367,14 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
             if (io.jump) hb = csub(hb, ccast<V>(io.jump[((size_t)col * io.n_leaves + leaf) * nb + b]));
             io.hout[((size_t)col * io.n_leaves + leaf) * nb + local_to_node(G, parity, b)] = hb;
         } else {
-            store_u(io, ((size_t)col * io.n_leaves + leaf) * G.m + b, uB);
+            store_u(io, col, leaf, G.m, b, uB);
         }
     }
     if (io.mode == LEAF_DOWN) {
         for (int e = tid; e < ni * CH; e += nt) {
             int q = e / CH, c = e % CH;
             if (c >= ncol) continue;
-            store_u(io, ((size_t)(c0 + c) * io.n_leaves + leaf) * G.m + nb + q, Y[e]);
+            store_u(io, c0 + c, leaf, G.m, nb + q, Y[e]);
         }
     }
 }
@@ -386,9 +387,23 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
 // M [g1; g2] = [h1; h2] (robin_tree_module.f90:425-467): it holds at every pair
 // of coincident leaf boundary points once the tree has solved.  A correction
 // solve takes (rf, rg) as (f, g) and subtracts rj from the leaf outgoing data.
+// column col of the current solution: complex128 through the column table, or
+// the complex64 first iterate [col][leaf][m]
 template <typename U>
-__global__ void residual_kernel(LeafGeom G, LeafCoef C, const U* __restrict__ u, const double2* __restrict__ kint,
-                                const double2* __restrict__ f, const double2* __restrict__ g,
+__device__ inline const U* ucolumn(double2* const* u64, const float2* u32, int col, int nl, int m);
+template <>
+__device__ inline const double2* ucolumn<double2>(double2* const* u64, const float2*, int col, int, int) {
+    return u64[col];
+}
+template <>
+__device__ inline const float2* ucolumn<float2>(double2* const*, const float2* u32, int col, int nl, int m) {
+    return u32 + (size_t)col * nl * m;
+}
+
+template <typename U>
+__global__ void residual_kernel(LeafGeom G, LeafCoef C, double2* const* __restrict__ u64,
+                                const float2* __restrict__ u32, const double2* __restrict__ kint,
+                                const double2* const* __restrict__ f, const double2* const* __restrict__ g,
                                 const int* __restrict__ nbr, double2 nu, double2 zeta, float2* __restrict__ rf,
                                 double2* __restrict__ rg, double2* __restrict__ rj, int n_leaves, int ncols,
                                 int cols_per_block) {
@@ -401,7 +416,8 @@ __global__ void residual_kernel(LeafGeom G, LeafCoef C, const U* __restrict__ u,
     const int p = e % G.m;
     if (col >= ncols) break;
     const size_t lc = (size_t)col * n_leaves + leaf;
-    const U* ul = u + lc * G.m;
+    const U* ucol = ucolumn<U>(u64, u32, col, n_leaves, G.m);
+    const U* ul = ucol + (size_t)leaf * G.m;
     const double2 sma = make_double2(0.0, -2.0 * G.kb);
     auto uval = [&](const U* uu, int i) -> double2 { return ccast<double2>(uu[i]); };
     auto incoming = [&](const U* uu, int b) -> double2 {
@@ -418,12 +434,12 @@ __global__ void residual_kernel(LeafGeom G, LeafCoef C, const U* __restrict__ u,
         const double2 gA = incoming(ul, p);
         const int nl2 = nbr[(size_t)leaf * 7 + fc];
         if (nl2 < 0) {
-            rg[lc * nb + p] = csub(g[lc * nb + p], gA);
+            rg[lc * nb + p] = csub(g[col][(size_t)leaf * nb + p], gA);
             rj[lc * nb + p] = make_double2(0.0, 0.0);
         } else {
             const int f2 = (fc & 1) ? fc + 1 : fc - 1;
             const int b2 = G.foff[f2] + (p - G.foff[fc]);
-            const double2 gN = incoming(u + ((size_t)col * n_leaves + nl2) * G.m, b2);
+            const double2 gN = incoming(ucol + (size_t)nl2 * G.m, b2);
             const double2 hA = cfma(sma, uval(ul, p), gA);
             rj[lc * nb + p] = csub(cadd(cmul(zeta, gA), cmul(nu, gN)), hA);
             rg[lc * nb + p] = make_double2(0.0, 0.0);
@@ -448,7 +464,7 @@ __global__ void residual_kernel(LeafGeom G, LeafCoef C, const U* __restrict__ u,
         Au = cadd(Au, cscale(um, wm * ih2));
         Au = cadd(Au, cscale(upp, wp * ih2));
     }
-    rf[lc * ni + q] = ccast<float2>(csub(f[lc * ni + q], Au));
+    rf[lc * ni + q] = ccast<float2>(csub(f[col][(size_t)leaf * ni + q], Au));
     }
 }
 
@@ -487,6 +503,9 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     const bool colok = mycol < io.ncols;
     const int parity = leaf & 1;
     const size_t lc = (size_t)(colok ? mycol : 0) * nl + leaf;
+    // this lane's column of f: complex128 through the column table, or complex64 correction data
+    const double2* f64p = (io.f64 && colok) ? io.f64[mycol] + (size_t)leaf * ni : nullptr;
+    const float2* f32p = io.f32 ? io.f32 + lc * ni : nullptr;
 
     const float2* sl = io.sinv + (size_t)leaf * K * s * s;
     for (int e = tid; e < K * 256; e += 256) {
@@ -544,7 +563,12 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) {
             const int row = 4 * grp + vv;
-            raw[vv] = (row < s && colok) ? load_f<double2>(io, lc * ni + k * s + row) : make_double2(0.0, 0.0);
+            double2 x = make_double2(0.0, 0.0);
+            if (row < s && colok) {
+                if (f64p) x = f64p[k * s + row];
+                else if (f32p) x = ccast<double2>(f32p[k * s + row]);
+            }
+            raw[vv] = x;
         }
     };
     // the interior data runs PF layers ahead of the sweep (memory latency is
@@ -589,14 +613,19 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     }
     if (!colok) return;
 
-    const size_t ub = lc * m;
+    double2* u64p = io.u64 ? io.u64[mycol] + (size_t)leaf * m : nullptr;
+    float2* u32p = io.u32 ? io.u32 + lc * m : nullptr;
+    auto put = [&](int p, float2 x) {
+        if (u64p) u64p[p] = ccast<double2>(x);
+        else u32p[p] = x;
+    };
     if (io.mode == LEAF_DOWN) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) {
                 const int row = 4 * grp + vv;
-                if (row < s) store_u(io, ub + nb + k * s + row, y[k][vv]);
+                if (row < s) put(nb + k * s + row, y[k][vv]);
             }
     }
     // boundary values: u_B = (g - a1 x1 - a2 x2) / dd, h = g + (sigma - alpha) u_B
@@ -611,7 +640,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
             hb = csub(hb, bd[ccol * nb + b]);
             io.hout[lc * nb + G.noff[parity][face] + t] = hb;
         } else {
-            store_u(io, ub + b, uB);
+            put(b, uB);
         }
     };
 #pragma unroll
@@ -631,18 +660,18 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
 
 }  // namespace
 
-void launch_residual(const LeafGeom& G, const double2* u64, const float2* u32, const double2* kint, const double2* f,
-                     const double2* g, const int* nbr, double2 nu, double2 zeta, float2* rf, double2* rg,
-                     double2* rj, int n_leaves, int ncols, hipStream_t st) {
+void launch_residual(const LeafGeom& G, double2* const* u64, const float2* u32, const double2* kint,
+                     const double2* const* f, const double2* const* g, const int* nbr, double2 nu, double2 zeta,
+                     float2* rf, double2* rg, double2* rj, int n_leaves, int ncols, hipStream_t st) {
     LeafCoef C = make_coef(G);
     const int cpb = 8;
     dim3 grid((unsigned)n_leaves * ((ncols + cpb - 1) / cpb));
     if (u64)
-        hipLaunchKernelGGL(residual_kernel<double2>, grid, dim3(256), 0, st, G, C, u64, kint, f, g, nbr, nu, zeta,
-                           rf, rg, rj, n_leaves, ncols, cpb);
+        hipLaunchKernelGGL(residual_kernel<double2>, grid, dim3(256), 0, st, G, C, u64, nullptr, kint, f, g, nbr,
+                           nu, zeta, rf, rg, rj, n_leaves, ncols, cpb);
     else
-        hipLaunchKernelGGL(residual_kernel<float2>, grid, dim3(256), 0, st, G, C, u32, kint, f, g, nbr, nu, zeta,
-                           rf, rg, rj, n_leaves, ncols, cpb);
+        hipLaunchKernelGGL(residual_kernel<float2>, grid, dim3(256), 0, st, G, C, nullptr, u32, kint, f, g, nbr,
+                           nu, zeta, rf, rg, rj, n_leaves, ncols, cpb);
 }
 
 size_t leaf_blocks_lds(const LeafGeom& G) {

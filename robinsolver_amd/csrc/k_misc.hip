@@ -37,13 +37,14 @@ __global__ void copy_blocks_kernel(double2* __restrict__ dst, size_t dstride, in
     dst[bat * dstride + r + (size_t)c * ldd] = cmul(scale, src[bat * sstride + r + (size_t)c * lds]);
 }
 
+// root boundary data from the physical data g (column table, [leaf][nbl] per column)
 template <typename V>
-__global__ void root_gather_kernel(V* __restrict__ out, const double2* __restrict__ gin, const int* __restrict__ leaf_of,
-                                   const int* __restrict__ loc_of, int P, int n_leaves, int nbl) {
+__global__ void root_gather_kernel(V* __restrict__ out, const double2* const* __restrict__ gin,
+                                   const int* __restrict__ leaf_of, const int* __restrict__ loc_of, int P, int nbl) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int col = blockIdx.y;
     if (p >= P) return;
-    out[(size_t)col * P + p] = ccast<V>(gin[((size_t)col * n_leaves + leaf_of[p]) * nbl + loc_of[p]]);
+    out[(size_t)col * P + p] = ccast<V>(gin[col][(size_t)leaf_of[p] * nbl + loc_of[p]]);
 }
 
 // child N-part gather: child (2n+c) rows [0, bn) <- parent n rows prow[parity(n)][c][j]
@@ -75,13 +76,25 @@ __global__ void axpy_kernel(double2* __restrict__ y, const double2* __restrict__
     if (e < n) y[e] = cadd(y[e], x[e]);
 }
 
-__global__ void combine_kernel(double2* u, const double2* base64, const float2* __restrict__ base32,
-                               const float2* __restrict__ d, size_t n) {
+// u[col] = base[col] + d[col]; blockIdx.y is the column, blocks stride over its n entries
+// (u and base64 may be the same table: no restrict on them)
+__global__ void combine_kernel(double2* const* u, double2* const* base64,
+                               const float2* __restrict__ base32, const float2* __restrict__ d, size_t n) {
+    const int col = blockIdx.y;
+    double2* uc = u[col];
+    const double2* b64 = base64 ? base64[col] : nullptr;
+    const float2* b32 = base32 ? base32 + (size_t)col * n : nullptr;
+    const float2* dc = d + (size_t)col * n;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
-        const double2 b = base64 ? base64[e] : ccast<double2>(base32[e]);
-        u[e] = cadd(b, ccast<double2>(d[e]));
+        const double2 b = b64 ? b64[e] : ccast<double2>(b32[e]);
+        uc[e] = cadd(b, ccast<double2>(dc[e]));
     }
+}
+
+__global__ void col_table_kernel(double2** __restrict__ tab, double2* base, size_t stride, int ncols) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncols) tab[c] = base + (size_t)c * stride;
 }
 
 inline unsigned nblk(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
@@ -101,10 +114,10 @@ void launch_copy_blocks(double2* dst, size_t dstride, int ldd, const double2* sr
 }
 
 template <typename V>
-void launch_root_gather(V* out, const double2* gin, const int* leaf_of, const int* loc_of, int P, int n_leaves,
-                        int nbl, int ncols, hipStream_t st) {
+void launch_root_gather(V* out, const double2* const* gin, const int* leaf_of, const int* loc_of, int P, int nbl,
+                        int ncols, hipStream_t st) {
     hipLaunchKernelGGL(root_gather_kernel<V>, dim3(nblk(P), ncols), dim3(256), 0, st, out, gin, leaf_of, loc_of, P,
-                       n_leaves, nbl);
+                       nbl);
 }
 
 template <typename V>
@@ -126,15 +139,21 @@ void launch_axpy(double2* y, const double2* x, size_t n, hipStream_t st) {
     if (n) hipLaunchKernelGGL(axpy_kernel, dim3(nblk(n)), dim3(256), 0, st, y, x, n);
 }
 
-void launch_combine(double2* u, const double2* base64, const float2* base32, const float2* d, size_t n,
-                    hipStream_t st) {
-    const unsigned blocks = (unsigned)std::min<size_t>(nblk(n), 256 * 64);
-    if (n) hipLaunchKernelGGL(combine_kernel, dim3(blocks), dim3(256), 0, st, u, base64, base32, d, n);
+void launch_combine(double2* const* u, double2* const* base64, const float2* base32, const float2* d, size_t n,
+                    int ncols, hipStream_t st) {
+    // about 16k workgroups in all, each striding over one column
+    const unsigned per_col = (unsigned)std::max<size_t>(1, std::min<size_t>(nblk(n), (16384 + ncols - 1) / ncols));
+    if (n && ncols > 0)
+        hipLaunchKernelGGL(combine_kernel, dim3(per_col, ncols), dim3(256), 0, st, u, base64, base32, d, n);
 }
 
-template void launch_root_gather<double2>(double2*, const double2*, const int*, const int*, int, int, int, int,
+void launch_col_table(double2** tab, double2* base, size_t stride, int ncols, hipStream_t st) {
+    if (ncols > 0) hipLaunchKernelGGL(col_table_kernel, dim3(nblk(ncols)), dim3(256), 0, st, tab, base, stride, ncols);
+}
+
+template void launch_root_gather<double2>(double2*, const double2* const*, const int*, const int*, int, int, int,
                                           hipStream_t);
-template void launch_root_gather<float2>(float2*, const double2*, const int*, const int*, int, int, int, int,
+template void launch_root_gather<float2>(float2*, const double2* const*, const int*, const int*, int, int, int,
                                          hipStream_t);
 template void launch_child_gather<double2>(double2*, const double2*, const int*, int, int, int, int, int, int,
                                            hipStream_t);

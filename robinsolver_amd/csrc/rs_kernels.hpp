@@ -39,24 +39,28 @@ void launch_leaf_blocks(const LeafGeom& G, const double2* kint, double2* sinv, i
                         hipStream_t st);
 template <typename R> int leaf_apply_chunk(const LeafGeom& G, int ncols, size_t lds_limit);
 // Arguments of a leaf pass (see LeafIO in k_leaf.hip).
+// Caller-side complex128 arrays (f, g, u) are reached through column tables:
+// column c at tab[c], laid out [leaf][len]; the solve's own complex64 arrays are
+// contiguous [col][leaf][len].
 template <typename R> struct LeafArgs {
     int mode = LEAF_UP, ncols = 0, n_leaves = 0;
     const c_t<R>* sinv = nullptr;
-    const double2* f64 = nullptr;
+    const double2* const* f64 = nullptr;
     const float2* f32 = nullptr;
     const double2* jump = nullptr;
     const c_t<R>* gin = nullptr;
     c_t<R>* hout = nullptr;
-    double2* u64 = nullptr;
+    double2* const* u64 = nullptr;
     float2* u32 = nullptr;
     double2* T = nullptr;
 };
 template <typename R>
 void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit, hipStream_t st);
-// u is complex128 (u64) or complex64 (u32); rf is written in complex64.
-void launch_residual(const LeafGeom& G, const double2* u64, const float2* u32, const double2* kint, const double2* f,
-                     const double2* g, const int* nbr, double2 nu, double2 zeta, float2* rf, double2* rg,
-                     double2* rj, int n_leaves, int ncols, hipStream_t st);
+// u is complex128 (u64, column table) or complex64 (u32); f and g are column
+// tables; rf is written in complex64.
+void launch_residual(const LeafGeom& G, double2* const* u64, const float2* u32, const double2* kint,
+                     const double2* const* f, const double2* const* g, const int* nbr, double2 nu, double2 zeta,
+                     float2* rf, double2* rg, double2* rj, int n_leaves, int ncols, hipStream_t st);
 
 // batched inverse
 struct InvPlan {
@@ -77,7 +81,7 @@ void launch_form_iface(double2* out, const double2* T, size_t tstride, int child
 void launch_copy_blocks(double2* dst, size_t dstride, int ldd, const double2* src, size_t sstride, int lds, int rows,
                         int cols, int batch, double2 scale, hipStream_t st);
 template <typename V>
-void launch_root_gather(V* out, const double2* gin, const int* leaf_of, const int* loc_of, int P, int n_leaves,
+void launch_root_gather(V* out, const double2* const* gin, const int* leaf_of, const int* loc_of, int P,
                         int nbl, int ncols, hipStream_t st);
 template <typename V>
 void launch_child_gather(V* child, const V* parent, const int* prow, int bn, int bc, int P, int nodes, int root,
@@ -85,8 +89,12 @@ void launch_child_gather(V* child, const V* parent, const int* prow, int bn, int
 void launch_to_c64(float2* dst, const double2* src, size_t n, hipStream_t st);
 void launch_zero(double2* p, size_t n, hipStream_t st);
 void launch_axpy(double2* y, const double2* x, size_t n, hipStream_t st);
-// u = base + d, base complex128 (base64, may alias u) or complex64 (base32), d complex64
-void launch_combine(double2* u, const double2* base64, const float2* base32, const float2* d, size_t n,
-                    hipStream_t st);
+// u = base + d over ncols columns of n entries: u and base64 are column tables
+// (base64 may equal u), base32 and d are contiguous complex64 [col][n]; base is
+// base64 when given, else base32.
+void launch_combine(double2* const* u, double2* const* base64, const float2* base32, const float2* d, size_t n,
+                    int ncols, hipStream_t st);
+// tab[c] = base + c * stride for c < ncols (column table of a contiguous array)
+void launch_col_table(double2** tab, double2* base, size_t stride, int ncols, hipStream_t st);
 
 }  // namespace rs

@@ -55,11 +55,21 @@ template <typename R> void SolveWork<R>::clear() {
     for (DevBuf* b : descs) delete b;
     descs.clear();
     d_up_g.clear(); d_up_h.clear(); d_dn_t.clear(); d_dn_g.clear();
-    rf.release(); rg.release(); rj.release(); uA.release(); du.release(); ws.release();
+    rf.release(); rg.release(); rj.release(); uA.release(); du.release(); ws.release(); tab_rg.release();
     cap = 0;
 }
 
 Tree::Tree() { rs_opts_default(&opts); }
+
+Slot& Tree::slot(int i) {
+    while ((int)slots.size() <= i) slots.push_back(new Slot);
+    return *slots[i];
+}
+
+void Tree::clear_slots() {
+    for (Slot* s : slots) delete s;
+    slots.clear();
+}
 
 int Tree::ensure_device() {
     if (device_ready) return 0;
@@ -90,8 +100,7 @@ int Tree::ensure_device() {
 
 Tree::~Tree() {
     if (stream) (void)hipStreamSynchronize(stream);
-    w64.clear();
-    w32.clear();
+    clear_slots();
     free_factor();
     for (DevBuf* b : d_prow) delete b;
     for (DevBuf* b : d_imap) delete b;
@@ -108,6 +117,7 @@ int Tree::alloc(DevBuf& b, size_t bytes) {
     b.bytes = bytes;
     b.counter = &dev_bytes;
     dev_bytes += (long long)bytes;
+    peak_bytes = std::max(peak_bytes, dev_bytes);
     return 0;
 }
 
@@ -151,8 +161,7 @@ int Tree::plan_tree(int d, int L, const double* domain, double h_tgt, double kb,
         // a re-plan on a live handle: drop everything sized for the old tree
         if (stream) (void)hipStreamSynchronize(stream);
         free_factor();
-        w64.clear();
-        w32.clear();
+        clear_slots();
         device_ready = false;
         if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
     }
@@ -174,21 +183,22 @@ int Tree::factor_device(const double2* k) {
     if (!planned) return fail("Error: rs_plan must be called before rs_factor.");
     if (ensure_device()) return 1;
     const double t0 = now_s();
-    free_factor();
-    w64.clear();
-    w32.clear();
+    // A refactor of the same plan overwrites the factor data in place, so the
+    // solve workspaces (whose product descriptors point into it) stay valid.
+    factored = false;
+    have32 = false;
     const Plan& P = plan;
     const int L = P.L, nl = P.n_leaves;
     // keep the potential resident (the FP64 residual of the mixed path reads it)
-    if (alloc(kint, (size_t)nl * P.n_int * sizeof(double2))) return 1;
+    if (ensure_buf(kint, (size_t)nl * P.n_int * sizeof(double2))) return 1;
     HIPCHK(hipMemcpyAsync(kint.p, k, (size_t)nl * P.n_int * sizeof(double2), hipMemcpyDeviceToDevice, stream));
-    if (alloc(sinv, (size_t)nl * G.K * G.s * G.s * sizeof(double2)) || alloc(info, sizeof(int))) return 1;
+    if (ensure_buf(sinv, (size_t)nl * G.K * G.s * G.s * sizeof(double2)) || ensure_buf(info, sizeof(int))) return 1;
     HIPCHK(hipMemsetAsync(info.p, 0, sizeof(int), stream));
     launch_leaf_blocks(G, kint.as<double2>(), sinv.as<double2>(), info.as<int>(), nl, stream);
-    T.assign(L + 1, nullptr);
-    invM.assign(L, nullptr);
-    T[L] = new DevBuf;
-    if (alloc(*T[L], (size_t)nl * P.nb * P.nb * sizeof(double2))) return 1;
+    T.resize(L + 1, nullptr);
+    invM.resize(L, nullptr);
+    if (!T[L]) T[L] = new DevBuf;
+    if (ensure_buf(*T[L], (size_t)nl * P.nb * P.nb * sizeof(double2))) return 1;
     {
         LeafArgs<double> a;
         a.mode = LEAF_FACTOR_T; a.ncols = P.nb; a.n_leaves = nl;
@@ -262,8 +272,8 @@ int Tree::factor_device(const double2* k) {
         launch_inverse(ip, Sm.as<double2>(), piv.as<int>(), W.as<double2>(), dd_inv2.as<GemmDesc>(), info.as<int>(),
                        stream);
         // invM
-        invM[l] = new DevBuf;
-        if (alloc(*invM[l], nodes * 4 * ii * 16)) return 1;
+        if (!invM[l]) invM[l] = new DevBuf;
+        if (ensure_buf(*invM[l], nodes * 4 * ii * 16)) return 1;
         double2* IM = invM[l]->as<double2>();
         {
             std::vector<GemmDesc> v;
@@ -314,8 +324,8 @@ int Tree::factor_device(const double2* k) {
         }
         // parent RtR
         if (l > 0 || !opts.skip_root_rtr) {
-            T[l] = new DevBuf;
-            if (alloc(*T[l], (size_t)nodes * Pp * Pp * 16)) return 1;
+            if (!T[l]) T[l] = new DevBuf;
+            if (ensure_buf(*T[l], (size_t)nodes * Pp * Pp * 16)) return 1;
             std::vector<GemmDesc> v;
             const int* prow = d_prow[l]->as<int>();
             for (int n = 0; n < nodes; ++n) {
@@ -353,19 +363,18 @@ int Tree::factor_device(const double2* k) {
 
 int Tree::make_c64() {
     const Plan& P = plan;
+    // in place when the buffers exist (a refactor), so descriptors stay valid
     auto conv = [&](DevBuf& dst, const DevBuf& src) -> int {
         size_t n = src.bytes / sizeof(double2);
-        if (alloc(dst, n * sizeof(float2))) return 1;
+        if (ensure_buf(dst, n * sizeof(float2))) return 1;
         launch_to_c64(dst.as<float2>(), src.as<double2>(), n, stream);
         return 0;
     };
-    for (DevBuf* b : T32) delete b;
-    for (DevBuf* b : invM32) delete b;
-    T32.assign(P.L + 1, nullptr);
-    invM32.assign(P.L, nullptr);
+    T32.resize(P.L + 1, nullptr);
+    invM32.resize(P.L, nullptr);
     if (conv(sinv32, sinv)) return 1;
-    for (int l = 1; l <= P.L; ++l) { T32[l] = new DevBuf; if (conv(*T32[l], *T[l])) return 1; }
-    for (int l = 0; l < P.L; ++l) { invM32[l] = new DevBuf; if (conv(*invM32[l], *invM[l])) return 1; }
+    for (int l = 1; l <= P.L; ++l) { if (!T32[l]) T32[l] = new DevBuf; if (conv(*T32[l], *T[l])) return 1; }
+    for (int l = 0; l < P.L; ++l) { if (!invM32[l]) invM32[l] = new DevBuf; if (conv(*invM32[l], *invM[l])) return 1; }
     HIPCHK(hipStreamSynchronize(stream));
     have32 = true;
     return 0;
@@ -463,6 +472,75 @@ template <typename R> int Tree::ensure_work(SolveWork<R>& w, int nrhs) {
     return 0;
 }
 
+long long Tree::work_bytes(int cap, int prec) const {
+    const Plan& P = plan;
+    const int L = P.L;
+    const double e = (prec == RS_MIXED) ? 8.0 : 16.0;
+    long long b = 0;
+    for (int l = 0; l <= L; ++l) {
+        const long long n = (long long)((double)cap * P.lv[l].nodes * P.lv[l].P * e);
+        b += (l > 0 || L == 0) ? 2 * n : n;
+    }
+    size_t wse = 0;
+    for (int l = 0; l < L; ++l) {
+        const Level& Vl = P.lv[l];
+        b += 2 * (long long)((double)cap * Vl.nodes * 2 * Vl.isz * e);
+        wse = std::max(wse, gemm_ws_elems(Vl.nodes, 2 * Vl.isz, cap, 2 * Vl.isz));
+        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.bn, cap, Vl.isz));
+        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.isz, cap, Vl.bn));
+        const long long nd = (long long)Vl.nodes * (l > 0 ? 6 : 4);
+        b += nd * (long long)sizeof(GemmDesc);
+    }
+    if (wse) b += (long long)(wse * e);
+    const long long nl = P.n_leaves;
+    if (prec == RS_MIXED && opts.n_refine > 0)
+        b += (long long)cap * nl * (P.n_int * 8 + P.nb * 32 + P.m * 16) + (long long)cap * (long long)sizeof(void*);
+    b += 3LL * cap * (long long)sizeof(void*);
+    return b;
+}
+
+void Tree::factor_bytes(long long* resident_out, long long* peak_out) const {
+    const Plan& P = plan;
+    const int L = P.L;
+    const long long nl = P.n_leaves;
+    long long res = 0;
+    for (int l = 0; l < L; ++l) res += (long long)(P.lv[l].prow.size() + P.lv[l].imap.size()) * 4;
+    res += (long long)(P.root_leaf.size() + P.root_loc.size() + P.leaf_nbr.size()) * 4;
+    res += nl * P.n_int * 16 + nl * (long long)G.K * G.s * G.s * 16 + 4 + nl * (long long)P.nb * P.nb * 16;
+    long long peak = res, c64 = nl * (long long)G.K * G.s * G.s * 8;
+    c64 += nl * (long long)P.nb * P.nb * 8;
+    const size_t lds = device_ready ? lds_limit : 163840;
+    const long long gd = (long long)sizeof(GemmDesc);
+    for (int l = L - 1; l >= 0; --l) {
+        const Level& V = P.lv[l];
+        const long long nodes = V.nodes, i = V.isz, bn = V.bn, Pp = V.P;
+        const long long ii = i * i;
+        res += nodes * 4 * ii * 16;
+        c64 += nodes * 4 * ii * 8;
+        if (l > 0 || !opts.skip_root_rtr) {
+            res += nodes * Pp * Pp * 16;
+            if (l > 0) c64 += nodes * Pp * Pp * 8;
+        }
+        size_t wse = 0;
+        wse = std::max(wse, gemm_ws_elems((int)nodes, (int)i, (int)i, (int)i));
+        wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, (int)i, (int)i, (int)i));
+        wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, 2 * (int)i, (int)bn, (int)i));
+        wse = std::max(wse, gemm_ws_elems(4 * (int)nodes, (int)bn, (int)bn, (int)i));
+        InvPlan ip;
+        ip.build((int)i, (int)nodes, (size_t)ii, lds);
+        long long tr = 3 * nodes * ii * 16 + nodes * 2 * i * 2 * bn * 16 + (long long)wse * 16;
+        if (!ip.small)
+            tr += (long long)ip.w_elems() * 16 + nodes * i * 4 + 2 * (long long)ip.panels() * nodes * gd +
+                  (long long)ip.panels() * (i - ip.nb) * 4;
+        tr += 3 * nodes * gd + 2 * nodes * gd + ((l > 0 || !opts.skip_root_rtr) ? 4 * nodes * gd : 0);
+        peak = std::max(peak, res + tr);
+    }
+    if (opts.precision == RS_MIXED) res += c64;
+    peak = std::max(peak, res);
+    if (resident_out) *resident_out = res;
+    if (peak_out) *peak_out = peak;
+}
+
 void Tree::pbeg(int cat, hipStream_t st) {
     if (!prof_on) return;
     ProfRec r;
@@ -532,7 +610,7 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     }
     pbeg(P_GATHER, st);
     launch_root_gather<V>(w.gF[0]->template as<V>(), io.g, d_root_leaf.as<int>(), d_root_loc.as<int>(), P.lv[0].P,
-                          nl, P.nb, nrhs, st);
+                          P.nb, nrhs, st);
     pend(nr * P.lv[0].P * (16.0 + e), 0, st);
     for (int l = 0; l < L; ++l) {
         const Level& Vl = P.lv[l];
@@ -557,54 +635,98 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
 int Tree::solve_device(int nrhs, const double2* f, const double2* g, double2* u, int prec, hipStream_t st) {
     if (!factored) return fail("Error: Robin Tree must be factored first.");
     if (nrhs < 1) return fail("Error: nrhs must be positive.");
+    if (!st) st = stream;
+    // column tables of the contiguous arrays, built on the stream
+    Slot& sl = slot(0);
+    if (sl.tabs_cap < nrhs) {
+        if (alloc(sl.tabs, (size_t)3 * nrhs * sizeof(void*))) return 1;
+        sl.tabs_cap = nrhs;
+    }
+    double2** tab = sl.tabs.as<double2*>();
+    const Plan& P = plan;
+    const size_t nl = P.n_leaves;
+    launch_col_table(tab, const_cast<double2*>(f), nl * P.n_int, nrhs, st);
+    launch_col_table(tab + nrhs, const_cast<double2*>(g), nl * P.nb, nrhs, st);
+    launch_col_table(tab + 2 * nrhs, u, nl * P.m, nrhs, st);
+    RhsCols c;
+    c.f = tab; c.g = tab + nrhs; c.u = tab + 2 * nrhs;
+    return solve_cols(nrhs, c, prec, st, 0);
+}
+
+int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, int slot_id) {
+    if (!factored) return fail("Error: Robin Tree must be factored first.");
+    if (nrhs < 1) return fail("Error: nrhs must be positive.");
     if (prec < 0) prec = opts.precision;
     if (!st) st = stream;
+    Slot& sl = slot(slot_id);
     if (prec == RS_MIXED) {
+        SolveWork<float>& w32 = sl.w32;
         if (!have32 && make_c64()) return 1;
         if (ensure_work(w32, nrhs)) return 1;
         PassIO io;
-        io.f64 = f; io.g = g;
+        io.f64 = cols.f; io.g = cols.g;
         if (opts.n_refine <= 0) {
-            io.u64 = u;
+            io.u64 = cols.u;
             return solve_pass(w32, nrhs, io, st);
         }
         // The first iterate is kept in complex64: its FP64 residual is exact for
         // the rounded iterate, and the correction restores full accuracy.  Then
         // u = u_{i} + K32^-1 (b - K u_{i}) for a fixed number of steps, so that a
         // right-hand side gets the same bits alone or batched.
-        const size_t nl = plan.n_leaves, cap = w32.cap;
-        if (!w32.rf.p) {
-            if (alloc(w32.rf, cap * nl * plan.n_int * 8) || alloc(w32.rg, cap * nl * plan.nb * 16) ||
-                alloc(w32.rj, cap * nl * plan.nb * 16) || alloc(w32.uA, cap * nl * plan.m * 8) ||
-                alloc(w32.du, cap * nl * plan.m * 8))
-                return 1;
-        }
+        const size_t nl = plan.n_leaves;
+        if (ensure_refine(w32)) return 1;
         float2* uA = w32.uA.as<float2>();
         io.u32 = uA;
         if (solve_pass(w32, nrhs, io, st)) return 1;
         for (int it = 0; it < opts.n_refine; ++it) {
-            if (residual(nrhs, it == 0 ? nullptr : u, it == 0 ? uA : nullptr, f, g, w32.rf.as<float2>(),
-                         w32.rg.as<double2>(), w32.rj.as<double2>(), st))
+            if (residual(nrhs, it == 0 ? nullptr : cols.u, it == 0 ? uA : nullptr, cols.f, cols.g,
+                         w32.rf.as<float2>(), w32.rg.as<double2>(), w32.rj.as<double2>(), st))
                 return 1;
             PassIO c;
-            c.f32 = w32.rf.as<float2>(); c.jump = w32.rj.as<double2>(); c.g = w32.rg.as<double2>();
+            c.f32 = w32.rf.as<float2>(); c.jump = w32.rj.as<double2>();
+            c.g = w32.tab_rg.as<const double2*>();
             c.u32 = w32.du.as<float2>();
             if (solve_pass(w32, nrhs, c, st)) return 1;
-            const size_t n = (size_t)nrhs * nl * plan.m;
+            const size_t n = nl * plan.m;
             pbeg(P_COMBINE, st);
-            launch_combine(u, it == 0 ? nullptr : u, it == 0 ? uA : nullptr, w32.du.as<float2>(), n, st);
-            pend((double)n * (8.0 + (it == 0 ? 8.0 : 16.0) + 16.0), 0, st);
+            launch_combine(cols.u, it == 0 ? nullptr : cols.u, it == 0 ? uA : nullptr, w32.du.as<float2>(), n,
+                           nrhs, st);
+            pend((double)n * nrhs * (8.0 + (it == 0 ? 8.0 : 16.0) + 16.0), 0, st);
         }
         return 0;
     }
+    SolveWork<double>& w64 = sl.w64;
     if (ensure_work(w64, nrhs)) return 1;
     PassIO io;
-    io.f64 = f; io.g = g; io.u64 = u;
+    io.f64 = cols.f; io.g = cols.g; io.u64 = cols.u;
     return solve_pass(w64, nrhs, io, st);
 }
 
-int Tree::residual(int nrhs, const double2* u64, const float2* u32, const double2* f, const double2* g, float2* rf,
-                   double2* rg, double2* rj, hipStream_t st) {
+int Tree::ensure_refine(SolveWork<float>& w) {
+    if (w.rf.p) return 0;
+    const size_t nl = plan.n_leaves, cap = w.cap;
+    if (alloc(w.rf, cap * nl * plan.n_int * 8) || alloc(w.rg, cap * nl * plan.nb * 16) ||
+        alloc(w.rj, cap * nl * plan.nb * 16) || alloc(w.uA, cap * nl * plan.m * 8) ||
+        alloc(w.du, cap * nl * plan.m * 8) || alloc(w.tab_rg, cap * sizeof(void*)))
+        return 1;
+    launch_col_table(w.tab_rg.as<double2*>(), w.rg.as<double2>(), nl * plan.nb, (int)cap, stream);
+    HIPCHK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+int Tree::reserve_slot(int slot_id, int cap) {
+    if (!factored) return fail("Error: Robin Tree must be factored first.");
+    Slot& sl = slot(slot_id);
+    if (opts.precision == RS_MIXED) {
+        if (!have32 && make_c64()) return 1;
+        if (ensure_work(sl.w32, cap)) return 1;
+        return opts.n_refine > 0 ? ensure_refine(sl.w32) : 0;
+    }
+    return ensure_work(sl.w64, cap);
+}
+
+int Tree::residual(int nrhs, double2* const* u64, const float2* u32, const double2* const* f,
+                   const double2* const* g, float2* rf, double2* rg, double2* rj, hipStream_t st) {
     const Plan& P = plan;
     pbeg(P_RESIDUAL, st);
     launch_residual(G, u64, u32, kint.as<double2>(), f, g, d_leaf_nbr.as<int>(), make_double2(P.nu[0], P.nu[1]),

@@ -36,6 +36,7 @@ template <typename R> struct SolveWork {
     std::vector<DevBuf*> hV, gF, G, Z;   // per level
     DevBuf ws;                           // K-split GEMM workspace
     DevBuf rf, rg, rj, uA, du;           // refinement: residuals (c64 / c128 / c128), first iterate, correction (c64)
+    DevBuf tab_rg;                       // column table of rg (the correction pass reads it as g)
     std::vector<DevBuf*> descs;          // per (level, kind)
     std::vector<GemmDesc*> d_up_g, d_up_h, d_dn_t, d_dn_g;
     ~SolveWork() { clear(); }
@@ -52,13 +53,33 @@ struct ProfRec {
 };
 
 // Inputs and outputs of one tree solve pass (up + down) over all right-hand sides.
+// Complex128 arrays are column tables (column c at tab[c], laid out [leaf][len]);
+// complex64 arrays are contiguous [col][leaf][len].
 struct PassIO {
-    const double2* f64 = nullptr;   // interior data, complex128 ...
-    const float2* f32 = nullptr;    //   ... or complex64
-    const double2* jump = nullptr;  // transmission residual subtracted from the leaf outgoing data
-    const double2* g = nullptr;     // physical boundary data (complex128)
-    double2* u64 = nullptr;         // solution out, complex128 ...
-    float2* u32 = nullptr;          //   ... or complex64
+    const double2* const* f64 = nullptr;  // interior data, complex128 ...
+    const float2* f32 = nullptr;          //   ... or complex64
+    const double2* jump = nullptr;        // transmission residual subtracted from the leaf outgoing data
+    const double2* const* g = nullptr;    // physical boundary data (complex128)
+    double2* const* u64 = nullptr;        // solution out, complex128 ...
+    float2* u32 = nullptr;                //   ... or complex64
+};
+
+// The right-hand sides of one solve as column tables in device memory:
+// f[c] -> [leaf][n_int], g[c] -> [leaf][n_bnd], u[c] -> [leaf][m], complex128.
+struct RhsCols {
+    const double2* const* f = nullptr;
+    const double2* const* g = nullptr;
+    double2* const* u = nullptr;
+};
+
+// Solve workspaces of one concurrent caller (a stream): both precisions, and
+// the column tables of the contiguous-array entry points.
+struct Slot {
+    SolveWork<double> w64;
+    SolveWork<float> w32;
+    DevBuf tabs;
+    int tabs_cap = 0;
+    void clear() { w64.clear(); w32.clear(); tabs.release(); tabs_cap = 0; }
 };
 
 struct Tree {
@@ -96,15 +117,20 @@ struct Tree {
 
     DevBuf st_f, st_g, st_u;     // staging of the host-memory API (double2)
 
-    SolveWork<double> w64;
-    SolveWork<float> w32;
+    // solve workspaces; slot 0 serves rs_solve / rs_solve_device, a server uses
+    // one slot per stream it runs this tree on
+    std::vector<Slot*> slots;
+    Slot& slot(int i);
+    void clear_slots();
     int last_nrhs = 0;
     int last_prec = RS_FP64;
+    long long peak_bytes = 0;    // high-water mark of dev_bytes
 
     Tree();
     ~Tree();
     int fail(const std::string& m) { err = m; return 1; }
     int alloc(DevBuf& b, size_t bytes);
+    int ensure_buf(DevBuf& b, size_t bytes) { return (b.p && b.bytes == bytes) ? 0 : alloc(b, bytes); }
     template <typename T_> int upload(DevBuf& b, const std::vector<T_>& v);
     void free_factor();
 
@@ -115,12 +141,25 @@ struct Tree {
     int make_c64();
     template <typename R> int ensure_work(SolveWork<R>& w, int nrhs);
     template <typename R> int solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st);
-    int residual(int nrhs, const double2* u64, const float2* u32, const double2* f, const double2* g, float2* rf,
-                 double2* rg, double2* rj, hipStream_t st);
+    int residual(int nrhs, double2* const* u64, const float2* u32, const double2* const* f,
+                 const double2* const* g, float2* rf, double2* rg, double2* rj, hipStream_t st);
     int solve_device(int nrhs, const double2* f, const double2* g, double2* u, int prec, hipStream_t st);
+    int solve_cols(int nrhs, const RhsCols& c, int prec, hipStream_t st, int slot_id);
+    int ensure_refine(SolveWork<float>& w);
+    // allocate slot slot_id's workspace for cap columns at the tree's precision
+    int reserve_slot(int slot_id, int cap);
+    // device bytes: a solve workspace of cap columns at precision prec (with the
+    // refinement buffers of the mixed path), and the factor's resident and peak bytes
+    long long work_bytes(int cap, int prec) const;
+    void factor_bytes(long long* resident, long long* peak) const;
     int solve_host(int nrhs, const double* f, const double* g, int prec);
     int get_solution(int rhs, double* u) const;
     int node_rtr(int lvl, int idx, double* out, int* P) const;
 };
 
 }  // namespace rs
+
+// the opaque handle of the C ABI
+struct rs_tree {
+    rs::Tree t;
+};

@@ -1,0 +1,173 @@
+"""The serving loop's scheduling policy on the host (simulated device): per-tree
+arrival order, coalescing of queued right-hand sides into passes of at most
+max_cols, factor requests ordered against the tree's solves, admission by
+memory with eviction of idle workspaces, and the latency statistics.  Trees
+are really planned (the memory figures come from their plans); passes and
+factorizations take modelled times instead of running on a GPU."""
+import time
+
+import pytest
+
+import robinsolver_amd as rsa
+from robinsolver_amd.serve import Server, RS_REQ_DONE, RS_REQ_FAILED
+
+UNIT = [[0, 1], [0, 1], [0, 1]]
+
+
+def _tree(n=64, L=4, prec=rsa.RS_MIXED):
+    return rsa.GpuTree(2, L, UNIT, 1.0 / n, -10.0, precision=prec, n_refine=1)
+
+
+def _server(**kw):
+    args = dict(max_cols=32, lanes=2, hbm_limit=1 << 40, simulate=True, sim_pass_s=0.01, sim_col_s=0.0,
+                sim_factor_s=0.02)
+    args.update(kw)
+    return Server(**args)
+
+
+def _factor(srv, tid):
+    r = srv.factor(tid, 0)
+    srv.wait(r)
+
+
+def test_coalescing_and_arrival_order():
+    t = _tree()
+    with _server() as srv:
+        tid = srv.add_tree(t)
+        _factor(srv, tid)
+        sizes = [1, 5, 7, 3, 16, 9, 2, 30, 4, 1, 8, 12]
+        reqs = [srv.solve(tid, n, 0, 0, 0) for n in sizes]
+        srv.drain()
+        info = [srv.info(r) for r in reqs]
+        passes = {}
+        for n, (ts, tb, td, p) in zip(sizes, info):
+            assert ts <= tb <= td
+            passes.setdefault(p, []).append(n)
+        # every pass within max_cols; requests of one tree run in arrival order
+        assert all(sum(v) <= 32 for v in passes.values())
+        pids = [p for (_, _, _, p) in info]
+        assert pids == sorted(pids)
+        # requests queued while the first pass ran were coalesced
+        assert len(passes) < len(sizes)
+        st = srv.stats()
+        assert st["rhs_done"] == sum(sizes) and st["requests_done"] == len(sizes)
+        assert st["lat_p50_s"] <= st["lat_p99_s"] <= st["lat_max_s"]
+        assert st["mean_cols"] > 1.0
+
+
+def test_factor_is_ordered_against_the_trees_solves():
+    t = _tree()
+    with _server() as srv:
+        tid = srv.add_tree(t)
+        _factor(srv, tid)
+        a = srv.solve(tid, 4, 0, 0, 0)
+        f = srv.factor(tid, 0)
+        b = srv.solve(tid, 4, 0, 0, 0)
+        srv.drain()
+        _, _, a_done, pa = srv.info(a)
+        _, f_start, f_done, _ = srv.info(f)
+        _, b_start, _, pb = srv.info(b)
+        assert a_done <= f_start and f_done <= b_start
+        assert pa != pb          # b was not coalesced across the factorization
+
+
+def test_solve_on_unfactored_tree_fails():
+    t = _tree()
+    with _server() as srv:
+        tid = srv.add_tree(t)
+        r = srv.solve(tid, 2, 0, 0, 0)
+        srv.drain()
+        assert srv.state(r) == RS_REQ_FAILED
+        assert "factored first" in srv.error(r)
+        with pytest.raises(rsa.RobinError, match="factored first"):
+            srv.wait(r)
+
+
+def test_nrhs_bound_and_bad_tree():
+    t = _tree()
+    with _server(max_cols=8) as srv:
+        tid = srv.add_tree(t)
+        with pytest.raises(rsa.RobinError, match="max_cols"):
+            srv.solve(tid, 9, 0, 0, 0)
+        with pytest.raises(rsa.RobinError, match="no such tree"):
+            srv.solve(tid + 1, 1, 0, 0, 0)
+        with pytest.raises(rsa.RobinError, match="already registered"):
+            srv.add_tree(t)
+
+
+def test_lanes_overlap_across_trees():
+    t1, t2 = _tree(), _tree(128, 6)
+    with _server(sim_pass_s=0.05) as srv:
+        a, b = srv.add_tree(t1), srv.add_tree(t2)
+        _factor(srv, a)
+        _factor(srv, b)
+        r1 = srv.solve(a, 3, 0, 0, 0)
+        r2 = srv.solve(b, 3, 0, 0, 0)
+        srv.drain()
+        _, s1, d1, _ = srv.info(r1)
+        _, s2, d2, _ = srv.info(r2)
+        assert s2 < d1 and s1 < d2          # both passes were in flight together
+
+
+def test_admission_evicts_idle_workspaces():
+    t1, t2 = _tree(), _tree()
+    r1, p1, w1 = t1.memory_estimate(16)
+    # room for both factorizations and one solve workspace at a time
+    limit = 2 * r1 + max(p1 - r1, w1) + w1 // 2
+    with _server(max_cols=16, lanes=1, hbm_limit=limit) as srv:
+        a, b = srv.add_tree(t1), srv.add_tree(t2)
+        _factor(srv, a)
+        _factor(srv, b)
+        reqs = []
+        for i in range(6):
+            reqs.append(srv.solve(a if i % 2 == 0 else b, 4, 0, 0, 0))
+            time.sleep(0.015)
+        srv.drain()
+        assert all(srv.state(r) == RS_REQ_DONE for r in reqs)
+        st = srv.stats()
+        assert st["evictions"] >= 1
+        assert st["held_bytes"] <= limit
+
+
+def test_admission_refuses_what_can_never_fit():
+    t = _tree()
+    r, p, w = t.memory_estimate(16)
+    with _server(max_cols=16, lanes=1, hbm_limit=r + (p - r) + w // 2) as srv:
+        tid = srv.add_tree(t)
+        _factor(srv, tid)
+        q = srv.solve(tid, 4, 0, 0, 0)
+        srv.drain()
+        assert srv.state(q) == RS_REQ_FAILED
+        assert "not enough device memory" in srv.error(q)
+    with _server(max_cols=16, lanes=1, hbm_limit=p // 2) as srv:
+        tid = srv.add_tree(t)
+        f = srv.factor(tid, 0)
+        srv.drain()
+        assert srv.state(f) == RS_REQ_FAILED
+        assert "not enough device memory" in srv.error(f)
+
+
+def test_memory_estimates_scale():
+    t = _tree(128, 6)
+    r, p, w16 = t.memory_estimate(16)
+    _, _, w32 = t.memory_estimate(32)
+    assert 0 < r <= p
+    assert 1.9 < w32 / w16 < 2.1         # the workspace is per right-hand side
+    t64 = rsa.GpuTree(2, 6, UNIT, 1.0 / 128, -10.0, precision=rsa.RS_FP64)
+    r64, _, w64 = t64.memory_estimate(16)
+    assert r64 < r                        # no complex64 copies
+    assert w64 < w16 * 2.0                # no refinement buffers in FP64
+
+
+def test_release_and_stats_reset():
+    t = _tree()
+    with _server() as srv:
+        tid = srv.add_tree(t)
+        _factor(srv, tid)
+        r = srv.solve(tid, 2, 0, 0, 0)
+        srv.wait(r)
+        srv.release(r)
+        with pytest.raises(rsa.RobinError, match="no such request"):
+            srv.state(r)
+        srv.reset_stats()
+        assert srv.stats()["requests_done"] == 0
