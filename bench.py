@@ -58,6 +58,245 @@ def gaussian_sources(torch, pts_int_dev, nrhs, seed, sigma=0.02, d=2):
     return f
 
 
+class URing:
+    """Output columns of one tree's requests: a ring of `cap` solution columns.
+    A request takes the next free run of columns; before a run is reused the
+    requests that held it must have completed (their u is the caller's data
+    until then)."""
+
+    def __init__(self, torch, tree, cap):
+        self.u = torch.zeros((cap, tree.n_leaves, tree.m), dtype=torch.complex128, device="cuda:0")
+        self.cap, self.pos, self.col_bytes = cap, 0, tree.n_leaves * tree.m * 16
+        self.out = []                   # (request id, start, end), oldest first
+        self.owner = [-1] * cap
+        self.waits = 0
+
+    def take(self, srv, nrhs, job):
+        if self.pos + nrhs > self.cap:
+            self.pos = 0
+        s, e = self.pos, self.pos + nrhs
+        while any(a < e and s < b for _, a, b in self.out):
+            rid, _, _ = self.out.pop(0)
+            if srv.state(rid) < 2:          # still queued or running: the client blocks
+                self.waits += 1
+            srv.wait(rid)
+        self.pos = e
+        for c in range(s, e):
+            self.owner[c] = job
+        return s
+
+    def ptr(self, col):
+        return self.u.data_ptr() + col * self.col_bytes
+
+
+def drive(srv, jobs, rate, seed):
+    """Submit jobs (dicts: tree id, kind, nrhs, input pointers, ring) at Poisson
+    arrival times of the given rate (None: as fast as the output rings allow),
+    then drain.  Returns wall seconds from the first submission to the last
+    completion and the request ids."""
+    rng = np.random.default_rng(seed)
+    t_arr = np.cumsum(rng.exponential(1.0 / rate, len(jobs))) if rate else np.zeros(len(jobs))
+    ids = []
+    t0 = time.perf_counter()
+    for i, jb in enumerate(jobs):
+        if rate:
+            dt = t0 + t_arr[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+        if jb["kind"] == "factor":
+            rid = srv.factor(jb["tree"], jb["k"])
+        else:
+            ring = jb["ring"]
+            c0 = ring.take(srv, jb["nrhs"], i)
+            jb["ucol"] = c0
+            rid = srv.solve(jb["tree"], jb["nrhs"], jb["f"], jb["g"], ring.ptr(c0))
+            ring.out.append((rid, c0, c0 + jb["nrhs"]))
+        ids.append(rid)
+    srv.drain()
+    for jb in jobs:
+        if "ring" in jb:
+            jb["ring"].out.clear()       # every request has completed
+    t_end = max(srv.info(r)[2] for r in ids)
+    t_first = min(srv.info(r)[0] for r in ids)
+    return t_end - t_first, ids
+
+
+def serve_cfg3(torch, rsa, t, f, g, u32, u64, args, sz_f, sz_g):
+    """The served path on the same tree and right-hand sides: requests of 1..32
+    columns of the 256-source batch (random offsets), a saturated client (at
+    most the output ring's columns outstanding) for sustained solves/s, then a
+    Poisson client at 70% of that rate for p50/p99 latency.  Served results of
+    requests whose output columns were not reused are compared bit for bit
+    with the batched results of the same columns."""
+    from robinsolver_amd.serve import Server
+    rng = np.random.default_rng(args.seed + 17)
+    out = {}
+    with Server(max_cols=args.nrhs, lanes=2) as srv:
+        tid = srv.add_tree(t)
+        ring = URing(torch, t, 3 * args.nrhs)
+
+        def jobs(n):
+            js = []
+            for _ in range(n):
+                nr = int(rng.integers(1, 33))
+                off = int(rng.integers(0, args.nrhs - nr + 1))
+                js.append({"tree": tid, "kind": "solve", "nrhs": nr, "off": off, "ring": ring,
+                           "f": f.data_ptr() + off * sz_f * 16, "g": g.data_ptr() + off * sz_g * 16})
+            return js
+
+        drive(srv, jobs(40), None, 1)            # warm: workspaces, both lanes
+        for name, frac in (("saturated", None), ("poisson70", 0.7)):
+            js = jobs(args.serve_requests)
+            srv.reset_stats()
+            rate = None if frac is None else frac * out["saturated"]["solves_per_s"] / 16.5
+            wall, ids = drive(srv, js, rate, 2)
+            st = srv.stats()
+            rhs = sum(j["nrhs"] for j in js)
+            # bitwise: served columns against the batched mixed result
+            same = tot = 0
+            worst = 0.0
+            for i, j in enumerate(js):
+                if any(ring.owner[c] != i for c in range(j["ucol"], j["ucol"] + j["nrhs"])):
+                    continue
+                us = ring.u[j["ucol"]:j["ucol"] + j["nrhs"]]
+                ub = u32[j["off"]:j["off"] + j["nrhs"]]
+                tot += 1
+                same += int(torch.equal(us, ub))
+                ref = u64[j["off"]:j["off"] + j["nrhs"]]
+                d = torch.linalg.vector_norm((us - ref).reshape(j["nrhs"], -1), dim=1)
+                worst = max(worst, float((d / torch.linalg.vector_norm(ref.reshape(j["nrhs"], -1), dim=1)).max()))
+            out[name] = {"requests": len(js), "rhs": rhs, "wall_s": wall, "solves_per_s": rhs / wall,
+                         "offered_requests_per_s": rate, "lat_p50_ms": 1e3 * st["lat_p50_s"],
+                         "lat_p99_ms": 1e3 * st["lat_p99_s"], "lat_max_ms": 1e3 * st["lat_max_s"],
+                         "passes": st["passes"], "mean_cols_per_pass": st["mean_cols"],
+                         "busy_s": st["busy_s"], "ring_waits": ring.waits, "failed": st["failed"],
+                         "checked_requests": tot, "bitwise_equal_to_batched": same,
+                         "max_rel_l2_vs_fp64": worst}
+            log("served %s: %.1f solves/s, p50 %.1f ms, p99 %.1f ms, %d passes (%.1f cols), "
+                "%d/%d bitwise equal, max rel l2 %.2e" %
+                (name, rhs / wall, 1e3 * st["lat_p50_s"], 1e3 * st["lat_p99_s"], st["passes"], st["mean_cols"],
+                 same, tot, worst))
+            for r in ids:
+                srv.release(r)
+    return out
+
+
+MIX_TREES = [(128, 6, 10.0), (256, 8, 10.0), (384, 10, 20.0), (512, 10, 20.0), (640, 10, 40.0), (768, 12, 40.0),
+             (896, 12, 80.0), (1024, 12, 80.0)]
+
+
+def plane_wave_g(tree, pts, face, kb, thetas):
+    """Impedance data of u = exp(i kb d.x), d = (cos t, sin t), on the physical
+    faces: g = a1 u + du/dn with a1 = -i kb, so g = i kb (d.n - 1) u.
+    Returns [len(thetas)][n_leaves][n_bnd] (zero away from the boundary)."""
+    nb = tree.nb
+    x = pts[:, :nb, :2]
+    fc = np.broadcast_to(face[:nb], x.shape[:2])
+    normal = np.zeros(x.shape)
+    phys = np.zeros(x.shape[:2], bool)
+    for fid, ax, sgn, edge in ((1, 0, -1.0, 0.0), (2, 0, 1.0, 1.0), (3, 1, -1.0, 0.0), (4, 1, 1.0, 1.0)):
+        on = (fc == fid) & (np.abs(x[..., ax] - edge) < 1e-12)
+        phys |= on
+        normal[..., ax] = np.where(fc == fid, sgn, normal[..., ax])
+    out = np.zeros((len(thetas),) + x.shape[:2], np.complex128)
+    for i, th in enumerate(thetas):
+        dv = np.array([np.cos(th), np.sin(th)])
+        u = np.exp(1j * kb * (x @ dv))
+        out[i] = np.where(phys, 1j * kb * ((normal @ dv) - 1.0) * u, 0.0)
+    return out
+
+
+def serve_mix(args):
+    """Eight resident trees (2D 128^2 .. 1024^2, leaves near 16^2, kb by grid),
+    requests of 1..32 plane-wave right-hand sides on a uniformly chosen tree,
+    2% of requests refactorizations with a new potential; a saturated client
+    for sustained solves/s, then Poisson arrivals at 70% of it for latency."""
+    import torch
+    import robinsolver_amd as rsa
+    from robinsolver_amd.serve import Server
+    dev = torch.device("cuda:0")
+    unit = [[0, 1], [0, 1], [0, 1]]
+    rng = np.random.default_rng(args.seed + 5)
+    trees, info = [], []
+    for gi, (n, L, kb) in enumerate(MIX_TREES):
+        t = rsa.GpuTree(2, L, unit, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=1)
+        pts, face = t.all_points()
+        ks = [torch.from_numpy(potential(pts[:, t.nb:], kb, args.seed + 100 * gi + j)).to(dev) for j in range(2)]
+        t.factor(ks[0].cpu().numpy())
+        gpool = torch.from_numpy(plane_wave_g(t, pts, face, kb, rng.uniform(0, 2 * np.pi, 32))).to(dev)
+        fpool = torch.zeros((32, t.n_leaves, t.n_int), dtype=torch.complex128, device=dev)
+        # accuracy of one refinement step on this tree, against its FP64 solve
+        t64 = rsa.GpuTree(2, L, unit, 1.0 / n, -kb, precision=rsa.RS_FP64)
+        t64.factor(ks[0].cpu().numpy())
+        ua = torch.zeros((4, t.n_leaves, t.m), dtype=torch.complex128, device=dev)
+        ub = torch.zeros_like(ua)
+        t.solve_device(4, fpool.data_ptr(), gpool.data_ptr(), ua.data_ptr())
+        t64.solve_device(4, fpool.data_ptr(), gpool.data_ptr(), ub.data_ptr())
+        torch.cuda.synchronize()
+        rel = float((torch.linalg.vector_norm((ua - ub).reshape(4, -1), dim=1) /
+                     torch.linalg.vector_norm(ub.reshape(4, -1), dim=1)).max())
+        if rel > 1e-8:
+            t.set_refine(2)
+        del t64, ua, ub
+        r, p, w = t.memory_estimate(args.nrhs)
+        info.append({"grid": n, "L": L, "kb": kb, "leaf": int(round((t.n_int) ** 0.5)), "rel_l2_1step": rel,
+                     "factor_s": t.timings()[0], "resident_GB": r / 1e9, "workspace_GB_per_lane": w / 1e9})
+        trees.append((t, ks, fpool, gpool))
+        log("tree %d^2 L=%d: factor %.3fs, rel l2 (1 step) %.2e" % (n, L, t.timings()[0], rel))
+    out = {"trees": info}
+    with Server(max_cols=args.nrhs, lanes=2) as srv:
+        ids = [srv.add_tree(t) for t, _, _, _ in trees]
+        rings = [URing(torch, t, args.nrhs) for t, _, _, _ in trees]
+        col = [(t.n_leaves * t.n_int * 16, t.n_leaves * t.nb * 16) for t, _, _, _ in trees]
+        nfac = [0] * len(trees)
+
+        def jobs(n):
+            js = []
+            for _ in range(n):
+                ti = int(rng.integers(0, len(trees)))
+                if rng.uniform() < 0.02:
+                    nfac[ti] += 1
+                    js.append({"tree": ids[ti], "kind": "factor", "k": trees[ti][1][nfac[ti] % 2].data_ptr()})
+                    continue
+                nr = int(rng.integers(1, 33))
+                off = int(rng.integers(0, 32 - nr + 1))
+                js.append({"tree": ids[ti], "ti": ti, "kind": "solve", "nrhs": nr, "ring": rings[ti],
+                           "f": trees[ti][2].data_ptr() + off * col[ti][0],
+                           "g": trees[ti][3].data_ptr() + off * col[ti][1]})
+            return js
+
+        drive(srv, jobs(60), None, 3)
+        for name, frac in (("saturated", None), ("poisson70", 0.7)):
+            js = jobs(args.serve_requests)
+            srv.reset_stats()
+            nreq = len(js)
+            rate = None if frac is None else frac * out["saturated"]["requests_per_s"]
+            wall, rids = drive(srv, js, rate, 4)
+            st = srv.stats()
+            rhs = sum(j.get("nrhs", 0) for j in js)
+            per = {}
+            for j, r in zip(js, rids):
+                if j["kind"] != "solve":
+                    continue
+                a, _, c, _ = srv.info(r)
+                per.setdefault(MIX_TREES[j["ti"]][0], []).append(c - a)
+            out[name] = {"requests": nreq, "factor_requests": sum(1 for j in js if j["kind"] == "factor"),
+                         "rhs": rhs, "wall_s": wall, "solves_per_s": rhs / wall, "requests_per_s": nreq / wall,
+                         "offered_requests_per_s": rate, "lat_p50_ms": 1e3 * st["lat_p50_s"],
+                         "lat_p99_ms": 1e3 * st["lat_p99_s"], "lat_max_ms": 1e3 * st["lat_max_s"],
+                         "passes": st["passes"], "mean_cols_per_pass": st["mean_cols"], "failed": st["failed"],
+                         "factors_done": st["factors_done"], "admission_waits": st["admission_waits"],
+                         "ring_waits": sum(r.waits for r in rings),
+                         "p99_ms_by_grid": {str(k): 1e3 * float(np.quantile(v, 0.99)) for k, v in sorted(per.items())}}
+            log("mix %s: %.1f solves/s, %.1f req/s, p50 %.1f ms, p99 %.1f ms, %d passes (%.1f cols), %d factors" %
+                (name, rhs / wall, nreq / wall, 1e3 * st["lat_p50_s"], 1e3 * st["lat_p99_s"], st["passes"],
+                 st["mean_cols"], st["factors_done"]))
+            for r in rids:
+                srv.release(r)
+        out["held_GB"] = srv.stats()["held_bytes"] / 1e9
+    return out
+
+
 def cpu_baseline(args, kb_phys, kb_opt):
     """The oracle (reference algorithm restated in C, OpenMP over leaves and
     nodes) on a bounded sample of the workload: the [0,1/4]^2 corner at the
@@ -113,11 +352,28 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--r1-solves", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-mix", action="store_true",
+                    help="only the multi-tree serving scenario (8 resident trees 128^2..1024^2)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(multi-GPU sharding is design-only)." % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.serve_mix:
+        mix = serve_mix(args)
+        sat = mix["saturated"]
+        print(json.dumps({
+            "metric": "served solves/s and latency, 8 resident 2D Helmholtz trees 128^2..1024^2 (mixed precision)",
+            "value": sat["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "c64", "data": "synthetic",
+            "config": {"workload": "8 trees (grid, L, kb) = %s; requests of 1..32 plane-wave RHS, 2%% refactors"
+                                   % [list(x) for x in MIX_TREES], "max_cols": args.nrhs, "lanes": 2,
+                       "requests": args.serve_requests},
+            "p50_ms": mix["poisson70"]["lat_p50_ms"], "p99_ms": mix["poisson70"]["lat_p99_ms"], "mix": mix}))
+        return
 
     import torch
     import robinsolver_amd as rsa
@@ -271,6 +527,15 @@ def main():
         "roofline": roof,
         "kernel_ms_per_step": {c: v[0] / args.steps for c, v in prof.items()},
     }
+    if not args.no_serve:
+        log("serving ...")
+        sv = serve_cfg3(torch, rsa, t, f, g, u32, u64, args, sz_f, sz_g)
+        out["served"] = sv
+        out["served_solves_per_s"] = sv["saturated"]["solves_per_s"]
+        out["served_over_fp64_r1"] = sv["saturated"]["solves_per_s"] / r1_rate
+        out["served_p50_ms"] = sv["poisson70"]["lat_p50_ms"]
+        out["served_p99_ms"] = sv["poisson70"]["lat_p99_ms"]
+
     # a plain copy of known size, the calibration point for hardware byte counters
     cal = torch.empty(1 << 25, dtype=torch.float64, device=dev)   # 256 MiB
     cal2 = torch.empty_like(cal)

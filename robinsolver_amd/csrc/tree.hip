@@ -133,8 +133,8 @@ void Tree::free_factor() {
         for (DevBuf* b : v) delete b;
         v.clear();
     };
-    drop(T); drop(invM); drop(T32); drop(invM32);
-    for (DevBuf* b : {&sinv, &sinv32, &info}) b->release();
+    drop(T); drop(invM); drop(T32); drop(invM32); drop(fdesc);
+    for (DevBuf* b : {&sinv, &sinv32, &info, &fscratch}) b->release();
     have32 = false;
     factored = false;
 }
@@ -179,26 +179,192 @@ int Tree::factor(const double* k_host) {
     return factor_device(tmp.as<double2>());
 }
 
+// Scratch of one level's merge, carved from the tree's merge arena (offsets in
+// bytes, 256-aligned): D, S, S D, Y, the K-split workspace of the level's
+// products, and the Gauss-Jordan panel and pivots of large interfaces.
+struct MergeLayout {
+    size_t D = 0, S = 0, SD = 0, Y = 0, ws = 0, W = 0, piv = 0, total = 0;
+    InvPlan ip;
+};
+
+static MergeLayout merge_layout(const Level& V, size_t lds_limit) {
+    MergeLayout m;
+    const size_t nodes = V.nodes, i = V.isz, bn = V.bn, ii = i * i;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+    m.D = take(nodes * ii * 16);
+    m.S = take(nodes * ii * 16);
+    m.SD = take(nodes * ii * 16);
+    m.Y = take(nodes * 2 * i * 2 * bn * 16);
+    size_t wse = 0;
+    wse = std::max(wse, gemm_ws_elems((int)nodes, (int)i, (int)i, (int)i));
+    wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, (int)i, (int)i, (int)i));
+    wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, 2 * (int)i, (int)bn, (int)i));
+    wse = std::max(wse, gemm_ws_elems(4 * (int)nodes, (int)bn, (int)bn, (int)i));
+    m.ws = take(wse * 16);
+    m.ip.build((int)i, (int)nodes, ii, lds_limit);
+    m.W = take(m.ip.w_elems() * 16);
+    m.piv = take(m.ip.small ? 0 : nodes * i * sizeof(int));
+    m.total = off;
+    return m;
+}
+
 int Tree::factor_device(const double2* k) {
     if (!planned) return fail("Error: rs_plan must be called before rs_factor.");
     if (ensure_device()) return 1;
     const double t0 = now_s();
-    // A refactor of the same plan overwrites the factor data in place, so the
-    // solve workspaces (whose product descriptors point into it) stay valid.
+    // A refactor of the same plan overwrites the factor data in place, with the
+    // merge scratch and the product descriptors kept from the first factor: no
+    // allocation, no host synchronisation until the end, and the solve
+    // workspaces (whose descriptors point into the factor data) stay valid.
     factored = false;
     have32 = false;
     const Plan& P = plan;
     const int L = P.L, nl = P.n_leaves;
+    bool moved = false;
+    auto need = [&](DevBuf& b, size_t bytes) -> int {
+        if (b.p && b.bytes == bytes) return 0;
+        moved = true;
+        return alloc(b, bytes);
+    };
     // keep the potential resident (the FP64 residual of the mixed path reads it)
-    if (ensure_buf(kint, (size_t)nl * P.n_int * sizeof(double2))) return 1;
-    HIPCHK(hipMemcpyAsync(kint.p, k, (size_t)nl * P.n_int * sizeof(double2), hipMemcpyDeviceToDevice, stream));
-    if (ensure_buf(sinv, (size_t)nl * G.K * G.s * G.s * sizeof(double2)) || ensure_buf(info, sizeof(int))) return 1;
-    HIPCHK(hipMemsetAsync(info.p, 0, sizeof(int), stream));
-    launch_leaf_blocks(G, kint.as<double2>(), sinv.as<double2>(), info.as<int>(), nl, stream);
+    if (need(kint, (size_t)nl * P.n_int * sizeof(double2)) ||
+        need(sinv, (size_t)nl * G.K * G.s * G.s * sizeof(double2)) || need(info, sizeof(int)))
+        return 1;
     T.resize(L + 1, nullptr);
     invM.resize(L, nullptr);
     if (!T[L]) T[L] = new DevBuf;
-    if (ensure_buf(*T[L], (size_t)nl * P.nb * P.nb * sizeof(double2))) return 1;
+    if (need(*T[L], (size_t)nl * P.nb * P.nb * sizeof(double2))) return 1;
+    std::vector<MergeLayout> ml(L);
+    size_t arena = 0;
+    for (int l = L - 1; l >= 0; --l) {
+        const Level& V = P.lv[l];
+        ml[l] = merge_layout(V, lds_limit);
+        arena = std::max(arena, ml[l].total);
+        if (!invM[l]) invM[l] = new DevBuf;
+        if (need(*invM[l], (size_t)V.nodes * 4 * V.isz * V.isz * 16)) return 1;
+        if (l > 0 || !opts.skip_root_rtr) {
+            if (!T[l]) T[l] = new DevBuf;
+            if (need(*T[l], (size_t)V.nodes * V.P * V.P * 16)) return 1;
+        }
+    }
+    if (need(fscratch, arena ? arena : 256)) return 1;
+    char* ar = fscratch.as<char>();
+
+    const double2 zeta = make_double2(P.zeta[0], P.zeta[1]);
+    const double2 nu = make_double2(P.nu[0], P.nu[1]);
+    const double2 nu2 = cmul(nu, nu);
+    // product descriptors of every level (kept for refactors)
+    if (moved || (int)fdesc.size() != 6 * L) {
+        for (DevBuf* b : fdesc) delete b;
+        fdesc.assign(6 * L, nullptr);
+        for (DevBuf*& b : fdesc) b = new DevBuf;
+        for (int l = L - 1; l >= 0; --l) {
+            const Level& V = P.lv[l];
+            const MergeLayout& m = ml[l];
+            const int nodes = V.nodes, i = V.isz, b = V.bc, bn = V.bn, Pp = V.P;
+            const size_t ii = (size_t)i * i, tstride = (size_t)b * b;
+            const double2* Tc = T[l + 1]->as<double2>();
+            double2* Dm = (double2*)(ar + m.D);
+            double2* Sm = (double2*)(ar + m.S);
+            double2* SD = (double2*)(ar + m.SD);
+            double2* Y = (double2*)(ar + m.Y);
+            double2* W = (double2*)(ar + m.W);
+            double2* IM = invM[l]->as<double2>();
+            DevBuf** dl = &fdesc[6 * l];   // inv D, inv S, S D / B21 / B22, Y, parent T, GJ column maps
+            if (!m.ip.small) {
+                const int n = i, nb_ = m.ip.nb;
+                std::vector<int> colmaps((size_t)m.ip.panels() * (n - nb_));
+                for (int pi = 0; pi < m.ip.panels(); ++pi) {
+                    const int j0 = pi * nb_;
+                    for (int c2 = 0; c2 < n - nb_; ++c2) colmaps[(size_t)pi * (n - nb_) + c2] = c2 < j0 ? c2 : c2 + nb_;
+                }
+                if (upload(*dl[5], colmaps)) return 1;
+                for (int which = 0; which < 2; ++which) {
+                    double2* mats = which == 0 ? Dm : Sm;
+                    std::vector<GemmDesc> v;
+                    for (int pi = 0; pi < m.ip.panels(); ++pi)
+                        for (int bt = 0; bt < nodes; ++bt) {
+                            GemmDesc g = desc0();
+                            double2* Mb = mats + (size_t)bt * ii;
+                            g.A = Mb + (size_t)pi * nb_ * n; g.lda = n;
+                            g.B = W + (size_t)bt * nb_ * (n - nb_); g.ldb = nb_;
+                            g.Out = Mb; g.ldo = n;
+                            g.cmap = dl[5]->as<int>() + (size_t)pi * (n - nb_);
+                            g.Din = Mb; g.ldd = n; g.dmap = 1; g.br = 1.0;
+                            v.push_back(g);
+                        }
+                    if (upload(*dl[which], v)) return 1;
+                }
+            }
+            {
+                std::vector<GemmDesc> v;
+                for (int n = 0; n < nodes; ++n) {  // SD = S D
+                    GemmDesc g = desc0();
+                    g.A = Sm + n * ii; g.lda = i;
+                    g.B = Dm + n * ii; g.ldb = i;
+                    g.Out = SD + n * ii; g.ldo = i;
+                    v.push_back(g);
+                }
+                for (int n = 0; n < nodes; ++n) {  // B21 = -nu D S
+                    GemmDesc g = desc0();
+                    g.A = Dm + n * ii; g.lda = i;
+                    g.B = Sm + n * ii; g.ldb = i;
+                    g.Out = IM + n * 4 * ii + i; g.ldo = 2 * i;
+                    g.ar = -nu.x; g.ai = -nu.y;
+                    v.push_back(g);
+                }
+                for (int n = 0; n < nodes; ++n) {  // B22 = D + nu^2 D (S D)
+                    GemmDesc g = desc0();
+                    g.A = Dm + n * ii; g.lda = i;
+                    g.B = SD + n * ii; g.ldb = i;
+                    g.Out = IM + n * 4 * ii + i + (size_t)i * 2 * i; g.ldo = 2 * i;
+                    g.ar = nu2.x; g.ai = nu2.y;
+                    g.Din = Dm + n * ii; g.ldd = i; g.br = 1.0;
+                    v.push_back(g);
+                }
+                if (upload(*dl[2], v)) return 1;
+            }
+            {  // Y = invM diag(T1_IN, T2_IN)
+                std::vector<GemmDesc> v;
+                for (int n = 0; n < nodes; ++n)
+                    for (int c = 0; c < 2; ++c) {
+                        GemmDesc g = desc0();
+                        g.A = IM + n * 4 * ii + (size_t)c * i * 2 * i; g.lda = 2 * i;
+                        g.B = Tc + (size_t)(2 * n + c) * tstride + bn; g.ldb = b;
+                        g.Out = Y + (size_t)n * 2 * i * 2 * bn + (size_t)c * bn * 2 * i; g.ldo = 2 * i;
+                        v.push_back(g);
+                    }
+                if (upload(*dl[3], v)) return 1;
+            }
+            if (l > 0 || !opts.skip_root_rtr) {  // parent RtR, scattered into the parent's node order
+                std::vector<GemmDesc> v;
+                const int* prow = d_prow[l]->as<int>();
+                for (int n = 0; n < nodes; ++n) {
+                    const int par = (l == 0) ? 0 : (n & 1);
+                    for (int c = 0; c < 2; ++c)
+                        for (int c2 = 0; c2 < 2; ++c2) {
+                            GemmDesc g = desc0();
+                            const double2* Tcn = Tc + (size_t)(2 * n + c) * tstride;
+                            g.A = Tcn + (size_t)bn * b; g.lda = b;
+                            g.B = Y + (size_t)n * 2 * i * 2 * bn + (size_t)c * i + (size_t)c2 * bn * 2 * i;
+                            g.ldb = 2 * i;
+                            g.Out = T[l]->as<double2>() + (size_t)n * Pp * Pp; g.ldo = Pp;
+                            g.rmap = prow + (size_t)(par * 2 + c) * bn;
+                            g.cmap = prow + (size_t)(par * 2 + c2) * bn;
+                            if (c == c2) { g.Din = Tcn; g.ldd = b; g.br = 1.0; }
+                            v.push_back(g);
+                        }
+                }
+                if (upload(*dl[4], v)) return 1;
+            }
+        }
+    }
+
+    // leaves: block inverses and RtR maps
+    HIPCHK(hipMemcpyAsync(kint.p, k, (size_t)nl * P.n_int * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+    HIPCHK(hipMemsetAsync(info.p, 0, sizeof(int), stream));
+    launch_leaf_blocks(G, kint.as<double2>(), sinv.as<double2>(), info.as<int>(), nl, stream);
     {
         LeafArgs<double> a;
         a.mode = LEAF_FACTOR_T; a.ncols = P.nb; a.n_leaves = nl;
@@ -206,153 +372,42 @@ int Tree::factor_device(const double2* k) {
         launch_leaf_apply<double>(G, a, lds_limit, stream);
     }
     HIPCHK(hipGetLastError());
-    const double2 zeta = make_double2(P.zeta[0], P.zeta[1]);
-    const double2 nu = make_double2(P.nu[0], P.nu[1]);
-    const double2 nu2 = cmul(nu, nu);
+    // merges, bottom-up; every level of one launch sequence on the tree's stream
     for (int l = L - 1; l >= 0; --l) {
         const Level& V = P.lv[l];
-        const int nodes = V.nodes, i = V.isz, b = V.bc, bn = V.bn, Pp = V.P;
+        const MergeLayout& m = ml[l];
+        const int nodes = V.nodes, i = V.isz, b = V.bc, bn = V.bn;
         const size_t ii = (size_t)i * i, tstride = (size_t)b * b;
         const double2* Tc = T[l + 1]->as<double2>();
-        DevBuf Dm, Sm, SD, Y, W, piv, dd_inv, dd_sd, dd_y, dd_t;
-        if (alloc(Dm, nodes * ii * 16) || alloc(Sm, nodes * ii * 16) || alloc(SD, nodes * ii * 16) ||
-            alloc(Y, (size_t)nodes * 2 * i * 2 * bn * 16))
-            return 1;
-        // K-split workspace for this level's products
-        size_t wse = 0;
-        wse = std::max(wse, gemm_ws_elems(nodes, i, i, i));
-        wse = std::max(wse, gemm_ws_elems(2 * nodes, i, i, i));
-        wse = std::max(wse, gemm_ws_elems(2 * nodes, 2 * i, bn, i));
-        wse = std::max(wse, gemm_ws_elems(4 * nodes, bn, bn, i));
-        DevBuf gws;
-        if (wse && alloc(gws, wse * sizeof(double2))) return 1;
-        InvPlan ip;
-        ip.build(i, nodes, ii, lds_limit);
-        std::vector<GemmDesc> inv_descs;
-        std::vector<int> colmaps;
-        if (!ip.small) {
-            if (alloc(W, ip.w_elems() * 16) || alloc(piv, (size_t)nodes * i * sizeof(int))) return 1;
-            colmaps.resize((size_t)ip.panels() * (i - ip.nb));
-        }
-        // the GJ panel descriptors differ between the D and S passes only in
-        // the matrix base pointer: build for Dm, patch for Sm
-        auto build_inv = [&](double2* mats, DevBuf& dbuf, DevBuf& cmbuf) -> int {
-            if (ip.small) return 0;
-            inv_descs.clear();
-            const int n = i, nb_ = ip.nb;
-            for (int pi = 0; pi < ip.panels(); ++pi) {
-                int j0 = pi * nb_;
-                for (int c2 = 0; c2 < n - nb_; ++c2) colmaps[(size_t)pi * (n - nb_) + c2] = c2 < j0 ? c2 : c2 + nb_;
-            }
-            if (!cmbuf.p && upload(cmbuf, colmaps)) return 1;
-            for (int pi = 0; pi < ip.panels(); ++pi)
-                for (int bt = 0; bt < nodes; ++bt) {
-                    GemmDesc g = desc0();
-                    double2* Mb = mats + (size_t)bt * ii;
-                    g.A = Mb + (size_t)pi * nb_ * n; g.lda = n;
-                    g.B = W.as<double2>() + (size_t)bt * nb_ * (n - nb_); g.ldb = nb_;
-                    g.Out = Mb; g.ldo = n;
-                    g.cmap = cmbuf.as<int>() + (size_t)pi * (n - nb_);
-                    g.Din = Mb; g.ldd = n; g.dmap = 1; g.br = 1.0;
-                    inv_descs.push_back(g);
-                }
-            return upload(dbuf, inv_descs);
-        };
-        DevBuf cmaps, dd_inv2;
-        // D
-        launch_form_iface(Dm.as<double2>(), Tc, tstride, 2, 1, b, bn, i, nodes, zeta, make_double2(0, 0), nullptr,
-                          stream);
-        if (build_inv(Dm.as<double2>(), dd_inv, cmaps)) return 1;
-        launch_inverse(ip, Dm.as<double2>(), piv.as<int>(), W.as<double2>(), dd_inv.as<GemmDesc>(), info.as<int>(),
-                       stream);
-        // S
-        launch_form_iface(Sm.as<double2>(), Tc, tstride, 2, 0, b, bn, i, nodes, zeta,
-                          make_double2(-nu2.x, -nu2.y), Dm.as<double2>(), stream);
-        if (build_inv(Sm.as<double2>(), dd_inv2, cmaps)) return 1;
-        launch_inverse(ip, Sm.as<double2>(), piv.as<int>(), W.as<double2>(), dd_inv2.as<GemmDesc>(), info.as<int>(),
-                       stream);
-        // invM
-        if (!invM[l]) invM[l] = new DevBuf;
-        if (ensure_buf(*invM[l], nodes * 4 * ii * 16)) return 1;
+        double2* Dm = (double2*)(ar + m.D);
+        double2* Sm = (double2*)(ar + m.S);
+        double2* SD = (double2*)(ar + m.SD);
+        double2* W = (double2*)(ar + m.W);
+        int* piv = (int*)(ar + m.piv);
+        void* gws = ar + m.ws;
         double2* IM = invM[l]->as<double2>();
-        {
-            std::vector<GemmDesc> v;
-            for (int n = 0; n < nodes; ++n) {  // SD = S D
-                GemmDesc g = desc0();
-                g.A = Sm.as<double2>() + n * ii; g.lda = i;
-                g.B = Dm.as<double2>() + n * ii; g.ldb = i;
-                g.Out = SD.as<double2>() + n * ii; g.ldo = i;
-                v.push_back(g);
-            }
-            for (int n = 0; n < nodes; ++n) {  // B21 = -nu D S
-                GemmDesc g = desc0();
-                g.A = Dm.as<double2>() + n * ii; g.lda = i;
-                g.B = Sm.as<double2>() + n * ii; g.ldb = i;
-                g.Out = IM + n * 4 * ii + i; g.ldo = 2 * i;
-                g.ar = -nu.x; g.ai = -nu.y;
-                v.push_back(g);
-            }
-            for (int n = 0; n < nodes; ++n) {  // B22 = D + nu^2 D (S D)
-                GemmDesc g = desc0();
-                g.A = Dm.as<double2>() + n * ii; g.lda = i;
-                g.B = SD.as<double2>() + n * ii; g.ldb = i;
-                g.Out = IM + n * 4 * ii + i + (size_t)i * 2 * i; g.ldo = 2 * i;
-                g.ar = nu2.x; g.ai = nu2.y;
-                g.Din = Dm.as<double2>() + n * ii; g.ldd = i; g.br = 1.0;
-                v.push_back(g);
-            }
-            if (upload(dd_sd, v)) return 1;
-            launch_gemm<double>(dd_sd.as<GemmDesc>(), nodes, i, i, i, stream, gws.p);
-            launch_gemm<double>(dd_sd.as<GemmDesc>() + nodes, 2 * nodes, i, i, i, stream, gws.p);
-            launch_copy_blocks(IM, 4 * ii, 2 * i, Sm.as<double2>(), ii, i, i, i, nodes, make_double2(1, 0), stream);
-            launch_copy_blocks(IM + (size_t)i * 2 * i, 4 * ii, 2 * i, SD.as<double2>(), ii, i, i, i, nodes,
-                               make_double2(-nu.x, -nu.y), stream);
-        }
-        // Y = invM diag(T1_IN, T2_IN)
-        {
-            std::vector<GemmDesc> v;
-            for (int n = 0; n < nodes; ++n)
-                for (int c = 0; c < 2; ++c) {
-                    GemmDesc g = desc0();
-                    g.A = IM + n * 4 * ii + (size_t)c * i * 2 * i; g.lda = 2 * i;
-                    g.B = Tc + (size_t)(2 * n + c) * tstride + bn; g.ldb = b;
-                    g.Out = Y.as<double2>() + (size_t)n * 2 * i * 2 * bn + (size_t)c * bn * 2 * i; g.ldo = 2 * i;
-                    v.push_back(g);
-                }
-            if (upload(dd_y, v)) return 1;
-            launch_gemm<double>(dd_y.as<GemmDesc>(), 2 * nodes, 2 * i, bn, i, stream, gws.p);
-        }
-        // parent RtR
-        if (l > 0 || !opts.skip_root_rtr) {
-            if (!T[l]) T[l] = new DevBuf;
-            if (ensure_buf(*T[l], (size_t)nodes * Pp * Pp * 16)) return 1;
-            std::vector<GemmDesc> v;
-            const int* prow = d_prow[l]->as<int>();
-            for (int n = 0; n < nodes; ++n) {
-                int par = (l == 0) ? 0 : (n & 1);
-                for (int c = 0; c < 2; ++c)
-                    for (int c2 = 0; c2 < 2; ++c2) {
-                        GemmDesc g = desc0();
-                        const double2* Tcn = Tc + (size_t)(2 * n + c) * tstride;
-                        g.A = Tcn + (size_t)bn * b; g.lda = b;
-                        g.B = Y.as<double2>() + (size_t)n * 2 * i * 2 * bn + (size_t)c * i + (size_t)c2 * bn * 2 * i;
-                        g.ldb = 2 * i;
-                        g.Out = T[l]->as<double2>() + (size_t)n * Pp * Pp; g.ldo = Pp;
-                        g.rmap = prow + (size_t)(par * 2 + c) * bn;
-                        g.cmap = prow + (size_t)(par * 2 + c2) * bn;
-                        if (c == c2) { g.Din = Tcn; g.ldd = b; g.br = 1.0; }
-                        v.push_back(g);
-                    }
-            }
-            if (upload(dd_t, v)) return 1;
-            launch_gemm<double>(dd_t.as<GemmDesc>(), 4 * nodes, bn, bn, i, stream, gws.p);
-        }
+        DevBuf** dl = &fdesc[6 * l];
+        // D = (zeta I - T2_II)^-1
+        launch_form_iface(Dm, Tc, tstride, 2, 1, b, bn, i, nodes, zeta, make_double2(0, 0), nullptr, stream);
+        launch_inverse(m.ip, Dm, piv, W, dl[0]->as<GemmDesc>(), info.as<int>(), stream);
+        // S = (zeta I - T1_II - nu^2 D)^-1
+        launch_form_iface(Sm, Tc, tstride, 2, 0, b, bn, i, nodes, zeta, make_double2(-nu2.x, -nu2.y), Dm, stream);
+        launch_inverse(m.ip, Sm, piv, W, dl[1]->as<GemmDesc>(), info.as<int>(), stream);
+        // invM = [S, -nu S D; -nu D S, D + nu^2 D S D]
+        launch_gemm<double>(dl[2]->as<GemmDesc>(), nodes, i, i, i, stream, gws);
+        launch_gemm<double>(dl[2]->as<GemmDesc>() + nodes, 2 * nodes, i, i, i, stream, gws);
+        launch_copy_blocks(IM, 4 * ii, 2 * i, Sm, ii, i, i, i, nodes, make_double2(1, 0), stream);
+        launch_copy_blocks(IM + (size_t)i * 2 * i, 4 * ii, 2 * i, SD, ii, i, i, i, nodes, make_double2(-nu.x, -nu.y),
+                           stream);
+        // Y = invM diag(T1_IN, T2_IN); parent T = diag(T_NN) + diag(T_NI) Y
+        launch_gemm<double>(dl[3]->as<GemmDesc>(), 2 * nodes, 2 * i, bn, i, stream, gws);
+        if (l > 0 || !opts.skip_root_rtr)
+            launch_gemm<double>(dl[4]->as<GemmDesc>(), 4 * nodes, bn, bn, i, stream, gws);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(stream));
     }
-    HIPCHK(hipStreamSynchronize(stream));
     int hinfo = 0;
-    HIPCHK(hipMemcpy(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
     if (hinfo & 1) return fail("Error: singular leaf block in the leaf factorization.");
     if (hinfo & 2) return fail("Error: singular interface matrix in the merge.");
     factored = true;
@@ -500,45 +555,40 @@ long long Tree::work_bytes(int cap, int prec) const {
 }
 
 void Tree::factor_bytes(long long* resident_out, long long* peak_out) const {
+    // mirrors factor_device: everything is allocated up front and kept, so the
+    // peak equals the resident total
     const Plan& P = plan;
     const int L = P.L;
     const long long nl = P.n_leaves;
     long long res = 0;
     for (int l = 0; l < L; ++l) res += (long long)(P.lv[l].prow.size() + P.lv[l].imap.size()) * 4;
     res += (long long)(P.root_leaf.size() + P.root_loc.size() + P.leaf_nbr.size()) * 4;
-    res += nl * P.n_int * 16 + nl * (long long)G.K * G.s * G.s * 16 + 4 + nl * (long long)P.nb * P.nb * 16;
-    long long peak = res, c64 = nl * (long long)G.K * G.s * G.s * 8;
-    c64 += nl * (long long)P.nb * P.nb * 8;
+    const long long leafblk = nl * (long long)G.K * G.s * G.s, tleaf = nl * (long long)P.nb * P.nb;
+    res += nl * P.n_int * 16 + leafblk * 16 + 4 + tleaf * 16;
+    long long c64 = (leafblk + tleaf) * 8;
     const size_t lds = device_ready ? lds_limit : 163840;
     const long long gd = (long long)sizeof(GemmDesc);
+    size_t arena = 0;
     for (int l = L - 1; l >= 0; --l) {
         const Level& V = P.lv[l];
-        const long long nodes = V.nodes, i = V.isz, bn = V.bn, Pp = V.P;
+        const long long nodes = V.nodes, i = V.isz, Pp = V.P;
         const long long ii = i * i;
+        const bool formT = l > 0 || !opts.skip_root_rtr;
         res += nodes * 4 * ii * 16;
         c64 += nodes * 4 * ii * 8;
-        if (l > 0 || !opts.skip_root_rtr) {
+        if (formT) {
             res += nodes * Pp * Pp * 16;
             if (l > 0) c64 += nodes * Pp * Pp * 8;
         }
-        size_t wse = 0;
-        wse = std::max(wse, gemm_ws_elems((int)nodes, (int)i, (int)i, (int)i));
-        wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, (int)i, (int)i, (int)i));
-        wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, 2 * (int)i, (int)bn, (int)i));
-        wse = std::max(wse, gemm_ws_elems(4 * (int)nodes, (int)bn, (int)bn, (int)i));
-        InvPlan ip;
-        ip.build((int)i, (int)nodes, (size_t)ii, lds);
-        long long tr = 3 * nodes * ii * 16 + nodes * 2 * i * 2 * bn * 16 + (long long)wse * 16;
-        if (!ip.small)
-            tr += (long long)ip.w_elems() * 16 + nodes * i * 4 + 2 * (long long)ip.panels() * nodes * gd +
-                  (long long)ip.panels() * (i - ip.nb) * 4;
-        tr += 3 * nodes * gd + 2 * nodes * gd + ((l > 0 || !opts.skip_root_rtr) ? 4 * nodes * gd : 0);
-        peak = std::max(peak, res + tr);
+        const MergeLayout m = merge_layout(V, lds);
+        arena = std::max(arena, m.total);
+        if (!m.ip.small) res += 2 * (long long)m.ip.panels() * nodes * gd + (long long)m.ip.panels() * (i - m.ip.nb) * 4;
+        res += 3 * nodes * gd + 2 * nodes * gd + (formT ? 4 * nodes * gd : 0);
     }
+    res += (long long)(arena ? arena : 256);
     if (opts.precision == RS_MIXED) res += c64;
-    peak = std::max(peak, res);
     if (resident_out) *resident_out = res;
-    if (peak_out) *peak_out = peak;
+    if (peak_out) *peak_out = res;
 }
 
 void Tree::pbeg(int cat, hipStream_t st) {

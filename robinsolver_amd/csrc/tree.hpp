@@ -97,6 +97,8 @@ struct Tree {
     DevBuf kint, sinv, info;
     std::vector<DevBuf*> T;      // per level: [nodes][P][P] node order
     std::vector<DevBuf*> invM;   // per level < L: [nodes][2i][2i]
+    DevBuf fscratch;              // merge scratch of the largest level (kept for refactors)
+    std::vector<DevBuf*> fdesc;   // per level: the merge's product descriptors (kept for refactors)
     // complex64 copies for the mixed solve
     DevBuf sinv32;
     std::vector<DevBuf*> T32, invM32;
