@@ -14,6 +14,9 @@
 // the pivot rows moved out (W), and the rest of the matrix gets the rank-nb
 // update A[:, R] += A[:, panel] W as a batched GEMM.  After the last panel the
 // interchanges are undone as column interchanges.
+#include <algorithm>
+#include <utility>
+
 #include "rs_common.hpp"
 #include "rs_kernels.hpp"
 
@@ -140,6 +143,125 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(double2* __restrict__ mat
     if (tid == 0 && bad) atomicOr(info, 2);
 }
 
+// Panel elimination with the panel held in registers: thread t owns rows
+// t, t + nt, ... (RPT of them) of the n x w panel (w <= NB columns).  One
+// Gauss-Jordan step is a wave-shuffle argmax over the candidate rows, one
+// barrier to combine the waves' results, the pivot row (scaled) and the old
+// row j published through LDS, a second barrier, and the rank-1 update of each
+// thread's own rows.  Same pivot choice as gj_panel_kernel.
+template <int NB, int RPT>
+struct PanelRegs {
+    double2 a[RPT][NB];
+};
+
+// Gauss-Jordan step JJ (a compile-time column, so the panel stays in registers)
+template <int JJ, int NB, int RPT>
+__device__ __forceinline__ void gj_reg_step(PanelRegs<NB, RPT>& P, int n, int j0, int w, int* piv, double* rv,
+                                            int* ri, double2* prow, double2* jrow, bool& bad) {
+    if (JJ >= w) return;   // w is uniform: every thread skips the same steps
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    const int j = j0 + JJ;
+    double v = -1.0;
+    int vi = n;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int r = tid + k * nt;
+        if (r >= j && r < n) {
+            const double x = cabs1(P.a[k][JJ]);
+            if (x > v) { v = x; vi = r; }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off);
+        const int oi = __shfl_xor(vi, off);
+        if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
+    }
+    if (lane == 0) { rv[wave] = v; ri[wave] = vi; }
+    __syncthreads();
+    double best = rv[0];
+    int p = ri[0];
+    for (int q = 1; q < nw; ++q)
+        if (rv[q] > best || (rv[q] == best && ri[q] < p)) { best = rv[q]; p = ri[q]; }
+    bad |= !(best > 0.0);
+    // publish old row j and the scaled pivot row
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int r = tid + k * nt;
+        if (r == j)
+#pragma unroll
+            for (int c = 0; c < NB; ++c) jrow[c] = P.a[k][c];
+        if (r == p) {
+            const double2 inv = cinv(P.a[k][JJ]);
+#pragma unroll
+            for (int c = 0; c < NB; ++c) prow[c] = (c == JJ) ? inv : cmul(P.a[k][c], inv);
+        }
+    }
+    if (tid == 0) piv[j] = p;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int r = tid + k * nt;
+        if (r == j) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) P.a[k][c] = prow[c];
+            continue;
+        }
+        if (r == p)
+#pragma unroll
+            for (int c = 0; c < NB; ++c) P.a[k][c] = jrow[c];
+        const double2 m = P.a[k][JJ];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const double2 pc = prow[c];
+            P.a[k][c] = csub((c == JJ) ? make_double2(0.0, 0.0) : P.a[k][c], cmul(m, pc));
+        }
+    }
+}
+
+template <int NB, int RPT, int... JJ>
+__device__ __forceinline__ void gj_reg_steps(std::integer_sequence<int, JJ...>, PanelRegs<NB, RPT>& P, int n,
+                                             int j0, int w, int* piv, double* rv, int* ri, double2* prow,
+                                             double2* jrow, bool& bad) {
+    (gj_reg_step<JJ, NB, RPT>(P, n, j0, w, piv, rv, ri, prow, jrow, bad), ...);
+}
+
+// Panel elimination with the panel held in registers: thread t owns rows
+// t, t + nt, ... (RPT of them) of the n x w panel (w <= NB columns).  One
+// Gauss-Jordan step is a wave-shuffle argmax over the candidate rows, one
+// barrier to combine the waves' results, the pivot row (scaled) and the old
+// row j published through LDS, a second barrier, and the rank-1 update of each
+// thread's own rows.  Same pivot choice as gj_panel_kernel.
+template <int NB, int RPT>
+__global__ void __launch_bounds__(1024) gj_panel_reg_kernel(double2* __restrict__ mats, int n, size_t stride,
+                                                            int j0, int w, int* __restrict__ pivs,
+                                                            int* __restrict__ info) {
+    __shared__ double rv[16];
+    __shared__ int ri[16];
+    __shared__ double2 prow[NB];   // new row j: the pivot row, scaled
+    __shared__ double2 jrow[NB];   // old row j (becomes row p)
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double2* G = mats + (size_t)blockIdx.x * stride + (size_t)j0 * n;
+    int* piv = pivs + (size_t)blockIdx.x * n;
+    PanelRegs<NB, RPT> P;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int r = tid + k * nt;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) P.a[k][c] = (r < n && c < w) ? G[r + (size_t)c * n] : make_double2(0.0, 0.0);
+    }
+    bool bad = false;
+    gj_reg_steps<NB, RPT>(std::make_integer_sequence<int, NB>{}, P, n, j0, w, piv, rv, ri, prow, jrow, bad);
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int r = tid + k * nt;
+        if (r < n)
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c < w) G[r + (size_t)c * n] = P.a[k][c];
+    }
+    if (bad && tid == 0) atomicOr(info, 2);
+}
+
 // Apply the panel's row interchanges to the columns outside the panel, move the
 // pivot rows of those columns to W (nb x (n-nb), compact) and zero them.
 __global__ void gj_swap_extract_kernel(double2* __restrict__ mats, int n, size_t stride, int j0, int nb,
@@ -172,9 +294,14 @@ __global__ void gj_unpermute_kernel(double2* __restrict__ mats, int n, size_t st
 
 size_t inv_small_lds(int n) { return (size_t)n * n * sizeof(double2) + (size_t)n * (sizeof(double2) + sizeof(int)); }
 
+// rows per thread of the register panel kernel (0: use the LDS panel kernel)
+static int gj_reg_rows(int n) { return n <= 1024 ? 1 : (n <= 2048 ? 2 : (n <= 4096 ? 4 : 0)); }
+
 int gj_panel_width(int n, size_t lds_limit) {
-    int nb = 16;
-    while (nb > 1 && ((size_t)n * nb + n) * sizeof(double2) + 1024 > lds_limit) nb >>= 1;
+    const int rpt = gj_reg_rows(n);
+    int nb = rpt ? 16 / rpt : 16;
+    if (!rpt)
+        while (nb > 1 && ((size_t)n * nb + n) * sizeof(double2) + 1024 > lds_limit) nb >>= 1;
     while (nb > 1 && n % nb) nb >>= 1;
     return nb;
 }
@@ -196,9 +323,21 @@ void launch_inverse(const InvPlan& ip, double2* mats, int* pivs, double2* W, con
     const int n = ip.n, nb = ip.nb;
     const size_t plds = ((size_t)n * nb + n) * sizeof(double2);
     const size_t wstride = (size_t)nb * (n - nb);
+    const int rpt = gj_reg_rows(n);
+    const int nt = rpt ? std::min(1024, ((n + rpt - 1) / rpt + 63) / 64 * 64) : 256;
     for (int j0 = 0, pi = 0; j0 < n; j0 += nb, ++pi) {
-        hipLaunchKernelGGL(gj_panel_kernel, dim3(ip.batch), dim3(256), plds, st, mats, n, ip.stride, j0, nb, pivs,
-                           info);
+        if (rpt == 1)
+            hipLaunchKernelGGL((gj_panel_reg_kernel<16, 1>), dim3(ip.batch), dim3(nt), 0, st, mats, n, ip.stride, j0,
+                               nb, pivs, info);
+        else if (rpt == 2)
+            hipLaunchKernelGGL((gj_panel_reg_kernel<8, 2>), dim3(ip.batch), dim3(nt), 0, st, mats, n, ip.stride, j0,
+                               nb, pivs, info);
+        else if (rpt == 4)
+            hipLaunchKernelGGL((gj_panel_reg_kernel<4, 4>), dim3(ip.batch), dim3(nt), 0, st, mats, n, ip.stride, j0,
+                               nb, pivs, info);
+        else
+            hipLaunchKernelGGL(gj_panel_kernel, dim3(ip.batch), dim3(256), plds, st, mats, n, ip.stride, j0, nb,
+                               pivs, info);
         hipLaunchKernelGGL(gj_swap_extract_kernel, dim3((n - nb + 255) / 256, ip.batch), dim3(256), 0, st, mats, n,
                            ip.stride, j0, nb, pivs, W, wstride);
         launch_gemm<double>(d_descs + (size_t)pi * ip.batch, ip.batch, n, n - nb, nb, st);

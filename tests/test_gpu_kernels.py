@@ -99,7 +99,8 @@ def test_gemm_split_and_scatter(prec, gpu):
     assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < tol
 
 
-@pytest.mark.parametrize("n,batch", [(16, 5), (48, 3), (64, 2), (128, 2), (200, 1), (512, 2)])
+@pytest.mark.parametrize("n,batch", [(16, 5), (48, 3), (64, 2), (128, 2), (200, 1), (512, 2), (1024, 2),
+                                     (1536, 1), (2048, 1)])
 def test_inverse_matches_numpy(n, batch, gpu):
     print("case: inverse", n, batch, flush=True)
     rng = np.random.default_rng(n)
@@ -114,3 +115,20 @@ def test_inverse_matches_numpy(n, batch, gpu):
         inv = flat[b * n * n:(b + 1) * n * n].reshape(n, n, order="F")
         err = np.linalg.norm(inv @ mats[b] - np.eye(n)) / np.sqrt(n)
         assert err < 1e-10, (b, err)
+
+
+@pytest.mark.parametrize("n", [40, 256])
+def test_inverse_needs_pivoting(n, gpu):
+    """A zero diagonal (a permutation plus a small perturbation): without row
+    interchanges the first pivot would be 0."""
+    print("case: inverse pivoting", n, flush=True)
+    rng = np.random.default_rng(n + 1)
+    perm = np.roll(np.arange(n), 1)
+    M = np.zeros((n, n), np.complex128)
+    M[np.arange(n), perm] = 1.0 + 0.5j
+    M += 1e-3 * _rc(rng, 1, n, n)[0]
+    np.fill_diagonal(M, 0.0)
+    flat = M.reshape(-1, order="F").copy()
+    assert _lib().rs_check_inverse(n, 1, _dp(flat.view(np.float64))) == 0
+    inv = flat.reshape(n, n, order="F")
+    assert np.linalg.norm(inv @ M - np.eye(n)) / np.sqrt(n) < 1e-10
