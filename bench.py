@@ -497,6 +497,30 @@ def main():
     roof["kernel"] = cat
     roof["avg_launch_ms"] = dur * 1e3
     roof["launches"] = nl
+    roof["algorithmic_bytes_per_launch"] = bpl
+    # HBM bytes per launch from the committed hardware-counter summary of this
+    # workload (FETCH_SIZE and WRITE_SIZE runs, scaled by the copy calibration)
+    tpath = os.path.join(ROOT, "profiles", "traffic_leaf_down_cfg3.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        if tr.get("category") == cat:
+            roof["traffic"] = tr["hbm_bytes_per_launch"]
+            roof["traffic_source"] = "profiles/traffic_leaf_down_cfg3.json"
+
+    # every category: time per step and achieved rates against its algorithmic counts
+    kernels = {}
+    for c, (cms, cn, cb, cf) in prof.items():
+        if cn == 0:
+            continue
+        kernels[c] = {"ms_per_step": cms / args.steps, "launches_per_step": cn / args.steps,
+                      "GB_per_s": cb / (cms / 1e3) / 1e9 if cms > 0 else 0.0,
+                      "TFLOP_per_s": cf / (cms / 1e3) / 1e12 if cms > 0 else 0.0}
+
+    # FD residual of the global system, FP64 and mixed results
+    res64 = t64.residual_norms(args.nrhs, f.data_ptr(), g.data_ptr(), u64.data_ptr())
+    res32 = t.residual_norms(args.nrhs, f.data_ptr(), g.data_ptr(), u32.data_ptr())
+    log("FD residual: fp64 max %.2e, mixed max %.2e" % (res64.max(), res32.max()))
 
     ff, fs = t.algorithmic_counts()
     out = {
@@ -526,6 +550,9 @@ def main():
         "reference_flops": {"factor_GF": ff / 1e9, "solve_GF_per_rhs": fs / 1e9},
         "roofline": roof,
         "kernel_ms_per_step": {c: v[0] / args.steps for c, v in prof.items()},
+        "kernels": kernels,
+        "fd_residual_fp64": {"max": float(res64.max()), "median": float(np.median(res64))},
+        "fd_residual_mixed": {"max": float(res32.max()), "median": float(np.median(res32))},
     }
     if not args.no_serve:
         log("serving ...")

@@ -71,6 +71,7 @@ def lib():
         llp = ctypes.POINTER(ctypes.c_longlong)
         L.rs_memory_estimate.argtypes = [vp, ctypes.c_int, llp, llp, llp]
         L.rs_memory_used.argtypes = [vp, llp, llp]
+        L.rs_residual_norms.argtypes = [vp, ctypes.c_int, vp, vp, vp, dp]
         from . import serve as _serve
         _serve.bind(L)
         _lib = L
@@ -173,6 +174,13 @@ class GpuTree:
         f, s = ctypes.c_double(), ctypes.c_double()
         lib().rs_timings(self.h, ctypes.byref(f), ctypes.byref(s))
         return f.value, s.value
+
+    def residual_norms(self, nrhs, f_ptr, g_ptr, u_ptr):
+        """||r|| / ||(f, g)|| of the global FD system for each of nrhs device-resident solutions."""
+        out = np.zeros(nrhs)
+        self._check(lib().rs_residual_norms(self.h, nrhs, ctypes.c_void_p(f_ptr), ctypes.c_void_p(g_ptr),
+                                            ctypes.c_void_p(u_ptr), _dp(out)))
+        return out
 
     def memory_estimate(self, ncols=1):
         """(factor resident bytes, factor peak bytes, solve workspace bytes of ncols RHS)."""

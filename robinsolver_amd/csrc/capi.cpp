@@ -127,6 +127,12 @@ int rs_inspect(const rs_tree* t, int* factored, int* nlvls, long long* device_by
     return 0;
 }
 
+int rs_residual_norms(rs_tree* t, int nrhs, const void* f_dev, const void* g_dev, const void* u_dev,
+                      double* rel) {
+    if (!t || !f_dev || !g_dev || !u_dev || !rel) return 1;
+    return t->t.residual_norms(nrhs, (const double2*)f_dev, (const double2*)g_dev, (const double2*)u_dev, rel);
+}
+
 int rs_memory_estimate(const rs_tree* t, int ncols, long long* factor_resident, long long* factor_peak,
                        long long* solve_workspace) {
     if (!t) return 1;
@@ -294,7 +300,7 @@ int rs_profile_ncat(void) { return rs::P_NCAT; }
 
 const char* rs_profile_name(int cat) {
     static const char* names[rs::P_NCAT] = {"leaf_up", "leaf_down", "gemm_up_g", "gemm_up_h",
-                                            "gemm_dn_t", "gemm_dn_g", "gather", "residual", "combine"};
+                                            "gemm_dn_t", "gemm_dn_g", "gather", "residual"};
     return (cat >= 0 && cat < rs::P_NCAT) ? names[cat] : "";
 }
 

@@ -250,6 +250,10 @@ template <typename R> struct LeafIO {
     c_t<R>* hout;             // UP: outgoing data h: [col][leaf][nb] node order
     double2* const* u64;      // DOWN: solution, complex128: column c at u64[c], [leaf][m] ...
     float2* u32;              //   ... or complex64 [col][leaf][m] (one set); written, never read
+    // complex128 solution written as a correction (refinement): u64 = base + x with
+    // base = ubase32 [col][leaf][m] (complex64) when given, else the current u64 when uacc
+    const float2* ubase32;
+    int uacc;
     double2* T;               // FACTOR: [leaf][nb][nb] node order, column-major
 };
 
@@ -263,8 +267,15 @@ __device__ inline V load_f(const LeafIO<R>& io, int col, int leaf, int ni, int q
 
 template <typename V, typename R>
 __device__ inline void store_u(const LeafIO<R>& io, int col, int leaf, int m, int p, V x) {
-    if (io.u64) io.u64[col][(size_t)leaf * m + p] = ccast<double2>(x);
-    else io.u32[((size_t)col * io.n_leaves + leaf) * m + p] = ccast<float2>(x);
+    const size_t i = (size_t)leaf * m + p;
+    if (io.u64) {
+        double2* u = io.u64[col] + i;
+        if (io.ubase32) *u = cadd(ccast<double2>(io.ubase32[(size_t)col * io.n_leaves * m + i]), ccast<double2>(x));
+        else if (io.uacc) *u = cadd(*u, ccast<double2>(x));
+        else *u = ccast<double2>(x);
+    } else {
+        io.u32[(size_t)col * io.n_leaves * m + i] = ccast<float2>(x);
+    }
 }
 
 
@@ -489,7 +500,30 @@ __device__ inline float2 pick4(float2 a0, float2 a1, float2 a2, float2 a3, int i
     return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
 }
 
-template <int K>
+// f element type of a leaf pass: the caller's complex128 data through the
+// column table, or the complex64 correction data (contiguous [col][leaf][n_int]).
+template <typename FT> struct FCol;
+template <> struct FCol<double2> {
+    static __device__ const double2* base(const LeafIO<float>& io, int col, int leaf, int ni, int) {
+        return io.f64[col] + (size_t)leaf * ni;
+    }
+};
+template <> struct FCol<float2> {
+    static __device__ const float2* base(const LeafIO<float>& io, int col, int leaf, int ni, int nl) {
+        return io.f32 + ((size_t)col * nl + leaf) * ni;
+    }
+};
+
+// Every global load in this kernel is unconditional (indices are clamped into
+// range and out-of-range lanes select zero afterwards), so that loads issued
+// ahead stay in flight: a load under a branch makes the compiler wait for
+// all outstanding loads where the branches join.
+// Output form of the down pass: the complex64 first iterate (contiguous), or
+// the complex128 solution through the column table, written plainly, as
+// u = base64 + x with a complex64 base, or as u += x.
+enum { OUT_U32 = 0, OUT_U64 = 1, OUT_U64_BASE32 = 2, OUT_U64_ACC = 3 };
+
+template <int K, int MODE, typename FT, int OUT>
 __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, LeafIO<float> io, int nchunks) {
     extern __shared__ float2 lds2[];
     const int s = G.s, ni = G.n_int, nb = G.nb, m = G.m, nl = io.n_leaves;
@@ -503,29 +537,64 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     const bool colok = mycol < io.ncols;
     const int parity = leaf & 1;
     const size_t lc = (size_t)(colok ? mycol : 0) * nl + leaf;
-    // this lane's column of f: complex128 through the column table, or complex64 correction data
-    const double2* f64p = (io.f64 && colok) ? io.f64[mycol] + (size_t)leaf * ni : nullptr;
-    const float2* f32p = io.f32 ? io.f32 + lc * ni : nullptr;
+    // Global loads and stores of interior data use a coalesced mapping: lane l
+    // handles row rr = l % 16 of the wave's columns cc(i) = 4 i + l / 16, so 16
+    // lanes cover 16 consecutive points of one column; tiles pass through the
+    // wave's LDS tile to and from the MFMA layout.
+    const int wcol0 = chunk * 64 + wave * 16;
+    const int rr = lane & 15, cq = lane >> 4;
+    const int rrc = rr < s ? rr : s - 1;
+    const FT* fcp[4];
+    bool fok[4];
+    float fmask[4];   // 1 or 0: lanes outside the leaf or the batch stage zeros (a multiply, not a branch)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int col = wcol0 + 4 * i + cq;
+        fok[i] = col < io.ncols && rr < s;
+        fmask[i] = fok[i] ? 1.f : 0.f;
+        fcp[i] = FCol<FT>::base(io, col < io.ncols ? col : 0, leaf, ni, nl);
+    }
 
-    const float2* sl = io.sinv + (size_t)leaf * K * s * s;
-    for (int e = tid; e < K * 256; e += 256) {
-        int k = e >> 8, r = e & 15, c = (e >> 4) & 15;
-        Bl[e] = (r < s && c < s) ? sl[(size_t)k * s * s + r + c * s] : make_float2(0.f, 0.f);
+    // the leaf's K block inverses into LDS (zero-padded to 16 x 16)
+    {
+        const float2* sl = io.sinv + (size_t)leaf * K * s * s;
+        const int r = tid & 15, c = tid >> 4;
+        const bool ok = r < s && c < s;
+        const int off = (r < s ? r : 0) + (c < s ? c : 0) * s;
+        float2 t[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) t[k] = sl[(size_t)k * s * s + off];
+#pragma unroll
+        for (int k = 0; k < K; ++k) Bl[k * 256 + tid] = ok ? t[k] : make_float2(0.f, 0.f);
     }
     // boundary data of the wave's 16 columns, read once with coalesced loads:
     // DOWN: incoming g (node order); UP: the transmission residual (local order)
     {
-        const int wcol0 = chunk * 64 + wave * 16;
-        for (int e = lane; e < 16 * nb; e += 64) {
-            const int c = e / nb, i = e - c * nb;
-            const int col = wcol0 + c;
-            float2 v = make_float2(0.f, 0.f);
-            if (col < io.ncols) {
+        const int ntot = 16 * nb;
+        if (MODE == LEAF_DOWN || io.jump) {
+            // stored in node order (g already is; the residual is in local order)
+            float2 t[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                int e = lane + 64 * j;
+                e = e < ntot ? e : ntot - 1;
+                const int c = e / nb, i = e - c * nb;
+                const int col = (wcol0 + c) < io.ncols ? wcol0 + c : 0;
                 const size_t gi = ((size_t)col * nl + leaf) * nb + i;
-                if (io.mode == LEAF_DOWN) v = io.gin[gi];
-                else if (io.jump) v = ccast<float2>(io.jump[gi]);
+                if (MODE == LEAF_DOWN) t[j] = io.gin[gi];
+                else t[j] = ccast<float2>(io.jump[gi]);
             }
-            bd[e] = v;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int e = lane + 64 * j;
+                if (e < ntot) {
+                    const int c = e / nb, i = e - c * nb;
+                    const int pos = (MODE == LEAF_DOWN) ? i : local_to_node(G, parity, i);
+                    bd[c * nb + pos] = ((wcol0 + c) < io.ncols) ? t[j] : make_float2(0.f, 0.f);
+                }
+            }
+        } else {
+            for (int e = lane; e < ntot; e += 64) bd[e] = make_float2(0.f, 0.f);
         }
     }
     __syncthreads();
@@ -557,34 +626,38 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
         buf ^= 1;
     };
 
-    // forward sweep; the interior data of layer k+1 is fetched while layer k is
-    // being processed
-    auto fetch = [&](int k, double2 (&raw)[4]) {
+    // raw[i] = f at row rr of layer k, column cc(i) (coalesced mapping; zero for
+    // lanes outside the leaf or the batch is applied when the tile is staged)
+    auto fetch = [&](int k, FT (&raw)[4]) {
 #pragma unroll
-        for (int vv = 0; vv < 4; ++vv) {
-            const int row = 4 * grp + vv;
-            double2 x = make_double2(0.0, 0.0);
-            if (row < s && colok) {
-                if (f64p) x = f64p[k * s + row];
-                else if (f32p) x = ccast<double2>(f32p[k * s + row]);
-            }
-            raw[vv] = x;
-        }
+        for (int i = 0; i < 4; ++i) raw[i] = fcp[i][k * s + rrc];
     };
     // the interior data runs PF layers ahead of the sweep (memory latency is
     // several layer steps long)
     constexpr int PF = 4;
-    double2 fbuf[PF][4];
+    FT fbuf[PF][4];
 #pragma unroll
     for (int k = 0; k < PF && k < K; ++k) fetch(k, fbuf[k]);
+    // forward sweep
 #pragma unroll
     for (int k = 0; k < K; ++k) {
+        // the layer's f tile into the wave's LDS tile, then read back in the MFMA layout
+        {
+            float2* sb = scr + buf * 256;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float2 x = ccast<float2>(fbuf[k % PF][i]);
+                sb[rr * 16 + 4 * i + cq] = make_float2(x.x * fmask[i], x.y * fmask[i]);
+            }
+            __syncthreads();
+        }
+        if (k + PF < K) fetch(k + PF, fbuf[k % PF]);
         float2 v[4];
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) {
             const int row = 4 * grp + vv;
-            float2 r = ccast<float2>(fbuf[k % PF][vv]);
-            if (row < s && colok && io.mode == LEAF_DOWN) {
+            float2 r = scr[buf * 256 + row * 16 + ccol];
+            if (MODE == LEAF_DOWN && row < s && colok) {
                 if (row == 0) r = cfma(cbx, gval(1, k), r);
                 if (row == s - 1) r = cfma(cbx, gval(2, k), r);
                 if (k == 0) r = cfma(cby, gval(3, row), r);
@@ -593,7 +666,6 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
             if (k > 0) r = csub(r, cmul(ccast<float2>(slow_lambda(G, C, k)), prev[vv]));
             v[vv] = r;
         }
-        if (k + PF < K) fetch(k + PF, fbuf[k % PF]);
         f4v aR, aI;
         product(k, v, aR, aI);
 #pragma unroll
@@ -611,40 +683,59 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) y[k][vv] = csub(y[k][vv], make_float2(aR[vv], aI[vv]));
     }
-    if (!colok) return;
 
-    double2* u64p = io.u64 ? io.u64[mycol] + (size_t)leaf * m : nullptr;
-    float2* u32p = io.u32 ? io.u32 + lc * m : nullptr;
-    auto put = [&](int p, float2 x) {
-        if (u64p) u64p[p] = ccast<double2>(x);
-        else u32p[p] = x;
+    // store of solution point p of column col; base: the complex64 iterate for OUT_U64_BASE32
+    auto put_col = [&](int col, double2* u64c, int p, float2 x, float2 base) {
+        if (OUT == OUT_U32) io.u32[((size_t)col * nl + leaf) * m + p] = x;
+        else if (OUT == OUT_U64) u64c[p] = ccast<double2>(x);
+        else if (OUT == OUT_U64_BASE32) u64c[p] = cadd(ccast<double2>(base), ccast<double2>(x));
+        else u64c[p] = cadd(u64c[p], ccast<double2>(x));
     };
-    if (io.mode == LEAF_DOWN) {
+    if (MODE == LEAF_DOWN) {
+        // interior solution: each layer's tile through LDS, stored 16 points per column at a time
+        double2* ucp[4];
+        const float2* bcp[4];
 #pragma unroll
-        for (int k = 0; k < K; ++k)
+        for (int i = 0; i < 4; ++i) {
+            const int col = wcol0 + 4 * i + cq;
+            const int colc = col < io.ncols ? col : 0;
+            ucp[i] = (OUT != OUT_U32) ? io.u64[colc] + (size_t)leaf * m : nullptr;
+            bcp[i] = (OUT == OUT_U64_BASE32) ? io.ubase32 + ((size_t)colc * nl + leaf) * m : nullptr;
+        }
 #pragma unroll
-            for (int vv = 0; vv < 4; ++vv) {
-                const int row = 4 * grp + vv;
-                if (row < s) put(nb + k * s + row, y[k][vv]);
+        for (int k = 0; k < K; ++k) {
+            float2 base[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                base[i] = (OUT == OUT_U64_BASE32) ? bcp[i][nb + k * s + rrc] : make_float2(0.f, 0.f);
+            float2* sb = scr + buf * 256;
+#pragma unroll
+            for (int vv = 0; vv < 4; ++vv) sb[(4 * grp + vv) * 16 + ccol] = y[k][vv];
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int col = wcol0 + 4 * i + cq;
+                if (fok[i]) put_col(col, ucp[i], nb + k * s + rr, sb[rr * 16 + 4 * i + cq], base[i]);
             }
+            buf ^= 1;
+        }
     }
-    // boundary values: u_B = (g - a1 x1 - a2 x2) / dd, h = g + (sigma - alpha) u_B
+    // boundary values: u_B = (g - a1 x1 - a2 x2) / dd (down) or h = g + (sigma - alpha) u_B
+    // less the transmission residual (up), computed into the column's boundary slots of
+    // the LDS area (node order: each slot is read and written by one lane only) and
+    // stored afterwards with coalesced writes
     const float2 sma = make_float2(0.f, (float)(-2.0 * G.kb));
     auto emit = [&](int face, int t, float2 x1, float2 x2, const AxisCoef& a) {
-        const int b = G.foff[face] + t;
-        const float2 gb = (io.mode == LEAF_DOWN) ? gval(face, t) : make_float2(0.f, 0.f);
+        float2* slot = bd + ccol * nb + G.noff[parity][face] + t;
+        const float2 gb = (MODE == LEAF_DOWN) ? *slot : make_float2(0.f, 0.f);
         float2 num = csub(csub(gb, cscale(x1, (float)a.a1)), cscale(x2, (float)a.a2));
         float2 uB = cmul(num, ccast<float2>(cinv(a.dd)));
-        if (io.mode == LEAF_UP) {
-            float2 hb = cfma(sma, uB, gb);
-            hb = csub(hb, bd[ccol * nb + b]);
-            io.hout[lc * nb + G.noff[parity][face] + t] = hb;
-        } else {
-            put(b, uB);
-        }
+        if (MODE == LEAF_UP) *slot = csub(cfma(sma, uB, gb), *slot);
+        else *slot = uB;
     };
 #pragma unroll
     for (int k = 0; k < K; ++k) {
+        if (!colok) break;
         if (grp == 0) emit(1, k, y[k][0], y[k][1], ax0);
         if (grp == (s - 1) / 4) emit(2, k, pick4(y[k][0], y[k][1], y[k][2], y[k][3], (s - 1) & 3),
                                         pick4(y[k][0], y[k][1], y[k][2], y[k][3], (s - 2) & 3), ax0);
@@ -652,9 +743,39 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
 #pragma unroll
     for (int vv = 0; vv < 4; ++vv) {
         const int row = 4 * grp + vv;
-        if (row >= s) continue;
+        if (row >= s || !colok) continue;
         emit(3, row, y[0][vv], y[1][vv], ax1);
         emit(4, row, y[K - 1][vv], y[K - 2][vv], ax1);
+    }
+    __syncthreads();
+    // coalesced store of the wave's boundary values: up, h in node order; down, u_B
+    // in local point order
+    const int ntot = 16 * nb;
+    for (int e0 = 0; e0 < ntot; e0 += 256) {
+        float2 base[4];
+        int cidx[4], pidx[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int e = e0 + lane + 64 * j;
+            e = e < ntot ? e : ntot - 1;
+            cidx[j] = e / nb;
+            pidx[j] = e - cidx[j] * nb;
+            const int col = (wcol0 + cidx[j]) < io.ncols ? wcol0 + cidx[j] : 0;
+            base[j] = (MODE == LEAF_DOWN && OUT == OUT_U64_BASE32)
+                          ? io.ubase32[((size_t)col * nl + leaf) * m + pidx[j]] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int e = e0 + lane + 64 * j;
+            const int col = wcol0 + cidx[j];
+            if (e >= ntot || col >= io.ncols) continue;
+            if (MODE == LEAF_UP) {
+                io.hout[((size_t)col * nl + leaf) * nb + pidx[j]] = bd[cidx[j] * nb + pidx[j]];
+            } else {
+                const float2 x = bd[cidx[j] * nb + local_to_node(G, parity, pidx[j])];
+                put_col(col, (OUT != OUT_U32) ? io.u64[col] + (size_t)leaf * m : nullptr, pidx[j], x, base[j]);
+            }
+        }
     }
 }
 
@@ -672,6 +793,54 @@ void launch_residual(const LeafGeom& G, double2* const* u64, const float2* u32, 
     else
         hipLaunchKernelGGL(residual_kernel<float2>, grid, dim3(256), 0, st, G, C, nullptr, u32, kint, f, g, nbr,
                            nu, zeta, rf, rg, rj, n_leaves, ncols, cpb);
+}
+
+// Per-column norms of the residual and of the data: one workgroup per column,
+// a fixed element order and a fixed tree reduction (the same bits every call).
+// out[2 col] = |rf|^2 + |rg|^2 + |rj|^2, out[2 col + 1] = |f|^2 + |g|^2 over
+// the physical boundary points.
+__global__ void __launch_bounds__(256) fd_norms_kernel(LeafGeom G, const float2* __restrict__ rf,
+                                                       const double2* __restrict__ rg,
+                                                       const double2* __restrict__ rj,
+                                                       const double2* const* __restrict__ f,
+                                                       const double2* const* __restrict__ g,
+                                                       const int* __restrict__ nbr, int n_leaves,
+                                                       double* __restrict__ out) {
+    __shared__ double sr[256], sd[256];
+    const int col = blockIdx.x, tid = threadIdx.x;
+    const size_t ni = (size_t)n_leaves * G.n_int, nbt = (size_t)n_leaves * G.nb;
+    double r = 0.0, d = 0.0;
+    for (size_t e = tid; e < ni; e += 256) {
+        const float2 a = rf[(size_t)col * ni + e];
+        r += (double)a.x * a.x + (double)a.y * a.y;
+        const double2 b = f[col][e];
+        d += b.x * b.x + b.y * b.y;
+    }
+    for (size_t e = tid; e < nbt; e += 256) {
+        const double2 a = rg[(size_t)col * nbt + e], c = rj[(size_t)col * nbt + e];
+        r += a.x * a.x + a.y * a.y + c.x * c.x + c.y * c.y;
+        const int leaf = (int)(e / G.nb), p = (int)(e % G.nb);
+        int fc = 1;
+        while (fc < 2 * G.d && p >= G.foff[fc] + G.pf[fc]) ++fc;
+        if (nbr[(size_t)leaf * 7 + fc] < 0) {
+            const double2 b = g[col][e];
+            d += b.x * b.x + b.y * b.y;
+        }
+    }
+    sr[tid] = r;
+    sd[tid] = d;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) { sr[tid] += sr[tid + h]; sd[tid] += sd[tid + h]; }
+        __syncthreads();
+    }
+    if (tid == 0) { out[2 * col] = sr[0]; out[2 * col + 1] = sd[0]; }
+}
+
+void launch_fd_norms(const LeafGeom& G, const float2* rf, const double2* rg, const double2* rj,
+                     const double2* const* f, const double2* const* g, const int* nbr, int n_leaves, int ncols,
+                     double* out, hipStream_t st) {
+    hipLaunchKernelGGL(fd_norms_kernel, dim3(ncols), dim3(256), 0, st, G, rf, rg, rj, f, g, nbr, n_leaves, out);
 }
 
 size_t leaf_blocks_lds(const LeafGeom& G) {
@@ -695,8 +864,14 @@ template <typename R> int leaf_apply_chunk(const LeafGeom& G, int ncols, size_t 
 }
 
 // The matrix-core path takes 2D leaves of 12x12, 14x14 or 16x16 interior points.
-static bool mfma_leaf_ok(const LeafGeom& G, int mode) {
-    return G.d == 2 && mode != LEAF_FACTOR_T && G.s == G.n[0] && G.s <= 16 && (G.s % 2) == 0 &&
+static bool mfma_leaf_ok(const LeafGeom& G, const LeafArgs<float>& a) {
+    // forms with a matrix-core kernel: up with f in either precision; down with the
+    // caller's f into u (complex64 or complex128), or with complex64 f as a correction
+    const bool form = a.mode == LEAF_UP ? (a.f64 || a.f32)
+                    : a.mode == LEAF_DOWN ? ((a.f64 && !a.ubase32 && !a.uacc) ||
+                                             (a.f32 && a.u64 && (a.ubase32 || a.uacc)))
+                    : false;
+    return G.d == 2 && form && G.s == G.n[0] && G.s <= 16 && (G.s % 2) == 0 &&
            (G.K == 12 || G.K == 14 || G.K == 16);
 }
 
@@ -708,18 +883,31 @@ void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit
     io.CH = leaf_apply_chunk<R>(G, a.ncols, lds_limit);
     io.sinv = a.sinv; io.f64 = a.f64; io.f32 = a.f32; io.jump = a.jump; io.gin = a.gin; io.hout = a.hout;
     io.u64 = a.u64; io.u32 = a.u32; io.T = a.T;
+    io.ubase32 = a.ubase32; io.uacc = a.uacc;
     const int mode = a.mode, ncols = a.ncols, n_leaves = a.n_leaves;
     if constexpr (std::is_same<R, float>::value) {
-        if (mfma_leaf_ok(G, mode)) {
+        if (mfma_leaf_ok(G, a)) {
             const int nchunks = (ncols + 63) / 64;
             const size_t lds = ((size_t)G.K * 256 + 4 * 512 + 4 * 16 * (size_t)G.nb) * sizeof(float2);
             dim3 grid((unsigned)n_leaves * nchunks);
-            if (G.K == 16)
-                hipLaunchKernelGGL(leaf_mfma_kernel<16>, grid, dim3(256), lds, st, G, C, io, nchunks);
-            else if (G.K == 14)
-                hipLaunchKernelGGL(leaf_mfma_kernel<14>, grid, dim3(256), lds, st, G, C, io, nchunks);
-            else
-                hipLaunchKernelGGL(leaf_mfma_kernel<12>, grid, dim3(256), lds, st, G, C, io, nchunks);
+#define RS_LEAF_MFMA(KK)                                                                                        \
+    do {                                                                                                         \
+        auto L = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, G, C, io, nchunks); };     \
+        if (mode == LEAF_UP) {                                                                                   \
+            if (io.f32) L(leaf_mfma_kernel<KK, LEAF_UP, float2, OUT_U32>);                                       \
+            else L(leaf_mfma_kernel<KK, LEAF_UP, double2, OUT_U32>);                                             \
+        } else if (io.f32) {                                                                                     \
+            if (io.ubase32) L(leaf_mfma_kernel<KK, LEAF_DOWN, float2, OUT_U64_BASE32>);                          \
+            else L(leaf_mfma_kernel<KK, LEAF_DOWN, float2, OUT_U64_ACC>);                                        \
+        } else {                                                                                                 \
+            if (io.u64) L(leaf_mfma_kernel<KK, LEAF_DOWN, double2, OUT_U64>);                                    \
+            else L(leaf_mfma_kernel<KK, LEAF_DOWN, double2, OUT_U32>);                                           \
+        }                                                                                                        \
+    } while (0)
+            if (G.K == 16) RS_LEAF_MFMA(16);
+            else if (G.K == 14) RS_LEAF_MFMA(14);
+            else RS_LEAF_MFMA(12);
+#undef RS_LEAF_MFMA
             return;
         }
     }

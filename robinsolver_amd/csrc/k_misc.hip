@@ -76,22 +76,6 @@ __global__ void axpy_kernel(double2* __restrict__ y, const double2* __restrict__
     if (e < n) y[e] = cadd(y[e], x[e]);
 }
 
-// u[col] = base[col] + d[col]; blockIdx.y is the column, blocks stride over its n entries
-// (u and base64 may be the same table: no restrict on them)
-__global__ void combine_kernel(double2* const* u, double2* const* base64,
-                               const float2* __restrict__ base32, const float2* __restrict__ d, size_t n) {
-    const int col = blockIdx.y;
-    double2* uc = u[col];
-    const double2* b64 = base64 ? base64[col] : nullptr;
-    const float2* b32 = base32 ? base32 + (size_t)col * n : nullptr;
-    const float2* dc = d + (size_t)col * n;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
-        const double2 b = b64 ? b64[e] : ccast<double2>(b32[e]);
-        uc[e] = cadd(b, ccast<double2>(dc[e]));
-    }
-}
-
 __global__ void col_table_kernel(double2** __restrict__ tab, double2* base, size_t stride, int ncols) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < ncols) tab[c] = base + (size_t)c * stride;
@@ -137,14 +121,6 @@ void launch_zero(double2* p, size_t n, hipStream_t st) {
 
 void launch_axpy(double2* y, const double2* x, size_t n, hipStream_t st) {
     if (n) hipLaunchKernelGGL(axpy_kernel, dim3(nblk(n)), dim3(256), 0, st, y, x, n);
-}
-
-void launch_combine(double2* const* u, double2* const* base64, const float2* base32, const float2* d, size_t n,
-                    int ncols, hipStream_t st) {
-    // about 16k workgroups in all, each striding over one column
-    const unsigned per_col = (unsigned)std::max<size_t>(1, std::min<size_t>(nblk(n), (16384 + ncols - 1) / ncols));
-    if (n && ncols > 0)
-        hipLaunchKernelGGL(combine_kernel, dim3(per_col, ncols), dim3(256), 0, st, u, base64, base32, d, n);
 }
 
 void launch_col_table(double2** tab, double2* base, size_t stride, int ncols, hipStream_t st) {

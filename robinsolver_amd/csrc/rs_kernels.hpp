@@ -53,6 +53,8 @@ template <typename R> struct LeafArgs {
     double2* const* u64 = nullptr;
     float2* u32 = nullptr;
     double2* T = nullptr;
+    const float2* ubase32 = nullptr;   // u64 = ubase32 + x (complex64 base, contiguous)
+    int uacc = 0;                      // u64 += x
 };
 template <typename R>
 void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit, hipStream_t st);
@@ -61,6 +63,12 @@ void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit
 void launch_residual(const LeafGeom& G, double2* const* u64, const float2* u32, const double2* kint,
                      const double2* const* f, const double2* const* g, const int* nbr, double2 nu, double2 zeta,
                      float2* rf, double2* rg, double2* rj, int n_leaves, int ncols, hipStream_t st);
+
+// per-column squared norms of the residual (rf, rg, rj) and of the data (f, g on
+// the physical boundary): out[2c], out[2c+1]
+void launch_fd_norms(const LeafGeom& G, const float2* rf, const double2* rg, const double2* rj,
+                     const double2* const* f, const double2* const* g, const int* nbr, int n_leaves, int ncols,
+                     double* out, hipStream_t st);
 
 // batched inverse
 struct InvPlan {
@@ -89,11 +97,6 @@ void launch_child_gather(V* child, const V* parent, const int* prow, int bn, int
 void launch_to_c64(float2* dst, const double2* src, size_t n, hipStream_t st);
 void launch_zero(double2* p, size_t n, hipStream_t st);
 void launch_axpy(double2* y, const double2* x, size_t n, hipStream_t st);
-// u = base + d over ncols columns of n entries: u and base64 are column tables
-// (base64 may equal u), base32 and d are contiguous complex64 [col][n]; base is
-// base64 when given, else base32.
-void launch_combine(double2* const* u, double2* const* base64, const float2* base32, const float2* d, size_t n,
-                    int ncols, hipStream_t st);
 // tab[c] = base + c * stride for c < ncols (column table of a contiguous array)
 void launch_col_table(double2** tab, double2* base, size_t stride, int ncols, hipStream_t st);
 

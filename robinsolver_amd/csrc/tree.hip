@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 
 namespace rs {
@@ -55,7 +56,7 @@ template <typename R> void SolveWork<R>::clear() {
     for (DevBuf* b : descs) delete b;
     descs.clear();
     d_up_g.clear(); d_up_h.clear(); d_dn_t.clear(); d_dn_g.clear();
-    rf.release(); rg.release(); rj.release(); uA.release(); du.release(); ws.release(); tab_rg.release();
+    rf.release(); rg.release(); rj.release(); uA.release(); ws.release(); tab_rg.release();
     cap = 0;
 }
 
@@ -549,7 +550,7 @@ long long Tree::work_bytes(int cap, int prec) const {
     if (wse) b += (long long)(wse * e);
     const long long nl = P.n_leaves;
     if (prec == RS_MIXED && opts.n_refine > 0)
-        b += (long long)cap * nl * (P.n_int * 8 + P.nb * 32 + P.m * 16) + (long long)cap * (long long)sizeof(void*);
+        b += (long long)cap * nl * (P.n_int * 8 + P.nb * 32 + P.m * 8) + (long long)cap * (long long)sizeof(void*);
     b += 3LL * cap * (long long)sizeof(void*);
     return b;
 }
@@ -636,7 +637,7 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     const V* sv = std::is_same<R, double>::value ? (const V*)sinv.p : (const V*)sinv32.p;
     const double e = sizeof(V), nr = nrhs;
     const double ef = io.f64 ? 16.0 : (io.f32 ? 8.0 : 0.0);
-    const double eu = io.u64 ? 16.0 : 8.0;
+    const double eu = (io.u64 ? 16.0 : 8.0) + (io.ubase32 ? 8.0 : (io.uacc ? 16.0 : 0.0));
     // algorithmic traffic of a leaf pass: the block inverses once, plus the
     // per-RHS vectors (f in; h out / g in, u out, u base in)
     const double sinv_b = (double)nl * G.K * G.s * G.s * e;
@@ -674,7 +675,7 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     LeafArgs<R> dn;
     dn.mode = LEAF_DOWN; dn.ncols = nrhs; dn.n_leaves = nl; dn.sinv = sv;
     dn.f64 = io.f64; dn.f32 = io.f32; dn.gin = w.gF[L]->template as<V>();
-    dn.u64 = io.u64; dn.u32 = io.u32;
+    dn.u64 = io.u64; dn.u32 = io.u32; dn.ubase32 = io.ubase32; dn.uacc = io.uacc;
     pbeg(P_LEAF_DOWN, st);
     launch_leaf_apply<R>(G, dn, lds_limit, st);
     pend(sinv_b + nr * nl * (ef * P.n_int + e * P.nb + eu * P.m), leaf_fl, st);
@@ -732,16 +733,15 @@ int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, in
             if (residual(nrhs, it == 0 ? nullptr : cols.u, it == 0 ? uA : nullptr, cols.f, cols.g,
                          w32.rf.as<float2>(), w32.rg.as<double2>(), w32.rj.as<double2>(), st))
                 return 1;
+            // the correction pass writes u = u_i + K32^-1 r directly (its leaf stage adds
+            // the correction to the current iterate as it stores the solution)
             PassIO c;
             c.f32 = w32.rf.as<float2>(); c.jump = w32.rj.as<double2>();
             c.g = w32.tab_rg.as<const double2*>();
-            c.u32 = w32.du.as<float2>();
+            c.u64 = cols.u;
+            if (it == 0) c.ubase32 = uA;
+            else c.uacc = 1;
             if (solve_pass(w32, nrhs, c, st)) return 1;
-            const size_t n = nl * plan.m;
-            pbeg(P_COMBINE, st);
-            launch_combine(cols.u, it == 0 ? nullptr : cols.u, it == 0 ? uA : nullptr, w32.du.as<float2>(), n,
-                           nrhs, st);
-            pend((double)n * nrhs * (8.0 + (it == 0 ? 8.0 : 16.0) + 16.0), 0, st);
         }
         return 0;
     }
@@ -757,7 +757,7 @@ int Tree::ensure_refine(SolveWork<float>& w) {
     const size_t nl = plan.n_leaves, cap = w.cap;
     if (alloc(w.rf, cap * nl * plan.n_int * 8) || alloc(w.rg, cap * nl * plan.nb * 16) ||
         alloc(w.rj, cap * nl * plan.nb * 16) || alloc(w.uA, cap * nl * plan.m * 8) ||
-        alloc(w.du, cap * nl * plan.m * 8) || alloc(w.tab_rg, cap * sizeof(void*)))
+        alloc(w.tab_rg, cap * sizeof(void*)))
         return 1;
     launch_col_table(w.tab_rg.as<double2*>(), w.rg.as<double2>(), nl * plan.nb, (int)cap, stream);
     HIPCHK(hipStreamSynchronize(stream));
@@ -786,6 +786,34 @@ int Tree::residual(int nrhs, double2* const* u64, const float2* u32, const doubl
              16.0 * P.n_leaves * P.n_int,
          (double)nrhs * P.n_leaves * P.m * 8.0 * (2 * G.d + 1), st);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int Tree::residual_norms(int nrhs, const double2* f, const double2* g, const double2* u, double* out) {
+    if (!factored) return fail("Error: Robin Tree must be factored first.");
+    if (nrhs < 1) return fail("Error: nrhs must be positive.");
+    const Plan& P = plan;
+    const size_t nl = P.n_leaves;
+    DevBuf rf, rg, rj, tabs, nrm;
+    if (alloc(rf, nrhs * nl * P.n_int * 8) || alloc(rg, nrhs * nl * P.nb * 16) || alloc(rj, nrhs * nl * P.nb * 16) ||
+        alloc(tabs, (size_t)3 * nrhs * sizeof(void*)) || alloc(nrm, (size_t)2 * nrhs * sizeof(double)))
+        return 1;
+    double2** tab = tabs.as<double2*>();
+    launch_col_table(tab, const_cast<double2*>(f), nl * P.n_int, nrhs, stream);
+    launch_col_table(tab + nrhs, const_cast<double2*>(g), nl * P.nb, nrhs, stream);
+    launch_col_table(tab + 2 * nrhs, const_cast<double2*>(u), nl * P.m, nrhs, stream);
+    const bool was = prof_on;
+    prof_on = false;
+    const int rc = residual(nrhs, tab + 2 * nrhs, nullptr, tab, tab + nrhs, rf.as<float2>(), rg.as<double2>(),
+                            rj.as<double2>(), stream);
+    prof_on = was;
+    if (rc) return 1;
+    launch_fd_norms(G, rf.as<float2>(), rg.as<double2>(), rj.as<double2>(), tab, tab + nrhs, d_leaf_nbr.as<int>(),
+                    P.n_leaves, nrhs, nrm.as<double>(), stream);
+    std::vector<double> h((size_t)2 * nrhs);
+    HIPCHK(hipMemcpyAsync(h.data(), nrm.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int c = 0; c < nrhs; ++c) out[c] = h[2 * c + 1] > 0 ? std::sqrt(h[2 * c] / h[2 * c + 1]) : std::sqrt(h[2 * c]);
     return 0;
 }
 

@@ -35,7 +35,7 @@ template <typename R> struct SolveWork {
     int cap = 0;
     std::vector<DevBuf*> hV, gF, G, Z;   // per level
     DevBuf ws;                           // K-split GEMM workspace
-    DevBuf rf, rg, rj, uA, du;           // refinement: residuals (c64 / c128 / c128), first iterate, correction (c64)
+    DevBuf rf, rg, rj, uA;               // refinement: residuals (c64 / c128 / c128), first iterate (c64)
     DevBuf tab_rg;                       // column table of rg (the correction pass reads it as g)
     std::vector<DevBuf*> descs;          // per (level, kind)
     std::vector<GemmDesc*> d_up_g, d_up_h, d_dn_t, d_dn_g;
@@ -44,7 +44,7 @@ template <typename R> struct SolveWork {
 };
 
 enum ProfCat { P_LEAF_UP = 0, P_LEAF_DOWN, P_GEMM_UP_G, P_GEMM_UP_H, P_GEMM_DN_T, P_GEMM_DN_G, P_GATHER,
-               P_RESIDUAL, P_COMBINE, P_NCAT };
+               P_RESIDUAL, P_NCAT };
 
 struct ProfRec {
     int cat;
@@ -62,6 +62,8 @@ struct PassIO {
     const double2* const* g = nullptr;    // physical boundary data (complex128)
     double2* const* u64 = nullptr;        // solution out, complex128 ...
     float2* u32 = nullptr;                //   ... or complex64
+    const float2* ubase32 = nullptr;      // complex128 out as a correction: u64 = ubase32 + x ...
+    int uacc = 0;                         //   ... or u64 += x
 };
 
 // The right-hand sides of one solve as column tables in device memory:
@@ -155,6 +157,8 @@ struct Tree {
     long long work_bytes(int cap, int prec) const;
     void factor_bytes(long long* resident, long long* peak) const;
     int solve_host(int nrhs, const double* f, const double* g, int prec);
+    // ||r||_2 / ||(f, g)||_2 of the global FD system per right-hand side (device arrays in)
+    int residual_norms(int nrhs, const double2* f, const double2* g, const double2* u, double* out_host);
     int get_solution(int rhs, double* u) const;
     int node_rtr(int lvl, int idx, double* out, int* P) const;
 };
