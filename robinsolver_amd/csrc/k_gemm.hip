@@ -65,8 +65,13 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
     constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
     constexpr int TM = WM / 16, TN = WN / 16;
     constexpr int NA = BM * BK / 256, NB = BK * BN / 256;
-    __shared__ V As[BK][BM];
-    __shared__ V Bs[BK][BN];
+    // operand tiles; after the K loop the same memory holds the output tile
+    // (column stride BM + 1, so the 16 columns of one MFMA row group fall in
+    // different banks)
+    constexpr int SMEM = (BK * (BM + BN) > BN * (BM + 1)) ? BK * (BM + BN) : BN * (BM + 1);
+    __shared__ V smem[SMEM];
+    V (*As)[BM] = (V (*)[BM])smem;
+    V (*Bs)[BN] = (V (*)[BN])(smem + BK * BM);
 
     const int tiles = tilesM * tilesN;
     const int per = tiles * kslices;
@@ -149,35 +154,85 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
         }
     }
 
+    // Epilogue through LDS: the accumulators (MFMA layout: 16 columns per wave
+    // instruction) go to a column-major tile, and each thread then handles
+    // consecutive rows of one column, so stores (and the Din reads) are
+    // contiguous runs of a column.  The K loop ended with a barrier, so the
+    // operand tiles are free.
+    V* Ct = smem;   // [BN][BM + 1]
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = wm + i * 16 + out_row<R>(lane, v);
+                const int col = wn + j * 16 + (lane & 15);
+                Ct[col * (BM + 1) + row] = mk<R>(accR[i][j][v], accI[i][j][v]);
+            }
+    __syncthreads();
+    constexpr int NE = BM * BN / 256;
     if (kslices > 1) {
         V* W = ws + ((size_t)p * kslices + ks) * (size_t)M * N;
-        for (int i = 0; i < TM; ++i)
-            for (int j = 0; j < TN; ++j)
-                for (int v = 0; v < 4; ++v) {
-                    const int row = m0 + wm + i * 16 + out_row<R>(lane, v);
-                    const int col = n0 + wn + j * 16 + (lane & 15);
-                    if (row < M && col < N) W[(size_t)row + (size_t)col * M] = mk<R>(accR[i][j][v], accI[i][j][v]);
-                }
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e % BM, c = e / BM;
+            const int row = m0 + r, col = n0 + c;
+            if (row < M && col < N) W[(size_t)row + (size_t)col * M] = Ct[c * (BM + 1) + r];
+        }
         return;
     }
     V* Out = (V*)D.Out;
     const V* Din = (const V*)D.Din;
     const V alpha = mk<R>((R)D.ar, (R)D.ai), beta = mk<R>((R)D.br, (R)D.bi);
-    for (int i = 0; i < TM; ++i)
-        for (int j = 0; j < TN; ++j)
-            for (int v = 0; v < 4; ++v) {
-                int row = m0 + wm + i * 16 + out_row<R>(lane, v);
-                int col = n0 + wn + j * 16 + (lane & 15);
-                if (row >= M || col >= N) continue;
-                V val = cmul(alpha, mk<R>(accR[i][j][v], accI[i][j][v]));
-                int orow = D.rmap ? D.rmap[row] : row;
-                int ocol = D.cmap ? D.cmap[col] : col;
-                if (Din) {
-                    size_t di = D.dmap ? (size_t)orow + (size_t)ocol * D.ldd : (size_t)row + (size_t)col * D.ldd;
-                    val = cfma(beta, Din[di], val);
-                }
-                Out[(size_t)orow + (size_t)ocol * D.ldo] = val;
-            }
+    // all index-map and Din loads of the thread are issued back to back (branches
+    // only around whole loops), then the results are formed and stored
+    int rowc[NE], colc[NE], orow[NE], ocol[NE];
+    bool ok[NE];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+        const int e = tid + 256 * q;
+        const int row = m0 + e % BM, col = n0 + e / BM;
+        ok[q] = row < M && col < N;
+        rowc[q] = row < M ? row : M - 1;
+        colc[q] = col < N ? col : N - 1;
+    }
+    if (D.rmap) {
+#pragma unroll
+        for (int q = 0; q < NE; ++q) orow[q] = D.rmap[rowc[q]];
+    } else {
+#pragma unroll
+        for (int q = 0; q < NE; ++q) orow[q] = rowc[q];
+    }
+    if (D.cmap) {
+#pragma unroll
+        for (int q = 0; q < NE; ++q) ocol[q] = D.cmap[colc[q]];
+    } else {
+#pragma unroll
+        for (int q = 0; q < NE; ++q) ocol[q] = colc[q];
+    }
+    V val[NE];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+        const int e = tid + 256 * q;
+        val[q] = cmul(alpha, Ct[(e / BM) * (BM + 1) + e % BM]);
+    }
+    if (Din) {
+        V din[NE];
+        if (D.dmap) {
+#pragma unroll
+            for (int q = 0; q < NE; ++q) din[q] = Din[(size_t)orow[q] + (size_t)ocol[q] * D.ldd];
+        } else {
+#pragma unroll
+            for (int q = 0; q < NE; ++q) din[q] = Din[(size_t)rowc[q] + (size_t)colc[q] * D.ldd];
+        }
+#pragma unroll
+        for (int q = 0; q < NE; ++q) val[q] = cfma(beta, din[q], val[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < NE; ++q)
+        if (ok[q]) Out[(size_t)orow[q] + (size_t)ocol[q] * D.ldo] = val[q];
 }
 
 // Sum of the K slices in slice order, then the epilogue of gemm_kernel.
