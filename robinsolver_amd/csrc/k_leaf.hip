@@ -94,22 +94,32 @@ __device__ inline int face_point(const LeafGeom& G, int f, const int g[3]) {
 
 // Decompose a local boundary index into (face, first interior point p1, axis j).
 __device__ inline void boundary_info(const LeafGeom& G, int b, int& f, int& j, int& q1, int& step) {
+    // scalar coordinates and selects only (a runtime index into a small array or
+    // into G.n would put it in scratch memory)
     f = 1;
     while (f < 2 * G.d && b >= G.foff[f] + G.pf[f]) ++f;
-    int t = b - G.foff[f];
+    const int t = b - G.foff[f];
     j = (f - 1) >> 1;
-    int g[3] = {1, 1, 1};
+    const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
+    int g0 = 1, g1 = 1, g2 = 1;
     if (G.d == 2) {
-        g[1 - j] = t + 1;
+        if (j == 0) g1 = t + 1;
+        else g0 = t + 1;
     } else {
-        int a0 = (j == 0) ? 1 : 0;
-        int a1 = (j == 2) ? 1 : 2;
-        g[a0] = t % G.n[a0] + 1;
-        g[a1] = t / G.n[a0] + 1;
+        // tangential axes a0 < a1 of face axis j; t runs fastest along a0
+        const int na0 = (j == 0) ? n1 : n0;
+        const int v0 = t % na0 + 1, v1 = t / na0 + 1;
+        if (j == 0) { g1 = v0; g2 = v1; }
+        else if (j == 1) { g0 = v0; g2 = v1; }
+        else { g0 = v0; g1 = v1; }
     }
-    g[j] = (f & 1) ? 1 : G.n[j];
-    q1 = (g[0] - 1) + G.n[0] * ((g[1] - 1) + (G.d == 3 ? G.n[1] * (g[2] - 1) : 0));
-    step = ((f & 1) ? 1 : -1) * stride_of(G, j);
+    const int nj = (j == 0) ? n0 : (j == 1 ? n1 : n2);
+    const int gj = (f & 1) ? 1 : nj;
+    if (j == 0) g0 = gj;
+    else if (j == 1) g1 = gj;
+    else g2 = gj;
+    q1 = (g0 - 1) + n0 * ((g1 - 1) + (G.d == 3 ? n1 * (g2 - 1) : 0));
+    step = ((f & 1) ? 1 : -1) * (j == 0 ? 1 : (j == 1 ? n0 : n0 * n1));
 }
 
 __device__ inline int local_to_node(const LeafGeom& G, int parity, int b) {
@@ -245,7 +255,7 @@ template <typename R> struct LeafIO {
     const c_t<R>* sinv;       // [leaf][K][s*s]
     const double2* const* f64;  // interior data f, complex128: column c at f64[c], [leaf][n_int] ...
     const float2* f32;        //   ... or complex64 [col][leaf][n_int] (at most one set; none = 0)
-    const double2* jump;      // UP: subtracted from the outgoing data h: [col][leaf][nb] local order
+    const float2* jump;       // UP: subtracted from the outgoing data h: [col][leaf][nb] local order
     const c_t<R>* gin;        // DOWN: incoming data g: [col][leaf][nb] node order
     c_t<R>* hout;             // UP: outgoing data h: [col][leaf][nb] node order
     double2* const* u64;      // DOWN: solution, complex128: column c at u64[c], [leaf][m] ...
@@ -398,8 +408,9 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
 // M [g1; g2] = [h1; h2] (robin_tree_module.f90:425-467): it holds at every pair
 // of coincident leaf boundary points once the tree has solved.  A correction
 // solve takes (rf, rg) as (f, g) and subtracts rj from the leaf outgoing data.
-// column col of the current solution: complex128 through the column table, or
-// the complex64 first iterate [col][leaf][m]
+// Evaluated in FP64; stored in complex64, the precision the correction pass
+// computes in (rf [col][leaf][n_int], rg and rj [col][leaf][n_bnd]).  One
+// workgroup covers one leaf and cols_per_block columns, a thread one row.
 template <typename U>
 __device__ inline const U* ucolumn(double2* const* u64, const float2* u32, int col, int nl, int m);
 template <>
@@ -416,7 +427,7 @@ __global__ void residual_kernel(LeafGeom G, LeafCoef C, double2* const* __restri
                                 const float2* __restrict__ u32, const double2* __restrict__ kint,
                                 const double2* const* __restrict__ f, const double2* const* __restrict__ g,
                                 const int* __restrict__ nbr, double2 nu, double2 zeta, float2* __restrict__ rf,
-                                double2* __restrict__ rg, double2* __restrict__ rj, int n_leaves, int ncols,
+                                float2* __restrict__ rg, float2* __restrict__ rj, int n_leaves, int ncols,
                                 int cols_per_block) {
     // one block: one leaf, cols_per_block columns; threads run over (column, point)
     const int leaf = blockIdx.x % n_leaves;
@@ -445,15 +456,15 @@ __global__ void residual_kernel(LeafGeom G, LeafCoef C, double2* const* __restri
         const double2 gA = incoming(ul, p);
         const int nl2 = nbr[(size_t)leaf * 7 + fc];
         if (nl2 < 0) {
-            rg[lc * nb + p] = csub(g[col][(size_t)leaf * nb + p], gA);
-            rj[lc * nb + p] = make_double2(0.0, 0.0);
+            rg[lc * nb + p] = ccast<float2>(csub(g[col][(size_t)leaf * nb + p], gA));
+            rj[lc * nb + p] = make_float2(0.f, 0.f);
         } else {
             const int f2 = (fc & 1) ? fc + 1 : fc - 1;
             const int b2 = G.foff[f2] + (p - G.foff[fc]);
             const double2 gN = incoming(ucol + (size_t)nl2 * G.m, b2);
             const double2 hA = cfma(sma, uval(ul, p), gA);
-            rj[lc * nb + p] = csub(cadd(cmul(zeta, gA), cmul(nu, gN)), hA);
-            rg[lc * nb + p] = make_double2(0.0, 0.0);
+            rj[lc * nb + p] = ccast<float2>(csub(cadd(cmul(zeta, gA), cmul(nu, gN)), hA));
+            rg[lc * nb + p] = make_float2(0.f, 0.f);
         }
         continue;
     }
@@ -461,7 +472,9 @@ __global__ void residual_kernel(LeafGeom G, LeafCoef C, double2* const* __restri
     int gq[3];
     interior_coords(G, q, gq);
     double2 Au = cmul(kint[(size_t)leaf * ni + q], uval(ul, p));
-    for (int j = 0; j < G.d; ++j) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        if (j >= G.d) break;
         const AxisCoef& a = C.ax[j];
         const double ih2 = a.mid;
         const int st = stride_of(G, j);
@@ -783,16 +796,16 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
 
 void launch_residual(const LeafGeom& G, double2* const* u64, const float2* u32, const double2* kint,
                      const double2* const* f, const double2* const* g, const int* nbr, double2 nu, double2 zeta,
-                     float2* rf, double2* rg, double2* rj, int n_leaves, int ncols, hipStream_t st) {
+                     float2* rf, float2* rg, float2* rj, int n_leaves, int ncols, hipStream_t st) {
     LeafCoef C = make_coef(G);
     const int cpb = 8;
     dim3 grid((unsigned)n_leaves * ((ncols + cpb - 1) / cpb));
     if (u64)
-        hipLaunchKernelGGL(residual_kernel<double2>, grid, dim3(256), 0, st, G, C, u64, nullptr, kint, f, g, nbr,
-                           nu, zeta, rf, rg, rj, n_leaves, ncols, cpb);
+        hipLaunchKernelGGL(residual_kernel<double2>, grid, dim3(256), 0, st, G, C, u64, nullptr, kint, f, g, nbr, nu,
+                           zeta, rf, rg, rj, n_leaves, ncols, cpb);
     else
-        hipLaunchKernelGGL(residual_kernel<float2>, grid, dim3(256), 0, st, G, C, nullptr, u32, kint, f, g, nbr,
-                           nu, zeta, rf, rg, rj, n_leaves, ncols, cpb);
+        hipLaunchKernelGGL(residual_kernel<float2>, grid, dim3(256), 0, st, G, C, nullptr, u32, kint, f, g, nbr, nu,
+                           zeta, rf, rg, rj, n_leaves, ncols, cpb);
 }
 
 // Per-column norms of the residual and of the data: one workgroup per column,
@@ -800,8 +813,8 @@ void launch_residual(const LeafGeom& G, double2* const* u64, const float2* u32, 
 // out[2 col] = |rf|^2 + |rg|^2 + |rj|^2, out[2 col + 1] = |f|^2 + |g|^2 over
 // the physical boundary points.
 __global__ void __launch_bounds__(256) fd_norms_kernel(LeafGeom G, const float2* __restrict__ rf,
-                                                       const double2* __restrict__ rg,
-                                                       const double2* __restrict__ rj,
+                                                       const float2* __restrict__ rg,
+                                                       const float2* __restrict__ rj,
                                                        const double2* const* __restrict__ f,
                                                        const double2* const* __restrict__ g,
                                                        const int* __restrict__ nbr, int n_leaves,
@@ -817,8 +830,8 @@ __global__ void __launch_bounds__(256) fd_norms_kernel(LeafGeom G, const float2*
         d += b.x * b.x + b.y * b.y;
     }
     for (size_t e = tid; e < nbt; e += 256) {
-        const double2 a = rg[(size_t)col * nbt + e], c = rj[(size_t)col * nbt + e];
-        r += a.x * a.x + a.y * a.y + c.x * c.x + c.y * c.y;
+        const float2 a = rg[(size_t)col * nbt + e], c = rj[(size_t)col * nbt + e];
+        r += (double)a.x * a.x + (double)a.y * a.y + (double)c.x * c.x + (double)c.y * c.y;
         const int leaf = (int)(e / G.nb), p = (int)(e % G.nb);
         int fc = 1;
         while (fc < 2 * G.d && p >= G.foff[fc] + G.pf[fc]) ++fc;
@@ -837,7 +850,7 @@ __global__ void __launch_bounds__(256) fd_norms_kernel(LeafGeom G, const float2*
     if (tid == 0) { out[2 * col] = sr[0]; out[2 * col + 1] = sd[0]; }
 }
 
-void launch_fd_norms(const LeafGeom& G, const float2* rf, const double2* rg, const double2* rj,
+void launch_fd_norms(const LeafGeom& G, const float2* rf, const float2* rg, const float2* rj,
                      const double2* const* f, const double2* const* g, const int* nbr, int n_leaves, int ncols,
                      double* out, hipStream_t st) {
     hipLaunchKernelGGL(fd_norms_kernel, dim3(ncols), dim3(256), 0, st, G, rf, rg, rj, f, g, nbr, n_leaves, out);

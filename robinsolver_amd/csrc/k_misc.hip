@@ -47,6 +47,16 @@ __global__ void root_gather_kernel(V* __restrict__ out, const double2* const* __
     out[(size_t)col * P + p] = ccast<V>(gin[col][(size_t)leaf_of[p] * nbl + loc_of[p]]);
 }
 
+template <typename V>
+__global__ void root_gather32_kernel(V* __restrict__ out, const float2* __restrict__ gin,
+                                     const int* __restrict__ leaf_of, const int* __restrict__ loc_of, int P,
+                                     int n_leaves, int nbl) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int col = blockIdx.y;
+    if (p >= P) return;
+    out[(size_t)col * P + p] = ccast<V>(gin[((size_t)col * n_leaves + leaf_of[p]) * nbl + loc_of[p]]);
+}
+
 // child N-part gather: child (2n+c) rows [0, bn) <- parent n rows prow[parity(n)][c][j]
 template <typename V>
 __global__ void child_gather_kernel(V* __restrict__ child, const V* __restrict__ parent, const int* __restrict__ prow,
@@ -105,6 +115,13 @@ void launch_root_gather(V* out, const double2* const* gin, const int* leaf_of, c
 }
 
 template <typename V>
+void launch_root_gather32(V* out, const float2* gin, const int* leaf_of, const int* loc_of, int P, int n_leaves,
+                          int nbl, int ncols, hipStream_t st) {
+    hipLaunchKernelGGL(root_gather32_kernel<V>, dim3(nblk(P), ncols), dim3(256), 0, st, out, gin, leaf_of, loc_of, P,
+                       n_leaves, nbl);
+}
+
+template <typename V>
 void launch_child_gather(V* child, const V* parent, const int* prow, int bn, int bc, int P, int nodes, int root,
                          int ncols, hipStream_t st) {
     hipLaunchKernelGGL(child_gather_kernel<V>, dim3(nblk(bn), 2 * nodes, ncols), dim3(256), 0, st, child, parent,
@@ -131,6 +148,10 @@ template void launch_root_gather<double2>(double2*, const double2* const*, const
                                           hipStream_t);
 template void launch_root_gather<float2>(float2*, const double2* const*, const int*, const int*, int, int, int,
                                          hipStream_t);
+template void launch_root_gather32<double2>(double2*, const float2*, const int*, const int*, int, int, int, int,
+                                            hipStream_t);
+template void launch_root_gather32<float2>(float2*, const float2*, const int*, const int*, int, int, int, int,
+                                           hipStream_t);
 template void launch_child_gather<double2>(double2*, const double2*, const int*, int, int, int, int, int, int,
                                            hipStream_t);
 template void launch_child_gather<float2>(float2*, const float2*, const int*, int, int, int, int, int, int,

@@ -47,7 +47,7 @@ template <typename R> struct LeafArgs {
     const c_t<R>* sinv = nullptr;
     const double2* const* f64 = nullptr;
     const float2* f32 = nullptr;
-    const double2* jump = nullptr;
+    const float2* jump = nullptr;
     const c_t<R>* gin = nullptr;
     c_t<R>* hout = nullptr;
     double2* const* u64 = nullptr;
@@ -59,14 +59,14 @@ template <typename R> struct LeafArgs {
 template <typename R>
 void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit, hipStream_t st);
 // u is complex128 (u64, column table) or complex64 (u32); f and g are column
-// tables; rf is written in complex64.
+// tables; rf, rg and rj are written in complex64.
 void launch_residual(const LeafGeom& G, double2* const* u64, const float2* u32, const double2* kint,
                      const double2* const* f, const double2* const* g, const int* nbr, double2 nu, double2 zeta,
-                     float2* rf, double2* rg, double2* rj, int n_leaves, int ncols, hipStream_t st);
+                     float2* rf, float2* rg, float2* rj, int n_leaves, int ncols, hipStream_t st);
 
 // per-column squared norms of the residual (rf, rg, rj) and of the data (f, g on
 // the physical boundary): out[2c], out[2c+1]
-void launch_fd_norms(const LeafGeom& G, const float2* rf, const double2* rg, const double2* rj,
+void launch_fd_norms(const LeafGeom& G, const float2* rf, const float2* rg, const float2* rj,
                      const double2* const* f, const double2* const* g, const int* nbr, int n_leaves, int ncols,
                      double* out, hipStream_t st);
 
@@ -91,6 +91,10 @@ void launch_copy_blocks(double2* dst, size_t dstride, int ldd, const double2* sr
 template <typename V>
 void launch_root_gather(V* out, const double2* const* gin, const int* leaf_of, const int* loc_of, int P,
                         int nbl, int ncols, hipStream_t st);
+// the same from complex64 data, contiguous [col][leaf][nbl]
+template <typename V>
+void launch_root_gather32(V* out, const float2* gin, const int* leaf_of, const int* loc_of, int P, int n_leaves,
+                          int nbl, int ncols, hipStream_t st);
 template <typename V>
 void launch_child_gather(V* child, const V* parent, const int* prow, int bn, int bc, int P, int nodes, int root,
                          int ncols, hipStream_t st);

@@ -56,7 +56,7 @@ template <typename R> void SolveWork<R>::clear() {
     for (DevBuf* b : descs) delete b;
     descs.clear();
     d_up_g.clear(); d_up_h.clear(); d_dn_t.clear(); d_dn_g.clear();
-    rf.release(); rg.release(); rj.release(); uA.release(); ws.release(); tab_rg.release();
+    rf.release(); rg.release(); rj.release(); uA.release(); ws.release();
     cap = 0;
 }
 
@@ -550,7 +550,7 @@ long long Tree::work_bytes(int cap, int prec) const {
     if (wse) b += (long long)(wse * e);
     const long long nl = P.n_leaves;
     if (prec == RS_MIXED && opts.n_refine > 0)
-        b += (long long)cap * nl * (P.n_int * 8 + P.nb * 32 + P.m * 8) + (long long)cap * (long long)sizeof(void*);
+        b += (long long)cap * nl * (P.n_int * 8 + P.nb * 16 + P.m * 8);
     b += 3LL * cap * (long long)sizeof(void*);
     return b;
 }
@@ -647,7 +647,7 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     up.f64 = io.f64; up.f32 = io.f32; up.jump = io.jump; up.hout = w.hV[L]->template as<V>();
     pbeg(P_LEAF_UP, st);
     launch_leaf_apply<R>(G, up, lds_limit, st);
-    pend(sinv_b + nr * nl * (ef * P.n_int + e * P.nb + (io.jump ? 16.0 * P.nb : 0.0)), leaf_fl, st);
+    pend(sinv_b + nr * nl * (ef * P.n_int + e * P.nb + (io.jump ? 8.0 * P.nb : 0.0)), leaf_fl, st);
     auto gemm = [&](int cat, const GemmDesc* d, int np, int M, int N, int K, bool din) {
         pbeg(cat, st);
         launch_gemm<R>(d, np, M, N, K, st, w.ws.p);
@@ -660,9 +660,13 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
         if (l > 0) gemm(P_GEMM_UP_H, w.d_up_h[l], 2 * Vl.nodes, Vl.bn, nrhs, Vl.isz, true);
     }
     pbeg(P_GATHER, st);
-    launch_root_gather<V>(w.gF[0]->template as<V>(), io.g, d_root_leaf.as<int>(), d_root_loc.as<int>(), P.lv[0].P,
-                          P.nb, nrhs, st);
-    pend(nr * P.lv[0].P * (16.0 + e), 0, st);
+    if (io.g32)
+        launch_root_gather32<V>(w.gF[0]->template as<V>(), io.g32, d_root_leaf.as<int>(), d_root_loc.as<int>(),
+                                P.lv[0].P, nl, P.nb, nrhs, st);
+    else
+        launch_root_gather<V>(w.gF[0]->template as<V>(), io.g, d_root_leaf.as<int>(), d_root_loc.as<int>(),
+                              P.lv[0].P, P.nb, nrhs, st);
+    pend(nr * P.lv[0].P * ((io.g32 ? 8.0 : 16.0) + e), 0, st);
     for (int l = 0; l < L; ++l) {
         const Level& Vl = P.lv[l];
         pbeg(P_GATHER, st);
@@ -731,13 +735,13 @@ int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, in
         if (solve_pass(w32, nrhs, io, st)) return 1;
         for (int it = 0; it < opts.n_refine; ++it) {
             if (residual(nrhs, it == 0 ? nullptr : cols.u, it == 0 ? uA : nullptr, cols.f, cols.g,
-                         w32.rf.as<float2>(), w32.rg.as<double2>(), w32.rj.as<double2>(), st))
+                         w32.rf.as<float2>(), w32.rg.as<float2>(), w32.rj.as<float2>(), st))
                 return 1;
             // the correction pass writes u = u_i + K32^-1 r directly (its leaf stage adds
             // the correction to the current iterate as it stores the solution)
             PassIO c;
-            c.f32 = w32.rf.as<float2>(); c.jump = w32.rj.as<double2>();
-            c.g = w32.tab_rg.as<const double2*>();
+            c.f32 = w32.rf.as<float2>(); c.jump = w32.rj.as<float2>();
+            c.g32 = w32.rg.as<float2>();
             c.u64 = cols.u;
             if (it == 0) c.ubase32 = uA;
             else c.uacc = 1;
@@ -755,12 +759,9 @@ int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, in
 int Tree::ensure_refine(SolveWork<float>& w) {
     if (w.rf.p) return 0;
     const size_t nl = plan.n_leaves, cap = w.cap;
-    if (alloc(w.rf, cap * nl * plan.n_int * 8) || alloc(w.rg, cap * nl * plan.nb * 16) ||
-        alloc(w.rj, cap * nl * plan.nb * 16) || alloc(w.uA, cap * nl * plan.m * 8) ||
-        alloc(w.tab_rg, cap * sizeof(void*)))
+    if (alloc(w.rf, cap * nl * plan.n_int * 8) || alloc(w.rg, cap * nl * plan.nb * 8) ||
+        alloc(w.rj, cap * nl * plan.nb * 8) || alloc(w.uA, cap * nl * plan.m * 8))
         return 1;
-    launch_col_table(w.tab_rg.as<double2*>(), w.rg.as<double2>(), nl * plan.nb, (int)cap, stream);
-    HIPCHK(hipStreamSynchronize(stream));
     return 0;
 }
 
@@ -776,13 +777,13 @@ int Tree::reserve_slot(int slot_id, int cap) {
 }
 
 int Tree::residual(int nrhs, double2* const* u64, const float2* u32, const double2* const* f,
-                   const double2* const* g, float2* rf, double2* rg, double2* rj, hipStream_t st) {
+                   const double2* const* g, float2* rf, float2* rg, float2* rj, hipStream_t st) {
     const Plan& P = plan;
     pbeg(P_RESIDUAL, st);
     launch_residual(G, u64, u32, kint.as<double2>(), f, g, d_leaf_nbr.as<int>(), make_double2(P.nu[0], P.nu[1]),
                     make_double2(P.zeta[0], P.zeta[1]), rf, rg, rj, P.n_leaves, nrhs, st);
     const double eu = u64 ? 16.0 : 8.0;
-    pend((double)nrhs * P.n_leaves * (eu * P.m + 16.0 * P.n_int + 8.0 * P.n_int + 16.0 * 3 * P.nb) +
+    pend((double)nrhs * P.n_leaves * (eu * P.m + 16.0 * P.n_int + 8.0 * P.n_int + 16.0 * P.nb + 8.0 * 2 * P.nb) +
              16.0 * P.n_leaves * P.n_int,
          (double)nrhs * P.n_leaves * P.m * 8.0 * (2 * G.d + 1), st);
     HIPCHK(hipGetLastError());
@@ -795,7 +796,7 @@ int Tree::residual_norms(int nrhs, const double2* f, const double2* g, const dou
     const Plan& P = plan;
     const size_t nl = P.n_leaves;
     DevBuf rf, rg, rj, tabs, nrm;
-    if (alloc(rf, nrhs * nl * P.n_int * 8) || alloc(rg, nrhs * nl * P.nb * 16) || alloc(rj, nrhs * nl * P.nb * 16) ||
+    if (alloc(rf, nrhs * nl * P.n_int * 8) || alloc(rg, nrhs * nl * P.nb * 8) || alloc(rj, nrhs * nl * P.nb * 8) ||
         alloc(tabs, (size_t)3 * nrhs * sizeof(void*)) || alloc(nrm, (size_t)2 * nrhs * sizeof(double)))
         return 1;
     double2** tab = tabs.as<double2*>();
@@ -804,11 +805,11 @@ int Tree::residual_norms(int nrhs, const double2* f, const double2* g, const dou
     launch_col_table(tab + 2 * nrhs, const_cast<double2*>(u), nl * P.m, nrhs, stream);
     const bool was = prof_on;
     prof_on = false;
-    const int rc = residual(nrhs, tab + 2 * nrhs, nullptr, tab, tab + nrhs, rf.as<float2>(), rg.as<double2>(),
-                            rj.as<double2>(), stream);
+    const int rc = residual(nrhs, tab + 2 * nrhs, nullptr, tab, tab + nrhs, rf.as<float2>(), rg.as<float2>(),
+                            rj.as<float2>(), stream);
     prof_on = was;
     if (rc) return 1;
-    launch_fd_norms(G, rf.as<float2>(), rg.as<double2>(), rj.as<double2>(), tab, tab + nrhs, d_leaf_nbr.as<int>(),
+    launch_fd_norms(G, rf.as<float2>(), rg.as<float2>(), rj.as<float2>(), tab, tab + nrhs, d_leaf_nbr.as<int>(),
                     P.n_leaves, nrhs, nrm.as<double>(), stream);
     std::vector<double> h((size_t)2 * nrhs);
     HIPCHK(hipMemcpyAsync(h.data(), nrm.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));

@@ -35,8 +35,7 @@ template <typename R> struct SolveWork {
     int cap = 0;
     std::vector<DevBuf*> hV, gF, G, Z;   // per level
     DevBuf ws;                           // K-split GEMM workspace
-    DevBuf rf, rg, rj, uA;               // refinement: residuals (c64 / c128 / c128), first iterate (c64)
-    DevBuf tab_rg;                       // column table of rg (the correction pass reads it as g)
+    DevBuf rf, rg, rj, uA;               // refinement: residuals and the first iterate (complex64)
     std::vector<DevBuf*> descs;          // per (level, kind)
     std::vector<GemmDesc*> d_up_g, d_up_h, d_dn_t, d_dn_g;
     ~SolveWork() { clear(); }
@@ -58,8 +57,9 @@ struct ProfRec {
 struct PassIO {
     const double2* const* f64 = nullptr;  // interior data, complex128 ...
     const float2* f32 = nullptr;          //   ... or complex64
-    const double2* jump = nullptr;        // transmission residual subtracted from the leaf outgoing data
-    const double2* const* g = nullptr;    // physical boundary data (complex128)
+    const float2* jump = nullptr;         // transmission residual subtracted from the leaf outgoing data
+    const double2* const* g = nullptr;    // physical boundary data (complex128) ...
+    const float2* g32 = nullptr;          //   ... or complex64, contiguous
     double2* const* u64 = nullptr;        // solution out, complex128 ...
     float2* u32 = nullptr;                //   ... or complex64
     const float2* ubase32 = nullptr;      // complex128 out as a correction: u64 = ubase32 + x ...
@@ -146,7 +146,7 @@ struct Tree {
     template <typename R> int ensure_work(SolveWork<R>& w, int nrhs);
     template <typename R> int solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st);
     int residual(int nrhs, double2* const* u64, const float2* u32, const double2* const* f,
-                 const double2* const* g, float2* rf, double2* rg, double2* rj, hipStream_t st);
+                 const double2* const* g, float2* rf, float2* rg, float2* rj, hipStream_t st);
     int solve_device(int nrhs, const double2* f, const double2* g, double2* u, int prec, hipStream_t st);
     int solve_cols(int nrhs, const RhsCols& c, int prec, hipStream_t st, int slot_id);
     int ensure_refine(SolveWork<float>& w);
