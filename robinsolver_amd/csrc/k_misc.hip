@@ -57,18 +57,30 @@ __global__ void root_gather32_kernel(V* __restrict__ out, const float2* __restri
     out[(size_t)col * P + p] = ccast<V>(gin[((size_t)col * n_leaves + leaf_of[p]) * nbl + loc_of[p]]);
 }
 
-// child N-part gather: child (2n+c) rows [0, bn) <- parent n rows prow[parity(n)][c][j]
+// child N-part gather: child (2n+c) rows [0, bn) <- parent n rows prow[parity(n)][c][j];
+// each thread moves one row for GATHER_CPT columns (the map is read once, the
+// column loads are independent)
+constexpr int GATHER_CPT = 8;
 template <typename V>
 __global__ void child_gather_kernel(V* __restrict__ child, const V* __restrict__ parent, const int* __restrict__ prow,
-                                    int bn, int bc, int P, int nodes, int root) {
+                                    int bn, int bc, int P, int nodes, int root, int ncols) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= bn) return;
     const int node = blockIdx.y >> 1, c = blockIdx.y & 1;
-    const int col = blockIdx.z;
+    const int col0 = blockIdx.z * GATHER_CPT;
     const int par = root ? 0 : (node & 1);
-    const int* map = prow + (size_t)(par * 2 + c) * bn;
-    child[(size_t)col * (2 * nodes) * bc + (size_t)(2 * node + c) * bc + j] =
-        parent[(size_t)col * nodes * P + (size_t)node * P + map[j]];
+    const int src = prow[(size_t)(par * 2 + c) * bn + j];
+    V v[GATHER_CPT];
+#pragma unroll
+    for (int q = 0; q < GATHER_CPT; ++q) {
+        const int col = min(col0 + q, ncols - 1);
+        v[q] = parent[(size_t)col * nodes * P + (size_t)node * P + src];
+    }
+#pragma unroll
+    for (int q = 0; q < GATHER_CPT; ++q) {
+        const int col = col0 + q;
+        if (col < ncols) child[(size_t)col * (2 * nodes) * bc + (size_t)(2 * node + c) * bc + j] = v[q];
+    }
 }
 
 __global__ void to_c64_kernel(float2* __restrict__ dst, const double2* __restrict__ src, size_t n) {
@@ -124,8 +136,9 @@ void launch_root_gather32(V* out, const float2* gin, const int* leaf_of, const i
 template <typename V>
 void launch_child_gather(V* child, const V* parent, const int* prow, int bn, int bc, int P, int nodes, int root,
                          int ncols, hipStream_t st) {
-    hipLaunchKernelGGL(child_gather_kernel<V>, dim3(nblk(bn), 2 * nodes, ncols), dim3(256), 0, st, child, parent,
-                       prow, bn, bc, P, nodes, root);
+    const int bs = bn >= 256 ? 256 : (bn + 63) / 64 * 64;
+    hipLaunchKernelGGL(child_gather_kernel<V>, dim3((bn + bs - 1) / bs, 2 * nodes, (ncols + GATHER_CPT - 1) / GATHER_CPT),
+                       dim3(bs), 0, st, child, parent, prow, bn, bc, P, nodes, root, ncols);
 }
 
 void launch_to_c64(float2* dst, const double2* src, size_t n, hipStream_t st) {
