@@ -409,8 +409,7 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
 // of coincident leaf boundary points once the tree has solved.  A correction
 // solve takes (rf, rg) as (f, g) and subtracts rj from the leaf outgoing data.
 // Evaluated in FP64; stored in complex64, the precision the correction pass
-// computes in (rf [col][leaf][n_int], rg and rj [col][leaf][n_bnd]).  One
-// workgroup covers one leaf and cols_per_block columns, a thread one row.
+// computes in (rf [col][leaf][n_int], rg and rj [col][leaf][n_bnd]).
 template <typename U>
 __device__ inline const U* ucolumn(double2* const* u64, const float2* u32, int col, int nl, int m);
 template <>
@@ -422,73 +421,127 @@ __device__ inline const float2* ucolumn<float2>(double2* const*, const float2* u
     return u32 + (size_t)col * nl * m;
 }
 
+// One thread per (leaf, point); the point's stencil and neighbour geometry is
+// worked out once and the loads of RES_CPT columns are issued together.
+constexpr int RES_CPT = 4;
+
 template <typename U>
-__global__ void residual_kernel(LeafGeom G, LeafCoef C, double2* const* __restrict__ u64,
-                                const float2* __restrict__ u32, const double2* __restrict__ kint,
-                                const double2* const* __restrict__ f, const double2* const* __restrict__ g,
-                                const int* __restrict__ nbr, double2 nu, double2 zeta, float2* __restrict__ rf,
-                                float2* __restrict__ rg, float2* __restrict__ rj, int n_leaves, int ncols,
-                                int cols_per_block) {
-    // one block: one leaf, cols_per_block columns; threads run over (column, point)
-    const int leaf = blockIdx.x % n_leaves;
-    const int cb = blockIdx.x / n_leaves;
-    const int nb = G.nb, ni = G.n_int;
-    for (int e = threadIdx.x; e < cols_per_block * G.m; e += blockDim.x) {
-    const int col = cb * cols_per_block + e / G.m;
-    const int p = e % G.m;
-    if (col >= ncols) break;
-    const size_t lc = (size_t)col * n_leaves + leaf;
-    const U* ucol = ucolumn<U>(u64, u32, col, n_leaves, G.m);
-    const U* ul = ucol + (size_t)leaf * G.m;
-    const double2 sma = make_double2(0.0, -2.0 * G.kb);
-    auto uval = [&](const U* uu, int i) -> double2 { return ccast<double2>(uu[i]); };
-    auto incoming = [&](const U* uu, int b) -> double2 {
-        int f_, j_, q1, st;
-        boundary_info(G, b, f_, j_, q1, st);
-        const AxisCoef& a = C.ax[j_];
-        double2 v = cmul(a.dd, uval(uu, b));
-        v = cadd(v, cscale(uval(uu, nb + q1), a.a1));
-        return cadd(v, cscale(uval(uu, nb + q1 + st), a.a2));
-    };
-    if (p < nb) {
-        int fc = 1;
-        while (fc < 2 * G.d && p >= G.foff[fc] + G.pf[fc]) ++fc;
-        const double2 gA = incoming(ul, p);
-        const int nl2 = nbr[(size_t)leaf * 7 + fc];
-        if (nl2 < 0) {
-            rg[lc * nb + p] = ccast<float2>(csub(g[col][(size_t)leaf * nb + p], gA));
-            rj[lc * nb + p] = make_float2(0.f, 0.f);
-        } else {
-            const int f2 = (fc & 1) ? fc + 1 : fc - 1;
-            const int b2 = G.foff[f2] + (p - G.foff[fc]);
-            const double2 gN = incoming(ucol + (size_t)nl2 * G.m, b2);
-            const double2 hA = cfma(sma, uval(ul, p), gA);
-            rj[lc * nb + p] = ccast<float2>(csub(cadd(cmul(zeta, gA), cmul(nu, gN)), hA));
-            rg[lc * nb + p] = make_float2(0.f, 0.f);
-        }
-        continue;
+__global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, double2* const* __restrict__ u64,
+                                                       const float2* __restrict__ u32,
+                                                       const double2* __restrict__ kint,
+                                                       const double2* const* __restrict__ f,
+                                                       const double2* const* __restrict__ g,
+                                                       const int* __restrict__ nbr, double2 nu, double2 zeta,
+                                                       float2* __restrict__ rf, float2* __restrict__ rg,
+                                                       float2* __restrict__ rj, int n_leaves, int ncols) {
+    const int m = G.m, nb = G.nb, ni = G.n_int;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n_leaves * m) return;
+    const int leaf = (int)(e / m), p = (int)(e - (size_t)leaf * m);
+    const int col0 = blockIdx.y * RES_CPT;
+    const U* ucols[RES_CPT];
+    bool cok[RES_CPT];
+#pragma unroll
+    for (int c = 0; c < RES_CPT; ++c) {
+        cok[c] = col0 + c < ncols;
+        ucols[c] = ucolumn<U>(u64, u32, cok[c] ? col0 + c : ncols - 1, n_leaves, m);
     }
+    const size_t ul0 = (size_t)leaf * m;
+    if (p < nb) {
+        int fc, j_, q1, st;
+        boundary_info(G, p, fc, j_, q1, st);
+        const int nl2 = nbr[(size_t)leaf * 7 + fc];
+        const int f2 = (fc & 1) ? fc + 1 : fc - 1;
+        const int b2 = G.foff[f2] + (p - G.foff[fc]);
+        int ff, jj, q1n, stn;
+        boundary_info(G, b2, ff, jj, q1n, stn);
+        const size_t un0 = (size_t)(nl2 >= 0 ? nl2 : leaf) * m;
+        U a0[RES_CPT], a1[RES_CPT], a2[RES_CPT], n0[RES_CPT], n1[RES_CPT], n2[RES_CPT];
+        double2 gv[RES_CPT];
+#pragma unroll
+        for (int c = 0; c < RES_CPT; ++c) {
+            const U* uc = ucols[c];
+            a0[c] = uc[ul0 + p];
+            a1[c] = uc[ul0 + nb + q1];
+            a2[c] = uc[ul0 + nb + q1 + st];
+            n0[c] = uc[un0 + b2];
+            n1[c] = uc[un0 + nb + q1n];
+            n2[c] = uc[un0 + nb + q1n + stn];
+            gv[c] = g[cok[c] ? col0 + c : ncols - 1][(size_t)leaf * nb + p];
+        }
+        const AxisCoef& A = C.ax[j_];
+        const AxisCoef& B = C.ax[jj];
+        const double2 sma = make_double2(0.0, -2.0 * G.kb);
+#pragma unroll
+        for (int c = 0; c < RES_CPT; ++c) {
+            if (!cok[c]) continue;
+            const size_t o = ((size_t)(col0 + c) * n_leaves + leaf) * nb + p;
+            const double2 up = ccast<double2>(a0[c]);
+            double2 gA = cmul(A.dd, up);
+            gA = cadd(gA, cscale(ccast<double2>(a1[c]), A.a1));
+            gA = cadd(gA, cscale(ccast<double2>(a2[c]), A.a2));
+            if (nl2 < 0) {
+                rg[o] = ccast<float2>(csub(gv[c], gA));
+                rj[o] = make_float2(0.f, 0.f);
+            } else {
+                double2 gN = cmul(B.dd, ccast<double2>(n0[c]));
+                gN = cadd(gN, cscale(ccast<double2>(n1[c]), B.a1));
+                gN = cadd(gN, cscale(ccast<double2>(n2[c]), B.a2));
+                const double2 hA = cfma(sma, up, gA);
+                rj[o] = ccast<float2>(csub(cadd(cmul(zeta, gA), cmul(nu, gN)), hA));
+                rg[o] = make_float2(0.f, 0.f);
+            }
+        }
+        return;
+    }
+    // interior row: k u + sum over axes of the three-point second differences
     const int q = p - nb;
     int gq[3];
     interior_coords(G, q, gq);
-    double2 Au = cmul(kint[(size_t)leaf * ni + q], uval(ul, p));
+    int im[3], ip[3];
+    double wm[3], w0[3], wp[3];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        if (j >= G.d) break;
-        const AxisCoef& a = C.ax[j];
-        const double ih2 = a.mid;
-        const int st = stride_of(G, j);
-        double wm, w0, wp;
-        if (gq[j] == 1) { wm = 8.0 / 3.0; w0 = -4.0; wp = 4.0 / 3.0; }
-        else if (gq[j] == G.n[j]) { wm = 4.0 / 3.0; w0 = -4.0; wp = 8.0 / 3.0; }
-        else { wm = 1.0; w0 = -2.0; wp = 1.0; }
-        const double2 um = (gq[j] == 1) ? uval(ul, face_point(G, 2 * j + 1, gq)) : uval(ul, nb + q - st);
-        const double2 upp = (gq[j] == G.n[j]) ? uval(ul, face_point(G, 2 * j + 2, gq)) : uval(ul, nb + q + st);
-        Au = cadd(Au, cscale(uval(ul, p), w0 * ih2));
-        Au = cadd(Au, cscale(um, wm * ih2));
-        Au = cadd(Au, cscale(upp, wp * ih2));
+    for (int jx = 0; jx < 3; ++jx) {
+        im[jx] = ip[jx] = p;
+        wm[jx] = w0[jx] = wp[jx] = 0.0;
+        if (jx >= G.d) continue;
+        const int stx = jx == 0 ? 1 : (jx == 1 ? G.n[0] : G.n[0] * G.n[1]);
+        const int nj = jx == 0 ? G.n[0] : (jx == 1 ? G.n[1] : G.n[2]);
+        const double ih2 = C.ax[jx].mid;
+        if (gq[jx] == 1) { wm[jx] = 8.0 / 3.0; w0[jx] = -4.0; wp[jx] = 4.0 / 3.0; }
+        else if (gq[jx] == nj) { wm[jx] = 4.0 / 3.0; w0[jx] = -4.0; wp[jx] = 8.0 / 3.0; }
+        else { wm[jx] = 1.0; w0[jx] = -2.0; wp[jx] = 1.0; }
+        wm[jx] *= ih2; w0[jx] *= ih2; wp[jx] *= ih2;
+        im[jx] = (gq[jx] == 1) ? face_point(G, 2 * jx + 1, gq) : nb + q - stx;
+        ip[jx] = (gq[jx] == nj) ? face_point(G, 2 * jx + 2, gq) : nb + q + stx;
     }
-    rf[lc * ni + q] = ccast<float2>(csub(f[col][(size_t)leaf * ni + q], Au));
+    const double2 kq = kint[(size_t)leaf * ni + q];
+    U uc0[RES_CPT], um[RES_CPT][3], upp[RES_CPT][3];
+    double2 fv[RES_CPT];
+#pragma unroll
+    for (int c = 0; c < RES_CPT; ++c) {
+        const U* uc = ucols[c] + ul0;
+        uc0[c] = uc[p];
+#pragma unroll
+        for (int jx = 0; jx < 3; ++jx) {
+            um[c][jx] = uc[im[jx]];
+            upp[c][jx] = uc[ip[jx]];
+        }
+        fv[c] = f[cok[c] ? col0 + c : ncols - 1][(size_t)leaf * ni + q];
+    }
+#pragma unroll
+    for (int c = 0; c < RES_CPT; ++c) {
+        if (!cok[c]) continue;
+        const double2 u0 = ccast<double2>(uc0[c]);
+        double2 Au = cmul(kq, u0);
+#pragma unroll
+        for (int jx = 0; jx < 3; ++jx) {
+            if (jx >= G.d) break;
+            Au = cadd(Au, cscale(u0, w0[jx]));
+            Au = cadd(Au, cscale(ccast<double2>(um[c][jx]), wm[jx]));
+            Au = cadd(Au, cscale(ccast<double2>(upp[c][jx]), wp[jx]));
+        }
+        rf[((size_t)(col0 + c) * n_leaves + leaf) * ni + q] = ccast<float2>(csub(fv[c], Au));
     }
 }
 
@@ -798,14 +851,13 @@ void launch_residual(const LeafGeom& G, double2* const* u64, const float2* u32, 
                      const double2* const* f, const double2* const* g, const int* nbr, double2 nu, double2 zeta,
                      float2* rf, float2* rg, float2* rj, int n_leaves, int ncols, hipStream_t st) {
     LeafCoef C = make_coef(G);
-    const int cpb = 8;
-    dim3 grid((unsigned)n_leaves * ((ncols + cpb - 1) / cpb));
+    dim3 grid((unsigned)(((size_t)n_leaves * G.m + 255) / 256), (unsigned)((ncols + RES_CPT - 1) / RES_CPT));
     if (u64)
         hipLaunchKernelGGL(residual_kernel<double2>, grid, dim3(256), 0, st, G, C, u64, nullptr, kint, f, g, nbr, nu,
-                           zeta, rf, rg, rj, n_leaves, ncols, cpb);
+                           zeta, rf, rg, rj, n_leaves, ncols);
     else
         hipLaunchKernelGGL(residual_kernel<float2>, grid, dim3(256), 0, st, G, C, nullptr, u32, kint, f, g, nbr, nu,
-                           zeta, rf, rg, rj, n_leaves, ncols, cpb);
+                           zeta, rf, rg, rj, n_leaves, ncols);
 }
 
 // Per-column norms of the residual and of the data: one workgroup per column,
