@@ -107,6 +107,15 @@ const char *rs_last_error(const rs_tree *t);
  * number of levels, device bytes held. */
 int rs_inspect(const rs_tree *t, int *factored, int *nlvls, long long *device_bytes);
 
+/* FD residual check (no reference counterpart; the reference's check_solution
+ * compares with a known solution, robin_output_module.f90:87-138): for each of
+ * nrhs solutions u of f, g (device arrays, the layouts of rs_solve_device),
+ * rel[r] = ||r||_2 / ||(f, g)||_2 of the global FD system, r = the interior rows
+ * (Delta_h + k) u - f (fd_module.f90:140-181), the physical Robin rows
+ * (fd_module.f90:187-216) and the transmission rows zeta g_l + nu g_nb - h_l
+ * between neighbouring leaves, all evaluated in FP64.  rel is host memory. */
+int rs_residual_norms(rs_tree *t, int nrhs, const void *f_dev, const void *g_dev, const void *u_dev, double *rel);
+
 /* Device memory (no reference counterpart; the reference's memsize is empty,
  * robin_tree_module.f90:243-250).  rs_memory_estimate: host-side, from the plan:
  * bytes the factorization keeps resident, its peak while it runs, and one solve
