@@ -768,12 +768,21 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
             ucp[i] = (OUT != OUT_U32) ? io.u64[colc] + (size_t)leaf * m : nullptr;
             bcp[i] = (OUT == OUT_U64_BASE32) ? io.ubase32 + ((size_t)colc * nl + leaf) * m : nullptr;
         }
+        // the complex64 base runs two layers ahead of the stores
+        float2 bbuf[2][4];
+        auto bfetch = [&](int k, float2 (&bb)[4]) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                bb[i] = (OUT == OUT_U64_BASE32) ? bcp[i][nb + k * s + rrc] : make_float2(0.f, 0.f);
+        };
+        bfetch(0, bbuf[0]);
+        if (K > 1) bfetch(1, bbuf[1]);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             float2 base[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                base[i] = (OUT == OUT_U64_BASE32) ? bcp[i][nb + k * s + rrc] : make_float2(0.f, 0.f);
+            for (int i = 0; i < 4; ++i) base[i] = bbuf[k & 1][i];
+            if (k + 2 < K) bfetch(k + 2, bbuf[k & 1]);
             float2* sb = scr + buf * 256;
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) sb[(4 * grp + vv) * 16 + ccol] = y[k][vv];
