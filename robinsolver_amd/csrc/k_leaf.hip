@@ -566,6 +566,15 @@ __device__ inline float2 pick4(float2 a0, float2 a1, float2 a2, float2 a3, int i
     return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
 }
 
+// LDS exchange among the lanes of one wave.  A wave's LDS instructions are
+// processed in order, so a write by one lane is seen by a later read of another
+// lane of the same wave; only the compiler must not move LDS accesses across.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // f element type of a leaf pass: the caller's complex128 data through the
 // column table, or the complex64 correction data (contiguous [col][leaf][n_int]).
 template <typename FT> struct FCol;
@@ -677,7 +686,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     auto product = [&](int k, const float2 v[4], f4v& aR, f4v& aI) {
         float2* sb = scr + buf * 256;
         for (int vv = 0; vv < 4; ++vv) sb[(4 * grp + vv) * 16 + ccol] = v[vv];
-        __syncthreads();
+        wave_lds_sync();
         aR = f4v{0.f, 0.f, 0.f, 0.f};
         aI = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -715,7 +724,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
                 const float2 x = ccast<float2>(fbuf[k % PF][i]);
                 sb[rr * 16 + 4 * i + cq] = make_float2(x.x * fmask[i], x.y * fmask[i]);
             }
-            __syncthreads();
+            wave_lds_sync();
         }
         if (k + PF < K) fetch(k + PF, fbuf[k % PF]);
         float2 v[4];
@@ -786,7 +795,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
             float2* sb = scr + buf * 256;
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) sb[(4 * grp + vv) * 16 + ccol] = y[k][vv];
-            __syncthreads();
+            wave_lds_sync();
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int col = wcol0 + 4 * i + cq;
@@ -822,7 +831,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
         emit(3, row, y[0][vv], y[1][vv], ax1);
         emit(4, row, y[K - 1][vv], y[K - 2][vv], ax1);
     }
-    __syncthreads();
+    wave_lds_sync();
     // coalesced store of the wave's boundary values: up, h in node order; down, u_B
     // in local point order
     const int ntot = 16 * nb;
