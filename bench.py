@@ -356,11 +356,29 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-mix", action="store_true",
                     help="only the multi-tree serving scenario (8 resident trees 128^2..1024^2)")
+    ap.add_argument("--factor-only", type=int, default=0, metavar="REPS",
+                    help="only time REPS factorizations of the workload tree (FP64), one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(multi-GPU sharding is design-only)." % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.factor_only:
+        import robinsolver_amd as rsa
+        t = rsa.GpuTree(2, args.levels, [[0, 1], [0, 1], [0, 1]], 1.0 / args.grid, -args.kb, precision=rsa.RS_FP64)
+        pts, _ = t.all_points()
+        k = potential(pts[:, t.nb:], args.kb, args.seed)
+        times = []
+        for _ in range(args.factor_only):
+            t.factor(k)
+            times.append(t.timings()[0])
+            log("factor %.4f s" % times[-1])
+        ff, _ = t.algorithmic_counts()
+        print(json.dumps({"metric": "factor seconds, 2D %d^2 Helmholtz (FP64)" % args.grid, "value": min(times),
+                          "unit": "s", "higher_is_better": False, "times": times,
+                          "reference_factor_GF": ff / 1e9}))
+        return
 
     if args.serve_mix:
         mix = serve_mix(args)
@@ -500,13 +518,13 @@ def main():
     roof["algorithmic_bytes_per_launch"] = bpl
     # HBM bytes per launch from the committed hardware-counter summary of this
     # workload (FETCH_SIZE and WRITE_SIZE runs, scaled by the copy calibration)
-    tpath = os.path.join(ROOT, "profiles", "traffic_leaf_down_cfg3.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_leaf_down_2d1024.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             tr = json.load(fh)
         if tr.get("category") == cat:
             roof["traffic"] = tr["hbm_bytes_per_launch"]
-            roof["traffic_source"] = "profiles/traffic_leaf_down_cfg3.json"
+            roof["traffic_source"] = "profiles/traffic_leaf_down_2d1024.json"
 
     # every category: time per step and achieved rates against its algorithmic counts
     kernels = {}
