@@ -569,6 +569,7 @@ def main():
         "roofline": roof,
         "kernel_ms_per_step": {c: v[0] / args.steps for c, v in prof.items()},
         "kernels": kernels,
+        "gemm_tiles_mixed": t.tile_choices(rsa.RS_MIXED).tolist(),
         "fd_residual_fp64": {"max": float(res64.max()), "median": float(np.median(res64))},
         "fd_residual_mixed": {"max": float(res32.max()), "median": float(np.median(res32))},
     }

@@ -41,9 +41,13 @@ typedef struct rs_opts {
     int n_refine;         /* refinement steps of the mixed path                 */
     int verbose;          /* 1: progress lines as the reference prints them     */
     long long hbm_budget; /* bytes this tree may hold on the device (0 = no cap) */
+    int autotune;         /* 1: when a solve workspace is set up, time the matrix-
+                             core tile shapes of each of its products and keep the
+                             fastest (the results do not depend on the choice)   */
 } rs_opts;
 
-/* Defaults: precision RS_FP64, skip_root_rtr 1, n_refine 2, verbose 0, no cap. */
+/* Defaults: precision RS_FP64, skip_root_rtr 1, n_refine 2, verbose 0, no cap,
+ * autotune 1. */
 void rs_opts_default(rs_opts *o);
 
 /* Create / destroy a tree handle (type robin_tree, robin_tree_module.f90:55-77;
@@ -125,6 +129,14 @@ int rs_memory_estimate(const rs_tree *t, int ncols, long long *factor_resident, 
                        long long *solve_workspace);
 int rs_memory_used(const rs_tree *t, long long *held, long long *peak);
 
+/* Autotuning result (no reference counterpart): the GEMM tile chosen for each
+ * product of the solve workspace of rs_solve / rs_solve_device at `precision`,
+ * tiles[4 * lvl + k] for levels 0..L-1 and products k = 0 up (interface),
+ * 1 up (parent data), 2 down (children's data), 3 down (interface); values as in
+ * rs_check_gemm_tile, -1 for the shape rule or a product the level does not have.
+ * Fails if that workspace does not exist yet (before the first solve). */
+int rs_tile_choices(const rs_tree *t, int precision, int *tiles);
+
 /* Per-level shape of the plan (no reference counterpart; host-side, needs no
  * GPU): nodes of level lvl, boundary points P of each, and for lvl < L the
  * interface size and the children's boundary size of its merge (0 at the
@@ -149,6 +161,12 @@ int rs_check_gemm(int precision, int M, int N, int K, const double *A, int lda, 
                   const double *B2, int ldb, int ksplit, double *C, int ldc, const int *rmap,
                   const double *alpha, const double *beta);
 int rs_check_inverse(int n, int batch, double *mats);
+/* rs_check_gemm with the output tile shape forced (0: 64x64, 1: 32x64, 2: 16x64,
+ * 3: 64x16, -1: by shape); a test hook for the autotuner's premise that the
+ * tile never changes the results. */
+int rs_check_gemm_tile(int precision, int M, int N, int K, const double *A, int lda, const double *B,
+                       const double *B2, int ldb, int ksplit, double *C, int ldc, const int *rmap,
+                       const double *alpha, const double *beta, int tile);
 
 /* Kernel timing (no reference counterpart; the reference's only timer is
  * time_module).  While enabled, every solve-pass launch is bracketed by HIP

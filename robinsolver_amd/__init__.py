@@ -33,7 +33,7 @@ class RobinError(RuntimeError):
 
 class RsOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("skip_root_rtr", ctypes.c_int), ("n_refine", ctypes.c_int),
-                ("verbose", ctypes.c_int), ("hbm_budget", ctypes.c_longlong)]
+                ("verbose", ctypes.c_int), ("hbm_budget", ctypes.c_longlong), ("autotune", ctypes.c_int)]
 
 
 def lib():
@@ -72,6 +72,7 @@ def lib():
         L.rs_memory_estimate.argtypes = [vp, ctypes.c_int, llp, llp, llp]
         L.rs_memory_used.argtypes = [vp, llp, llp]
         L.rs_residual_norms.argtypes = [vp, ctypes.c_int, vp, vp, vp, dp]
+        L.rs_tile_choices.argtypes = [vp, ctypes.c_int, ip]
         from . import serve as _serve
         _serve.bind(L)
         _lib = L
@@ -90,11 +91,12 @@ class GpuTree:
     """One factored tree resident on the GPU (C ABI handle)."""
 
     def __init__(self, d, L, domain, h_tgt, kb, bndry_params=None, precision=RS_FP64, skip_root_rtr=True,
-                 n_refine=2, hbm_budget=0):
+                 n_refine=2, hbm_budget=0, autotune=True):
         L_ = lib()
         o = RsOpts()
         L_.rs_opts_default(ctypes.byref(o))
         o.precision, o.skip_root_rtr, o.n_refine, o.hbm_budget = precision, int(skip_root_rtr), n_refine, hbm_budget
+        o.autotune = int(autotune)
         h = ctypes.c_void_p()
         if L_.rs_tree_create(ctypes.byref(h), ctypes.byref(o)):
             raise RobinError(L_.rs_last_error(None).decode())
@@ -181,6 +183,12 @@ class GpuTree:
         self._check(lib().rs_residual_norms(self.h, nrhs, ctypes.c_void_p(f_ptr), ctypes.c_void_p(g_ptr),
                                             ctypes.c_void_p(u_ptr), _dp(out)))
         return out
+
+    def tile_choices(self, precision):
+        """GEMM tile chosen per level and product of the solve workspace: [L][4] (-1: shape rule / none)."""
+        out = np.zeros(4 * self.L, np.int32)
+        self._check(lib().rs_tile_choices(self.h, precision, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return out.reshape(self.L, 4)
 
     def memory_estimate(self, ncols=1):
         """(factor resident bytes, factor peak bytes, solve workspace bytes of ncols RHS)."""

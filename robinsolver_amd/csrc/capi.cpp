@@ -21,6 +21,7 @@ void rs_opts_default(rs_opts* o) {
     o->n_refine = 2;
     o->verbose = 0;
     o->hbm_budget = 0;
+    o->autotune = 1;
 }
 
 int rs_tree_create(rs_tree** t, const rs_opts* o) {
@@ -152,6 +153,21 @@ int rs_memory_used(const rs_tree* t, long long* held, long long* peak) {
     return 0;
 }
 
+int rs_tile_choices(const rs_tree* t, int precision, int* tiles) {
+    if (!t || !tiles) return 1;
+    const rs::Tree& T = t->t;
+    if (T.slots.empty()) return const_cast<rs_tree*>(t)->t.fail("Error: no solve workspace yet.");
+    const rs::Slot& sl = *T.slots[0];
+    const int L = T.plan.L;
+    auto put = [&](const std::vector<int>(&tile)[4]) -> int {
+        if (tile[0].size() != (size_t)L) return const_cast<rs_tree*>(t)->t.fail("Error: no solve workspace yet.");
+        for (int l = 0; l < L; ++l)
+            for (int k = 0; k < 4; ++k) tiles[4 * l + k] = (k == 1 && l == 0) ? -1 : tile[k][l];
+        return 0;
+    };
+    return precision == RS_MIXED ? put(sl.w32.tile) : put(sl.w64.tile);
+}
+
 int rs_level_info(const rs_tree* t, int lvl, int* nodes, int* P, int* isz, int* child_P) {
     if (!t || !t->t.planned || lvl < 0 || lvl > t->t.plan.L) return 1;
     const rs::Level& V = t->t.plan.lv[lvl];
@@ -179,7 +195,7 @@ std::vector<V> to_prec(const double* p, size_t n) {
 }
 
 template <typename R>
-int check_gemm_t(int M, int N, int K, const double* A, int lda, const double* B, const double* B2, int ldb,
+int check_gemm_t(int tile, int M, int N, int K, const double* A, int lda, const double* B, const double* B2, int ldb,
                  int ksplit, double* C, int ldc, const int* rmap, const double* alpha, const double* beta) {
     using V = rs::c_t<R>;
     size_t na = (size_t)lda * K, nb = (size_t)ldb * N, nc = (size_t)ldc * N;
@@ -214,7 +230,7 @@ int check_gemm_t(int M, int N, int K, const double* A, int lda, const double* B,
     void* ws = nullptr;
     size_t wse = rs::gemm_ws_elems(1, M, N, K);
     if (wse) (void)hipMalloc(&ws, wse * sizeof(V));
-    rs::launch_gemm<R>(dd, 1, M, N, K, nullptr, ws);
+    rs::launch_gemm<R>(dd, 1, M, N, K, nullptr, ws, tile);
     hipError_t e = hipDeviceSynchronize();
     if (ws) (void)hipFree(ws);
     if (e == hipSuccess) e = hipMemcpy(hC.data(), dC, nc * sizeof(V), hipMemcpyDeviceToHost);
@@ -228,13 +244,19 @@ int check_gemm_t(int M, int N, int K, const double* A, int lda, const double* B,
 
 extern "C" {
 
-int rs_check_gemm(int precision, int M, int N, int K, const double* A, int lda, const double* B, const double* B2,
-                  int ldb, int ksplit, double* C, int ldc, const int* rmap, const double* alpha, const double* beta) {
+int rs_check_gemm_tile(int precision, int M, int N, int K, const double* A, int lda, const double* B,
+                       const double* B2, int ldb, int ksplit, double* C, int ldc, const int* rmap,
+                       const double* alpha, const double* beta, int tile) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_err = "Error: no HIP device."; return 1; }
     if (precision == RS_MIXED)
-        return check_gemm_t<float>(M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta);
-    return check_gemm_t<double>(M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta);
+        return check_gemm_t<float>(tile, M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta);
+    return check_gemm_t<double>(tile, M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta);
+}
+
+int rs_check_gemm(int precision, int M, int N, int K, const double* A, int lda, const double* B, const double* B2,
+                  int ldb, int ksplit, double* C, int ldc, const int* rmap, const double* alpha, const double* beta) {
+    return rs_check_gemm_tile(precision, M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta, -1);
 }
 
 int rs_check_inverse(int n, int batch, double* mats) {

@@ -278,33 +278,29 @@ size_t gemm_ws_elems(int nprob, int M, int N, int K) {
     return ks > 1 ? (size_t)nprob * ks * M * N : 0;
 }
 
+int gemm_tile_rule(int M, int N) { return N <= 16 ? GEMM_TILE_64x16 : (M <= 16 ? GEMM_TILE_16x64 : (M <= 32 ? GEMM_TILE_32x64 : GEMM_TILE_64x64)); }
+
 template <typename R>
-void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws) {
+void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile) {
     if (nprob <= 0 || M <= 0 || N <= 0) return;
     int ks = gemm_kslices(nprob, M, K);
     if (!ws) ks = 1;
     const int kchunk = ((K + ks - 1) / ks + BK - 1) / BK * BK;
     ks = (K + kchunk - 1) / kchunk;
     c_t<R>* W = (c_t<R>*)ws;
-    // tile shape from M and N only; every 16x16 output block runs the same
-    // MFMA sequence over K whatever the tile, so results do not depend on it
-    if (N <= 16) {
-        const int tilesM = (M + 63) / 64, tilesN = (N + 15) / 16;
-        hipLaunchKernelGGL((gemm_kernel<R, 64, 16>), dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs,
-                           M, N, K, tilesM, tilesN, ks, kchunk, W);
-    } else if (M <= 16) {
-        const int tilesM = (M + 15) / 16, tilesN = (N + 63) / 64;
-        hipLaunchKernelGGL((gemm_kernel<R, 16, 64>), dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs,
-                           M, N, K, tilesM, tilesN, ks, kchunk, W);
-    } else if (M <= 32) {
-        const int tilesM = (M + 31) / 32, tilesN = (N + 63) / 64;
-        hipLaunchKernelGGL((gemm_kernel<R, 32, 64>), dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs,
-                           M, N, K, tilesM, tilesN, ks, kchunk, W);
-    } else {
-        const int tilesM = (M + 63) / 64, tilesN = (N + 63) / 64;
-        hipLaunchKernelGGL((gemm_kernel<R, 64, 64>), dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs,
-                           M, N, K, tilesM, tilesN, ks, kchunk, W);
-    }
+    // Every 16x16 output block runs the same MFMA sequence over K whatever the
+    // tile, so the tile shape (by rule from M and N, or chosen by the caller's
+    // autotuning) never changes the results.
+    if (tile < 0 || tile >= GEMM_TILES) tile = gemm_tile_rule(M, N);
+    auto run = [&](auto kern, int bm, int bn) {
+        const int tilesM = (M + bm - 1) / bm, tilesN = (N + bn - 1) / bn;
+        hipLaunchKernelGGL(kern, dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs, M, N, K, tilesM, tilesN,
+                           ks, kchunk, W);
+    };
+    if (tile == GEMM_TILE_64x16) run(gemm_kernel<R, 64, 16>, 64, 16);
+    else if (tile == GEMM_TILE_16x64) run(gemm_kernel<R, 16, 64>, 16, 64);
+    else if (tile == GEMM_TILE_32x64) run(gemm_kernel<R, 32, 64>, 32, 64);
+    else run(gemm_kernel<R, 64, 64>, 64, 64);
     if (ks > 1) {
         const size_t mn = (size_t)M * N;
         hipLaunchKernelGGL(gemm_reduce_kernel<R>, dim3((unsigned)((mn + 255) / 256), nprob), dim3(256), 0, st,
@@ -312,7 +308,7 @@ void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStr
     }
 }
 
-template void launch_gemm<double>(const GemmDesc*, int, int, int, int, hipStream_t, void*);
-template void launch_gemm<float>(const GemmDesc*, int, int, int, int, hipStream_t, void*);
+template void launch_gemm<double>(const GemmDesc*, int, int, int, int, hipStream_t, void*, int);
+template void launch_gemm<float>(const GemmDesc*, int, int, int, int, hipStream_t, void*, int);
 
 }  // namespace rs

@@ -57,6 +57,7 @@ template <typename R> void SolveWork<R>::clear() {
     descs.clear();
     d_up_g.clear(); d_up_h.clear(); d_dn_t.clear(); d_dn_g.clear();
     rf.release(); rg.release(); rj.release(); uA.release(); ws.release();
+    for (auto& t : tile) t.clear();
     cap = 0;
 }
 
@@ -525,7 +526,48 @@ template <typename R> int Tree::ensure_work(SolveWork<R>& w, int nrhs) {
         w.d_dn_g[l] = w.descs[k0 + 3]->template as<GemmDesc>();
     }
     w.cap = cap;
+    for (auto& t : w.tile) t.assign(L, -1);
+    if (opts.autotune && tune_work(w)) return 1;
     return 0;
+}
+
+template <typename R> int Tree::tune_work(SolveWork<R>& w) {
+    // Each product is run on the workspace's own buffers; every one of them
+    // writes a buffer it does not read, so repeating it is harmless.
+    const Plan& P = plan;
+    const int L = P.L, N = w.cap;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = 0;
+    for (int l = 0; l < L && !rc; ++l) {
+        const Level& V = P.lv[l];
+        struct Site { int kind; const GemmDesc* d; int np, M, K; };
+        const Site sites[4] = {{0, w.d_up_g[l], V.nodes, 2 * V.isz, 2 * V.isz},
+                               {1, w.d_up_h[l], 2 * V.nodes, V.bn, V.isz},
+                               {2, w.d_dn_t[l], 2 * V.nodes, V.isz, V.bn},
+                               {3, w.d_dn_g[l], V.nodes, 2 * V.isz, 2 * V.isz}};
+        for (const Site& st_ : sites) {
+            if (st_.kind == 1 && l == 0) continue;   // no parent: the root level has no up_h
+            float best = 0.f;
+            int bt = -1;
+            for (int t = 0; t < GEMM_TILES; ++t) {
+                launch_gemm<R>(st_.d, st_.np, st_.M, N, st_.K, stream, w.ws.p, t);
+                (void)hipEventRecord(e0, stream);
+                for (int r = 0; r < 3; ++r) launch_gemm<R>(st_.d, st_.np, st_.M, N, st_.K, stream, w.ws.p, t);
+                (void)hipEventRecord(e1, stream);
+                if (hipEventSynchronize(e1) != hipSuccess) { rc = fail("Error: GEMM autotuning failed."); break; }
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (bt < 0 || ms < best) { best = ms; bt = t; }
+            }
+            if (rc) break;
+            w.tile[st_.kind][l] = bt;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
 }
 
 long long Tree::work_bytes(int cap, int prec) const {
@@ -648,16 +690,17 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     pbeg(P_LEAF_UP, st);
     launch_leaf_apply<R>(G, up, lds_limit, st);
     pend(sinv_b + nr * nl * (ef * P.n_int + e * P.nb + (io.jump ? 8.0 * P.nb : 0.0)), leaf_fl, st);
-    auto gemm = [&](int cat, const GemmDesc* d, int np, int M, int N, int K, bool din) {
+    auto gemm = [&](int cat, int lvl, const GemmDesc* d, int np, int M, int N, int K, bool din) {
+        const std::vector<int>& tiles = w.tile[cat - P_GEMM_UP_G];   // up_g, up_h, dn_t, dn_g
         pbeg(cat, st);
-        launch_gemm<R>(d, np, M, N, K, st, w.ws.p);
+        launch_gemm<R>(d, np, M, N, K, st, w.ws.p, lvl < (int)tiles.size() ? tiles[lvl] : -1);
         pend((double)np * e * ((double)M * K + (double)K * N + (double)M * N * (din ? 2 : 1)),
              8.0 * np * (double)M * N * K, st);
     };
     for (int l = L - 1; l >= 0; --l) {
         const Level& Vl = P.lv[l];
-        gemm(P_GEMM_UP_G, w.d_up_g[l], Vl.nodes, 2 * Vl.isz, nrhs, 2 * Vl.isz, false);
-        if (l > 0) gemm(P_GEMM_UP_H, w.d_up_h[l], 2 * Vl.nodes, Vl.bn, nrhs, Vl.isz, true);
+        gemm(P_GEMM_UP_G, l, w.d_up_g[l], Vl.nodes, 2 * Vl.isz, nrhs, 2 * Vl.isz, false);
+        if (l > 0) gemm(P_GEMM_UP_H, l, w.d_up_h[l], 2 * Vl.nodes, Vl.bn, nrhs, Vl.isz, true);
     }
     pbeg(P_GATHER, st);
     if (io.g32)
@@ -673,8 +716,8 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
         launch_child_gather<V>(w.gF[l + 1]->template as<V>(), w.gF[l]->template as<V>(), d_prow[l]->as<int>(),
                                Vl.bn, Vl.bc, Vl.P, Vl.nodes, l == 0, nrhs, st);
         pend(nr * 2.0 * Vl.nodes * Vl.bn * 2 * e, 0, st);
-        gemm(P_GEMM_DN_T, w.d_dn_t[l], 2 * Vl.nodes, Vl.isz, nrhs, Vl.bn, false);
-        gemm(P_GEMM_DN_G, w.d_dn_g[l], Vl.nodes, 2 * Vl.isz, nrhs, 2 * Vl.isz, true);
+        gemm(P_GEMM_DN_T, l, w.d_dn_t[l], 2 * Vl.nodes, Vl.isz, nrhs, Vl.bn, false);
+        gemm(P_GEMM_DN_G, l, w.d_dn_g[l], Vl.nodes, 2 * Vl.isz, nrhs, 2 * Vl.isz, true);
     }
     LeafArgs<R> dn;
     dn.mode = LEAF_DOWN; dn.ncols = nrhs; dn.n_leaves = nl; dn.sinv = sv;
@@ -866,6 +909,8 @@ int Tree::node_rtr(int lvl, int idx, double* out, int* Pout) const {
 }
 
 template int Tree::ensure_work<double>(SolveWork<double>&, int);
+template int Tree::tune_work<double>(SolveWork<double>&);
+template int Tree::tune_work<float>(SolveWork<float>&);
 template int Tree::ensure_work<float>(SolveWork<float>&, int);
 
 }  // namespace rs

@@ -38,6 +38,7 @@ template <typename R> struct SolveWork {
     DevBuf rf, rg, rj, uA;               // refinement: residuals and the first iterate (complex64)
     std::vector<DevBuf*> descs;          // per (level, kind)
     std::vector<GemmDesc*> d_up_g, d_up_h, d_dn_t, d_dn_g;
+    std::vector<int> tile[4];            // per product (up_g, up_h, dn_t, dn_g) and level: GEMM tile, -1 = by shape
     ~SolveWork() { clear(); }
     void clear();
 };
@@ -144,6 +145,8 @@ struct Tree {
     int factor_device(const double2* k_int_dev);
     int make_c64();
     template <typename R> int ensure_work(SolveWork<R>& w, int nrhs);
+    // time the tile shapes of every product of the workspace (at its capacity) and keep the fastest
+    template <typename R> int tune_work(SolveWork<R>& w);
     template <typename R> int solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st);
     int residual(int nrhs, double2* const* u64, const float2* u32, const double2* const* f,
                  const double2* const* g, float2* rf, float2* rg, float2* rj, hipStream_t st);

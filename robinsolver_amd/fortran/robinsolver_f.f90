@@ -131,6 +131,7 @@ module rs_capi
         integer(c_int) :: n_refine
         integer(c_int) :: verbose
         integer(c_long_long) :: hbm_budget
+        integer(c_int) :: autotune
     end type rs_opts_t
 
     interface

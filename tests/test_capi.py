@@ -64,3 +64,12 @@ def test_solve_before_factor_fails():
     t = rsa.GpuTree(2, 2, UNIT, 1.0 / 16, 1.0)
     with pytest.raises(rsa.RobinError, match="factored first"):
         t.solve(np.zeros((t.n_leaves, t.n_int)), np.zeros((t.n_leaves, t.nb)))
+
+
+def test_opts_defaults_and_layout():
+    """rs_opts as the header documents it (the Fortran and Python mirrors bind
+    this layout): FP64, skip the root RtR, two refinement steps, autotuning on."""
+    o = rsa.RsOpts()
+    rsa.lib().rs_opts_default(ctypes.byref(o))
+    assert (o.precision, o.skip_root_rtr, o.n_refine, o.verbose, o.hbm_budget, o.autotune) == (0, 1, 2, 0, 0, 1)
+    assert ctypes.sizeof(rsa.RsOpts) == 32

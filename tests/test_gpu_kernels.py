@@ -132,3 +132,55 @@ def test_inverse_needs_pivoting(n, gpu):
     assert _lib().rs_check_inverse(n, 1, _dp(flat.view(np.float64))) == 0
     inv = flat.reshape(n, n, order="F")
     assert np.linalg.norm(inv @ M - np.eye(n)) / np.sqrt(n) < 1e-10
+
+
+@pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED])
+@pytest.mark.parametrize("M,N,K,scatter", [(48, 256, 16, True), (100, 70, 777, False), (200, 33, 1000, False),
+                                           (16, 64, 32, True)])
+def test_gemm_tile_shapes_give_identical_bits(prec, M, N, K, scatter, gpu):
+    """The autotuner may pick any tile shape: every shape must give the same bits."""
+    print("case: gemm tiles", prec, M, N, K, scatter, flush=True)
+    L = _lib()
+    L.rs_check_gemm_tile.argtypes = L.rs_check_gemm.argtypes + [ctypes.c_int]
+    rng = np.random.default_rng(M + N + K)
+    A = _rc(rng, M, K)
+    B = _rc(rng, K, N)
+    ldc = M + 5 if scatter else M
+    C0 = _rc(rng, ldc, N)
+    rmap = np.ascontiguousarray(rng.permutation(ldc)[:M].astype(np.int32)) if scatter else None
+    outs = []
+    for tile in (-1, 0, 1, 2, 3):
+        C = np.asfortranarray(C0.copy())
+        a = np.asfortranarray(A)
+        b = np.asfortranarray(B)
+        alpha = np.array([1.0, 0.5])
+        beta = np.array([0.25, -1.0])
+        rc = L.rs_check_gemm_tile(prec, M, N, K, _dp(a.reshape(-1, order="F").view(np.float64)), M,
+                                  _dp(b.reshape(-1, order="F").view(np.float64)), None, K, K,
+                                  _dp(C.reshape(-1, order="F").view(np.float64)), ldc,
+                                  rmap.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) if scatter else None,
+                                  _dp(alpha), _dp(beta), tile)
+        assert rc == 0
+        outs.append(C.copy())
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
+def test_autotuned_solve_matches_untuned_bits(gpu):
+    print("case: autotuned solve bits", flush=True)
+    import robinsolver_amd as rsa2
+    res = []
+    for tune in (True, False):
+        t = rsa2.GpuTree(2, 6, [[0, 1], [0, 1], [0, 1]], 1.0 / 128, -12.0, precision=rsa2.RS_MIXED, n_refine=1,
+                         autotune=tune)
+        pts, _ = t.all_points()
+        t.factor(144.0 * (1 + 0.2 * np.sin(3 * pts[:, t.nb:, 0])) + 1.0j)
+        rng = np.random.default_rng(2)
+        f = _rc(rng, 20, t.n_leaves, t.n_int)
+        g = _rc(rng, 20, t.n_leaves, t.nb)
+        res.append(t.solve(f, g))
+        if tune:
+            ch = t.tile_choices(rsa2.RS_MIXED)
+            print("tiles:", ch.tolist(), flush=True)
+            assert ch.shape == (6, 4) and (ch[1:] >= 0).all()
+    assert np.array_equal(res[0], res[1])
