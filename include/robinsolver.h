@@ -162,8 +162,8 @@ int rs_check_gemm(int precision, int M, int N, int K, const double *A, int lda, 
                   const double *alpha, const double *beta);
 int rs_check_inverse(int n, int batch, double *mats);
 /* rs_check_gemm with the output tile shape forced (0: 64x64, 1: 32x64, 2: 16x64,
- * 3: 64x16, -1: by shape); a test hook for the autotuner's premise that the
- * tile never changes the results. */
+ * 3: 64x16, 4: 32x32, 5: 64x32, 6: 128x16, -1: by shape); a test hook for the
+ * autotuner's premise that the tile never changes the results. */
 int rs_check_gemm_tile(int precision, int M, int N, int K, const double *A, int lda, const double *B,
                        const double *B2, int ldb, int ksplit, double *C, int ldc, const int *rmap,
                        const double *alpha, const double *beta, int tile);

@@ -59,7 +59,8 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
                                                   c_t<R>* __restrict__ ws) {
     using V = c_t<R>;
     using A4 = typename acc_t<R>::T;
-    // four waves: 2x2 for 64x64 and 32x64 tiles, 4x1 for 64x16, 1x4 for 16x64
+    // four waves: 2x2 for 64x64, 32x64, 32x32 and 64x32 tiles, 4x1 for 64x16 and
+    // 128x16, 1x4 for 16x64
     constexpr int WAVES_N = (BN == 16) ? 1 : (BM == 16 ? 4 : 2);
     constexpr int WAVES_M = 4 / WAVES_N;
     constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
@@ -297,10 +298,15 @@ void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStr
         hipLaunchKernelGGL(kern, dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs, M, N, K, tilesM, tilesN,
                            ks, kchunk, W);
     };
-    if (tile == GEMM_TILE_64x16) run(gemm_kernel<R, 64, 16>, 64, 16);
-    else if (tile == GEMM_TILE_16x64) run(gemm_kernel<R, 16, 64>, 16, 64);
-    else if (tile == GEMM_TILE_32x64) run(gemm_kernel<R, 32, 64>, 32, 64);
-    else run(gemm_kernel<R, 64, 64>, 64, 64);
+    switch (tile) {
+        case GEMM_TILE_64x16: run(gemm_kernel<R, 64, 16>, 64, 16); break;
+        case GEMM_TILE_16x64: run(gemm_kernel<R, 16, 64>, 16, 64); break;
+        case GEMM_TILE_32x64: run(gemm_kernel<R, 32, 64>, 32, 64); break;
+        case GEMM_TILE_32x32: run(gemm_kernel<R, 32, 32>, 32, 32); break;
+        case GEMM_TILE_64x32: run(gemm_kernel<R, 64, 32>, 64, 32); break;
+        case GEMM_TILE_128x16: run(gemm_kernel<R, 128, 16>, 128, 16); break;
+        default: run(gemm_kernel<R, 64, 64>, 64, 64); break;
+    }
     if (ks > 1) {
         const size_t mn = (size_t)M * N;
         hipLaunchKernelGGL(gemm_reduce_kernel<R>, dim3((unsigned)((mn + 255) / 256), nprob), dim3(256), 0, st,

@@ -29,7 +29,8 @@ struct GemmDesc {
 // ws: workspace of gemm_ws_elems(...) complex elements of precision R for the
 // K-split form (NULL: no split).  tile: one of GEMM_TILE_* (output tile rows x
 // columns), or -1 for the shape rule gemm_tile_rule; results do not depend on it.
-enum { GEMM_TILE_64x64 = 0, GEMM_TILE_32x64 = 1, GEMM_TILE_16x64 = 2, GEMM_TILE_64x16 = 3, GEMM_TILES = 4 };
+enum { GEMM_TILE_64x64 = 0, GEMM_TILE_32x64 = 1, GEMM_TILE_16x64 = 2, GEMM_TILE_64x16 = 3, GEMM_TILE_32x32 = 4,
+       GEMM_TILE_64x32 = 5, GEMM_TILE_128x16 = 6, GEMM_TILES = 7 };
 int gemm_tile_rule(int M, int N);
 template <typename R>
 void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws = nullptr,
