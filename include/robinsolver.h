@@ -44,10 +44,13 @@ typedef struct rs_opts {
     int autotune;         /* 1: when a solve workspace is set up, time the matrix-
                              core tile shapes of each of its products and keep the
                              fastest (the results do not depend on the choice)   */
+    int graphs;           /* 1: rs_solve / rs_solve_device capture the launch
+                             sequence of each solve shape (precision, nrhs) once
+                             as a hipGraph and replay it                          */
 } rs_opts;
 
 /* Defaults: precision RS_FP64, skip_root_rtr 1, n_refine 2, verbose 0, no cap,
- * autotune 1. */
+ * autotune 1, graphs 1. */
 void rs_opts_default(rs_opts *o);
 
 /* Create / destroy a tree handle (type robin_tree, robin_tree_module.f90:55-77;

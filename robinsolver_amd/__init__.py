@@ -33,7 +33,8 @@ class RobinError(RuntimeError):
 
 class RsOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("skip_root_rtr", ctypes.c_int), ("n_refine", ctypes.c_int),
-                ("verbose", ctypes.c_int), ("hbm_budget", ctypes.c_longlong), ("autotune", ctypes.c_int)]
+                ("verbose", ctypes.c_int), ("hbm_budget", ctypes.c_longlong), ("autotune", ctypes.c_int),
+                ("graphs", ctypes.c_int)]
 
 
 def lib():
@@ -91,12 +92,13 @@ class GpuTree:
     """One factored tree resident on the GPU (C ABI handle)."""
 
     def __init__(self, d, L, domain, h_tgt, kb, bndry_params=None, precision=RS_FP64, skip_root_rtr=True,
-                 n_refine=2, hbm_budget=0, autotune=True):
+                 n_refine=2, hbm_budget=0, autotune=True, graphs=True):
         L_ = lib()
         o = RsOpts()
         L_.rs_opts_default(ctypes.byref(o))
         o.precision, o.skip_root_rtr, o.n_refine, o.hbm_budget = precision, int(skip_root_rtr), n_refine, hbm_budget
         o.autotune = int(autotune)
+        o.graphs = int(graphs)
         h = ctypes.c_void_p()
         if L_.rs_tree_create(ctypes.byref(h), ctypes.byref(o)):
             raise RobinError(L_.rs_last_error(None).decode())

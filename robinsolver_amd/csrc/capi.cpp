@@ -22,6 +22,7 @@ void rs_opts_default(rs_opts* o) {
     o->verbose = 0;
     o->hbm_budget = 0;
     o->autotune = 1;
+    o->graphs = 1;
 }
 
 int rs_tree_create(rs_tree** t, const rs_opts* o) {

@@ -748,19 +748,70 @@ int Tree::solve_device(int nrhs, const double2* f, const double2* g, double2* u,
     launch_col_table(tab + 2 * nrhs, u, nl * P.m, nrhs, st);
     RhsCols c;
     c.f = tab; c.g = tab + nrhs; c.u = tab + 2 * nrhs;
-    return solve_cols(nrhs, c, prec, st, 0);
+    return solve_cols(nrhs, c, prec, st, 0, true);
 }
 
-int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, int slot_id) {
+int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, int slot_id, bool graph) {
     if (!factored) return fail("Error: Robin Tree must be factored first.");
     if (nrhs < 1) return fail("Error: nrhs must be positive.");
     if (prec < 0) prec = opts.precision;
     if (!st) st = stream;
     Slot& sl = slot(slot_id);
+    // workspaces first: allocation and autotuning cannot be captured; a workspace
+    // that moves invalidates the slot's captured solves
+    if (prec == RS_MIXED) {
+        if (!have32 && make_c64()) return 1;
+        const void* before = sl.w32.ws.p;
+        const int cap = sl.w32.cap;
+        if (ensure_work(sl.w32, nrhs)) return 1;
+        if (opts.n_refine > 0 && ensure_refine(sl.w32)) return 1;
+        if (sl.w32.cap != cap || sl.w32.ws.p != before) sl.drop_graphs();
+    } else {
+        const void* before = sl.w64.ws.p;
+        const int cap = sl.w64.cap;
+        if (ensure_work(sl.w64, nrhs)) return 1;
+        if (sl.w64.cap != cap || sl.w64.ws.p != before) sl.drop_graphs();
+    }
+    if (!graph || !opts.graphs || prof_on) return solve_launches(nrhs, cols, prec, st, sl);
+    const int nref = prec == RS_MIXED ? opts.n_refine : 0;
+    GraphEnt* hit = nullptr;
+    for (GraphEnt& g : sl.graphs)
+        if (g.prec == prec && g.nrhs == nrhs && g.n_refine == nref && g.tabs == (const void*)cols.f) hit = &g;
+    if (!hit) {
+        // capture this shape's launch sequence once
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = solve_launches(nrhs, cols, prec, st, sl);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &gr);
+        if (rc || e != hipSuccess) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc ? 1 : fail(std::string("HIP error: stream capture: ") + hipGetErrorString(e));
+        }
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e2 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e2 != hipSuccess) return fail(std::string("HIP error: graph instantiation: ") + hipGetErrorString(e2));
+        if ((int)sl.graphs.size() >= Slot::kMaxGraphs) {
+            auto lru = std::min_element(sl.graphs.begin(), sl.graphs.end(),
+                                        [](const GraphEnt& a, const GraphEnt& b) { return a.used < b.used; });
+            (void)hipGraphExecDestroy(lru->exec);
+            sl.graphs.erase(lru);
+        }
+        GraphEnt g;
+        g.prec = prec; g.nrhs = nrhs; g.n_refine = nref; g.tabs = cols.f; g.exec = ex;
+        sl.graphs.push_back(g);
+        hit = &sl.graphs.back();
+    }
+    hit->used = ++sl.tick;
+    HIPCHK(hipGraphLaunch(hit->exec, st));
+    return 0;
+}
+
+// The launch sequence of one solve (no allocation, no synchronisation): the
+// workspaces of `sl` are ready.
+int Tree::solve_launches(int nrhs, const RhsCols& cols, int prec, hipStream_t st, Slot& sl) {
     if (prec == RS_MIXED) {
         SolveWork<float>& w32 = sl.w32;
-        if (!have32 && make_c64()) return 1;
-        if (ensure_work(w32, nrhs)) return 1;
         PassIO io;
         io.f64 = cols.f; io.g = cols.g;
         if (opts.n_refine <= 0) {
@@ -771,8 +822,6 @@ int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, in
         // the rounded iterate, and the correction restores full accuracy.  Then
         // u = u_{i} + K32^-1 (b - K u_{i}) for a fixed number of steps, so that a
         // right-hand side gets the same bits alone or batched.
-        const size_t nl = plan.n_leaves;
-        if (ensure_refine(w32)) return 1;
         float2* uA = w32.uA.as<float2>();
         io.u32 = uA;
         if (solve_pass(w32, nrhs, io, st)) return 1;
@@ -792,11 +841,9 @@ int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, in
         }
         return 0;
     }
-    SolveWork<double>& w64 = sl.w64;
-    if (ensure_work(w64, nrhs)) return 1;
     PassIO io;
     io.f64 = cols.f; io.g = cols.g; io.u64 = cols.u;
-    return solve_pass(w64, nrhs, io, st);
+    return solve_pass(sl.w64, nrhs, io, st);
 }
 
 int Tree::ensure_refine(SolveWork<float>& w) {

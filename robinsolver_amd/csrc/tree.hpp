@@ -77,12 +77,29 @@ struct RhsCols {
 
 // Solve workspaces of one concurrent caller (a stream): both precisions, and
 // the column tables of the contiguous-array entry points.
+// A captured solve: the launch sequence of one shape, replayed from a hipGraph.
+struct GraphEnt {
+    int prec = 0, nrhs = 0, n_refine = 0;
+    const void* tabs = nullptr;    // the column tables the graph reads (fixed addresses)
+    hipGraphExec_t exec = nullptr;
+    unsigned long long used = 0;
+};
+
 struct Slot {
     SolveWork<double> w64;
     SolveWork<float> w32;
     DevBuf tabs;
     int tabs_cap = 0;
-    void clear() { w64.clear(); w32.clear(); tabs.release(); tabs_cap = 0; }
+    std::vector<GraphEnt> graphs;  // at most kMaxGraphs, least recently used replaced
+    unsigned long long tick = 0;
+    static constexpr int kMaxGraphs = 8;
+    void drop_graphs() {
+        for (GraphEnt& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
+    void clear() { drop_graphs(); w64.clear(); w32.clear(); tabs.release(); tabs_cap = 0; }
+    ~Slot() { drop_graphs(); }
 };
 
 struct Tree {
@@ -151,7 +168,8 @@ struct Tree {
     int residual(int nrhs, double2* const* u64, const float2* u32, const double2* const* f,
                  const double2* const* g, float2* rf, float2* rg, float2* rj, hipStream_t st);
     int solve_device(int nrhs, const double2* f, const double2* g, double2* u, int prec, hipStream_t st);
-    int solve_cols(int nrhs, const RhsCols& c, int prec, hipStream_t st, int slot_id);
+    int solve_cols(int nrhs, const RhsCols& c, int prec, hipStream_t st, int slot_id, bool graph = false);
+    int solve_launches(int nrhs, const RhsCols& c, int prec, hipStream_t st, Slot& sl);
     int ensure_refine(SolveWork<float>& w);
     // allocate slot slot_id's workspace for cap columns at the tree's precision
     int reserve_slot(int slot_id, int cap);

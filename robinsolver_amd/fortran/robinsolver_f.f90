@@ -132,6 +132,7 @@ module rs_capi
         integer(c_int) :: verbose
         integer(c_long_long) :: hbm_budget
         integer(c_int) :: autotune
+        integer(c_int) :: graphs
     end type rs_opts_t
 
     interface

@@ -71,5 +71,6 @@ def test_opts_defaults_and_layout():
     this layout): FP64, skip the root RtR, two refinement steps, autotuning on."""
     o = rsa.RsOpts()
     rsa.lib().rs_opts_default(ctypes.byref(o))
-    assert (o.precision, o.skip_root_rtr, o.n_refine, o.verbose, o.hbm_budget, o.autotune) == (0, 1, 2, 0, 0, 1)
+    assert (o.precision, o.skip_root_rtr, o.n_refine, o.verbose, o.hbm_budget, o.autotune, o.graphs) == \
+        (0, 1, 2, 0, 0, 1, 1)
     assert ctypes.sizeof(rsa.RsOpts) == 32
