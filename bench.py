@@ -350,6 +350,8 @@ def main():
     ap.add_argument("--kb", type=float, default=80.0)
     ap.add_argument("--nrhs", type=int, default=256)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--dim", type=int, default=2, choices=(2, 3),
+                    help="3: the 3D workload (e.g. --dim 3 --grid 64 --levels 9 --kb 8 --nrhs 16)")
     ap.add_argument("--r1-solves", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=600)
@@ -400,14 +402,18 @@ def main():
     kb_phys = args.kb
     kb_opt = -args.kb          # Robin a1 = -i kb realised as opts%kb = -kb
     unit = [[0, 1], [0, 1], [0, 1]]
-    log("plan: 2D %d^2, L=%d, kb=%g, nrhs=%d" % (args.grid, args.levels, kb_phys, args.nrhs))
-    t = rsa.GpuTree(2, args.levels, unit, 1.0 / args.grid, kb_opt, precision=rsa.RS_MIXED)
+    d = args.dim
+    if d == 3:
+        args.no_cpu_baseline = True    # the CPU sample and the serving scenarios are 2D
+        args.no_serve = True
+    log("plan: %dD %d^%d, L=%d, kb=%g, nrhs=%d" % (d, args.grid, d, args.levels, kb_phys, args.nrhs))
+    t = rsa.GpuTree(d, args.levels, unit, 1.0 / args.grid, kb_opt, precision=rsa.RS_MIXED)
     pts, _ = t.all_points()
     pint = pts[:, t.nb:, :]
-    k = potential(pint, kb_phys, args.seed)
+    k = potential(pint, kb_phys, args.seed, nmodes=16 if d == 2 else 24, d=d)
     k_dev = torch.from_numpy(k).to(dev)
     pint_dev = torch.from_numpy(np.ascontiguousarray(pint)).to(dev)
-    f = gaussian_sources(torch, pint_dev, args.nrhs, args.seed)
+    f = gaussian_sources(torch, pint_dev, args.nrhs, args.seed, sigma=0.02 if d == 2 else 0.05, d=d)
     g = torch.zeros((args.nrhs, t.n_leaves, t.nb), dtype=torch.complex128, device=dev)
     u64 = torch.zeros((args.nrhs, t.n_leaves, t.m), dtype=torch.complex128, device=dev)
     u32 = torch.zeros_like(u64)
@@ -419,7 +425,7 @@ def main():
     f_time_mixed = t.timings()[0]
     t.factor(k)
     f_time_mixed = min(f_time_mixed, t.timings()[0])
-    t64 = rsa.GpuTree(2, args.levels, unit, 1.0 / args.grid, kb_opt, precision=rsa.RS_FP64)
+    t64 = rsa.GpuTree(d, args.levels, unit, 1.0 / args.grid, kb_opt, precision=rsa.RS_FP64)
     t64.factor(k)
     f_time_64 = t64.timings()[0]
     t64.factor(k)
@@ -542,7 +548,8 @@ def main():
 
     ff, fs = t.algorithmic_counts()
     out = {
-        "metric": "solves/s + factor s, 2D 1024^2 Helmholtz; mixed vs FP64 GPU at rel.l2<=1e-8",
+        "metric": ("solves/s + factor s, 2D 1024^2 Helmholtz; mixed vs FP64 GPU at rel.l2<=1e-8" if d == 2 else
+                   "solves/s + factor s, 3D %d^3 Helmholtz; mixed vs FP64 GPU at rel.l2<=1e-8" % args.grid),
         "value": mixed_rate,
         "unit": "solves/s",
         "n_gpus": 1,
@@ -554,9 +561,12 @@ def main():
         "vs_baseline": None,
         "dtype": "c64",
         "data": "synthetic",
-        "config": {"workload": "2D %d^2 Helmholtz, L=%d (16x16 leaves), kb=%g, seeded 16-mode potential, "
-                               "%d Gaussian-source RHS per step" % (args.grid, args.levels, kb_phys, args.nrhs),
-                   "grid": args.grid, "levels": args.levels, "kb": kb_phys, "nrhs": args.nrhs, "seed": args.seed},
+        "config": {"workload": "%dD %d^%d Helmholtz, L=%d (%s leaves), kb=%g, seeded %d-mode potential, "
+                               "%d Gaussian-source RHS per step"
+                               % (d, args.grid, d, args.levels, "x".join(str(x) for x in t.leaf_grid()), kb_phys,
+                                  16 if d == 2 else 24, args.nrhs),
+                   "dim": d, "grid": args.grid, "levels": args.levels, "kb": kb_phys, "nrhs": args.nrhs,
+                   "seed": args.seed},
         "factor_s_fp64": f_time_64,
         "factor_s_mixed": f_time_mixed,
         "fp64_r1_solves_per_s": r1_rate,

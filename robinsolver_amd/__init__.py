@@ -74,6 +74,7 @@ def lib():
         L.rs_memory_used.argtypes = [vp, llp, llp]
         L.rs_residual_norms.argtypes = [vp, ctypes.c_int, vp, vp, vp, dp]
         L.rs_tile_choices.argtypes = [vp, ctypes.c_int, ip]
+        L.rs_leaf_grid.argtypes = [vp, ip, dp]
         from . import serve as _serve
         _serve.bind(L)
         _lib = L
@@ -185,6 +186,13 @@ class GpuTree:
         self._check(lib().rs_residual_norms(self.h, nrhs, ctypes.c_void_p(f_ptr), ctypes.c_void_p(g_ptr),
                                             ctypes.c_void_p(u_ptr), _dp(out)))
         return out
+
+    def leaf_grid(self):
+        """Points per axis of every leaf (d entries)."""
+        pb = np.zeros(3, np.int32)
+        h = np.zeros(3)
+        self._check(lib().rs_leaf_grid(self.h, pb.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(h)))
+        return tuple(int(x) for x in pb[:self.d])
 
     def tile_choices(self, precision):
         """GEMM tile chosen per level and product of the solve workspace: [L][4] (-1: shape rule / none)."""

@@ -65,6 +65,7 @@ __global__ void __launch_bounds__(256) inv_small_kernel(double2* __restrict__ ma
         double best;
         int p;
         block_argmax(v, vi, rv, ri, best, p);
+        if (!(best > 0.0) || p >= n) p = j;   // singular: no interchange, reported through info
         if (tid == 0) { piv[j] = p; if (!(best > 0.0)) bad = 1; }
         if (p != j)
             for (int c = tid; c < n; c += nt) {
@@ -120,6 +121,7 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(double2* __restrict__ mat
         double best;
         int p;
         block_argmax(v, vi, rv, ri, best, p);
+        if (!(best > 0.0) || p >= n) p = j;   // singular: no interchange, reported through info
         if (tid == 0) { piv[j] = p; if (!(best > 0.0)) bad = 1; }
         if (p != j)
             for (int c = tid; c < nb; c += nt) {
@@ -182,7 +184,7 @@ __device__ __forceinline__ void gj_reg_step(PanelRegs<NB, RPT>& P, int n, int j0
     int p = ri[0];
     for (int q = 1; q < nw; ++q)
         if (rv[q] > best || (rv[q] == best && ri[q] < p)) { best = rv[q]; p = ri[q]; }
-    bad |= !(best > 0.0);
+    if (!(best > 0.0) || p >= n) { bad = true; p = j; }   // singular: no interchange, reported through info
     // publish old row j and the scaled pivot row
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {

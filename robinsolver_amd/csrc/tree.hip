@@ -136,6 +136,7 @@ void Tree::free_factor() {
         v.clear();
     };
     drop(T); drop(invM); drop(T32); drop(invM32); drop(fdesc);
+    ftile.clear();
     for (DevBuf* b : {&sinv, &sinv32, &info, &fscratch}) b->release();
     have32 = false;
     factored = false;
@@ -374,7 +375,35 @@ int Tree::factor_device(const double2* k) {
         launch_leaf_apply<double>(G, a, lds_limit, stream);
     }
     HIPCHK(hipGetLastError());
-    // merges, bottom-up; every level of one launch sequence on the tree's stream
+    // merges, bottom-up; every level of one launch sequence on the tree's stream.
+    // The merge products are idempotent (each writes a buffer it does not read),
+    // so the first factorization of a plan times their tile shapes and keeps the
+    // fastest; later factorizations reuse the choice (results do not depend on it).
+    const bool tune = opts.autotune && (int)ftile.size() != 4 * L;
+    if (tune) ftile.assign(4 * L, -1);
+    hipEvent_t te0 = nullptr, te1 = nullptr;
+    if (tune) {
+        HIPCHK(hipEventCreate(&te0));
+        HIPCHK(hipEventCreate(&te1));
+    }
+    auto fgemm = [&](int site, const GemmDesc* d, int np, int M, int N, int K, void* gws) -> int {
+        int tile = site < (int)ftile.size() ? ftile[site] : -1;
+        if (tune) {
+            float best = 0.f;
+            for (int t = 0; t < GEMM_TILES; ++t) {
+                (void)hipEventRecord(te0, stream);
+                launch_gemm<double>(d, np, M, N, K, stream, gws, t);
+                (void)hipEventRecord(te1, stream);
+                if (hipEventSynchronize(te1) != hipSuccess) return fail("Error: GEMM autotuning failed.");
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, te0, te1);
+                if (t == 0 || ms < best) { best = ms; tile = t; }
+            }
+            ftile[site] = tile;
+        }
+        launch_gemm<double>(d, np, M, N, K, stream, gws, tile);
+        return 0;
+    };
     for (int l = L - 1; l >= 0; --l) {
         const Level& V = P.lv[l];
         const MergeLayout& m = ml[l];
@@ -396,17 +425,20 @@ int Tree::factor_device(const double2* k) {
         launch_form_iface(Sm, Tc, tstride, 2, 0, b, bn, i, nodes, zeta, make_double2(-nu2.x, -nu2.y), Dm, stream);
         launch_inverse(m.ip, Sm, piv, W, dl[1]->as<GemmDesc>(), info.as<int>(), stream);
         // invM = [S, -nu S D; -nu D S, D + nu^2 D S D]
-        launch_gemm<double>(dl[2]->as<GemmDesc>(), nodes, i, i, i, stream, gws);
-        launch_gemm<double>(dl[2]->as<GemmDesc>() + nodes, 2 * nodes, i, i, i, stream, gws);
+        if (fgemm(4 * l + 0, dl[2]->as<GemmDesc>(), nodes, i, i, i, gws) ||
+            fgemm(4 * l + 1, dl[2]->as<GemmDesc>() + nodes, 2 * nodes, i, i, i, gws))
+            return 1;
         launch_copy_blocks(IM, 4 * ii, 2 * i, Sm, ii, i, i, i, nodes, make_double2(1, 0), stream);
         launch_copy_blocks(IM + (size_t)i * 2 * i, 4 * ii, 2 * i, SD, ii, i, i, i, nodes, make_double2(-nu.x, -nu.y),
                            stream);
         // Y = invM diag(T1_IN, T2_IN); parent T = diag(T_NN) + diag(T_NI) Y
-        launch_gemm<double>(dl[3]->as<GemmDesc>(), 2 * nodes, 2 * i, bn, i, stream, gws);
-        if (l > 0 || !opts.skip_root_rtr)
-            launch_gemm<double>(dl[4]->as<GemmDesc>(), 4 * nodes, bn, bn, i, stream, gws);
+        if (fgemm(4 * l + 2, dl[3]->as<GemmDesc>(), 2 * nodes, 2 * i, bn, i, gws)) return 1;
+        if ((l > 0 || !opts.skip_root_rtr) && fgemm(4 * l + 3, dl[4]->as<GemmDesc>(), 4 * nodes, bn, bn, i, gws))
+            return 1;
         HIPCHK(hipGetLastError());
     }
+    if (te0) (void)hipEventDestroy(te0);
+    if (te1) (void)hipEventDestroy(te1);
     int hinfo = 0;
     HIPCHK(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));

@@ -118,6 +118,7 @@ struct Tree {
     std::vector<DevBuf*> T;      // per level: [nodes][P][P] node order
     std::vector<DevBuf*> invM;   // per level < L: [nodes][2i][2i]
     DevBuf fscratch;              // merge scratch of the largest level (kept for refactors)
+    std::vector<int> ftile;       // per level, 4 merge products: GEMM tile chosen by the first factor's timing
     std::vector<DevBuf*> fdesc;   // per level: the merge's product descriptors (kept for refactors)
     // complex64 copies for the mixed solve
     DevBuf sinv32;

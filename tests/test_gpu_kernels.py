@@ -184,3 +184,15 @@ def test_autotuned_solve_matches_untuned_bits(gpu):
             print("tiles:", ch.tolist(), flush=True)
             assert ch.shape == (6, 4) and (ch[1:] >= 0).all()
     assert np.array_equal(res[0], res[1])
+
+
+@pytest.mark.parametrize("n", [40, 256, 1024])
+def test_inverse_reports_singular_matrix(n, gpu):
+    """A column with no nonzero candidate: the elimination makes no interchange
+    (no out-of-range row index) and the call reports the singular matrix."""
+    print("case: inverse singular", n, flush=True)
+    rng = np.random.default_rng(n + 7)
+    M = _rc(rng, 1, n, n)[0]
+    M[:, n // 3] = 0.0
+    flat = np.ascontiguousarray(np.tile(M.reshape(-1, order="F"), 2))
+    assert _lib().rs_check_inverse(n, 2, _dp(flat.view(np.float64))) == 2
