@@ -603,9 +603,16 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     extern __shared__ float2 lds2[];
     const int s = G.s, ni = G.n_int, nb = G.nb, m = G.m, nl = io.n_leaves;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    float2* Bl = lds2;                                   // [K][16 x 16], column-major
-    float2* scr = lds2 + K * 256 + wave * 512;           // two 16x16 tiles, row-major
-    float2* bd = lds2 + K * 256 + 4 * 512 + wave * 16 * nb;  // per-column boundary data (g or jump)
+    // LDS: the K blocks [K][16 x 16] column-major; per wave one 16 x 16 tile,
+    // row-major with row stride LT (= 17: rows fall in different banks), and the
+    // boundary data of its 16 columns, column stride BS = nb + 1.  The tile is
+    // reused from one step to the next without a second copy: a wave's LDS
+    // accesses are processed in order.
+    constexpr int LT = 17, LTS = 16 * LT;
+    const int BS = nb + 1;
+    float2* Bl = lds2;
+    float2* scr = lds2 + K * 256 + wave * LTS;
+    float2* bd = lds2 + K * 256 + 4 * LTS + wave * 16 * BS;
     const int leaf = blockIdx.x / nchunks, chunk = blockIdx.x - leaf * nchunks;
     const int ccol = lane & 15, grp = lane >> 4;
     const int mycol = chunk * 64 + wave * 16 + ccol;
@@ -665,11 +672,11 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
                 if (e < ntot) {
                     const int c = e / nb, i = e - c * nb;
                     const int pos = (MODE == LEAF_DOWN) ? i : local_to_node(G, parity, i);
-                    bd[c * nb + pos] = ((wcol0 + c) < io.ncols) ? t[j] : make_float2(0.f, 0.f);
+                    bd[c * BS + pos] = ((wcol0 + c) < io.ncols) ? t[j] : make_float2(0.f, 0.f);
                 }
             }
         } else {
-            for (int e = lane; e < ntot; e += 64) bd[e] = make_float2(0.f, 0.f);
+            for (int e = lane; e < ntot; e += 64) bd[(e / nb) * BS + e % nb] = make_float2(0.f, 0.f);
         }
     }
     __syncthreads();
@@ -678,27 +685,25 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     const AxisCoef& ax1 = C.ax[1];
     const float2 cbx = ccast<float2>(cscale(cinv(ax0.dd), -ax0.cib));
     const float2 cby = ccast<float2>(cscale(cinv(ax1.dd), -ax1.cib));
-    auto gval = [&](int face, int t) -> float2 { return bd[ccol * nb + G.noff[parity][face] + t]; };
+    auto gval = [&](int face, int t) -> float2 { return bd[ccol * BS + G.noff[parity][face] + t]; };
 
     float2 y[K][4];
     float2 prev[4];
-    int buf = 0;
     auto product = [&](int k, const float2 v[4], f4v& aR, f4v& aI) {
-        float2* sb = scr + buf * 256;
-        for (int vv = 0; vv < 4; ++vv) sb[(4 * grp + vv) * 16 + ccol] = v[vv];
+        float2* sb = scr;
+        for (int vv = 0; vv < 4; ++vv) sb[(4 * grp + vv) * LT + ccol] = v[vv];
         wave_lds_sync();
         aR = f4v{0.f, 0.f, 0.f, 0.f};
         aI = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const float2 a = Bl[k * 256 + ccol + 16 * (4 * q + grp)];
-            const float2 b = sb[(4 * q + grp) * 16 + ccol];
+            const float2 b = sb[(4 * q + grp) * LT + ccol];
             aR = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, aR, 0, 0, 0);
             aR = __builtin_amdgcn_mfma_f32_16x16x4f32(-a.y, b.y, aR, 0, 0, 0);
             aI = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.y, aI, 0, 0, 0);
             aI = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.x, aI, 0, 0, 0);
         }
-        buf ^= 1;
     };
 
     // raw[i] = f at row rr of layer k, column cc(i) (coalesced mapping; zero for
@@ -718,11 +723,11 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     for (int k = 0; k < K; ++k) {
         // the layer's f tile into the wave's LDS tile, then read back in the MFMA layout
         {
-            float2* sb = scr + buf * 256;
+            float2* sb = scr;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const float2 x = ccast<float2>(fbuf[k % PF][i]);
-                sb[rr * 16 + 4 * i + cq] = make_float2(x.x * fmask[i], x.y * fmask[i]);
+                sb[rr * LT + 4 * i + cq] = make_float2(x.x * fmask[i], x.y * fmask[i]);
             }
             wave_lds_sync();
         }
@@ -731,7 +736,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) {
             const int row = 4 * grp + vv;
-            float2 r = scr[buf * 256 + row * 16 + ccol];
+            float2 r = scr[row * LT + ccol];
             if (MODE == LEAF_DOWN && row < s && colok) {
                 if (row == 0) r = cfma(cbx, gval(1, k), r);
                 if (row == s - 1) r = cfma(cbx, gval(2, k), r);
@@ -792,17 +797,16 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
 #pragma unroll
             for (int i = 0; i < 4; ++i) base[i] = bbuf[k & 1][i];
             if (k + 2 < K) bfetch(k + 2, bbuf[k & 1]);
-            float2* sb = scr + buf * 256;
+            float2* sb = scr;
 #pragma unroll
-            for (int vv = 0; vv < 4; ++vv) sb[(4 * grp + vv) * 16 + ccol] = y[k][vv];
+            for (int vv = 0; vv < 4; ++vv) sb[(4 * grp + vv) * LT + ccol] = y[k][vv];
             wave_lds_sync();
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int col = wcol0 + 4 * i + cq;
-                if (fok[i]) put_col(col, ucp[i], nb + k * s + rr, sb[rr * 16 + 4 * i + cq], base[i]);
+                if (fok[i]) put_col(col, ucp[i], nb + k * s + rr, sb[rr * LT + 4 * i + cq], base[i]);
             }
-            buf ^= 1;
-        }
+            }
     }
     // boundary values: u_B = (g - a1 x1 - a2 x2) / dd (down) or h = g + (sigma - alpha) u_B
     // less the transmission residual (up), computed into the column's boundary slots of
@@ -810,7 +814,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     // stored afterwards with coalesced writes
     const float2 sma = make_float2(0.f, (float)(-2.0 * G.kb));
     auto emit = [&](int face, int t, float2 x1, float2 x2, const AxisCoef& a) {
-        float2* slot = bd + ccol * nb + G.noff[parity][face] + t;
+        float2* slot = bd + ccol * BS + G.noff[parity][face] + t;
         const float2 gb = (MODE == LEAF_DOWN) ? *slot : make_float2(0.f, 0.f);
         float2 num = csub(csub(gb, cscale(x1, (float)a.a1)), cscale(x2, (float)a.a2));
         float2 uB = cmul(num, ccast<float2>(cinv(a.dd)));
@@ -854,9 +858,9 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
             const int col = wcol0 + cidx[j];
             if (e >= ntot || col >= io.ncols) continue;
             if (MODE == LEAF_UP) {
-                io.hout[((size_t)col * nl + leaf) * nb + pidx[j]] = bd[cidx[j] * nb + pidx[j]];
+                io.hout[((size_t)col * nl + leaf) * nb + pidx[j]] = bd[cidx[j] * BS + pidx[j]];
             } else {
-                const float2 x = bd[cidx[j] * nb + local_to_node(G, parity, pidx[j])];
+                const float2 x = bd[cidx[j] * BS + local_to_node(G, parity, pidx[j])];
                 put_col(col, (OUT != OUT_U32) ? io.u64[col] + (size_t)leaf * m : nullptr, pidx[j], x, base[j]);
             }
         }
@@ -971,7 +975,7 @@ void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit
     if constexpr (std::is_same<R, float>::value) {
         if (mfma_leaf_ok(G, a)) {
             const int nchunks = (ncols + 63) / 64;
-            const size_t lds = ((size_t)G.K * 256 + 4 * 512 + 4 * 16 * (size_t)G.nb) * sizeof(float2);
+            const size_t lds = ((size_t)G.K * 256 + 4 * 16 * 17 + 4 * 16 * (size_t)(G.nb + 1)) * sizeof(float2);
             dim3 grid((unsigned)n_leaves * nchunks);
 #define RS_LEAF_MFMA(KK)                                                                                        \
     do {                                                                                                         \
