@@ -69,10 +69,11 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
     // operand tiles; after the K loop the same memory holds the output tile
     // (column stride BM + 1, so the 16 columns of one MFMA row group fall in
     // different banks)
-    constexpr int SMEM = (BK * (BM + BN) > BN * (BM + 1)) ? BK * (BM + BN) : BN * (BM + 1);
+    // (operand rows padded by one element against bank conflicts)
+    constexpr int SMEM = (BK * (BM + BN + 2) > BN * (BM + 1)) ? BK * (BM + BN + 2) : BN * (BM + 1);
     __shared__ V smem[SMEM];
-    V (*As)[BM] = (V (*)[BM])smem;
-    V (*Bs)[BN] = (V (*)[BN])(smem + BK * BM);
+    V (*As)[BM + 1] = (V (*)[BM + 1])smem;
+    V (*Bs)[BN + 1] = (V (*)[BN + 1])(smem + BK * (BM + 1));
 
     const int tiles = tilesM * tilesN;
     const int per = tiles * kslices;
