@@ -421,9 +421,10 @@ __device__ inline const float2* ucolumn<float2>(double2* const*, const float2* u
     return u32 + (size_t)col * nl * m;
 }
 
-// One thread per (leaf, point); the point's stencil and neighbour geometry is
-// worked out once and the loads of RES_CPT columns are issued together.
-constexpr int RES_CPT = 4;
+// One thread per (leaf, point, RES_CPT columns); the loads of a thread are issued
+// together.  More columns per thread share the geometry but lower occupancy:
+// at 1024^2 with 256 right-hand sides 1, 2, 4, 8 measured 2.82, 2.86, 3.11, 3.65 ms.
+constexpr int RES_CPT = 1;
 
 template <typename U>
 __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, double2* const* __restrict__ u64,
