@@ -546,6 +546,25 @@ def main():
     res32 = t.residual_norms(args.nrhs, f.data_ptr(), g.data_ptr(), u32.data_ptr())
     log("FD residual: fp64 max %.2e, mixed max %.2e" % (res64.max(), res32.max()))
 
+    # the same factorization with complex64 merge products at every level
+    # (merge_fp64_levels 0): its time, and the refinement it then needs
+    tc = rsa.GpuTree(d, args.levels, unit, 1.0 / args.grid, kb_opt, precision=rsa.RS_MIXED, merge_fp64_levels=0)
+    tc.factor(k)
+    tc.factor(k)
+    f_time_c64 = tc.timings()[0]
+    c64_err = {}
+    for nr in (0, 1, 2):
+        tc.set_refine(nr)
+        solve(tc, args.nrhs, f.data_ptr(), g.data_ptr(), u32.data_ptr(), rsa.RS_MIXED)
+        torch.cuda.synchronize()
+        c64_err[nr] = float(max_rel(u32, u64).max())
+    tc.close()
+    c64_need = min([nr for nr in c64_err if c64_err[nr] <= 1e-8], default=None)
+    log("complex64 merges: factor %.4f s, max rel l2 by refinement steps %s" % (f_time_c64, c64_err))
+    # u32 holds the timed mixed results again
+    solve(t, args.nrhs, f.data_ptr(), g.data_ptr(), u32.data_ptr(), rsa.RS_MIXED)
+    torch.cuda.synchronize()
+
     ff, fs = t.algorithmic_counts()
     out = {
         "metric": ("solves/s + factor s, 2D 1024^2 Helmholtz; mixed vs FP64 GPU at rel.l2<=1e-8" if d == 2 else
@@ -575,6 +594,8 @@ def main():
         "mixed_rel_l2_vs_fp64": {"max": float(rel.max()), "median": float(np.median(rel))},
         "refinement_steps": n_ref,
         "mixed_max_rel_l2_by_refinement_steps": refine_err,
+        "complex64_merges": {"merge_fp64_levels": 0, "factor_s": f_time_c64,
+                             "max_rel_l2_by_refinement_steps": c64_err, "refinement_steps_needed": c64_need},
         "reference_flops": {"factor_GF": ff / 1e9, "solve_GF_per_rhs": fs / 1e9},
         "roofline": roof,
         "kernel_ms_per_step": {c: v[0] / args.steps for c, v in prof.items()},

@@ -31,12 +31,21 @@ typedef struct rs_tree rs_tree;
 
 enum {
     RS_FP64 = 0,   /* complex128 everywhere: the plain FP64 GPU path          */
-    RS_MIXED = 1   /* complex64 solve passes on the matrix cores, FP64 factor
-                      interface inverses, FP64 residual + iterative refinement */
+    RS_MIXED = 1,  /* complex64 merge products and solve passes on the matrix
+                      cores (FP32 accumulation), FP64 leaf factorization and
+                      pivoted interface inverses, FP64 residual + iterative
+                      refinement                                             */
+    RS_MIXED_MERGE = 2  /* rs_check_gemm only: complex64 products of complex128
+                           operands, complex128 result (the mixed merge)     */
 };
 
 typedef struct rs_opts {
-    int precision;        /* RS_FP64 or RS_MIXED (default solve precision)      */
+    int precision;        /* RS_FP64 or RS_MIXED: the factorization's merge
+                             products and the default solve precision.  A tree
+                             factored in FP64 also takes RS_MIXED solves (FP64
+                             factor data rounded to complex64 for the passes);
+                             a tree with complex64 merges refuses RS_FP64
+                             solves, whose accuracy it cannot give            */
     int skip_root_rtr;    /* 1: do not form the root RtR, which no solve reads  */
     int n_refine;         /* refinement steps of the mixed path                 */
     int verbose;          /* 1: progress lines as the reference prints them     */
@@ -47,10 +56,14 @@ typedef struct rs_opts {
     int graphs;           /* 1: rs_solve / rs_solve_device capture the launch
                              sequence of each solve shape (precision, nrhs) once
                              as a hipGraph and replay it                          */
+    int merge_fp64_levels;/* RS_MIXED trees: k >= 0 keeps complex128 merge
+                             products at the top k levels (root first) and
+                             runs the levels below in complex64; -1 (or k >= L)
+                             keeps the whole factorization FP64               */
 } rs_opts;
 
 /* Defaults: precision RS_FP64, skip_root_rtr 1, n_refine 2, verbose 0, no cap,
- * autotune 1, graphs 1. */
+ * autotune 1, graphs 1, merge_fp64_levels -1. */
 void rs_opts_default(rs_opts *o);
 
 /* Create / destroy a tree handle (type robin_tree, robin_tree_module.f90:55-77;
@@ -155,7 +168,9 @@ int rs_node_rtr(const rs_tree *t, int lvl, int idx, double *out, int *P);
 /* Kernel checks (test hooks, no reference counterpart), host memory in and out:
  *   rs_check_gemm: C = alpha A B + beta C for one column-major complex problem
  *     (M x K times K x N) through the batched MFMA GEMM, precision RS_FP64
- *     (complex128) or RS_MIXED (complex64 arithmetic on complex128 data);
+ *     (complex128), RS_MIXED (the complex64 GEMM of the solve passes: data
+ *     rounded to complex64 on the host, complex64 result) or RS_MIXED_MERGE
+ *     (complex64 products of complex128 data, complex128 result);
  *     rows k >= ksplit of B are taken from B2 (ksplit >= K: unused); rmap, when
  *     not NULL, scatters output row r to row rmap[r] of C (ldc rows).
  *   rs_check_inverse: in-place inverse of `batch` n x n complex128 matrices,

@@ -23,6 +23,7 @@ void rs_opts_default(rs_opts* o) {
     o->hbm_budget = 0;
     o->autotune = 1;
     o->graphs = 1;
+    o->merge_fp64_levels = -1;
 }
 
 int rs_tree_create(rs_tree** t, const rs_opts* o) {
@@ -195,10 +196,11 @@ std::vector<V> to_prec(const double* p, size_t n) {
     return v;
 }
 
-template <typename R>
+// S: precision of the operands and result, R: of the products
+template <typename S, typename R>
 int check_gemm_t(int tile, int M, int N, int K, const double* A, int lda, const double* B, const double* B2, int ldb,
                  int ksplit, double* C, int ldc, const int* rmap, const double* alpha, const double* beta) {
-    using V = rs::c_t<R>;
+    using V = rs::c_t<S>;
     size_t na = (size_t)lda * K, nb = (size_t)ldb * N, nc = (size_t)ldc * N;
     std::vector<V> hA = to_prec<V>(A, na), hB = to_prec<V>(B, nb), hC = to_prec<V>(C, nc);
     std::vector<V> hB2 = B2 ? to_prec<V>(B2, nb) : std::vector<V>();
@@ -230,8 +232,11 @@ int check_gemm_t(int tile, int M, int N, int K, const double* A, int lda, const 
     (void)hipMemcpy(dd, &d, sizeof d, hipMemcpyHostToDevice);
     void* ws = nullptr;
     size_t wse = rs::gemm_ws_elems(1, M, N, K);
-    if (wse) (void)hipMalloc(&ws, wse * sizeof(V));
-    rs::launch_gemm<R>(dd, 1, M, N, K, nullptr, ws, tile);
+    if (wse) (void)hipMalloc(&ws, wse * sizeof(rs::c_t<R>));
+    if (sizeof(S) != sizeof(R))
+        rs::launch_gemm_c64(dd, 1, M, N, K, nullptr, ws, tile);
+    else
+        rs::launch_gemm<R>(dd, 1, M, N, K, nullptr, ws, tile);
     hipError_t e = hipDeviceSynchronize();
     if (ws) (void)hipFree(ws);
     if (e == hipSuccess) e = hipMemcpy(hC.data(), dC, nc * sizeof(V), hipMemcpyDeviceToHost);
@@ -251,8 +256,10 @@ int rs_check_gemm_tile(int precision, int M, int N, int K, const double* A, int 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_err = "Error: no HIP device."; return 1; }
     if (precision == RS_MIXED)
-        return check_gemm_t<float>(tile, M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta);
-    return check_gemm_t<double>(tile, M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta);
+        return check_gemm_t<float, float>(tile, M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta);
+    if (precision == RS_MIXED_MERGE)
+        return check_gemm_t<double, float>(tile, M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta);
+    return check_gemm_t<double, double>(tile, M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta);
 }
 
 int rs_check_gemm(int precision, int M, int N, int K, const double* A, int lda, const double* B, const double* B2,

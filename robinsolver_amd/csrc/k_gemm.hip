@@ -53,11 +53,17 @@ constexpr int BK = 32;
 // current ones, staged in LDS, feed the matrix cores.  With kslices > 1 the raw
 // partial products go to a workspace and gemm_reduce_kernel sums the slices in
 // a fixed order.
-template <typename R, int BM, int BN>
+//
+// S is the precision of A, B, Din and Out, R the precision of the products:
+// S = double, R = float is the complex64 merge of the mixed-precision
+// factorization (operands rounded to complex64 as they are staged, FP32
+// accumulation, the epilogue and the stored result in FP64).
+template <typename S, typename R, int BM, int BN>
 __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ descs, int M, int N, int K,
                                                   int tilesM, int tilesN, int kslices, int kchunk,
                                                   c_t<R>* __restrict__ ws) {
     using V = c_t<R>;
+    using VS = c_t<S>;
     using A4 = typename acc_t<R>::T;
     // four waves: 2x2 for 64x64, 32x64, 32x32 and 64x32 tiles, 4x1 for 64x16 and
     // 128x16, 1x4 for 16x64
@@ -83,9 +89,9 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
     const int t = rem - ks * tiles;
     const int tm = t % tilesM, tn = t / tilesM;
     const GemmDesc& D = descs[p];
-    const V* __restrict__ A = (const V*)D.A;
-    const V* __restrict__ B = (const V*)D.B;
-    const V* __restrict__ B2 = (const V*)D.B2;
+    const VS* __restrict__ A = (const VS*)D.A;
+    const VS* __restrict__ B = (const VS*)D.B;
+    const VS* __restrict__ B2 = (const VS*)D.B2;
     const int m0 = tm * BM, n0 = tn * BN;
     const int kbeg = ks * kchunk, kend = min(K, kbeg + kchunk);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -103,18 +109,19 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
             const int e = tid + 256 * q;
             const int r = e % BM, k = e / BM;
             const int gr = m0 + r, gk = k0 + k;
-            ra[q] = (gr < M && gk < kend) ? A[(size_t)gr + (size_t)gk * D.lda] : cz<R>();
+            const VS a = (gr < M && gk < kend) ? A[(size_t)gr + (size_t)gk * D.lda] : cz<S>();
+            ra[q] = mk<R>((R)a.x, (R)a.y);
         }
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
             const int e = tid + 256 * q;
             const int k = e % BK, c = e / BK;
             const int gk = k0 + k, gc = n0 + c;
-            V v = cz<R>();
+            VS v = cz<S>();
             if (gk < kend && gc < N)
                 v = (gk < D.ksplit) ? B[(size_t)gk + (size_t)gc * D.ldb]
                                     : B2[(size_t)(gk - D.ksplit) + (size_t)gc * D.ldb];
-            rb[q] = v;
+            rb[q] = mk<R>((R)v.x, (R)v.y);
         }
     };
     auto stash = [&]() {
@@ -185,9 +192,9 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
         }
         return;
     }
-    V* Out = (V*)D.Out;
-    const V* Din = (const V*)D.Din;
-    const V alpha = mk<R>((R)D.ar, (R)D.ai), beta = mk<R>((R)D.br, (R)D.bi);
+    VS* Out = (VS*)D.Out;
+    const VS* Din = (const VS*)D.Din;
+    const VS alpha = mk<S>((S)D.ar, (S)D.ai), beta = mk<S>((S)D.br, (S)D.bi);
     // all index-map and Din loads of the thread are issued back to back (branches
     // only around whole loops), then the results are formed and stored
     int rowc[NE], colc[NE], orow[NE], ocol[NE];
@@ -214,14 +221,15 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
 #pragma unroll
         for (int q = 0; q < NE; ++q) ocol[q] = colc[q];
     }
-    V val[NE];
+    VS val[NE];
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
         const int e = tid + 256 * q;
-        val[q] = cmul(alpha, Ct[(e / BM) * (BM + 1) + e % BM]);
+        const V c = Ct[(e / BM) * (BM + 1) + e % BM];
+        val[q] = cmul(alpha, mk<S>((S)c.x, (S)c.y));
     }
     if (Din) {
-        V din[NE];
+        VS din[NE];
         if (D.dmap) {
 #pragma unroll
             for (int q = 0; q < NE; ++q) din[q] = Din[(size_t)orow[q] + (size_t)ocol[q] * D.ldd];
@@ -238,10 +246,12 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
 }
 
 // Sum of the K slices in slice order, then the epilogue of gemm_kernel.
-template <typename R>
+// (The slices are summed in precision R, the epilogue runs in S.)
+template <typename S, typename R>
 __global__ void gemm_reduce_kernel(const GemmDesc* __restrict__ descs, int M, int N, int kslices,
                                    const c_t<R>* __restrict__ ws) {
     using V = c_t<R>;
+    using VS = c_t<S>;
     const size_t mn = (size_t)M * N;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int p = blockIdx.y;
@@ -251,15 +261,15 @@ __global__ void gemm_reduce_kernel(const GemmDesc* __restrict__ descs, int M, in
     const V* W = ws + (size_t)p * kslices * mn + e;
     V sum = W[0];
     for (int k = 1; k < kslices; ++k) sum = cadd(sum, W[(size_t)k * mn]);
-    V val = cmul(mk<R>((R)D.ar, (R)D.ai), sum);
+    VS val = cmul(mk<S>((S)D.ar, (S)D.ai), mk<S>((S)sum.x, (S)sum.y));
     const int orow = D.rmap ? D.rmap[row] : row;
     const int ocol = D.cmap ? D.cmap[col] : col;
-    const V* Din = (const V*)D.Din;
+    const VS* Din = (const VS*)D.Din;
     if (Din) {
         size_t di = D.dmap ? (size_t)orow + (size_t)ocol * D.ldd : (size_t)row + (size_t)col * D.ldd;
-        val = cfma(mk<R>((R)D.br, (R)D.bi), Din[di], val);
+        val = cfma(mk<S>((S)D.br, (S)D.bi), Din[di], val);
     }
-    ((V*)D.Out)[(size_t)orow + (size_t)ocol * D.ldo] = val;
+    ((VS*)D.Out)[(size_t)orow + (size_t)ocol * D.ldo] = val;
 }
 
 }  // namespace
@@ -282,8 +292,10 @@ size_t gemm_ws_elems(int nprob, int M, int N, int K) {
 
 int gemm_tile_rule(int M, int N) { return N <= 16 ? GEMM_TILE_64x16 : (M <= 16 ? GEMM_TILE_16x64 : (M <= 32 ? GEMM_TILE_32x64 : GEMM_TILE_64x64)); }
 
-template <typename R>
-void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile) {
+namespace {
+
+template <typename S, typename R>
+void launch_gemm_sr(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile) {
     if (nprob <= 0 || M <= 0 || N <= 0) return;
     int ks = gemm_kslices(nprob, M, K);
     if (!ws) ks = 1;
@@ -300,19 +312,30 @@ void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStr
                            ks, kchunk, W);
     };
     switch (tile) {
-        case GEMM_TILE_64x16: run(gemm_kernel<R, 64, 16>, 64, 16); break;
-        case GEMM_TILE_16x64: run(gemm_kernel<R, 16, 64>, 16, 64); break;
-        case GEMM_TILE_32x64: run(gemm_kernel<R, 32, 64>, 32, 64); break;
-        case GEMM_TILE_32x32: run(gemm_kernel<R, 32, 32>, 32, 32); break;
-        case GEMM_TILE_64x32: run(gemm_kernel<R, 64, 32>, 64, 32); break;
-        case GEMM_TILE_128x16: run(gemm_kernel<R, 128, 16>, 128, 16); break;
-        default: run(gemm_kernel<R, 64, 64>, 64, 64); break;
+        case GEMM_TILE_64x16: run(gemm_kernel<S, R, 64, 16>, 64, 16); break;
+        case GEMM_TILE_16x64: run(gemm_kernel<S, R, 16, 64>, 16, 64); break;
+        case GEMM_TILE_32x64: run(gemm_kernel<S, R, 32, 64>, 32, 64); break;
+        case GEMM_TILE_32x32: run(gemm_kernel<S, R, 32, 32>, 32, 32); break;
+        case GEMM_TILE_64x32: run(gemm_kernel<S, R, 64, 32>, 64, 32); break;
+        case GEMM_TILE_128x16: run(gemm_kernel<S, R, 128, 16>, 128, 16); break;
+        default: run(gemm_kernel<S, R, 64, 64>, 64, 64); break;
     }
     if (ks > 1) {
         const size_t mn = (size_t)M * N;
-        hipLaunchKernelGGL(gemm_reduce_kernel<R>, dim3((unsigned)((mn + 255) / 256), nprob), dim3(256), 0, st,
+        hipLaunchKernelGGL((gemm_reduce_kernel<S, R>), dim3((unsigned)((mn + 255) / 256), nprob), dim3(256), 0, st,
                            d_descs, M, N, ks, W);
     }
+}
+
+}  // namespace
+
+template <typename R>
+void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile) {
+    launch_gemm_sr<R, R>(d_descs, nprob, M, N, K, st, ws, tile);
+}
+
+void launch_gemm_c64(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile) {
+    launch_gemm_sr<double, float>(d_descs, nprob, M, N, K, st, ws, tile);
 }
 
 template void launch_gemm<double>(const GemmDesc*, int, int, int, int, hipStream_t, void*, int);

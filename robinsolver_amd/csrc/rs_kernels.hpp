@@ -35,6 +35,11 @@ int gemm_tile_rule(int M, int N);
 template <typename R>
 void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws = nullptr,
                  int tile = -1);
+// The same products on complex128 operands and results with complex64 MFMA
+// products and FP32 accumulation (the merge of the mixed-precision
+// factorization).  ws as for launch_gemm<float>.
+void launch_gemm_c64(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws = nullptr,
+                     int tile = -1);
 int gemm_kslices(int nprob, int M, int K);
 size_t gemm_ws_elems(int nprob, int M, int N, int K);
 

@@ -386,13 +386,25 @@ int Tree::factor_device(const double2* k) {
         HIPCHK(hipEventCreate(&te0));
         HIPCHK(hipEventCreate(&te1));
     }
+    // In a mixed-precision tree with merge_fp64_levels = k >= 0 the merge
+    // products of the levels below the top k run in complex64 on the matrix
+    // cores (FP32 accumulation, complex128 operands and results); the interface
+    // inverses around them stay FP64 with pivoting.
+    const int c64_from = (opts.precision == RS_MIXED && opts.merge_fp64_levels >= 0) ? opts.merge_fp64_levels : L;
+    merge_c64 = c64_from < L;
+    auto gemm = [&](int lvl, const GemmDesc* d, int np, int M, int N, int K, void* gws, int t) {
+        if (lvl >= c64_from)
+            launch_gemm_c64(d, np, M, N, K, stream, gws, t);
+        else
+            launch_gemm<double>(d, np, M, N, K, stream, gws, t);
+    };
     auto fgemm = [&](int site, const GemmDesc* d, int np, int M, int N, int K, void* gws) -> int {
         int tile = site < (int)ftile.size() ? ftile[site] : -1;
         if (tune) {
             float best = 0.f;
             for (int t = 0; t < GEMM_TILES; ++t) {
                 (void)hipEventRecord(te0, stream);
-                launch_gemm<double>(d, np, M, N, K, stream, gws, t);
+                gemm(site / 4, d, np, M, N, K, gws, t);
                 (void)hipEventRecord(te1, stream);
                 if (hipEventSynchronize(te1) != hipSuccess) return fail("Error: GEMM autotuning failed.");
                 float ms = 0.f;
@@ -401,7 +413,7 @@ int Tree::factor_device(const double2* k) {
             }
             ftile[site] = tile;
         }
-        launch_gemm<double>(d, np, M, N, K, stream, gws, tile);
+        gemm(site / 4, d, np, M, N, K, gws, tile);
         return 0;
     };
     for (int l = L - 1; l >= 0; --l) {
@@ -787,6 +799,9 @@ int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, in
     if (!factored) return fail("Error: Robin Tree must be factored first.");
     if (nrhs < 1) return fail("Error: nrhs must be positive.");
     if (prec < 0) prec = opts.precision;
+    if (prec == RS_FP64 && merge_c64)
+        return fail("Error: an FP64 solve needs a tree factored in FP64 (this tree's merge products are complex64).");
+    if (prec != RS_FP64 && prec != RS_MIXED) return fail("Error: precision must be RS_FP64 or RS_MIXED.");
     if (!st) st = stream;
     Slot& sl = slot(slot_id);
     // workspaces first: allocation and autotuning cannot be captured; a workspace

@@ -124,6 +124,7 @@ struct Tree {
     DevBuf sinv32;
     std::vector<DevBuf*> T32, invM32;
     bool have32 = false;
+    bool merge_c64 = false;      // some merge products of the factorization ran in complex64
     // maps
     std::vector<DevBuf*> d_prow, d_imap;
     DevBuf d_root_leaf, d_root_loc, d_leaf_nbr;

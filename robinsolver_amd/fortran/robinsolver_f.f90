@@ -133,6 +133,7 @@ module rs_capi
         integer(c_long_long) :: hbm_budget
         integer(c_int) :: autotune
         integer(c_int) :: graphs
+        integer(c_int) :: merge_fp64_levels
     end type rs_opts_t
 
     interface

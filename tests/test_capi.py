@@ -68,9 +68,10 @@ def test_solve_before_factor_fails():
 
 def test_opts_defaults_and_layout():
     """rs_opts as the header documents it (the Fortran and Python mirrors bind
-    this layout): FP64, skip the root RtR, two refinement steps, autotuning on."""
+    this layout): FP64, skip the root RtR, two refinement steps, autotuning and
+    graphs on, an FP64 factorization in a mixed tree."""
     o = rsa.RsOpts()
     rsa.lib().rs_opts_default(ctypes.byref(o))
-    assert (o.precision, o.skip_root_rtr, o.n_refine, o.verbose, o.hbm_budget, o.autotune, o.graphs) == \
-        (0, 1, 2, 0, 0, 1, 1)
-    assert ctypes.sizeof(rsa.RsOpts) == 32
+    assert (o.precision, o.skip_root_rtr, o.n_refine, o.verbose, o.hbm_budget, o.autotune, o.graphs,
+            o.merge_fp64_levels) == (0, 1, 2, 0, 0, 1, 1, -1)
+    assert ctypes.sizeof(rsa.RsOpts) == 40

@@ -50,13 +50,13 @@ def test_graph_replays_match_direct_launches(prec, gpu):
             g = _dev(_rc(rng, nrhs, tg.n_leaves, tg.nb))
             assert np.array_equal(_solve(tg, f, g), _solve(td, f, g)), (rep, nrhs)
         if rep == 1:
-            # other work between replays: the other precision, then a refactor in place
-            other = rsa.RS_FP64 if prec == rsa.RS_MIXED else rsa.RS_MIXED
+            # other work between replays: a mixed solve of an FP64 tree (a mixed
+            # tree takes no FP64 solves) or another shape, then a refactor in place
             f = _dev(_rc(rng, 3, tg.n_leaves, tg.n_int))
             g = _dev(_rc(rng, 3, tg.n_leaves, tg.nb))
             import torch
             u = torch.zeros((3, tg.n_leaves, tg.m), dtype=torch.complex128, device="cuda:0")
-            tg.solve_device(3, f.data_ptr(), g.data_ptr(), u.data_ptr(), precision=other)
+            tg.solve_device(3, f.data_ptr(), g.data_ptr(), u.data_ptr(), precision=rsa.RS_MIXED)
             torch.cuda.synchronize()
         if rep == 2:
             for t in (tg, td):

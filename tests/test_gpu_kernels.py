@@ -55,7 +55,7 @@ def _run_gemm(prec, A, B, C, alpha, beta, B2=None, ksplit=None, rmap=None, ldc=N
     return c_flat.reshape(Cf.shape, order="F")
 
 
-@pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED])
+@pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED, rsa.RS_MIXED_MERGE])
 @pytest.mark.parametrize("M,N,K", [(16, 16, 16), (64, 64, 16), (37, 5, 23), (130, 200, 70), (48, 1, 96),
                                    (7, 65, 3), (64, 20, 2048), (200, 1, 1000), (100, 70, 777), (16, 40, 64), (30, 100, 50), (12, 300, 2000)])
 def test_gemm_matches_numpy(prec, M, N, K, gpu):
@@ -80,6 +80,28 @@ def test_gemm_column_results_do_not_depend_on_width(prec, gpu):
     for c in (0, 17, 79):
         one = _run_gemm(prec, A, B[:, c:c + 1], np.zeros((M, 1), complex), 1.0 + 0j, 0j)
         assert np.array_equal(one[:, 0], wide[:, c])
+
+
+@pytest.mark.parametrize("M,N,K", [(96, 40, 1536), (64, 20, 2048), (37, 5, 23)])
+def test_mixed_merge_gemm_is_complex64_products_with_complex128_result(M, N, K, gpu):
+    """The merge form of the mixed factorization: with alpha = 1, beta = 0 its
+    result is bitwise the complex64 GEMM's (same operand rounding, same FP32
+    accumulation sequence, K split or not); its error against complex128 is
+    that of complex64 products; and its epilogue keeps complex128 (alpha = 0,
+    beta = 1 returns C bit for bit, which complex64 storage cannot)."""
+    print("case: mixed merge gemm", M, N, K, flush=True)
+    rng = np.random.default_rng(M + 7 * N + K)
+    A, B, C = _rc(rng, M, K), _rc(rng, K, N), _rc(rng, M, N)
+    merge = _run_gemm(rsa.RS_MIXED_MERGE, A, B, np.zeros((M, N), complex), 1.0 + 0j, 0j)
+    c64 = _run_gemm(rsa.RS_MIXED, A, B, np.zeros((M, N), complex), 1.0 + 0j, 0j)
+    assert np.array_equal(merge, c64)
+    exact = A @ B
+    err = np.linalg.norm(merge - exact) / np.linalg.norm(exact)
+    assert 1e-9 < err < 2e-6, err
+    keep = _run_gemm(rsa.RS_MIXED_MERGE, A, B, C, 0j, 1.0 + 0j)
+    assert np.array_equal(keep, C)
+    rounded = _run_gemm(rsa.RS_MIXED, A, B, C, 0j, 1.0 + 0j)
+    assert not np.array_equal(rounded, C)
 
 
 @pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED])
@@ -134,7 +156,7 @@ def test_inverse_needs_pivoting(n, gpu):
     assert np.linalg.norm(inv @ M - np.eye(n)) / np.sqrt(n) < 1e-10
 
 
-@pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED])
+@pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED, rsa.RS_MIXED_MERGE])
 @pytest.mark.parametrize("M,N,K,scatter", [(48, 256, 16, True), (100, 70, 777, False), (200, 33, 1000, False),
                                            (16, 64, 32, True)])
 def test_gemm_tile_shapes_give_identical_bits(prec, M, N, K, scatter, gpu):

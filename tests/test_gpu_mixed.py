@@ -87,3 +87,38 @@ def test_mixed_batch_invariance(gpu):
     u = t.solve(f, g)
     for r in (0, 11, 19):
         assert np.array_equal(t.solve(f[r], g[r]), u[r])
+
+
+
+@pytest.mark.parametrize("fp64_levels", [0, 2])
+def test_complex64_merges_and_precision_rules(fp64_levels, gpu):
+    """merge_fp64_levels = k >= 0: the merges below the top k levels run in
+    complex64, so the RtR maps differ from an FP64 tree's by complex64 rounding
+    (not more); such a tree refuses FP64 solves and its refined mixed solves
+    reach the mixed-path bound.  The default mixed tree factors in FP64: its
+    RtR maps are bitwise an FP64 tree's and it takes FP64 solves."""
+    print("case: complex64 merges, fp64 levels", fp64_levels, flush=True)
+    kb, n, L = 12.0, 64, 4
+    o = OracleTree(2, L, UNIT, 1.0 / n, -kb)
+    tm = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=2, merge_fp64_levels=fp64_levels)
+    tdef = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=1)
+    td = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_FP64)
+    pts, _ = o.all_points()
+    k = _k(pts[:, o.nb:], kb)
+    for t in (o, tm, tdef, td):
+        t.factor(k)
+    for lvl, idx in ((L - 1, 0), (1, 1)):
+        a, b = tm.node_rtr(lvl, idx), td.node_rtr(lvl, idx)
+        d = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print("RtR level", lvl, "complex64 merges vs FP64:", d, flush=True)
+        assert 1e-12 < d < 1e-4, (lvl, d)
+        assert np.array_equal(tdef.node_rtr(lvl, idx), b)
+    rng = np.random.default_rng(12)
+    f = _rand_c(rng, (2, o.n_leaves, o.n_int))
+    g = _rand_c(rng, (2, o.n_leaves, o.nb))
+    with pytest.raises(rsa.RobinError, match="FP64 solve"):
+        tm.solve(f, g, precision=rsa.RS_FP64)
+    assert np.array_equal(tdef.solve(f, g, precision=rsa.RS_FP64), td.solve(f, g))
+    for u in (tm.solve(f, g), td.solve(f, g, precision=rsa.RS_MIXED)):
+        for r in range(2):
+            assert _rel(u[r], o.solve(f[r], g[r])) <= TOL_MIXED
