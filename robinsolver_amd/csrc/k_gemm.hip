@@ -5,10 +5,14 @@
 // block products of the merge (robin_tree_module.f90:482-534), which this
 // build expresses as products with explicit interface inverses.
 //
-// Complex products run as four real MFMA products (4M) on interleaved storage:
-//   Cr += Ar*Br - Ai*Bi,  Ci += Ar*Bi + Ai*Br,
+// Complex products run as three real MFMA products (Gauss's 3M) on interleaved
+// storage:
+//   P1 += Ar*Br,  P2 += Ai*Bi,  P3 += (Ar+Ai)*(Br+Bi),
+//   Cr = P1 - P2,  Ci = P3 - P1 - P2,
 // with v_mfma_f64_16x16x4f64 (complex128) or v_mfma_f32_16x16x4f32
-// (complex64, FP32 accumulation).  A 256-thread workgroup computes a 64 x BN
+// (complex64, FP32 accumulation); the sums Ar+Ai and Br+Bi are formed from the
+// operands as they are read from LDS.  The error bound of Ci carries
+// (|Ar|+|Ai|)(|Br|+|Bi|) in place of |A||B|, at most twice as large.  A 256-thread workgroup computes a 64 x BN
 // output tile from 64 x 16 and 16 x BN operand tiles staged in LDS; BN = 64 for
 // wide products (merge), BN = 16 for the few-RHS solve passes, which then stay
 // HBM-bound rather than MFMA-bound.  The K reduction of every output column is
@@ -97,10 +101,10 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave % WAVES_M) * WM, wn = (wave / WAVES_M) * WN;
 
-    A4 accR[TM][TN], accI[TM][TN];
+    A4 p1[TM][TN], p2[TM][TN], p3[TM][TN];
     for (int i = 0; i < TM; ++i)
         for (int j = 0; j < TN; ++j)
-            for (int v = 0; v < 4; ++v) { accR[i][j][v] = R(0); accI[i][j][v] = R(0); }
+            for (int v = 0; v < 4; ++v) { p1[i][j][v] = R(0); p2[i][j][v] = R(0); p3[i][j][v] = R(0); }
 
     V ra[NA], rb[NB];
     auto fetch = [&](int k0) {
@@ -142,18 +146,24 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             V af[TM], bf[TN];
+            R as[TM], bs[TN];
 #pragma unroll
-            for (int i = 0; i < TM; ++i) af[i] = As[kk + (lane >> 4)][wm + i * 16 + (lane & 15)];
+            for (int i = 0; i < TM; ++i) {
+                af[i] = As[kk + (lane >> 4)][wm + i * 16 + (lane & 15)];
+                as[i] = af[i].x + af[i].y;
+            }
 #pragma unroll
-            for (int j = 0; j < TN; ++j) bf[j] = Bs[kk + (lane >> 4)][wn + j * 16 + (lane & 15)];
+            for (int j = 0; j < TN; ++j) {
+                bf[j] = Bs[kk + (lane >> 4)][wn + j * 16 + (lane & 15)];
+                bs[j] = bf[j].x + bf[j].y;
+            }
 #pragma unroll
             for (int i = 0; i < TM; ++i)
 #pragma unroll
                 for (int j = 0; j < TN; ++j) {
-                    accR[i][j] = mfma(af[i].x, bf[j].x, accR[i][j]);
-                    accR[i][j] = mfma(-af[i].y, bf[j].y, accR[i][j]);
-                    accI[i][j] = mfma(af[i].x, bf[j].y, accI[i][j]);
-                    accI[i][j] = mfma(af[i].y, bf[j].x, accI[i][j]);
+                    p1[i][j] = mfma(af[i].x, bf[j].x, p1[i][j]);
+                    p2[i][j] = mfma(af[i].y, bf[j].y, p2[i][j]);
+                    p3[i][j] = mfma(as[i], bs[j], p3[i][j]);
                 }
         }
         __syncthreads();
@@ -177,7 +187,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
             for (int v = 0; v < 4; ++v) {
                 const int row = wm + i * 16 + out_row<R>(lane, v);
                 const int col = wn + j * 16 + (lane & 15);
-                Ct[col * (BM + 1) + row] = mk<R>(accR[i][j][v], accI[i][j][v]);
+                Ct[col * (BM + 1) + row] = mk<R>(p1[i][j][v] - p2[i][j][v], (p3[i][j][v] - p1[i][j][v]) - p2[i][j][v]);
             }
     __syncthreads();
     constexpr int NE = BM * BN / 256;
