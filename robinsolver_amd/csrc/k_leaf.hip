@@ -552,7 +552,7 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
 // columns; a workgroup covers 64 columns of one leaf.  The layer recursion
 //   forward   y_k = Shat_k^-1 (r_k - lambda_k y_{k-1})
 //   backward  x_k = y_k - mu_k Shat_k^-1 x_{k+1}
-// is one 16x16x16 complex product per layer (v_mfma_f32_16x16x4f32, four real
+// is one 16x16x16 complex product per layer (v_mfma_f32_16x16x4f32, three real
 // products per complex k-chunk).  The leaf's K blocks are staged once in LDS;
 // y_k stays in registers in the MFMA output layout (lane l holds rows
 // 4(l/16)..4(l/16)+3 of column l%16), and between layers the 16x16 operand
@@ -694,17 +694,19 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
         float2* sb = scr;
         for (int vv = 0; vv < 4; ++vv) sb[(4 * grp + vv) * LT + ccol] = v[vv];
         wave_lds_sync();
-        aR = f4v{0.f, 0.f, 0.f, 0.f};
-        aI = f4v{0.f, 0.f, 0.f, 0.f};
+        // three real products per complex k-chunk (Gauss): three independent
+        // accumulation chains of four instead of two of eight
+        f4v p1 = f4v{0.f, 0.f, 0.f, 0.f}, p2 = p1, p3 = p1;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const float2 a = Bl[k * 256 + ccol + 16 * (4 * q + grp)];
             const float2 b = sb[(4 * q + grp) * LT + ccol];
-            aR = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, aR, 0, 0, 0);
-            aR = __builtin_amdgcn_mfma_f32_16x16x4f32(-a.y, b.y, aR, 0, 0, 0);
-            aI = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.y, aI, 0, 0, 0);
-            aI = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.x, aI, 0, 0, 0);
+            p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, p1, 0, 0, 0);
+            p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, p2, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x + a.y, b.x + b.y, p3, 0, 0, 0);
         }
+        aR = p1 - p2;
+        aI = (p3 - p1) - p2;
     };
 
     // raw[i] = f at row rr of layer k, column cc(i) (coalesced mapping; zero for
