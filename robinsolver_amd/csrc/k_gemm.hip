@@ -49,7 +49,7 @@ template <typename R> __device__ inline int out_row(int lane, int v);
 template <> __device__ inline int out_row<float>(int lane, int v) { return 4 * (lane >> 4) + v; }
 template <> __device__ inline int out_row<double>(int lane, int v) { return (lane >> 4) + 4 * v; }
 
-constexpr int BMX = 64;   // largest tile height
+constexpr int BMX = 64;   // row-tile height assumed by the K-split rule (whatever the tile)
 constexpr int BK = 32;
 
 // One workgroup computes a 64 x BN tile over the K range of its slice.  The
@@ -93,9 +93,11 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
     const int t = rem - ks * tiles;
     const int tm = t % tilesM, tn = t / tilesM;
     const GemmDesc& D = descs[p];
-    const VS* __restrict__ A = (const VS*)D.A;
-    const VS* __restrict__ B = (const VS*)D.B;
-    const VS* __restrict__ B2 = (const VS*)D.B2;
+    // descriptor fields into registers once (the loop's stores could alias them)
+    const VS* const A = (const VS*)D.A;
+    const VS* const B = (const VS*)D.B;
+    const VS* const B2 = (const VS*)D.B2;
+    const int lda = D.lda, ldb = D.ldb, ksplit = D.ksplit;
     const int m0 = tm * BM, n0 = tn * BN;
     const int kbeg = ks * kchunk, kend = min(K, kbeg + kchunk);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -106,33 +108,43 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
         for (int j = 0; j < TN; ++j)
             for (int v = 0; v < 4; ++v) { p1[i][j][v] = R(0); p2[i][j][v] = R(0); p3[i][j][v] = R(0); }
 
-    V ra[NA], rb[NB];
+    // Operand prefetch: every load is issued unconditionally from an index
+    // clamped into range (a load under a branch makes the compiler wait for all
+    // outstanding loads where the branches join); elements outside the matrix
+    // or the K range are replaced by zero when staged.
+    VS ra[NA], rb[NB];
+    bool oka[NA], okb[NB];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
             const int e = tid + 256 * q;
             const int r = e % BM, k = e / BM;
             const int gr = m0 + r, gk = k0 + k;
-            const VS a = (gr < M && gk < kend) ? A[(size_t)gr + (size_t)gk * D.lda] : cz<S>();
-            ra[q] = mk<R>((R)a.x, (R)a.y);
+            oka[q] = gr < M && gk < kend;
+            ra[q] = gld(A + ((size_t)min(gr, M - 1) + (size_t)min(gk, kend - 1) * lda));
         }
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
             const int e = tid + 256 * q;
             const int k = e % BK, c = e / BK;
-            const int gk = k0 + k, gc = n0 + c;
-            VS v = cz<S>();
-            if (gk < kend && gc < N)
-                v = (gk < D.ksplit) ? B[(size_t)gk + (size_t)gc * D.ldb]
-                                    : B2[(size_t)(gk - D.ksplit) + (size_t)gc * D.ldb];
-            rb[q] = mk<R>((R)v.x, (R)v.y);
+            const int gk = min(k0 + k, kend - 1), gc = min(n0 + c, N - 1);
+            okb[q] = k0 + k < kend && n0 + c < N;
+            const VS* src = (gk < ksplit) ? B + ((size_t)gk + (size_t)gc * ldb)
+                                          : B2 + ((size_t)(gk - ksplit) + (size_t)gc * ldb);
+            rb[q] = gld(src);
         }
     };
     auto stash = [&]() {
 #pragma unroll
-        for (int q = 0; q < NA; ++q) { const int e = tid + 256 * q; As[e / BM][e % BM] = ra[q]; }
+        for (int q = 0; q < NA; ++q) {
+            const int e = tid + 256 * q;
+            As[e / BM][e % BM] = oka[q] ? mk<R>((R)ra[q].x, (R)ra[q].y) : cz<R>();
+        }
 #pragma unroll
-        for (int q = 0; q < NB; ++q) { const int e = tid + 256 * q; Bs[e % BK][e / BK] = rb[q]; }
+        for (int q = 0; q < NB; ++q) {
+            const int e = tid + 256 * q;
+            Bs[e % BK][e / BK] = okb[q] ? mk<R>((R)rb[q].x, (R)rb[q].y) : cz<R>();
+        }
     };
 
     if (kbeg < kend) {
@@ -202,57 +214,68 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
         }
         return;
     }
-    VS* Out = (VS*)D.Out;
-    const VS* Din = (const VS*)D.Din;
+    VS* const Out = (VS*)D.Out;
+    const VS* const Din = (const VS*)D.Din;
+    const int* const rmap = D.rmap;
+    const int* const cmap = D.cmap;
+    const int ldd = D.ldd, ldo = D.ldo;
     const VS alpha = mk<S>((S)D.ar, (S)D.ai), beta = mk<S>((S)D.br, (S)D.bi);
-    // all index-map and Din loads of the thread are issued back to back (branches
-    // only around whole loops), then the results are formed and stored
-    int rowc[NE], colc[NE], orow[NE], ocol[NE];
-    bool ok[NE];
+    // In groups of up to eight elements: all index-map and Din loads of a group
+    // are issued back to back (branches only around whole loops), then the
+    // results are formed and stored.  (Whole-tile groups would hold too many
+    // registers for the largest tiles.)
+    constexpr int NG = NE < 8 ? NE : 8;
+    static_assert(NE % NG == 0, "epilogue group size");
+    const bool dmap = D.dmap != 0;
 #pragma unroll
-    for (int q = 0; q < NE; ++q) {
-        const int e = tid + 256 * q;
-        const int row = m0 + e % BM, col = n0 + e / BM;
-        ok[q] = row < M && col < N;
-        rowc[q] = row < M ? row : M - 1;
-        colc[q] = col < N ? col : N - 1;
-    }
-    if (D.rmap) {
+    for (int g0 = 0; g0 < NE; g0 += NG) {
+        int rowc[NG], colc[NG], orow[NG], ocol[NG];
+        bool ok[NG];
 #pragma unroll
-        for (int q = 0; q < NE; ++q) orow[q] = D.rmap[rowc[q]];
-    } else {
+        for (int q = 0; q < NG; ++q) {
+            const int e = tid + 256 * (g0 + q);
+            const int row = m0 + e % BM, col = n0 + e / BM;
+            ok[q] = row < M && col < N;
+            rowc[q] = row < M ? row : M - 1;
+            colc[q] = col < N ? col : N - 1;
+        }
+        if (rmap) {
 #pragma unroll
-        for (int q = 0; q < NE; ++q) orow[q] = rowc[q];
-    }
-    if (D.cmap) {
-#pragma unroll
-        for (int q = 0; q < NE; ++q) ocol[q] = D.cmap[colc[q]];
-    } else {
-#pragma unroll
-        for (int q = 0; q < NE; ++q) ocol[q] = colc[q];
-    }
-    VS val[NE];
-#pragma unroll
-    for (int q = 0; q < NE; ++q) {
-        const int e = tid + 256 * q;
-        const V c = Ct[(e / BM) * (BM + 1) + e % BM];
-        val[q] = cmul(alpha, mk<S>((S)c.x, (S)c.y));
-    }
-    if (Din) {
-        VS din[NE];
-        if (D.dmap) {
-#pragma unroll
-            for (int q = 0; q < NE; ++q) din[q] = Din[(size_t)orow[q] + (size_t)ocol[q] * D.ldd];
+            for (int q = 0; q < NG; ++q) orow[q] = gldi(rmap + rowc[q]);
         } else {
 #pragma unroll
-            for (int q = 0; q < NE; ++q) din[q] = Din[(size_t)rowc[q] + (size_t)colc[q] * D.ldd];
+            for (int q = 0; q < NG; ++q) orow[q] = rowc[q];
+        }
+        if (cmap) {
+#pragma unroll
+            for (int q = 0; q < NG; ++q) ocol[q] = gldi(cmap + colc[q]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < NG; ++q) ocol[q] = colc[q];
+        }
+        VS val[NG];
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            const int e = tid + 256 * (g0 + q);
+            const V c = Ct[(e / BM) * (BM + 1) + e % BM];
+            val[q] = cmul(alpha, mk<S>((S)c.x, (S)c.y));
+        }
+        if (Din) {
+            VS din[NG];
+            if (dmap) {
+#pragma unroll
+                for (int q = 0; q < NG; ++q) din[q] = gld(Din + ((size_t)orow[q] + (size_t)ocol[q] * ldd));
+            } else {
+#pragma unroll
+                for (int q = 0; q < NG; ++q) din[q] = gld(Din + ((size_t)rowc[q] + (size_t)colc[q] * ldd));
+            }
+#pragma unroll
+            for (int q = 0; q < NG; ++q) val[q] = cfma(beta, din[q], val[q]);
         }
 #pragma unroll
-        for (int q = 0; q < NE; ++q) val[q] = cfma(beta, din[q], val[q]);
+        for (int q = 0; q < NG; ++q)
+            if (ok[q]) gst(Out + ((size_t)orow[q] + (size_t)ocol[q] * ldo), val[q]);
     }
-#pragma unroll
-    for (int q = 0; q < NE; ++q)
-        if (ok[q]) Out[(size_t)orow[q] + (size_t)ocol[q] * D.ldo] = val[q];
 }
 
 // Sum of the K slices in slice order, then the epilogue of gemm_kernel.
@@ -272,14 +295,14 @@ __global__ void gemm_reduce_kernel(const GemmDesc* __restrict__ descs, int M, in
     V sum = W[0];
     for (int k = 1; k < kslices; ++k) sum = cadd(sum, W[(size_t)k * mn]);
     VS val = cmul(mk<S>((S)D.ar, (S)D.ai), mk<S>((S)sum.x, (S)sum.y));
-    const int orow = D.rmap ? D.rmap[row] : row;
-    const int ocol = D.cmap ? D.cmap[col] : col;
+    const int orow = D.rmap ? gldi(D.rmap + row) : row;
+    const int ocol = D.cmap ? gldi(D.cmap + col) : col;
     const VS* Din = (const VS*)D.Din;
     if (Din) {
         size_t di = D.dmap ? (size_t)orow + (size_t)ocol * D.ldd : (size_t)row + (size_t)col * D.ldd;
-        val = cfma(mk<S>((S)D.br, (S)D.bi), Din[di], val);
+        val = cfma(mk<S>((S)D.br, (S)D.bi), gld(Din + di), val);
     }
-    ((VS*)D.Out)[(size_t)orow + (size_t)ocol * D.ldo] = val;
+    gst((VS*)D.Out + ((size_t)orow + (size_t)ocol * D.ldo), val);
 }
 
 }  // namespace

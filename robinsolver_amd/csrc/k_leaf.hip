@@ -270,7 +270,7 @@ template <typename R> struct LeafIO {
 // f at interior point q of (col, leaf); u at point p of (col, leaf)
 template <typename V, typename R>
 __device__ inline V load_f(const LeafIO<R>& io, int col, int leaf, int ni, int q) {
-    if (io.f64) return ccast<V>(io.f64[col][(size_t)leaf * ni + q]);
+    if (io.f64) return ccast<V>(gld(io.f64[col] + ((size_t)leaf * ni + q)));
     if (io.f32) return ccast<V>(io.f32[((size_t)col * io.n_leaves + leaf) * ni + q]);
     return V{};
 }
@@ -280,9 +280,9 @@ __device__ inline void store_u(const LeafIO<R>& io, int col, int leaf, int m, in
     const size_t i = (size_t)leaf * m + p;
     if (io.u64) {
         double2* u = io.u64[col] + i;
-        if (io.ubase32) *u = cadd(ccast<double2>(io.ubase32[(size_t)col * io.n_leaves * m + i]), ccast<double2>(x));
-        else if (io.uacc) *u = cadd(*u, ccast<double2>(x));
-        else *u = ccast<double2>(x);
+        if (io.ubase32) gst(u, cadd(ccast<double2>(io.ubase32[(size_t)col * io.n_leaves * m + i]), ccast<double2>(x)));
+        else if (io.uacc) gst(u, cadd(gld(u), ccast<double2>(x)));
+        else gst(u, ccast<double2>(x));
     } else {
         io.u32[(size_t)col * io.n_leaves * m + i] = ccast<float2>(x);
     }
@@ -462,13 +462,13 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
 #pragma unroll
         for (int c = 0; c < RES_CPT; ++c) {
             const U* uc = ucols[c];
-            a0[c] = uc[ul0 + p];
-            a1[c] = uc[ul0 + nb + q1];
-            a2[c] = uc[ul0 + nb + q1 + st];
-            n0[c] = uc[un0 + b2];
-            n1[c] = uc[un0 + nb + q1n];
-            n2[c] = uc[un0 + nb + q1n + stn];
-            gv[c] = g[cok[c] ? col0 + c : ncols - 1][(size_t)leaf * nb + p];
+            a0[c] = gld(uc + (ul0 + p));
+            a1[c] = gld(uc + (ul0 + nb + q1));
+            a2[c] = gld(uc + (ul0 + nb + q1 + st));
+            n0[c] = gld(uc + (un0 + b2));
+            n1[c] = gld(uc + (un0 + nb + q1n));
+            n2[c] = gld(uc + (un0 + nb + q1n + stn));
+            gv[c] = gld(g[cok[c] ? col0 + c : ncols - 1] + ((size_t)leaf * nb + p));
         }
         const AxisCoef& A = C.ax[j_];
         const AxisCoef& B = C.ax[jj];
@@ -522,13 +522,13 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
 #pragma unroll
     for (int c = 0; c < RES_CPT; ++c) {
         const U* uc = ucols[c] + ul0;
-        uc0[c] = uc[p];
+        uc0[c] = gld(uc + p);
 #pragma unroll
         for (int jx = 0; jx < 3; ++jx) {
-            um[c][jx] = uc[im[jx]];
-            upp[c][jx] = uc[ip[jx]];
+            um[c][jx] = gld(uc + im[jx]);
+            upp[c][jx] = gld(uc + ip[jx]);
         }
-        fv[c] = f[cok[c] ? col0 + c : ncols - 1][(size_t)leaf * ni + q];
+        fv[c] = gld(f[cok[c] ? col0 + c : ncols - 1] + ((size_t)leaf * ni + q));
     }
 #pragma unroll
     for (int c = 0; c < RES_CPT; ++c) {
@@ -713,7 +713,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     // lanes outside the leaf or the batch is applied when the tile is staged)
     auto fetch = [&](int k, FT (&raw)[4]) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) raw[i] = fcp[i][k * s + rrc];
+        for (int i = 0; i < 4; ++i) raw[i] = gld(fcp[i] + (k * s + rrc));
     };
     // the interior data runs PF layers ahead of the sweep (memory latency is
     // several layer steps long)
@@ -770,9 +770,9 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     // store of solution point p of column col; base: the complex64 iterate for OUT_U64_BASE32
     auto put_col = [&](int col, double2* u64c, int p, float2 x, float2 base) {
         if (OUT == OUT_U32) io.u32[((size_t)col * nl + leaf) * m + p] = x;
-        else if (OUT == OUT_U64) u64c[p] = ccast<double2>(x);
-        else if (OUT == OUT_U64_BASE32) u64c[p] = cadd(ccast<double2>(base), ccast<double2>(x));
-        else u64c[p] = cadd(u64c[p], ccast<double2>(x));
+        else if (OUT == OUT_U64) gst(u64c + p, ccast<double2>(x));
+        else if (OUT == OUT_U64_BASE32) gst(u64c + p, cadd(ccast<double2>(base), ccast<double2>(x)));
+        else gst(u64c + p, cadd(gld(u64c + p), ccast<double2>(x)));
     };
     if (MODE == LEAF_DOWN) {
         // interior solution: each layer's tile through LDS, stored 16 points per column at a time
@@ -903,7 +903,7 @@ __global__ void __launch_bounds__(256) fd_norms_kernel(LeafGeom G, const float2*
     for (size_t e = tid; e < ni; e += 256) {
         const float2 a = rf[(size_t)col * ni + e];
         r += (double)a.x * a.x + (double)a.y * a.y;
-        const double2 b = f[col][e];
+        const double2 b = gld(f[col] + e);
         d += b.x * b.x + b.y * b.y;
     }
     for (size_t e = tid; e < nbt; e += 256) {
@@ -913,7 +913,7 @@ __global__ void __launch_bounds__(256) fd_norms_kernel(LeafGeom G, const float2*
         int fc = 1;
         while (fc < 2 * G.d && p >= G.foff[fc] + G.pf[fc]) ++fc;
         if (nbr[(size_t)leaf * 7 + fc] < 0) {
-            const double2 b = g[col][e];
+            const double2 b = gld(g[col] + e);
             d += b.x * b.x + b.y * b.y;
         }
     }

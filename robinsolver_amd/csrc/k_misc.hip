@@ -44,7 +44,7 @@ __global__ void root_gather_kernel(V* __restrict__ out, const double2* const* __
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int col = blockIdx.y;
     if (p >= P) return;
-    out[(size_t)col * P + p] = ccast<V>(gin[col][(size_t)leaf_of[p] * nbl + loc_of[p]]);
+    out[(size_t)col * P + p] = ccast<V>(gld(gin[col] + ((size_t)leaf_of[p] * nbl + loc_of[p])));
 }
 
 template <typename V>

@@ -34,6 +34,28 @@ template <typename V> __host__ __device__ inline V cinv(V a) {
 }
 template <typename V> __host__ __device__ inline auto cabs1(V a) { return fabs(a.x) + fabs(a.y); }
 
+// Device pointers that the compiler cannot prove global (read from descriptors
+// or column tables) would be accessed with flat instructions, which count
+// against the LDS wait counter as well: every wait for an LDS result would also
+// wait for outstanding prefetches.  Such pointers are cast to the global address
+// space before use.
+template <typename T> using gptr = T __attribute__((address_space(1)))*;
+template <typename V> struct vec_of;
+template <> struct vec_of<float2> { typedef float T __attribute__((ext_vector_type(2))); };
+template <> struct vec_of<double2> { typedef double T __attribute__((ext_vector_type(2))); };
+// complex load / store through the global address space
+template <typename V> __device__ inline V gld(const V* p) {
+    using T = typename vec_of<V>::T;
+    const T t = *(gptr<const T>)p;
+    V v; v.x = t.x; v.y = t.y; return v;
+}
+template <typename V> __device__ inline void gst(V* p, V v) {
+    using T = typename vec_of<V>::T;
+    T t; t.x = v.x; t.y = v.y;
+    *(gptr<T>)p = t;
+}
+__device__ inline int gldi(const int* p) { return *(gptr<const int>)p; }
+
 template <typename Dst, typename Src> __host__ __device__ inline Dst ccast(Src a) {
     Dst z; z.x = (decltype(z.x))a.x; z.y = (decltype(z.y))a.y; return z;
 }
