@@ -50,7 +50,7 @@ template <> __device__ inline int out_row<float>(int lane, int v) { return 4 * (
 template <> __device__ inline int out_row<double>(int lane, int v) { return (lane >> 4) + 4 * v; }
 
 constexpr int BMX = 64;   // row-tile height assumed by the K-split rule (whatever the tile)
-constexpr int BK = 32;
+constexpr int KALIGN = 32; // K-split slices start at multiples of this (whatever the chunk depth)
 
 // One workgroup computes a 64 x BN tile over the K range of its slice.  The
 // next 64 x 32 / 32 x BN operand tiles are fetched into registers while the
@@ -62,7 +62,14 @@ constexpr int BK = 32;
 // S = double, R = float is the complex64 merge of the mixed-precision
 // factorization (operands rounded to complex64 as they are staged, FP32
 // accumulation, the epilogue and the stored result in FP64).
-template <typename S, typename R, int BM, int BN>
+//
+// BK: K depth of one staged chunk; DB: two LDS chunk buffers (one barrier per
+// chunk) instead of one (two).  Neither changes the order in which a column's
+// products are accumulated.
+// (Measured on the solve-pass shapes at N = 256: BK = 16 with two buffers beat
+// BK = 32 or 64 with one by 10-25%.)  The MFMA steps of a chunk are always fully
+// unrolled.
+template <typename S, typename R, int BM, int BN, int BK = 16, bool DB = true>
 __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ descs, int M, int N, int K,
                                                   int tilesM, int tilesN, int kslices, int kchunk,
                                                   c_t<R>* __restrict__ ws) {
@@ -80,10 +87,9 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
     // (column stride BM + 1, so the 16 columns of one MFMA row group fall in
     // different banks)
     // (operand rows padded by one element against bank conflicts)
-    constexpr int SMEM = (BK * (BM + BN + 2) > BN * (BM + 1)) ? BK * (BM + BN + 2) : BN * (BM + 1);
+    constexpr int CHUNK = BK * (BM + BN + 2);
+    constexpr int SMEM = ((DB ? 2 : 1) * CHUNK > BN * (BM + 1)) ? (DB ? 2 : 1) * CHUNK : BN * (BM + 1);
     __shared__ V smem[SMEM];
-    V (*As)[BM + 1] = (V (*)[BM + 1])smem;
-    V (*Bs)[BN + 1] = (V (*)[BN + 1])(smem + BK * (BM + 1));
 
     const int tiles = tilesM * tilesN;
     const int per = tiles * kslices;
@@ -134,7 +140,10 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
             rb[q] = gld(src);
         }
     };
-    auto stash = [&]() {
+    // chunk buffer b: As [BK][BM + 1], then Bs [BK][BN + 1]
+    auto stash = [&](int b) {
+        V (*As)[BM + 1] = (V (*)[BM + 1])(smem + b * CHUNK);
+        V (*Bs)[BN + 1] = (V (*)[BN + 1])(smem + b * CHUNK + BK * (BM + 1));
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
             const int e = tid + 256 * q;
@@ -149,12 +158,15 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
 
     if (kbeg < kend) {
         fetch(kbeg);
-        stash();
+        stash(0);
         __syncthreads();
     }
+    int cur = 0;
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
         const bool more = k0 + BK < kend;
         if (more) fetch(k0 + BK);
+        V (*As)[BM + 1] = (V (*)[BM + 1])(smem + cur * CHUNK);
+        V (*Bs)[BN + 1] = (V (*)[BN + 1])(smem + cur * CHUNK + BK * (BM + 1));
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             V af[TM], bf[TN];
@@ -178,10 +190,17 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
                     p3[i][j] = mfma(as[i], bs[j], p3[i][j]);
                 }
         }
-        __syncthreads();
-        if (more) {
-            stash();
+        if constexpr (DB) {
+            // the other buffer was last read before the previous barrier
+            if (more) stash(cur ^ 1);
             __syncthreads();
+            cur ^= 1;
+        } else {
+            __syncthreads();
+            if (more) {
+                stash(0);
+                __syncthreads();
+            }
         }
     }
 
@@ -332,7 +351,7 @@ void launch_gemm_sr(const GemmDesc* d_descs, int nprob, int M, int N, int K, hip
     if (nprob <= 0 || M <= 0 || N <= 0) return;
     int ks = gemm_kslices(nprob, M, K);
     if (!ws) ks = 1;
-    const int kchunk = ((K + ks - 1) / ks + BK - 1) / BK * BK;
+    const int kchunk = ((K + ks - 1) / ks + KALIGN - 1) / KALIGN * KALIGN;
     ks = (K + kchunk - 1) / kchunk;
     c_t<R>* W = (c_t<R>*)ws;
     // Every 16x16 output block runs the same MFMA sequence over K whatever the
