@@ -195,6 +195,9 @@ bool rs_server::dispatch_factors() {
         if (r.kind != RQ_FACTOR) continue;
         long long resident = 0, peak = 0;
         tr.t->factor_bytes(&resident, &peak);
+        // the tile tuning after a first factorization holds a solve workspace
+        if (!o.simulate && tr.t->tiles_pending(o.max_cols))
+            peak = std::max(peak, resident + tr.t->work_bytes(o.max_cols, tr.t->opts.precision));
         const long long need = std::max(0LL, resident - tr.fbytes) + (peak - resident);
         if (!evict_for(need, nullptr, -1)) {
             if (!any_busy() && factors_running == 0) {
@@ -224,7 +227,8 @@ bool rs_server::dispatch_factors() {
         TreeRec* trp = &tr;
         const size_t wi = fworkers.size();
         fworker_done.push_back(0);
-        fworkers.emplace_back([this, t, k, sim, sim_s, trp, id, wi]() {
+        const int mc = o.max_cols;
+        fworkers.emplace_back([this, t, k, sim, sim_s, trp, id, wi, mc]() {
             int rc = 0;
             std::string msg;
             if (sim) {
@@ -232,6 +236,8 @@ bool rs_server::dispatch_factors() {
             } else {
                 rc = t->factor_device(k);
                 if (rc) msg = t->err;
+                // the solve products' tile tuning, here rather than in the dispatch path
+                else (void)t->prepare_tiles(mc);
             }
             std::lock_guard<std::mutex> g(mu);
             trp->factoring = false;
@@ -488,6 +494,9 @@ int rs_server_destroy(rs_server* s) {
 
 int rs_server_add_tree(rs_server* s, rs_tree* t, int* tree_id) {
     if (!s || !t) return 1;
+    // tile tuning of a factored tree's solve passes here (outside the lock: the
+    // tree is not registered yet), not in the dispatch path
+    if (!s->o.simulate && t->t.planned && t->t.factored) (void)t->t.prepare_tiles(s->o.max_cols);
     std::lock_guard<std::mutex> g(s->mu);
     if (!t->t.planned) return s->fail("Error: rs_plan must be called before rs_server_add_tree.");
     for (const auto& tr : s->trees)

@@ -165,6 +165,7 @@ int Tree::plan_tree(int d, int L, const double* domain, double h_tgt, double kb,
         if (stream) (void)hipStreamSynchronize(stream);
         free_factor();
         clear_slots();
+        tile_cache.clear();
         device_ready = false;
         if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
     }
@@ -570,8 +571,40 @@ template <typename R> int Tree::ensure_work(SolveWork<R>& w, int nrhs) {
         w.d_dn_g[l] = w.descs[k0 + 3]->template as<GemmDesc>();
     }
     w.cap = cap;
+    const long long key = 2LL * cap + (std::is_same<R, double>::value ? 1 : 0);
+    auto it = tile_cache.find(key);
+    if (it != tile_cache.end()) {
+        for (int c = 0; c < 4; ++c) w.tile[c] = it->second[c];
+        return 0;
+    }
     for (auto& t : w.tile) t.assign(L, -1);
     if (opts.autotune && tune_work(w)) return 1;
+    std::array<std::vector<int>, 4>& e = tile_cache[key];
+    for (int c = 0; c < 4; ++c) e[c] = w.tile[c];
+    return 0;
+}
+
+int Tree::prepare_tiles(int cap) {
+    if (!factored || !opts.autotune) return 0;
+    const bool mixed = opts.precision == RS_MIXED;
+    const long long key = 2LL * cap + (mixed ? 0 : 1);
+    if (tile_cache.count(key)) return 0;
+    int rc;
+    if (mixed) {
+        SolveWork<float> w;
+        rc = ensure_work(w, cap);
+        w.clear();
+    } else {
+        SolveWork<double> w;
+        rc = ensure_work(w, cap);
+        w.clear();
+    }
+    if (rc) {
+        // no room for a workspace now: the tile rule, and no later tuning
+        std::array<std::vector<int>, 4>& e = tile_cache[key];
+        for (auto& v : e) v.assign(plan.L, -1);
+        err.clear();
+    }
     return 0;
 }
 

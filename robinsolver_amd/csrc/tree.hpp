@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -166,6 +168,15 @@ struct Tree {
     template <typename R> int ensure_work(SolveWork<R>& w, int nrhs);
     // time the tile shapes of every product of the workspace (at its capacity) and keep the fastest
     template <typename R> int tune_work(SolveWork<R>& w);
+    // GEMM tile choices by (capacity, precision): they depend on the plan's
+    // shapes only, so a workspace of a capacity already tuned copies them
+    std::map<long long, std::array<std::vector<int>, 4>> tile_cache;
+    // tune the solve products of capacity cap once (a workspace that is freed
+    // again); used to keep the tuning out of a serving loop's dispatch path
+    int prepare_tiles(int cap);
+    bool tiles_pending(int cap) const {
+        return opts.autotune && !tile_cache.count(2LL * cap + (opts.precision == RS_MIXED ? 0 : 1));
+    }
     template <typename R> int solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st);
     int residual(int nrhs, double2* const* u64, const float2* u32, const double2* const* f,
                  const double2* const* g, float2* rf, float2* rg, float2* rj, hipStream_t st);
