@@ -326,19 +326,19 @@ __global__ void gemm_reduce_kernel(const GemmDesc* __restrict__ descs, int M, in
 
 }  // namespace
 
-int gemm_kslices(int nprob, int M, int K) {
+int gemm_kslices(int nprob, int M, int K, int ks_target) {
     // split K only when the row tiles of all problems cannot fill the chip;
     // the choice never depends on N
     const long long rowtiles = (long long)nprob * ((M + BMX - 1) / BMX);
-    if (rowtiles >= 128 || K < 256) return 1;
-    long long ks = (256 + rowtiles - 1) / rowtiles;
+    if (rowtiles >= ks_target / 2 || K < 256) return 1;
+    long long ks = (ks_target + rowtiles - 1) / rowtiles;
     if (ks > K / 128) ks = K / 128;
     if (ks > 16) ks = 16;
     return ks < 1 ? 1 : (int)ks;
 }
 
-size_t gemm_ws_elems(int nprob, int M, int N, int K) {
-    const int ks = gemm_kslices(nprob, M, K);
+size_t gemm_ws_elems(int nprob, int M, int N, int K, int ks_target) {
+    const int ks = gemm_kslices(nprob, M, K, ks_target);
     return ks > 1 ? (size_t)nprob * ks * M * N : 0;
 }
 
@@ -347,9 +347,10 @@ int gemm_tile_rule(int M, int N) { return N <= 16 ? GEMM_TILE_64x16 : (M <= 16 ?
 namespace {
 
 template <typename S, typename R>
-void launch_gemm_sr(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile) {
+void launch_gemm_sr(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile,
+                    int ks_target) {
     if (nprob <= 0 || M <= 0 || N <= 0) return;
-    int ks = gemm_kslices(nprob, M, K);
+    int ks = gemm_kslices(nprob, M, K, ks_target);
     if (!ws) ks = 1;
     const int kchunk = ((K + ks - 1) / ks + KALIGN - 1) / KALIGN * KALIGN;
     ks = (K + kchunk - 1) / kchunk;
@@ -382,15 +383,16 @@ void launch_gemm_sr(const GemmDesc* d_descs, int nprob, int M, int N, int K, hip
 }  // namespace
 
 template <typename R>
-void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile) {
-    launch_gemm_sr<R, R>(d_descs, nprob, M, N, K, st, ws, tile);
+void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile,
+                 int ks_target) {
+    launch_gemm_sr<R, R>(d_descs, nprob, M, N, K, st, ws, tile, ks_target);
 }
 
 void launch_gemm_c64(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile) {
-    launch_gemm_sr<double, float>(d_descs, nprob, M, N, K, st, ws, tile);
+    launch_gemm_sr<double, float>(d_descs, nprob, M, N, K, st, ws, tile, 256);
 }
 
-template void launch_gemm<double>(const GemmDesc*, int, int, int, int, hipStream_t, void*, int);
-template void launch_gemm<float>(const GemmDesc*, int, int, int, int, hipStream_t, void*, int);
+template void launch_gemm<double>(const GemmDesc*, int, int, int, int, hipStream_t, void*, int, int);
+template void launch_gemm<float>(const GemmDesc*, int, int, int, int, hipStream_t, void*, int, int);
 
 }  // namespace rs

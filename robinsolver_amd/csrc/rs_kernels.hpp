@@ -32,16 +32,22 @@ struct GemmDesc {
 enum { GEMM_TILE_64x64 = 0, GEMM_TILE_32x64 = 1, GEMM_TILE_16x64 = 2, GEMM_TILE_64x16 = 3, GEMM_TILE_32x32 = 4,
        GEMM_TILE_64x32 = 5, GEMM_TILE_128x16 = 6, GEMM_TILES = 7 };
 int gemm_tile_rule(int M, int N);
+// ks_target: the K split aims at this many (row tile x slice) workgroups per
+// launch; it never depends on N.
 template <typename R>
 void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws = nullptr,
-                 int tile = -1);
+                 int tile = -1, int ks_target = 256);
 // The same products on complex128 operands and results with complex64 MFMA
 // products and FP32 accumulation (the merge of the mixed-precision
 // factorization).  ws as for launch_gemm<float>.
 void launch_gemm_c64(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws = nullptr,
                      int tile = -1);
-int gemm_kslices(int nprob, int M, int K);
-size_t gemm_ws_elems(int nprob, int M, int N, int K);
+int gemm_kslices(int nprob, int M, int K, int ks_target = 256);
+size_t gemm_ws_elems(int nprob, int M, int N, int K, int ks_target = 256);
+// K-split target of the solve passes: with few right-hand sides a long-K product
+// (a top-level 3D map, K >= 4096) is a matrix-vector stream that needs more
+// workgroups than its row tiles give.  The factor's products keep the default.
+inline int solve_ks_target(int K) { return K >= 4096 ? 1024 : 256; }
 
 // leaf stage
 size_t leaf_blocks_lds(const LeafGeom& G);
