@@ -870,6 +870,195 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Multi-RHS 3D leaf solve on the matrix cores (complex64; layers of s <= 64
+// points, K layers).  A workgroup takes 16 RHS columns of one leaf; wave w owns
+// rows 16w..16w+15 of a layer (rows padded to S, a multiple of 16).  The same
+// block recursion as the 2D kernel,
+//   forward   y_k = Shat_k^-1 (r_k - lambda_k y_{k-1})
+//   backward  x_k = y_k - mu_k Shat_k^-1 x_{k+1},
+// where each layer product is an S x S by S x 16 product: the layer's operand
+// goes through an LDS tile shared by the four waves, and each wave multiplies
+// its 16 block rows, read from global memory one layer ahead, on
+// v_mfma_f32_16x16x4f32 (three real products per complex k-chunk).  The
+// right-hand sides of all layers, and then the solution, are held in LDS
+// ([column][point], stride n_int + 1), so the interior loads and stores and the
+// boundary values (from the first two interior points along each normal) are
+// plain coalesced loops.
+template <int K, int MODE, typename FT, int OUT>
+__global__ void __launch_bounds__(256) leaf3_mfma_kernel(LeafGeom G, LeafCoef C, LeafIO<float> io) {
+    extern __shared__ float2 lds3[];
+    const int s = G.s, ni = G.n_int, nb = G.nb, m = G.m, nl = io.n_leaves;
+    const int S = (s + 15) & ~15;
+    const int XS = ni + 1;                 // column stride of X
+    constexpr int VS = 17;                 // row stride of the operand tile
+    float2* X = lds3;                      // [16][XS]
+    float2* Vt = lds3 + 16 * XS;           // [S][VS]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nch = (io.ncols + 15) / 16;
+    const int leaf = blockIdx.x / nch, c0 = (blockIdx.x - leaf * nch) * 16;
+    const int parity = leaf & 1;
+    const int ncol = min(16, io.ncols - c0);
+
+    // 1. right-hand sides r = f - A_IB A_BB^-1 g of all layers into X
+    for (int e = tid; e < ni * 16; e += 256) {
+        const int q = e % ni, c = e / ni;
+        float2 v = make_float2(0.f, 0.f);
+        if (c < ncol) {
+            v = ccast<float2>(gld(FCol<FT>::base(io, c0 + c, leaf, ni, nl) + q));
+            if (MODE == LEAF_DOWN) {
+                int g[3];
+                interior_coords(G, q, g);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const AxisCoef& a = C.ax[j];
+                    const float2 w = ccast<float2>(cscale(cinv(a.dd), -a.cib));
+                    const size_t gb = ((size_t)(c0 + c) * nl + leaf) * nb;
+                    if (g[j] == 1) v = cfma(w, io.gin[gb + local_to_node(G, parity, face_point(G, 2 * j + 1, g))], v);
+                    if (g[j] == G.n[j]) v = cfma(w, io.gin[gb + local_to_node(G, parity, face_point(G, 2 * j + 2, g))], v);
+                }
+            }
+        }
+        X[c * XS + q] = v;
+    }
+    __syncthreads();
+
+    // wave rows and MFMA lane roles
+    const int r0 = 16 * wave;
+    const bool active = r0 < S;
+    const int ccol = lane & 15, grp = lane >> 4;
+    const int arow = min(r0 + ccol, s - 1);             // A operand row of this lane (clamped)
+    const bool arow_ok = r0 + ccol < s;
+    const int nq = S / 4;                               // k-chunks of a layer product
+    const float2* sl = io.sinv + (size_t)leaf * K * s * s;
+    // A operand of layer k, chunk q: Shat_k^-1[arow][4q + grp] (zero outside s x s)
+    float2 abuf[2][16];
+    auto afetch = [&](int k, float2 (&a)[16]) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int col = 4 * q + grp;
+            const int colc = min(col, s - 1);
+            const float2 t = sl[(size_t)k * s * s + arow + (size_t)colc * s];
+            a[q] = (q < nq && col < s && arow_ok) ? t : make_float2(0.f, 0.f);
+        }
+    };
+    // y = Shat_k^-1 Vt (this wave's 16 rows, 16 columns)
+    auto product = [&](const float2 (&a)[16], float2 (&out)[4]) {
+        f4v p1 = f4v{0.f, 0.f, 0.f, 0.f}, p2 = p1, p3 = p1;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            if (q >= nq) break;
+            const float2 b = Vt[(4 * q + grp) * VS + ccol];
+            p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q].x, b.x, p1, 0, 0, 0);
+            p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q].y, b.y, p2, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q].x + a[q].y, b.x + b.y, p3, 0, 0, 0);
+        }
+#pragma unroll
+        for (int vv = 0; vv < 4; ++vv) out[vv] = make_float2(p1[vv] - p2[vv], (p3[vv] - p1[vv]) - p2[vv]);
+    };
+
+    // 2. forward sweep; y_k replaces r_k in X (each wave reads and writes only
+    // its own rows of X, in its own order)
+    auto xat = [&](int k, int R) -> float2& { return X[ccol * XS + k * s + R]; };
+    afetch(0, abuf[0]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (k + 1 < K) afetch(k + 1, abuf[(k + 1) & 1]);
+        const float2 lam = ccast<float2>(slow_lambda(G, C, k));
+        if (active) {
+#pragma unroll
+            for (int vv = 0; vv < 4; ++vv) {
+                const int R = r0 + 4 * grp + vv;
+                float2 v = make_float2(0.f, 0.f);
+                if (R < s) {
+                    v = xat(k, R);
+                    if (k > 0) v = csub(v, cmul(lam, xat(k - 1, R)));
+                }
+                Vt[R * VS + ccol] = v;
+            }
+        }
+        __syncthreads();
+        if (active) {
+            float2 yk[4];
+            product(abuf[k & 1], yk);
+#pragma unroll
+            for (int vv = 0; vv < 4; ++vv) {
+                const int R = r0 + 4 * grp + vv;
+                if (R < s) xat(k, R) = yk[vv];
+            }
+        }
+        __syncthreads();
+    }
+    // 3. backward sweep: x_k replaces y_k in X
+    if (K >= 2) afetch(K - 2, abuf[(K - 2) & 1]);
+#pragma unroll
+    for (int k = K - 2; k >= 0; --k) {
+        if (k >= 1) afetch(k - 1, abuf[(k - 1) & 1]);
+        const float2 mu = ccast<float2>(slow_mu(G, C, k));
+        if (active) {
+#pragma unroll
+            for (int vv = 0; vv < 4; ++vv) {
+                const int R = r0 + 4 * grp + vv;
+                Vt[R * VS + ccol] = (R < s) ? cmul(mu, xat(k + 1, R)) : make_float2(0.f, 0.f);
+            }
+        }
+        __syncthreads();
+        if (active) {
+            float2 pr[4];
+            product(abuf[k & 1], pr);
+#pragma unroll
+            for (int vv = 0; vv < 4; ++vv) {
+                const int R = r0 + 4 * grp + vv;
+                if (R < s) xat(k, R) = csub(xat(k, R), pr[vv]);
+            }
+        }
+        __syncthreads();
+    }
+
+    // 4. outputs
+    auto put = [&](int col, int p, float2 x) {
+        const size_t i32 = ((size_t)col * nl + leaf) * m + p;
+        if (OUT == OUT_U32) io.u32[i32] = x;
+        else {
+            double2* u = io.u64[col] + (size_t)leaf * m + p;
+            if (OUT == OUT_U64) gst(u, ccast<double2>(x));
+            else if (OUT == OUT_U64_BASE32) gst(u, cadd(ccast<double2>(io.ubase32[i32]), ccast<double2>(x)));
+            else gst(u, cadd(gld(u), ccast<double2>(x)));
+        }
+    };
+    if (MODE == LEAF_DOWN) {
+        for (int e = tid; e < ni * 16; e += 256) {
+            const int q = e % ni, c = e / ni;
+            if (c < ncol) put(c0 + c, nb + q, X[c * XS + q]);
+        }
+    }
+    const float2 sma = make_float2(0.f, (float)(-2.0 * G.kb));
+    for (int e = tid; e < nb * 16; e += 256) {
+        const int b = e % nb, c = e / nb;
+        if (c >= ncol) continue;
+        int f, j, q1, step;
+        boundary_info(G, b, f, j, q1, step);
+        // the axis constants by selects (a runtime index would go through scratch)
+        const double2 dd = j == 0 ? C.ax[0].dd : (j == 1 ? C.ax[1].dd : C.ax[2].dd);
+        const double a1 = j == 0 ? C.ax[0].a1 : (j == 1 ? C.ax[1].a1 : C.ax[2].a1);
+        const double a2 = j == 0 ? C.ax[0].a2 : (j == 1 ? C.ax[1].a2 : C.ax[2].a2);
+        const int col = c0 + c;
+        const size_t gi = ((size_t)col * nl + leaf) * nb;
+        const int pos = local_to_node(G, parity, b);
+        const float2 gb = (MODE == LEAF_DOWN) ? io.gin[gi + pos] : make_float2(0.f, 0.f);
+        const float2 x1 = X[c * XS + q1], x2 = X[c * XS + q1 + step];
+        const float2 num = csub(csub(gb, cscale(x1, (float)a1)), cscale(x2, (float)a2));
+        const float2 uB = cmul(num, ccast<float2>(cinv(dd)));
+        if (MODE == LEAF_UP) {
+            float2 h = cfma(sma, uB, gb);
+            if (io.jump) h = csub(h, io.jump[gi + b]);
+            io.hout[gi + pos] = h;
+        } else {
+            put(col, b, uB);
+        }
+    }
+}
+
 }  // namespace
 
 void launch_residual(const LeafGeom& G, double2* const* u64, const float2* u32, const double2* kint,
@@ -953,6 +1142,16 @@ template <typename R> int leaf_apply_chunk(const LeafGeom& G, int ncols, size_t 
     return ch;
 }
 
+// The 3D matrix-core path: layers of at most 64 points, 4, 6 or 8 layers, the
+// same pass forms as the 2D kernel.
+static bool mfma_leaf3_ok(const LeafGeom& G, const LeafArgs<float>& a) {
+    const bool form = a.mode == LEAF_UP ? (a.f64 || a.f32)
+                    : a.mode == LEAF_DOWN ? ((a.f64 && !a.ubase32 && !a.uacc) ||
+                                             (a.f32 && a.u64 && (a.ubase32 || a.uacc)))
+                    : false;
+    return G.d == 3 && form && G.s <= 64 && (G.K == 4 || G.K == 6 || G.K == 8);
+}
+
 // The matrix-core path takes 2D leaves of 12x12, 14x14 or 16x16 interior points.
 static bool mfma_leaf_ok(const LeafGeom& G, const LeafArgs<float>& a) {
     // forms with a matrix-core kernel: up with f in either precision; down with the
@@ -998,6 +1197,32 @@ void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit
             else if (G.K == 14) RS_LEAF_MFMA(14);
             else RS_LEAF_MFMA(12);
 #undef RS_LEAF_MFMA
+            return;
+        }
+    }
+    if constexpr (std::is_same<R, float>::value) {
+        if (mfma_leaf3_ok(G, a)) {
+            const int S = (G.s + 15) & ~15;
+            const size_t lds = ((size_t)16 * (G.n_int + 1) + (size_t)S * 17) * sizeof(float2);
+            dim3 grid((unsigned)(n_leaves * ((ncols + 15) / 16)));
+#define RS_LEAF3_MFMA(KK)                                                                                       \
+    do {                                                                                                         \
+        auto L = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, G, C, io); };              \
+        if (mode == LEAF_UP) {                                                                                   \
+            if (io.f32) L(leaf3_mfma_kernel<KK, LEAF_UP, float2, OUT_U32>);                                      \
+            else L(leaf3_mfma_kernel<KK, LEAF_UP, double2, OUT_U32>);                                            \
+        } else if (io.f32) {                                                                                     \
+            if (io.ubase32) L(leaf3_mfma_kernel<KK, LEAF_DOWN, float2, OUT_U64_BASE32>);                         \
+            else L(leaf3_mfma_kernel<KK, LEAF_DOWN, float2, OUT_U64_ACC>);                                       \
+        } else {                                                                                                 \
+            if (io.u64) L(leaf3_mfma_kernel<KK, LEAF_DOWN, double2, OUT_U64>);                                   \
+            else L(leaf3_mfma_kernel<KK, LEAF_DOWN, double2, OUT_U32>);                                          \
+        }                                                                                                        \
+    } while (0)
+            if (G.K == 8) RS_LEAF3_MFMA(8);
+            else if (G.K == 6) RS_LEAF3_MFMA(6);
+            else RS_LEAF3_MFMA(4);
+#undef RS_LEAF3_MFMA
             return;
         }
     }

@@ -31,7 +31,9 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("d,n,L,kb,n_refine", [(2, 64, 4, 10.0, 2), (2, 128, 6, 30.0, 2), (3, 16, 6, 4.0, 2),
-                                                 (2, 64, 4, 10.0, 0)])
+                                                 (2, 64, 4, 10.0, 0),
+                                                 # 3D leaves of 8^3 (64-point layers) and 6^3 (36, padded)
+                                                 (3, 16, 3, 4.0, 2), (3, 12, 3, 4.0, 2), (3, 16, 3, 4.0, 0)])
 def test_mixed_matches_oracle(d, n, L, kb, n_refine, gpu):
     print("case: mixed", d, n, L, kb, n_refine, flush=True)
     o = OracleTree(d, L, UNIT, 1.0 / n, -kb)
@@ -122,3 +124,20 @@ def test_complex64_merges_and_precision_rules(fp64_levels, gpu):
     for u in (tm.solve(f, g), td.solve(f, g, precision=rsa.RS_MIXED)):
         for r in range(2):
             assert _rel(u[r], o.solve(f[r], g[r])) <= TOL_MIXED
+
+
+@pytest.mark.parametrize("n,L", [(16, 3), (16, 6)])   # 8^3 and 4^3 leaves
+def test_mixed_3d_batch_invariance(n, L, gpu):
+    """3D leaves: more columns than one workgroup's 16; each column alone gives
+    the bits it gets in the batch."""
+    print("case: mixed 3d batch invariance", n, L, flush=True)
+    t = rsa.GpuTree(3, L, UNIT, 1.0 / n, -4.0, precision=rsa.RS_MIXED, n_refine=1)
+    pts, _ = t.all_points()
+    t.factor(_k(pts[:, t.nb:], 4.0))
+    rng = np.random.default_rng(14)
+    nrhs = 37
+    f = _rand_c(rng, (nrhs, t.n_leaves, t.n_int))
+    g = _rand_c(rng, (nrhs, t.n_leaves, t.nb))
+    u = t.solve(f, g)
+    for r in (0, 16, 36):
+        assert np.array_equal(t.solve(f[r], g[r]), u[r])
