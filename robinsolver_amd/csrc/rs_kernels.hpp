@@ -45,9 +45,12 @@ void launch_gemm_c64(const GemmDesc* d_descs, int nprob, int M, int N, int K, hi
 int gemm_kslices(int nprob, int M, int K, int ks_target = 256);
 size_t gemm_ws_elems(int nprob, int M, int N, int K, int ks_target = 256);
 // K-split target of the solve passes: with few right-hand sides a long-K product
-// (a top-level 3D map, K >= 4096) is a matrix-vector stream that needs more
-// workgroups than its row tiles give.  The factor's products keep the default.
-inline int solve_ks_target(int K) { return K >= 4096 ? 1024 : 256; }
+// (the top-level 3D maps: K >= 4096, or K >= 2048 over at least 4096 rows in
+// all) is a matrix-vector stream that needs more workgroups than its row tiles
+// give.  The factor's products keep the default.
+inline int solve_ks_target(int nprob, int M, int K) {
+    return (K >= 4096 || (K >= 2048 && (long long)nprob * M >= 4096)) ? 1024 : 256;
+}
 
 // leaf stage
 size_t leaf_blocks_lds(const LeafGeom& G);

@@ -508,9 +508,10 @@ template <typename R> int Tree::ensure_work(SolveWork<R>& w, int nrhs) {
     size_t wse = 0;
     for (int l = 0; l < L; ++l) {
         const Level& Vl = P.lv[l];
-        wse = std::max(wse, gemm_ws_elems(Vl.nodes, 2 * Vl.isz, cap, 2 * Vl.isz, solve_ks_target(2 * Vl.isz)));
-        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.bn, cap, Vl.isz, solve_ks_target(Vl.isz)));
-        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.isz, cap, Vl.bn, solve_ks_target(Vl.bn)));
+        wse = std::max(wse, gemm_ws_elems(Vl.nodes, 2 * Vl.isz, cap, 2 * Vl.isz,
+                                          solve_ks_target(Vl.nodes, 2 * Vl.isz, 2 * Vl.isz)));
+        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.bn, cap, Vl.isz, solve_ks_target(2 * Vl.nodes, Vl.bn, Vl.isz)));
+        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.isz, cap, Vl.bn, solve_ks_target(2 * Vl.nodes, Vl.isz, Vl.bn)));
     }
     if (wse && alloc(w.ws, wse * sizeof(V))) return 1;
     for (int l = 0; l < L; ++l) {
@@ -629,7 +630,7 @@ template <typename R> int Tree::tune_work(SolveWork<R>& w) {
             float best = 0.f;
             int bt = -1;
             for (int t = 0; t < GEMM_TILES; ++t) {
-                const int kt = solve_ks_target(st_.K);
+                const int kt = solve_ks_target(st_.np, st_.M, st_.K);
                 launch_gemm<R>(st_.d, st_.np, st_.M, N, st_.K, stream, w.ws.p, t, kt);
                 (void)hipEventRecord(e0, stream);
                 for (int r = 0; r < 3; ++r) launch_gemm<R>(st_.d, st_.np, st_.M, N, st_.K, stream, w.ws.p, t, kt);
@@ -661,9 +662,10 @@ long long Tree::work_bytes(int cap, int prec) const {
     for (int l = 0; l < L; ++l) {
         const Level& Vl = P.lv[l];
         b += 2 * (long long)((double)cap * Vl.nodes * 2 * Vl.isz * e);
-        wse = std::max(wse, gemm_ws_elems(Vl.nodes, 2 * Vl.isz, cap, 2 * Vl.isz, solve_ks_target(2 * Vl.isz)));
-        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.bn, cap, Vl.isz, solve_ks_target(Vl.isz)));
-        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.isz, cap, Vl.bn, solve_ks_target(Vl.bn)));
+        wse = std::max(wse, gemm_ws_elems(Vl.nodes, 2 * Vl.isz, cap, 2 * Vl.isz,
+                                          solve_ks_target(Vl.nodes, 2 * Vl.isz, 2 * Vl.isz)));
+        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.bn, cap, Vl.isz, solve_ks_target(2 * Vl.nodes, Vl.bn, Vl.isz)));
+        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.isz, cap, Vl.bn, solve_ks_target(2 * Vl.nodes, Vl.isz, Vl.bn)));
         const long long nd = (long long)Vl.nodes * (l > 0 ? 6 : 4);
         b += nd * (long long)sizeof(GemmDesc);
     }
@@ -771,7 +773,7 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     auto gemm = [&](int cat, int lvl, const GemmDesc* d, int np, int M, int N, int K, bool din) {
         const std::vector<int>& tiles = w.tile[cat - P_GEMM_UP_G];   // up_g, up_h, dn_t, dn_g
         pbeg(cat, st);
-        launch_gemm<R>(d, np, M, N, K, st, w.ws.p, lvl < (int)tiles.size() ? tiles[lvl] : -1, solve_ks_target(K));
+        launch_gemm<R>(d, np, M, N, K, st, w.ws.p, lvl < (int)tiles.size() ? tiles[lvl] : -1, solve_ks_target(np, M, K));
         pend((double)np * e * ((double)M * K + (double)K * N + (double)M * N * (din ? 2 : 1)),
              8.0 * np * (double)M * N * K, st);
     };
