@@ -66,10 +66,12 @@ constexpr int KALIGN = 32; // K-split slices start at multiples of this (whateve
 // BK: K depth of one staged chunk; DB: two LDS chunk buffers (one barrier per
 // chunk) instead of one (two).  Neither changes the order in which a column's
 // products are accumulated.
-// (Measured on the solve-pass shapes at N = 256: BK = 16 with two buffers beat
-// BK = 32 or 64 with one by 10-25%.)  The MFMA steps of a chunk are always fully
-// unrolled.
-template <typename S, typename R, int BM, int BN, int BK = 16, bool DB = true>
+// (Measured on the solve-pass shapes: at N = 256, BK = 16 with two buffers beat
+// BK = 32 or 64 with one by 10-25%; the 16-column tiles, which carry the
+// few-RHS matrix streams, do best with BK = 32 and one buffer, 12% ahead of
+// BK = 16 with two on the 3D top levels at N = 16.)  The MFMA steps of a chunk
+// are always fully unrolled.
+template <typename S, typename R, int BM, int BN, int BK = (BN == 16 ? 32 : 16), bool DB = (BN != 16)>
 __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ descs, int M, int N, int K,
                                                   int tilesM, int tilesN, int kslices, int kchunk,
                                                   c_t<R>* __restrict__ ws) {

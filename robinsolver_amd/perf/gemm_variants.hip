@@ -1,10 +1,11 @@
 // gemm_variants.hip -- times K-loop variants of the batched complex GEMM
-// (robinsolver_amd/csrc/k_gemm.hip) on the solve-pass shapes of the 2D 1024^2
-// tree (complex64 operands, N = 256 right-hand sides) and checks that every
+// (robinsolver_amd/csrc/k_gemm.hip) on solve-pass shapes and checks that every
 // variant gives the bits of the first.  Not part of the library.
+//   (no argument)  the 2D 1024^2 tree's shapes, N = 256 right-hand sides
+//   3d             the 3D 64^3 tree's top-level shapes, N = 16
 //
 //   hipcc -x hip --offload-arch=gfx950 -Xclang -target-feature -Xclang -packed-fp32-ops -O3 -std=c++17 \
-//         robinsolver_amd/perf/gemm_variants.hip -o gemm_variants && ./gemm_variants
+//         robinsolver_amd/perf/gemm_variants.hip -o gemm_variants && ./gemm_variants [3d]
 #include "../csrc/k_gemm.hip"
 
 #include <cstdio>
@@ -25,7 +26,7 @@ using namespace rs;
 
 template <int BM, int BN, int BK, bool DB>
 void launch_var(const GemmDesc* d, int np, int M, int N, int K, void* ws) {
-    int ks = gemm_kslices(np, M, K);
+    int ks = gemm_kslices(np, M, K, solve_ks_target(np, M, K));
     const int kchunk = ((K + ks - 1) / ks + 31) / 32 * 32;
     ks = (K + kchunk - 1) / kchunk;
     const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
@@ -43,17 +44,35 @@ struct Var {
     void (*f)(const GemmDesc*, int, int, int, int, void*);
 };
 
-int main() {
-    struct Shape {
-        const char* name;
-        int M, K, np;
-        bool din;
-    } shapes[] = {{"upg_l0", 2048, 2048, 1, false}, {"upg_l1", 1024, 1024, 2, false},
+struct Shape {
+    const char* name;
+    int M, K, np;
+    bool din;
+};
+
+int main(int argc, char** argv) {
+    const bool d3 = argc > 1 && std::strcmp(argv[1], "3d") == 0;
+    // 3D 64^3, 8^3 leaves: level 0 splits x (i = 4096), level 1 y (2048), level 2 z (1024)
+    const Shape shapes3[] = {{"upg_l0", 8192, 8192, 1, false}, {"dnt_l0", 4096, 12288, 2, false},
+                             {"uph_l0", 12288, 4096, 2, true}, {"upg_l1", 4096, 4096, 2, false},
+                             {"dnt_l1", 2048, 8192, 4, false}, {"upg_l2", 2048, 2048, 4, false},
+                             {"dnt_l2", 1024, 6144, 8, false}, {"upg_l3", 2048, 2048, 8, false}};
+    const Var vars3[] = {{"64x16 bk16 2buf", launch_var<64, 16, 16, true>},
+                         {"64x16 bk32 2buf", launch_var<64, 16, 32, true>},
+                         {"64x16 bk64 2buf", launch_var<64, 16, 64, true>},
+                         {"64x16 bk32 1buf", launch_var<64, 16, 32, false>},
+                         {"64x16 bk64 1buf", launch_var<64, 16, 64, false>},
+                         {"128x16 bk16 2buf", launch_var<128, 16, 16, true>},
+                         {"128x16 bk32 2buf", launch_var<128, 16, 32, true>},
+                         {"128x16 bk64 1buf", launch_var<128, 16, 64, false>},
+                         {"32x32 bk16 2buf", launch_var<32, 32, 16, true>},
+                         {"64x64 bk16 2buf", launch_var<64, 64, 16, true>}};
+    const Shape shapes2[] = {{"upg_l0", 2048, 2048, 1, false}, {"upg_l1", 1024, 1024, 2, false},
                   {"upg_l2", 1024, 1024, 4, false}, {"upg_l4", 512, 512, 16, false},
                   {"upg_l6", 256, 256, 64, false},  {"upg_l8", 128, 128, 256, false},
                   {"upg_l10", 64, 64, 1024, false}, {"uph_l2", 1024, 512, 8, true},
                   {"dnt_l2", 512, 1024, 8, false},  {"uph_l6", 256, 128, 128, true}};
-    const Var vars[] = {{"64x64 bk16 2buf", launch_var<64, 64, 16, true>},
+    const Var vars2[] = {{"64x64 bk16 2buf", launch_var<64, 64, 16, true>},
                         {"64x64 bk32 1buf", launch_var<64, 64, 32, false>},
                         {"64x64 bk32 2buf", launch_var<64, 64, 32, true>},
                         {"64x64 bk64 1buf", launch_var<64, 64, 64, false>},
@@ -65,16 +84,19 @@ int main() {
                         {"64x16 bk16 2buf", launch_var<64, 16, 16, true>},
                         {"16x64 bk16 2buf", launch_var<16, 64, 16, true>},
                         {"128x16 bk16 2buf", launch_var<128, 16, 16, true>}};
-    const int N = 256, REPS = 20;
+    const Shape* shapes = d3 ? shapes3 : shapes2;
+    const Var* vars = d3 ? vars3 : vars2;
+    const int ns = d3 ? (int)(sizeof(shapes3) / sizeof(shapes3[0])) : (int)(sizeof(shapes2) / sizeof(shapes2[0]));
+    const int nv = d3 ? (int)(sizeof(vars3) / sizeof(vars3[0])) : (int)(sizeof(vars2) / sizeof(vars2[0]));
+    const int N = d3 ? 16 : 256, REPS = 20;
     hipEvent_t e0, e1;
     CHK(hipEventCreate(&e0));
     CHK(hipEventCreate(&e1));
     std::printf("| variant |");
-    for (const Shape& sh : shapes) std::printf(" %s |", sh.name);
+    for (int si = 0; si < ns; ++si) std::printf(" %s |", shapes[si].name);
     std::printf(" sum |\n|---|");
-    for (size_t i = 0; i <= sizeof(shapes) / sizeof(shapes[0]); ++i) std::printf("---:|");
+    for (int i = 0; i <= ns; ++i) std::printf("---:|");
     std::printf("\n");
-    const int nv = sizeof(vars) / sizeof(vars[0]), ns = sizeof(shapes) / sizeof(shapes[0]);
     std::vector<std::vector<double>> us(nv, std::vector<double>(ns));
     std::vector<std::vector<int>> same(nv, std::vector<int>(ns, 1));
     for (int si = 0; si < ns; ++si) {
@@ -90,7 +112,7 @@ int main() {
         CHK(hipMalloc(&A, hA.size() * sizeof(float2)));
         CHK(hipMalloc(&B, hB.size() * sizeof(float2)));
         CHK(hipMalloc(&C, nc * sizeof(float2)));
-        const size_t wse = gemm_ws_elems(np, M, N, K);
+        const size_t wse = gemm_ws_elems(np, M, N, K, solve_ks_target(np, M, K));
         if (wse) CHK(hipMalloc(&ws, wse * sizeof(float2)));
         CHK(hipMemcpy(A, hA.data(), hA.size() * sizeof(float2), hipMemcpyHostToDevice));
         CHK(hipMemcpy(B, hB.data(), hB.size() * sizeof(float2), hipMemcpyHostToDevice));
