@@ -267,7 +267,7 @@ def serve_mix(args):
 
         drive(srv, jobs(60), None, 3)
         for name, frac in (("saturated", None), ("poisson70", 0.7)):
-            js = jobs(args.serve_requests)
+            js = jobs(args.mix_requests)
             srv.reset_stats()
             nreq = len(js)
             rate = None if frac is None else frac * out["saturated"]["requests_per_s"]
@@ -354,7 +354,10 @@ def main():
                     help="3: the 3D workload (e.g. --dim 3 --grid 64 --levels 9 --kb 8 --nrhs 16)")
     ap.add_argument("--r1-solves", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--serve-requests", type=int, default=600,
+                    help="requests per phase of the single-tree serving scenario")
+    ap.add_argument("--mix-requests", type=int, default=2000,
+                    help="requests per phase of the 8-tree serving scenario (--serve-mix)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-mix", action="store_true",
                     help="only the multi-tree serving scenario (8 resident trees 128^2..1024^2)")
@@ -391,7 +394,7 @@ def main():
             "dtype": "c64", "data": "synthetic",
             "config": {"workload": "8 trees (grid, L, kb) = %s; requests of 1..32 plane-wave RHS, 2%% refactors"
                                    % [list(x) for x in MIX_TREES], "max_cols": args.nrhs, "lanes": 2,
-                       "requests": args.serve_requests},
+                       "requests": args.mix_requests},
             "p50_ms": mix["poisson70"]["lat_p50_ms"], "p99_ms": mix["poisson70"]["lat_p99_ms"], "mix": mix}))
         return
 
