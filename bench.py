@@ -504,22 +504,29 @@ def main():
     c64_need = min([nr for nr in c64_err if c64_err[nr] <= 1e-8], default=None)
     log("complex64 merges: factor %.4f s, max rel l2 by refinement steps %s" % (f_time_c64, c64_err))
 
-    # mixed path, timed steps
+    # mixed path, timed steps (replayed as a captured graph, no per-launch events)
     for _ in range(args.warmup):
         solve(t, args.nrhs, f.data_ptr(), g.data_ptr(), u32.data_ptr(), rsa.RS_MIXED)
-    torch.cuda.synchronize()
-    t.profile(True)
-    t.profile_read()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         solve(t, args.nrhs, f.data_ptr(), g.data_ptr(), u32.data_ptr(), rsa.RS_MIXED)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    prof = t.profile_read()
-    t.profile(False)
     mixed_rate = args.steps * args.nrhs / elapsed
     log("mixed: %.1f solves/s (%.2f ms/step)" % (mixed_rate, 1e3 * elapsed / args.steps))
+    # the same steps again with HIP events around every launch (on the stream the
+    # kernels run on), for the per-kernel times and the roofline
+    nprof = args.steps
+    t.profile(True)
+    t.profile_read()
+    for _ in range(nprof):
+        solve(t, args.nrhs, f.data_ptr(), g.data_ptr(), u32.data_ptr(), rsa.RS_MIXED)
+    torch.cuda.synchronize()
+    prof = t.profile_read()
+    t.profile(False)
+    prof_ms = sum(v[0] for v in prof.values()) / nprof
+    log("kernel time per step (events): %.2f ms" % prof_ms)
 
     # accuracy of every mixed RHS against the FP64 GPU solution
     diff = (u32 - u64).reshape(args.nrhs, -1)
@@ -557,7 +564,7 @@ def main():
     for c, (cms, cn, cb, cf) in prof.items():
         if cn == 0:
             continue
-        kernels[c] = {"ms_per_step": cms / args.steps, "launches_per_step": cn / args.steps,
+        kernels[c] = {"ms_per_step": cms / nprof, "launches_per_step": cn / nprof,
                       "GB_per_s": cb / (cms / 1e3) / 1e9 if cms > 0 else 0.0,
                       "TFLOP_per_s": cf / (cms / 1e3) / 1e12 if cms > 0 else 0.0}
 
@@ -599,7 +606,8 @@ def main():
                              "max_rel_l2_by_refinement_steps": c64_err, "refinement_steps_needed": c64_need},
         "reference_flops": {"factor_GF": ff / 1e9, "solve_GF_per_rhs": fs / 1e9},
         "roofline": roof,
-        "kernel_ms_per_step": {c: v[0] / args.steps for c, v in prof.items()},
+        "kernel_ms_per_step": {c: v[0] / nprof for c, v in prof.items()},
+        "kernel_profile": "HIP events around every launch in %d separate steps after the timed ones" % nprof,
         "kernels": kernels,
         "gemm_tiles_mixed": t.tile_choices(rsa.RS_MIXED).tolist(),
         "fd_residual_fp64": {"max": float(res64.max()), "median": float(np.median(res64))},
