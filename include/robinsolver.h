@@ -50,9 +50,12 @@ typedef struct rs_opts {
     int n_refine;         /* refinement steps of the mixed path                 */
     int verbose;          /* 1: progress lines as the reference prints them     */
     long long hbm_budget; /* bytes this tree may hold on the device (0 = no cap) */
-    int autotune;         /* 1: when a solve workspace is set up, time the matrix-
-                             core tile shapes of each of its products and keep the
-                             fastest (the results do not depend on the choice)   */
+    int autotune;         /* 1: time the matrix-core tile shapes of each solve
+                             product once per workspace capacity (and of the merge
+                             products on the first factorization) and keep the
+                             fastest; the results do not depend on the choice.
+                             A server tunes a tree when it is registered or after
+                             its first factorization, not while dispatching     */
     int graphs;           /* 1: rs_solve / rs_solve_device capture the launch
                              sequence of each solve shape (precision, nrhs) once
                              as a hipGraph and replay it                          */
