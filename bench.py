@@ -47,6 +47,31 @@ def potential(pts_int, kb, seed, nmodes=16, d=2):
     return kb ** 2 * (1 + 0.3 * q) + 1j * 0.01 * kb ** 2
 
 
+def potential_dev(torch, pts_int_dev, kb, seed, nmodes=16, d=2):
+    """potential() evaluated on the GPU at device coordinates (the same seeded
+    modes; complex128 [n_leaves][n_int])."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-1, 1, nmodes)
+    p = rng.integers(1, 9, size=(nmodes, d))
+    ph = rng.uniform(0, 2 * np.pi, nmodes)
+    x = pts_int_dev[..., :d]
+    q = torch.zeros(x.shape[:-1], dtype=torch.float64, device=x.device)
+    for j in range(nmodes):
+        pj = torch.tensor(p[j], dtype=torch.float64, device=x.device)
+        q += float(a[j]) * torch.cos(2 * np.pi * (x @ pj) + float(ph[j]))
+    q /= float(np.abs(a).sum())
+    return (kb ** 2 * (1 + 0.3 * q)).to(torch.complex128) + 1j * 0.01 * kb ** 2
+
+
+def device_points(torch, tree):
+    """All points' coordinates [n_leaves][m][3] and face ids [n_leaves][m] on the GPU."""
+    xyz = torch.empty((tree.n_leaves, tree.m, 3), dtype=torch.float64, device="cuda:0")
+    face = torch.empty((tree.n_leaves, tree.m), dtype=torch.int32, device="cuda:0")
+    tree.points_device(xyz.data_ptr(), face.data_ptr())
+    torch.cuda.synchronize()
+    return xyz, face
+
+
 def gaussian_sources(torch, pts_int_dev, nrhs, seed, sigma=0.02, d=2):
     rng = np.random.default_rng(seed + 1000)
     c = torch.tensor(rng.uniform(0.1, 0.9, size=(nrhs, d)), device=pts_int_dev.device, dtype=torch.float64)
@@ -185,24 +210,26 @@ MIX_TREES = [(128, 6, 10.0), (256, 8, 10.0), (384, 10, 20.0), (512, 10, 20.0), (
              (896, 12, 80.0), (1024, 12, 80.0)]
 
 
-def plane_wave_g(tree, pts, face, kb, thetas):
+def plane_wave_g_dev(torch, tree, xyz, face, kb, thetas):
     """Impedance data of u = exp(i kb d.x), d = (cos t, sin t), on the physical
-    faces: g = a1 u + du/dn with a1 = -i kb, so g = i kb (d.n - 1) u.
-    Returns [len(thetas)][n_leaves][n_bnd] (zero away from the boundary)."""
+    faces, g = a1 u + du/dn with a1 = -i kb, so g = i kb (d.n - 1) u; evaluated
+    on the GPU from device coordinates and face ids.  Returns
+    [len(thetas)][n_leaves][n_bnd] (zero away from the boundary)."""
     nb = tree.nb
-    x = pts[:, :nb, :2]
-    fc = np.broadcast_to(face[:nb], x.shape[:2])
-    normal = np.zeros(x.shape)
-    phys = np.zeros(x.shape[:2], bool)
+    x = xyz[:, :nb, :2]
+    fc = face[:, :nb]
+    normal = torch.zeros_like(x)
+    phys = torch.zeros(x.shape[:2], dtype=torch.bool, device=x.device)
     for fid, ax, sgn, edge in ((1, 0, -1.0, 0.0), (2, 0, 1.0, 1.0), (3, 1, -1.0, 0.0), (4, 1, 1.0, 1.0)):
-        on = (fc == fid) & (np.abs(x[..., ax] - edge) < 1e-12)
+        on = (fc == fid) & ((x[..., ax] - edge).abs() < 1e-12)
         phys |= on
-        normal[..., ax] = np.where(fc == fid, sgn, normal[..., ax])
-    out = np.zeros((len(thetas),) + x.shape[:2], np.complex128)
+        normal[..., ax] = torch.where(fc == fid, torch.full_like(normal[..., ax], sgn), normal[..., ax])
+    out = torch.zeros((len(thetas),) + tuple(x.shape[:2]), dtype=torch.complex128, device=x.device)
     for i, th in enumerate(thetas):
-        dv = np.array([np.cos(th), np.sin(th)])
-        u = np.exp(1j * kb * (x @ dv))
-        out[i] = np.where(phys, 1j * kb * ((normal @ dv) - 1.0) * u, 0.0)
+        dv = torch.tensor([np.cos(th), np.sin(th)], dtype=torch.float64, device=x.device)
+        u = torch.exp(1j * kb * (x @ dv).to(torch.complex128))
+        val = 1j * kb * ((normal @ dv) - 1.0).to(torch.complex128) * u
+        out[i] = torch.where(phys, val, torch.zeros_like(val))
     return out
 
 
@@ -220,14 +247,14 @@ def serve_mix(args):
     trees, info = [], []
     for gi, (n, L, kb) in enumerate(MIX_TREES):
         t = rsa.GpuTree(2, L, unit, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=1)
-        pts, face = t.all_points()
-        ks = [torch.from_numpy(potential(pts[:, t.nb:], kb, args.seed + 100 * gi + j)).to(dev) for j in range(2)]
-        t.factor(ks[0].cpu().numpy())
-        gpool = torch.from_numpy(plane_wave_g(t, pts, face, kb, rng.uniform(0, 2 * np.pi, 32))).to(dev)
+        xyz, face = device_points(torch, t)
+        ks = [potential_dev(torch, xyz[:, t.nb:], kb, args.seed + 100 * gi + j).contiguous() for j in range(2)]
+        t.factor_device(ks[0].data_ptr())
+        gpool = plane_wave_g_dev(torch, t, xyz, face, kb, rng.uniform(0, 2 * np.pi, 32))
         fpool = torch.zeros((32, t.n_leaves, t.n_int), dtype=torch.complex128, device=dev)
         # accuracy of one refinement step on this tree, against its FP64 solve
         t64 = rsa.GpuTree(2, L, unit, 1.0 / n, -kb, precision=rsa.RS_FP64)
-        t64.factor(ks[0].cpu().numpy())
+        t64.factor_device(ks[0].data_ptr())
         ua = torch.zeros((4, t.n_leaves, t.m), dtype=torch.complex128, device=dev)
         ub = torch.zeros_like(ua)
         t.solve_device(4, fpool.data_ptr(), gpool.data_ptr(), ua.data_ptr())
@@ -371,12 +398,13 @@ def main():
 
     if args.factor_only:
         import robinsolver_amd as rsa
+        import torch
         t = rsa.GpuTree(2, args.levels, [[0, 1], [0, 1], [0, 1]], 1.0 / args.grid, -args.kb, precision=rsa.RS_FP64)
-        pts, _ = t.all_points()
-        k = potential(pts[:, t.nb:], args.kb, args.seed)
+        xyz, _ = device_points(torch, t)
+        k_dev = potential_dev(torch, xyz[:, t.nb:], args.kb, args.seed).contiguous()
         times = []
         for _ in range(args.factor_only):
-            t.factor(k)
+            t.factor_device(k_dev.data_ptr())
             times.append(t.timings()[0])
             log("factor %.4f s" % times[-1])
         ff, _ = t.algorithmic_counts()
@@ -411,11 +439,10 @@ def main():
         args.no_serve = True
     log("plan: %dD %d^%d, L=%d, kb=%g, nrhs=%d" % (d, args.grid, d, args.levels, kb_phys, args.nrhs))
     t = rsa.GpuTree(d, args.levels, unit, 1.0 / args.grid, kb_opt, precision=rsa.RS_MIXED)
-    pts, _ = t.all_points()
-    pint = pts[:, t.nb:, :]
-    k = potential(pint, kb_phys, args.seed, nmodes=16 if d == 2 else 24, d=d)
-    k_dev = torch.from_numpy(k).to(dev)
-    pint_dev = torch.from_numpy(np.ascontiguousarray(pint)).to(dev)
+    # inputs sampled on the GPU at the device coordinates of the points
+    xyz, _ = device_points(torch, t)
+    pint_dev = xyz[:, t.nb:, :]
+    k_dev = potential_dev(torch, pint_dev, kb_phys, args.seed, nmodes=16 if d == 2 else 24, d=d).contiguous()
     f = gaussian_sources(torch, pint_dev, args.nrhs, args.seed, sigma=0.02 if d == 2 else 0.05, d=d)
     g = torch.zeros((args.nrhs, t.n_leaves, t.nb), dtype=torch.complex128, device=dev)
     u64 = torch.zeros((args.nrhs, t.n_leaves, t.m), dtype=torch.complex128, device=dev)
@@ -424,14 +451,14 @@ def main():
 
     # factor (FP64 interface inverses; the mixed tree also converts the solve data)
     log("factor ...")
-    t.factor(k)
+    t.factor_device(k_dev.data_ptr())
     f_time_mixed = t.timings()[0]
-    t.factor(k)
+    t.factor_device(k_dev.data_ptr())
     f_time_mixed = min(f_time_mixed, t.timings()[0])
     t64 = rsa.GpuTree(d, args.levels, unit, 1.0 / args.grid, kb_opt, precision=rsa.RS_FP64)
-    t64.factor(k)
+    t64.factor_device(k_dev.data_ptr())
     f_time_64 = t64.timings()[0]
-    t64.factor(k)
+    t64.factor_device(k_dev.data_ptr())
     f_time_64 = min(f_time_64, t64.timings()[0])
     log("factor s: fp64 %.4f, mixed %.4f" % (f_time_64, f_time_mixed))
 
@@ -491,8 +518,8 @@ def main():
     # (merge_fp64_levels 0): its time, and the refinement it then needs (run
     # before the timed steps, whose launches end the kernel trace)
     tc = rsa.GpuTree(d, args.levels, unit, 1.0 / args.grid, kb_opt, precision=rsa.RS_MIXED, merge_fp64_levels=0)
-    tc.factor(k)
-    tc.factor(k)
+    tc.factor_device(k_dev.data_ptr())
+    tc.factor_device(k_dev.data_ptr())
     f_time_c64 = tc.timings()[0]
     c64_err = {}
     for nr in (0, 1, 2):

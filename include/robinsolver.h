@@ -86,6 +86,18 @@ int rs_plan(rs_tree *t, int d, int L, const double *domain, double h_tgt, double
  * fd_module.f90:669-700) and the face id (1..6, 0 = interior) of each point.
  * face may be NULL. */
 int rs_leaf_points(const rs_tree *t, int leaf, double *xyz, int *face);
+
+/* rs_leaf_points of every leaf at once, written to device memory on `stream`
+ * (a hipStream_t; NULL: the tree's stream): xyz [n_leaves][m][3], face
+ * [n_leaves][m] (may be NULL); the same bits as rs_leaf_points.  So that
+ * callers sample k, f and g on the GPU instead of in host callbacks (the
+ * reference evaluates them on the host: fd_module.f90:136-145, 427-431,
+ * 645-651). */
+int rs_points_device(rs_tree *t, double *xyz, int *face, void *stream);
+
+/* rs_factor with k at the interior points already in device memory
+ * ([n_leaves][n_int] complex128, DFS leaf order), synchronous. */
+int rs_factor_device(rs_tree *t, const double *k_int);
 /* Leaf grid: points per axis (ptbox) and spacing h, the values the reference
  * stores in node%ptbox and T%opts%h (fd_module.f90:1060-1084,
  * robin_tree_module.f90:392).  h(3) = 0 and ptbox(3) = -1 in 2D. */

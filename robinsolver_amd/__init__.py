@@ -76,6 +76,8 @@ def lib():
         L.rs_residual_norms.argtypes = [vp, ctypes.c_int, vp, vp, vp, dp]
         L.rs_tile_choices.argtypes = [vp, ctypes.c_int, ip]
         L.rs_leaf_grid.argtypes = [vp, ip, dp]
+        L.rs_points_device.argtypes = [vp, vp, vp, vp]
+        L.rs_factor_device.argtypes = [vp, vp]
         from . import serve as _serve
         _serve.bind(L)
         _lib = L
@@ -145,6 +147,16 @@ class GpuTree:
         for q in range(self.n_leaves):
             pts[q], face = self.leaf_points(q)
         return pts, face
+
+    def points_device(self, xyz_ptr, face_ptr=0, stream=None):
+        """Every point's coordinates into device memory at xyz_ptr ([n_leaves][m][3]
+        float64) and, when face_ptr is given, the face ids ([n_leaves][m] int32)."""
+        self._check(lib().rs_points_device(self.h, ctypes.c_void_p(xyz_ptr), ctypes.c_void_p(face_ptr or 0),
+                                           ctypes.c_void_p(stream or 0)))
+
+    def factor_device(self, k_ptr):
+        """rs_factor from a device array of k at the interior points ([n_leaves][n_int] complex128)."""
+        self._check(lib().rs_factor_device(self.h, ctypes.c_void_p(k_ptr)))
 
     def leaf_box(self, leaf):
         b = np.zeros(6)

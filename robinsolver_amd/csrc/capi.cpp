@@ -64,6 +64,18 @@ int rs_leaf_points(const rs_tree* t, int leaf, double* xyz, int* face) {
     return 0;
 }
 
+int rs_points_device(rs_tree* t, double* xyz, int* face, void* stream) {
+    if (!t) { g_err = "Error: null handle."; return 1; }
+    return t->t.points_device(xyz, face, (hipStream_t)stream);
+}
+
+int rs_factor_device(rs_tree* t, const double* k_int) {
+    if (!t || !k_int) { g_err = "Error: null argument."; return 1; }
+    if (!t->t.planned) return t->t.fail("Error: rs_plan must be called before rs_factor.");
+    if (t->t.ensure_device()) return 1;
+    return t->t.factor_device((const double2*)k_int);
+}
+
 int rs_leaf_grid(const rs_tree* t, int* ptbox, double* h) {
     if (!t || !t->t.planned) return 1;
     const rs::Plan& P = t->t.plan;

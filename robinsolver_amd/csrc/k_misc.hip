@@ -153,6 +153,40 @@ void launch_axpy(double2* y, const double2* x, size_t n, hipStream_t st) {
     if (n) hipLaunchKernelGGL(axpy_kernel, dim3(nblk(n)), dim3(256), 0, st, y, x, n);
 }
 
+// Coordinates of every point of every leaf (the host's Plan::leaf_points on
+// the device): xyz [leaf][m][3], face [leaf][m].  box: [leaf][lo x, lo y, lo z,
+// hi x, hi y, hi z]; pat: [m][g0, g1, g2, face].  The coordinate expressions are
+// evaluated without contraction, so the bits are the host's.
+__global__ void leaf_points_kernel(const double* __restrict__ box, const int* __restrict__ pat, double3 h, int3 n,
+                                   int d, int m, int nl, double* __restrict__ xyz, int* __restrict__ face) {
+#pragma clang fp contract(off)
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nl * m) return;
+    const int leaf = (int)(e / m), p = (int)(e - (size_t)leaf * m);
+    const double* b = box + 6 * (size_t)leaf;
+    const int* g = pat + 4 * p;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const double hj = j == 0 ? h.x : (j == 1 ? h.y : h.z);
+        const int nj = j == 0 ? n.x : (j == 1 ? n.y : n.z);
+        const int gj = g[j];
+        double v;
+        if (j < d && gj == 0) v = b[j];
+        else if (j < d && gj == nj + 1) v = b[3 + j];
+        else v = b[j] + gj * hj - hj / 2;
+        xyz[3 * e + j] = v;
+    }
+    if (face) face[e] = g[3];
+}
+
+void launch_leaf_points(const double* box, const int* pat, const double* h, const int* n, int d, int m, int nl,
+                        double* xyz, int* face, hipStream_t st) {
+    const size_t tot = (size_t)nl * m;
+    if (tot == 0) return;
+    hipLaunchKernelGGL(leaf_points_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, box, pat,
+                       make_double3(h[0], h[1], h[2]), make_int3(n[0], n[1], n[2]), d, m, nl, xyz, face);
+}
+
 void launch_col_table(double2** tab, double2* base, size_t stride, int ncols, hipStream_t st) {
     if (ncols > 0) hipLaunchKernelGGL(col_table_kernel, dim3(nblk(ncols)), dim3(256), 0, st, tab, base, stride, ncols);
 }

@@ -265,6 +265,34 @@ void Plan::leaf_points(int leaf, double* xyz, int* face) const {
     }
 }
 
+std::vector<int> Plan::point_pattern() const {
+    const int* n = lv[L].ptbox;
+    std::vector<int> pat;
+    pat.reserve((size_t)m * 4);
+    // boundary points (lin2fgrid): face f, remaining axes first-fastest
+    for (int f = 1; f <= 2 * d; ++f) {
+        int ax = (f - 1) / 2;
+        int rem[2], nr = 0;
+        for (int j = 0; j < d; ++j)
+            if (j != ax) rem[nr++] = j;
+        for (int t = 0; t < lv[L].pf[f]; ++t) {
+            int g[3] = {1, 1, 1};
+            g[ax] = (f % 2 == 1) ? 0 : n[ax] + 1;
+            if (nr >= 1) g[rem[0]] = (nr == 1) ? t + 1 : t % n[rem[0]] + 1;
+            if (nr == 2) g[rem[1]] = t / n[rem[0]] + 1;
+            pat.insert(pat.end(), {g[0], g[1], g[2], f});
+        }
+    }
+    for (int q = 0; q < n_int; ++q) {
+        int g[3] = {1, 1, 1};
+        g[0] = q % n[0] + 1;
+        if (d == 2) g[1] = q / n[0] + 1;
+        else { g[1] = (q / n[0]) % n[1] + 1; g[2] = q / (n[0] * n[1]) + 1; }
+        pat.insert(pat.end(), {g[0], g[1], g[2], 0});
+    }
+    return pat;
+}
+
 std::vector<int> Plan::node_to_face_order(int lvl, int parity) const {
     const Level& V = lv[lvl];
     std::vector<int> perm(V.P);

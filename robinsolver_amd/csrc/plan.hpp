@@ -55,6 +55,9 @@ struct Plan {
     // returns empty string on success, else the error message
     std::string build(int d, int L, const double* domain, double h_tgt, double kb, const double* bp);
     void leaf_points(int leaf, double* xyz, int* face) const;
+    // the leaf grid index (g0, g1, g2) and face id of every point of a leaf, in
+    // the order of leaf_points (the same for all leaves): [m][4]
+    std::vector<int> point_pattern() const;
     // node-order index -> face-order index for a node of level lvl with given parity
     std::vector<int> node_to_face_order(int lvl, int parity) const;
 };

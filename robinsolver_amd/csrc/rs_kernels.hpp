@@ -122,5 +122,7 @@ void launch_zero(double2* p, size_t n, hipStream_t st);
 void launch_axpy(double2* y, const double2* x, size_t n, hipStream_t st);
 // tab[c] = base + c * stride for c < ncols (column table of a contiguous array)
 void launch_col_table(double2** tab, double2* base, size_t stride, int ncols, hipStream_t st);
+void launch_leaf_points(const double* box, const int* pat, const double* h, const int* n, int d, int m, int nl,
+                        double* xyz, int* face, hipStream_t st);
 
 }  // namespace rs

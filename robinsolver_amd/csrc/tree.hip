@@ -166,10 +166,32 @@ int Tree::plan_tree(int d, int L, const double* domain, double h_tgt, double kb,
         free_factor();
         clear_slots();
         tile_cache.clear();
+        d_pts_box.release();
+        d_pts_pat.release();
         device_ready = false;
         if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
     }
     planned = true;
+    return 0;
+}
+
+int Tree::points_device(double* xyz, int* face, hipStream_t st) {
+    if (!planned) return fail("Error: rs_plan must be called first.");
+    if (!xyz) return fail("Error: null argument.");
+    if (ensure_device()) return 1;
+    const Plan& P = plan;
+    if (!d_pts_box.p) {
+        std::vector<double> box((size_t)P.n_leaves * 6);
+        for (int q = 0; q < P.n_leaves; ++q)
+            for (int j = 0; j < 3; ++j) {
+                box[6 * (size_t)q + j] = P.nodes[P.L][q].box[0][j];
+                box[6 * (size_t)q + 3 + j] = P.nodes[P.L][q].box[1][j];
+            }
+        if (upload(d_pts_box, box) || upload(d_pts_pat, P.point_pattern())) return 1;
+    }
+    launch_leaf_points(d_pts_box.as<double>(), d_pts_pat.as<int>(), P.h, P.lv[P.L].ptbox, P.d, P.m, P.n_leaves, xyz,
+                       face, st ? st : stream);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 

@@ -174,6 +174,9 @@ struct Tree {
     // tune the solve products of capacity cap once (a workspace that is freed
     // again); used to keep the tuning out of a serving loop's dispatch path
     int prepare_tiles(int cap);
+    // every point's coordinates and face id into device memory (leaf_points of all leaves)
+    DevBuf d_pts_box, d_pts_pat;
+    int points_device(double* xyz, int* face, hipStream_t st);
     bool tiles_pending(int cap) const {
         return opts.autotune && !tile_cache.count(2LL * cap + (opts.precision == RS_MIXED ? 0 : 1));
     }

@@ -218,3 +218,36 @@ def test_inverse_reports_singular_matrix(n, gpu):
     M[:, n // 3] = 0.0
     flat = np.ascontiguousarray(np.tile(M.reshape(-1, order="F"), 2))
     assert _lib().rs_check_inverse(n, 2, _dp(flat.view(np.float64))) == 2
+
+
+@pytest.mark.parametrize("d,L,h,domain", [(2, 4, 1 / 24, [[0, 1], [0, 1], [0, 1]]),
+                                          (2, 5, 1 / 24, [[0.25, 1.5], [-1.0, 0.5], [0, 1]]),
+                                          (3, 4, 1 / 12, [[0, 1], [0, 2], [-0.5, 0.5]])])
+def test_points_device_match_host_points(d, L, h, domain, gpu):
+    """rs_points_device writes the bits of rs_leaf_points for every leaf."""
+    print("case: points device", d, L, h, flush=True)
+    import torch
+    t = rsa.GpuTree(d, L, domain, h, -5.0)
+    pts, face = t.all_points()
+    xyz = torch.zeros((t.n_leaves, t.m, 3), dtype=torch.float64, device="cuda:0")
+    fc = torch.zeros((t.n_leaves, t.m), dtype=torch.int32, device="cuda:0")
+    t.points_device(xyz.data_ptr(), fc.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(xyz.cpu().numpy(), pts)
+    assert np.array_equal(fc.cpu().numpy()[0], face)
+    assert np.array_equal(fc.cpu().numpy()[-1], face)
+
+
+def test_factor_device_matches_host_factor(gpu):
+    print("case: factor device", flush=True)
+    import torch
+    trees = [rsa.GpuTree(2, 4, [[0, 1], [0, 1], [0, 1]], 1.0 / 64, -10.0) for _ in range(2)]
+    pts, _ = trees[0].all_points()
+    x = pts[:, trees[0].nb:]
+    k = 100.0 * (1 + 0.3 * np.sin(3 * x[..., 0]) * np.cos(2 * x[..., 1])) + 1j
+    trees[0].factor(k)
+    trees[1].factor_device(torch.from_numpy(np.ascontiguousarray(k)).to("cuda:0").data_ptr())
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal((2, trees[0].n_leaves, trees[0].n_int)) + 0j
+    g = np.zeros((2, trees[0].n_leaves, trees[0].nb), complex)
+    assert np.array_equal(trees[0].solve(f, g), trees[1].solve(f, g))
