@@ -21,6 +21,7 @@
 // all that the solve streams per leaf.  The outgoing data needs no stencil
 // pass: the boundary row gives (alpha + D_n) u = g, so
 //   h = (sigma + D_n) u = g + (sigma - alpha) u_B.
+#include <algorithm>
 #include <type_traits>
 
 #include "rs_common.hpp"
@@ -425,6 +426,11 @@ __device__ inline const float2* ucolumn<float2>(double2* const*, const float2* u
 // together.  More columns per thread share the geometry but lower occupancy:
 // at 1024^2 with 256 right-hand sides 1, 2, 4, 8 measured 2.82, 2.86, 3.11, 3.65 ms.
 constexpr int RES_CPT = 1;
+// Column groups in the grid: each thread loops over its share of the columns
+// with the point's geometry and k loaded once (k was otherwise fetched again
+// for every column).  At 1024^2 with 256 right-hand sides: 256 groups (one
+// column per thread) 2.85 ms, 16 groups 2.44, 4 groups 2.45, 1 group 2.99.
+constexpr int RES_GROUPS = 16;
 
 template <typename U>
 __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, double2* const* __restrict__ u64,
@@ -439,14 +445,17 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)n_leaves * m) return;
     const int leaf = (int)(e / m), p = (int)(e - (size_t)leaf * m);
-    const int col0 = blockIdx.y * RES_CPT;
+    // the thread's columns: col0 = blockIdx.y * RES_CPT + k * gridDim.y * RES_CPT
+    const int cstep = gridDim.y * RES_CPT;
     const U* ucols[RES_CPT];
     bool cok[RES_CPT];
+    auto columns = [&](int col0) {
 #pragma unroll
-    for (int c = 0; c < RES_CPT; ++c) {
-        cok[c] = col0 + c < ncols;
-        ucols[c] = ucolumn<U>(u64, u32, cok[c] ? col0 + c : ncols - 1, n_leaves, m);
-    }
+        for (int c = 0; c < RES_CPT; ++c) {
+            cok[c] = col0 + c < ncols;
+            ucols[c] = ucolumn<U>(u64, u32, cok[c] ? col0 + c : ncols - 1, n_leaves, m);
+        }
+    };
     const size_t ul0 = (size_t)leaf * m;
     if (p < nb) {
         int fc, j_, q1, st;
@@ -457,6 +466,11 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
         int ff, jj, q1n, stn;
         boundary_info(G, b2, ff, jj, q1n, stn);
         const size_t un0 = (size_t)(nl2 >= 0 ? nl2 : leaf) * m;
+        const AxisCoef& A = C.ax[j_];
+        const AxisCoef& B = C.ax[jj];
+        const double2 sma = make_double2(0.0, -2.0 * G.kb);
+        for (int col0 = blockIdx.y * RES_CPT; col0 < ncols; col0 += cstep) {
+        columns(col0);
         U a0[RES_CPT], a1[RES_CPT], a2[RES_CPT], n0[RES_CPT], n1[RES_CPT], n2[RES_CPT];
         double2 gv[RES_CPT];
 #pragma unroll
@@ -470,9 +484,6 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
             n2[c] = gld(uc + (un0 + nb + q1n + stn));
             gv[c] = gld(g[cok[c] ? col0 + c : ncols - 1] + ((size_t)leaf * nb + p));
         }
-        const AxisCoef& A = C.ax[j_];
-        const AxisCoef& B = C.ax[jj];
-        const double2 sma = make_double2(0.0, -2.0 * G.kb);
 #pragma unroll
         for (int c = 0; c < RES_CPT; ++c) {
             if (!cok[c]) continue;
@@ -492,6 +503,7 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
                 rj[o] = ccast<float2>(csub(cadd(cmul(zeta, gA), cmul(nu, gN)), hA));
                 rg[o] = make_float2(0.f, 0.f);
             }
+        }
         }
         return;
     }
@@ -517,6 +529,8 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
         ip[jx] = (gq[jx] == nj) ? face_point(G, 2 * jx + 2, gq) : nb + q + stx;
     }
     const double2 kq = kint[(size_t)leaf * ni + q];
+    for (int col0 = blockIdx.y * RES_CPT; col0 < ncols; col0 += cstep) {
+    columns(col0);
     U uc0[RES_CPT], um[RES_CPT][3], upp[RES_CPT][3];
     double2 fv[RES_CPT];
 #pragma unroll
@@ -543,6 +557,7 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
             Au = cadd(Au, cscale(ccast<double2>(upp[c][jx]), wp[jx]));
         }
         rf[((size_t)(col0 + c) * n_leaves + leaf) * ni + q] = ccast<float2>(csub(fv[c], Au));
+    }
     }
 }
 
@@ -1065,7 +1080,8 @@ void launch_residual(const LeafGeom& G, double2* const* u64, const float2* u32, 
                      const double2* const* f, const double2* const* g, const int* nbr, double2 nu, double2 zeta,
                      float2* rf, float2* rg, float2* rj, int n_leaves, int ncols, hipStream_t st) {
     LeafCoef C = make_coef(G);
-    dim3 grid((unsigned)(((size_t)n_leaves * G.m + 255) / 256), (unsigned)((ncols + RES_CPT - 1) / RES_CPT));
+    const int groups = std::min((ncols + RES_CPT - 1) / RES_CPT, RES_GROUPS);
+    dim3 grid((unsigned)(((size_t)n_leaves * G.m + 255) / 256), (unsigned)groups);
     if (u64)
         hipLaunchKernelGGL(residual_kernel<double2>, grid, dim3(256), 0, st, G, C, u64, nullptr, kint, f, g, nbr, nu,
                            zeta, rf, rg, rj, n_leaves, ncols);
