@@ -635,6 +635,22 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     const bool colok = mycol < io.ncols;
     const int parity = leaf & 1;
     const size_t lc = (size_t)(colok ? mycol : 0) * nl + leaf;
+    // The faces' offsets for this leaf's parity, read from the kernel arguments
+    // once: a run-time index into them is a memory read, and such reads inside
+    // the layer sweeps each came with a wait on the LDS counter as well.
+    // Plain variables and arithmetic, not arrays or selection chains: a choice
+    // among several such values can be compiled into an indexed load from a
+    // table in scratch memory.
+    // fe_f: end of face f in local order; fn_f: its start in node order.
+    const int fe1 = G.foff[1] + G.pf[1], fe2 = G.foff[2] + G.pf[2], fe3 = G.foff[3] + G.pf[3];
+    const int fn1 = G.noff[parity][1], fn2 = G.noff[parity][2], fn3 = G.noff[parity][3],
+              fn4 = G.noff[parity][4];
+    const int fs1 = fn1 - G.foff[1], fs2 = fn2 - G.foff[2], fs3 = fn3 - G.foff[3], fs4 = fn4 - G.foff[4];
+    // local_to_node for the four faces of a 2D leaf (faces are consecutive in
+    // local order, so fe1 <= fe2 <= fe3)
+    auto l2n = [&](int b) {
+        return b + fs1 + (b >= fe1 ? fs2 - fs1 : 0) + (b >= fe2 ? fs3 - fs2 : 0) + (b >= fe3 ? fs4 - fs3 : 0);
+    };
     // Global loads and stores of interior data use a coalesced mapping: lane l
     // handles row rr = l % 16 of the wave's columns cc(i) = 4 i + l / 16, so 16
     // lanes cover 16 consecutive points of one column; tiles pass through the
@@ -687,7 +703,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
                 const int e = lane + 64 * j;
                 if (e < ntot) {
                     const int c = e / nb, i = e - c * nb;
-                    const int pos = (MODE == LEAF_DOWN) ? i : local_to_node(G, parity, i);
+                    const int pos = (MODE == LEAF_DOWN) ? i : l2n(i);
                     bd[c * BS + pos] = ((wcol0 + c) < io.ncols) ? t[j] : make_float2(0.f, 0.f);
                 }
             }
@@ -701,7 +717,14 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     const AxisCoef& ax1 = C.ax[1];
     const float2 cbx = ccast<float2>(cscale(cinv(ax0.dd), -ax0.cib));
     const float2 cby = ccast<float2>(cscale(cinv(ax1.dd), -ax1.cib));
-    auto gval = [&](int face, int t) -> float2 { return bd[ccol * BS + G.noff[parity][face] + t]; };
+    // incoming g at point t of a face whose node offset is fn
+    auto gval = [&](int fn, int t) -> float2 { return bd[ccol * BS + fn + t]; };
+    // slow-axis couplings (axis 1, K layers): lambda_k couples layer k to k-1,
+    // mu_k layer k to k+1; the end value applies at the last / first layer
+    const float2 cmid = ccast<float2>(make_double2(ax1.mid, 0.0));
+    const float2 cend = ccast<float2>(ax1.eoff);
+    auto lam = [&](int k) { return (k + 1 == K) ? cend : cmid; };
+    auto muk = [&](int k) { return (k == 0) ? cend : cmid; };
 
     float2 y[K][4];
     float2 prev[4];
@@ -756,12 +779,12 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
             const int row = 4 * grp + vv;
             float2 r = scr[row * LT + ccol];
             if (MODE == LEAF_DOWN && row < s && colok) {
-                if (row == 0) r = cfma(cbx, gval(1, k), r);
-                if (row == s - 1) r = cfma(cbx, gval(2, k), r);
-                if (k == 0) r = cfma(cby, gval(3, row), r);
-                if (k == K - 1) r = cfma(cby, gval(4, row), r);
+                if (row == 0) r = cfma(cbx, gval(fn1, k), r);
+                if (row == s - 1) r = cfma(cbx, gval(fn2, k), r);
+                if (k == 0) r = cfma(cby, gval(fn3, row), r);
+                if (k == K - 1) r = cfma(cby, gval(fn4, row), r);
             }
-            if (k > 0) r = csub(r, cmul(ccast<float2>(slow_lambda(G, C, k)), prev[vv]));
+            if (k > 0) r = csub(r, cmul(lam(k), prev[vv]));
             v[vv] = r;
         }
         f4v aR, aI;
@@ -773,7 +796,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
 #pragma unroll
     for (int k = K - 2; k >= 0; --k) {
         float2 w[4];
-        const float2 mu = ccast<float2>(slow_mu(G, C, k));
+        const float2 mu = muk(k);
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) w[vv] = cmul(mu, y[k + 1][vv]);
         f4v aR, aI;
@@ -831,27 +854,30 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     // the LDS area (node order: each slot is read and written by one lane only) and
     // stored afterwards with coalesced writes
     const float2 sma = make_float2(0.f, (float)(-2.0 * G.kb));
-    auto emit = [&](int face, int t, float2 x1, float2 x2, const AxisCoef& a) {
-        float2* slot = bd + ccol * BS + G.noff[parity][face] + t;
+    // boundary-row coefficients of axes 0 and 1 (x1, x2 weights and 1/dd)
+    const float a1x = (float)ax0.a1, a2x = (float)ax0.a2, a1y = (float)ax1.a1, a2y = (float)ax1.a2;
+    const float2 iddx = ccast<float2>(cinv(ax0.dd)), iddy = ccast<float2>(cinv(ax1.dd));
+    auto emit = [&](int fn, int t, float2 x1, float2 x2, float a1, float a2, float2 idd) {
+        float2* slot = bd + ccol * BS + fn + t;
         const float2 gb = (MODE == LEAF_DOWN) ? *slot : make_float2(0.f, 0.f);
-        float2 num = csub(csub(gb, cscale(x1, (float)a.a1)), cscale(x2, (float)a.a2));
-        float2 uB = cmul(num, ccast<float2>(cinv(a.dd)));
+        float2 num = csub(csub(gb, cscale(x1, a1)), cscale(x2, a2));
+        float2 uB = cmul(num, idd);
         if (MODE == LEAF_UP) *slot = csub(cfma(sma, uB, gb), *slot);
         else *slot = uB;
     };
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (!colok) break;
-        if (grp == 0) emit(1, k, y[k][0], y[k][1], ax0);
-        if (grp == (s - 1) / 4) emit(2, k, pick4(y[k][0], y[k][1], y[k][2], y[k][3], (s - 1) & 3),
-                                        pick4(y[k][0], y[k][1], y[k][2], y[k][3], (s - 2) & 3), ax0);
+        if (grp == 0) emit(fn1, k, y[k][0], y[k][1], a1x, a2x, iddx);
+        if (grp == (s - 1) / 4) emit(fn2, k, pick4(y[k][0], y[k][1], y[k][2], y[k][3], (s - 1) & 3),
+                                        pick4(y[k][0], y[k][1], y[k][2], y[k][3], (s - 2) & 3), a1x, a2x, iddx);
     }
 #pragma unroll
     for (int vv = 0; vv < 4; ++vv) {
         const int row = 4 * grp + vv;
         if (row >= s || !colok) continue;
-        emit(3, row, y[0][vv], y[1][vv], ax1);
-        emit(4, row, y[K - 1][vv], y[K - 2][vv], ax1);
+        emit(fn3, row, y[0][vv], y[1][vv], a1y, a2y, iddy);
+        emit(fn4, row, y[K - 1][vv], y[K - 2][vv], a1y, a2y, iddy);
     }
     wave_lds_sync();
     // coalesced store of the wave's boundary values: up, h in node order; down, u_B
@@ -878,7 +904,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
             if (MODE == LEAF_UP) {
                 io.hout[((size_t)col * nl + leaf) * nb + pidx[j]] = bd[cidx[j] * BS + pidx[j]];
             } else {
-                const float2 x = bd[cidx[j] * BS + local_to_node(G, parity, pidx[j])];
+                const float2 x = bd[cidx[j] * BS + l2n(pidx[j])];
                 put_col(col, (OUT != OUT_U32) ? io.u64[col] + (size_t)leaf * m : nullptr, pidx[j], x, base[j]);
             }
         }
@@ -975,11 +1001,15 @@ __global__ void __launch_bounds__(256) leaf3_mfma_kernel(LeafGeom G, LeafCoef C,
     // 2. forward sweep; y_k replaces r_k in X (each wave reads and writes only
     // its own rows of X, in its own order)
     auto xat = [&](int k, int R) -> float2& { return X[ccol * XS + k * s + R]; };
+    // slow-axis couplings (axis 2, K layers), read once: lambda_k couples layer k
+    // to k-1, mu_k layer k to k+1; the end value applies at the last / first layer
+    const float2 cmid = ccast<float2>(make_double2(C.ax[2].mid, 0.0));
+    const float2 cend = ccast<float2>(C.ax[2].eoff);
     afetch(0, abuf[0]);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (k + 1 < K) afetch(k + 1, abuf[(k + 1) & 1]);
-        const float2 lam = ccast<float2>(slow_lambda(G, C, k));
+        const float2 lam = (k + 1 == K) ? cend : cmid;
         if (active) {
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) {
@@ -1009,7 +1039,7 @@ __global__ void __launch_bounds__(256) leaf3_mfma_kernel(LeafGeom G, LeafCoef C,
 #pragma unroll
     for (int k = K - 2; k >= 0; --k) {
         if (k >= 1) afetch(k - 1, abuf[(k - 1) & 1]);
-        const float2 mu = ccast<float2>(slow_mu(G, C, k));
+        const float2 mu = (k == 0) ? cend : cmid;
         if (active) {
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) {
