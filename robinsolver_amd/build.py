@@ -19,7 +19,13 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
           "-Wno-unused-result"]
 # Packed FP32 instructions stay off (hipcc would otherwise form v_pk_*_f32 from float2 code).
-DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# Matrix-core accumulators live in ordinary VGPRs (amdgpu-mfma-vgpr-form): with the
+# accumulation registers as the default, every MFMA result was copied out with
+# v_accvgpr_read (12 per layer step of the leaf kernels), and the GEMM kernels held
+# both copies (fewer registers with the VGPR form: <double,double,64,64> 276 -> 180,
+# two waves per SIMD instead of one).  The MFMA operations themselves are unchanged.
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
 def _compile_cmd(src):

@@ -563,7 +563,7 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
 
 // ---------------------------------------------------------------------------
 // Multi-RHS leaf solve on the matrix cores (complex64, 2D leaves with an even
-// number s <= 16 of points per layer and K layers).  Each wave owns 16 RHS
+// number 8 <= s <= 16 of points per layer and K layers).  Each wave owns 16 RHS
 // columns; a workgroup covers 64 columns of one leaf.  The layer recursion
 //   forward   y_k = Shat_k^-1 (r_k - lambda_k y_{k-1})
 //   backward  x_k = y_k - mu_k Shat_k^-1 x_{k+1}
@@ -658,16 +658,23 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     const int wcol0 = chunk * 64 + wave * 16;
     const int rr = lane & 15, cq = lane >> 4;
     const int rrc = rr < s ? rr : s - 1;
+    // Lanes outside the leaf (rr >= s) or the batch load a valid point instead
+    // (clamped indices) and their values are not masked: the rows of the layer
+    // blocks beyond s are zero, so rows of the operand beyond s add nothing to
+    // the product, and columns never mix, so a column beyond the batch only
+    // computes values that are not stored.
     const FT* fcp[4];
     bool fok[4];
-    float fmask[4];   // 1 or 0: lanes outside the leaf or the batch stage zeros (a multiply, not a branch)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int col = wcol0 + 4 * i + cq;
         fok[i] = col < io.ncols && rr < s;
-        fmask[i] = fok[i] ? 1.f : 0.f;
         fcp[i] = FCol<FT>::base(io, col < io.ncols ? col : 0, leaf, ni, nl);
     }
+    // this lane's boundary point in the per-column loops over boundary data
+    // (nb = 2 s + 2 K <= 64 for this kernel, so one point per lane)
+    const bool bok = lane < nb;
+    const int bi = bok ? lane : nb - 1;
 
     // the leaf's K block inverses into LDS (zero-padded to 16 x 16)
     {
@@ -684,31 +691,24 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     // boundary data of the wave's 16 columns, read once with coalesced loads:
     // DOWN: incoming g (node order); UP: the transmission residual (local order)
     {
-        const int ntot = 16 * nb;
         if (MODE == LEAF_DOWN || io.jump) {
             // stored in node order (g already is; the residual is in local order)
             float2 t[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                int e = lane + 64 * j;
-                e = e < ntot ? e : ntot - 1;
-                const int c = e / nb, i = e - c * nb;
+            for (int c = 0; c < 16; ++c) {
                 const int col = (wcol0 + c) < io.ncols ? wcol0 + c : 0;
-                const size_t gi = ((size_t)col * nl + leaf) * nb + i;
-                if (MODE == LEAF_DOWN) t[j] = io.gin[gi];
-                else t[j] = ccast<float2>(io.jump[gi]);
+                const size_t gi = ((size_t)col * nl + leaf) * nb + bi;
+                if (MODE == LEAF_DOWN) t[c] = io.gin[gi];
+                else t[c] = ccast<float2>(io.jump[gi]);
             }
+            const int pos = (MODE == LEAF_DOWN) ? bi : l2n(bi);
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const int e = lane + 64 * j;
-                if (e < ntot) {
-                    const int c = e / nb, i = e - c * nb;
-                    const int pos = (MODE == LEAF_DOWN) ? i : l2n(i);
-                    bd[c * BS + pos] = ((wcol0 + c) < io.ncols) ? t[j] : make_float2(0.f, 0.f);
-                }
-            }
+            for (int c = 0; c < 16; ++c)
+                if (bok) bd[c * BS + pos] = ((wcol0 + c) < io.ncols) ? t[c] : make_float2(0.f, 0.f);
         } else {
-            for (int e = lane; e < ntot; e += 64) bd[(e / nb) * BS + e % nb] = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (bok) bd[c * BS + bi] = make_float2(0.f, 0.f);
         }
     }
     __syncthreads();
@@ -721,10 +721,11 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     auto gval = [&](int fn, int t) -> float2 { return bd[ccol * BS + fn + t]; };
     // slow-axis couplings (axis 1, K layers): lambda_k couples layer k to k-1,
     // mu_k layer k to k+1; the end value applies at the last / first layer
-    const float2 cmid = ccast<float2>(make_double2(ax1.mid, 0.0));
+    // (the inner layers' value is real: a scaling, not a complex product)
+    const float cmid = (float)ax1.mid;
     const float2 cend = ccast<float2>(ax1.eoff);
-    auto lam = [&](int k) { return (k + 1 == K) ? cend : cmid; };
-    auto muk = [&](int k) { return (k == 0) ? cend : cmid; };
+    auto lam_times = [&](int k, float2 x) { return (k + 1 == K) ? cmul(cend, x) : cscale(x, cmid); };
+    auto mu_times = [&](int k, float2 x) { return (k == 0) ? cmul(cend, x) : cscale(x, cmid); };
 
     float2 y[K][4];
     float2 prev[4];
@@ -747,8 +748,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
         aI = (p3 - p1) - p2;
     };
 
-    // raw[i] = f at row rr of layer k, column cc(i) (coalesced mapping; zero for
-    // lanes outside the leaf or the batch is applied when the tile is staged)
+    // raw[i] = f at row rr of layer k, column cc(i) (coalesced mapping)
     auto fetch = [&](int k, FT (&raw)[4]) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) raw[i] = gld(fcp[i] + (k * s + rrc));
@@ -767,8 +767,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
             float2* sb = scr;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float2 x = ccast<float2>(fbuf[k % PF][i]);
-                sb[rr * LT + 4 * i + cq] = make_float2(x.x * fmask[i], x.y * fmask[i]);
+                sb[rr * LT + 4 * i + cq] = ccast<float2>(fbuf[k % PF][i]);
             }
             wave_lds_sync();
         }
@@ -784,7 +783,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
                 if (k == 0) r = cfma(cby, gval(fn3, row), r);
                 if (k == K - 1) r = cfma(cby, gval(fn4, row), r);
             }
-            if (k > 0) r = csub(r, cmul(lam(k), prev[vv]));
+            if (k > 0) r = csub(r, lam_times(k, prev[vv]));
             v[vv] = r;
         }
         f4v aR, aI;
@@ -796,9 +795,8 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
 #pragma unroll
     for (int k = K - 2; k >= 0; --k) {
         float2 w[4];
-        const float2 mu = muk(k);
 #pragma unroll
-        for (int vv = 0; vv < 4; ++vv) w[vv] = cmul(mu, y[k + 1][vv]);
+        for (int vv = 0; vv < 4; ++vv) w[vv] = mu_times(k, y[k + 1][vv]);
         f4v aR, aI;
         product(k, w, aR, aI);
 #pragma unroll
@@ -865,12 +863,17 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
         if (MODE == LEAF_UP) *slot = csub(cfma(sma, uB, gb), *slot);
         else *slot = uB;
     };
+    // x faces: rows 0, 1 (face 1) sit in the lanes of group 0 and rows s-1, s-2
+    // (face 2) in those of group (s-1)/4, a different group for s >= 8, so one
+    // emit per layer serves both faces
+    const bool xf1 = grp == 0, xf2 = grp == (s - 1) / 4;
+    const int fnx = xf1 ? fn1 : fn2;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (!colok) break;
-        if (grp == 0) emit(fn1, k, y[k][0], y[k][1], a1x, a2x, iddx);
-        if (grp == (s - 1) / 4) emit(fn2, k, pick4(y[k][0], y[k][1], y[k][2], y[k][3], (s - 1) & 3),
-                                        pick4(y[k][0], y[k][1], y[k][2], y[k][3], (s - 2) & 3), a1x, a2x, iddx);
+        if (xf1 || xf2)
+            emit(fnx, k, xf1 ? y[k][0] : pick4(y[k][0], y[k][1], y[k][2], y[k][3], (s - 1) & 3),
+                 xf1 ? y[k][1] : pick4(y[k][0], y[k][1], y[k][2], y[k][3], (s - 2) & 3), a1x, a2x, iddx);
     }
 #pragma unroll
     for (int vv = 0; vv < 4; ++vv) {
@@ -882,30 +885,25 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     wave_lds_sync();
     // coalesced store of the wave's boundary values: up, h in node order; down, u_B
     // in local point order
-    const int ntot = 16 * nb;
-    for (int e0 = 0; e0 < ntot; e0 += 256) {
+    // (four columns at a time, their loads issued together)
+    const int bnode = l2n(bi);
+    for (int c0 = 0; c0 < 16; c0 += 4) {
         float2 base[4];
-        int cidx[4], pidx[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            int e = e0 + lane + 64 * j;
-            e = e < ntot ? e : ntot - 1;
-            cidx[j] = e / nb;
-            pidx[j] = e - cidx[j] * nb;
-            const int col = (wcol0 + cidx[j]) < io.ncols ? wcol0 + cidx[j] : 0;
+            const int col = (wcol0 + c0 + j) < io.ncols ? wcol0 + c0 + j : 0;
             base[j] = (MODE == LEAF_DOWN && OUT == OUT_U64_BASE32)
-                          ? io.ubase32[((size_t)col * nl + leaf) * m + pidx[j]] : make_float2(0.f, 0.f);
+                          ? io.ubase32[((size_t)col * nl + leaf) * m + bi] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int e = e0 + lane + 64 * j;
-            const int col = wcol0 + cidx[j];
-            if (e >= ntot || col >= io.ncols) continue;
+            const int c = c0 + j, col = wcol0 + c;
+            if (!bok || col >= io.ncols) continue;
             if (MODE == LEAF_UP) {
-                io.hout[((size_t)col * nl + leaf) * nb + pidx[j]] = bd[cidx[j] * BS + pidx[j]];
+                io.hout[((size_t)col * nl + leaf) * nb + bi] = bd[c * BS + bi];
             } else {
-                const float2 x = bd[cidx[j] * BS + l2n(pidx[j])];
-                put_col(col, (OUT != OUT_U32) ? io.u64[col] + (size_t)leaf * m : nullptr, pidx[j], x, base[j]);
+                const float2 x = bd[c * BS + bnode];
+                put_col(col, (OUT != OUT_U32) ? io.u64[col] + (size_t)leaf * m : nullptr, bi, x, base[j]);
             }
         }
     }
@@ -1198,7 +1196,8 @@ static bool mfma_leaf3_ok(const LeafGeom& G, const LeafArgs<float>& a) {
     return G.d == 3 && form && G.s <= 64 && (G.K == 4 || G.K == 6 || G.K == 8);
 }
 
-// The matrix-core path takes 2D leaves of 12x12, 14x14 or 16x16 interior points.
+// The matrix-core path takes 2D leaves with 8 to 16 (even) points per layer and
+// 12, 14 or 16 layers.
 static bool mfma_leaf_ok(const LeafGeom& G, const LeafArgs<float>& a) {
     // forms with a matrix-core kernel: up with f in either precision; down with the
     // caller's f into u (complex64 or complex128), or with complex64 f as a correction
@@ -1206,7 +1205,7 @@ static bool mfma_leaf_ok(const LeafGeom& G, const LeafArgs<float>& a) {
                     : a.mode == LEAF_DOWN ? ((a.f64 && !a.ubase32 && !a.uacc) ||
                                              (a.f32 && a.u64 && (a.ubase32 || a.uacc)))
                     : false;
-    return G.d == 2 && form && G.s == G.n[0] && G.s <= 16 && (G.s % 2) == 0 &&
+    return G.d == 2 && form && G.s == G.n[0] && G.s >= 8 && G.s <= 16 && (G.s % 2) == 0 &&
            (G.K == 12 || G.K == 14 || G.K == 16);
 }
 
