@@ -562,8 +562,8 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
 }
 
 // ---------------------------------------------------------------------------
-// Multi-RHS leaf solve on the matrix cores (complex64, 2D leaves with an even
-// number 8 <= s <= 16 of points per layer and K layers).  Each wave owns 16 RHS
+// Multi-RHS leaf solve on the matrix cores (complex64, square 2D leaves of
+// s = K = 12, 14 or 16 points per axis: K layers of s points).  Each wave owns 16 RHS
 // columns; a workgroup covers 64 columns of one leaf.  The layer recursion
 //   forward   y_k = Shat_k^-1 (r_k - lambda_k y_{k-1})
 //   backward  x_k = y_k - mu_k Shat_k^-1 x_{k+1}
@@ -617,7 +617,9 @@ enum { OUT_U32 = 0, OUT_U64 = 1, OUT_U64_BASE32 = 2, OUT_U64_ACC = 3 };
 template <int K, int MODE, typename FT, int OUT>
 __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, LeafIO<float> io, int nchunks) {
     extern __shared__ float2 lds2[];
-    const int s = G.s, ni = G.n_int, nb = G.nb, m = G.m, nl = io.n_leaves;
+    // square leaves: K layers of s = K points (the launcher checks the geometry)
+    constexpr int s = K, ni = K * K, nb = 4 * K, m = ni + nb;
+    const int nl = io.n_leaves;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // LDS: the K blocks [K][16 x 16] column-major; per wave one 16 x 16 tile,
     // row-major with row stride LT (= 17: rows fall in different banks), and the
@@ -625,7 +627,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     // reused from one step to the next without a second copy: a wave's LDS
     // accesses are processed in order.
     constexpr int LT = 17, LTS = 16 * LT;
-    const int BS = nb + 1;
+    constexpr int BS = nb + 1;
     float2* Bl = lds2;
     float2* scr = lds2 + K * 256 + wave * LTS;
     float2* bd = lds2 + K * 256 + 4 * LTS + wave * 16 * BS;
@@ -651,6 +653,11 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     auto l2n = [&](int b) {
         return b + fs1 + (b >= fe1 ? fs2 - fs1 : 0) + (b >= fe2 ? fs3 - fs2 : 0) + (b >= fe3 ? fs4 - fs3 : 0);
     };
+    // x faces: rows 0, 1 (face 1) sit in the lanes of group 0 and rows s-1, s-2
+    // (face 2) in those of group (s-1)/4, a different group (s >= 8): a lane
+    // handles at most one x face
+    const bool xf1 = grp == 0, xf2 = grp == (s - 1) / 4;
+    const int fnx = xf1 ? fn1 : fn2;
     // Global loads and stores of interior data use a coalesced mapping: lane l
     // handles row rr = l % 16 of the wave's columns cc(i) = 4 i + l / 16, so 16
     // lanes cover 16 consecutive points of one column; tiles pass through the
@@ -772,16 +779,23 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
             wave_lds_sync();
         }
         if (k + PF < K) fetch(k + PF, fbuf[k % PF]);
+        // incoming data of the boundary rows (down pass), added by selects:
+        // the x face of the lane's group at row 0 / s-1 of every layer (one LDS
+        // read), the y faces at every row of the first and last layer.  Columns
+        // beyond the batch hold zeros here and are not stored anyway.
+        const float2 gx = (MODE == LEAF_DOWN) ? gval(fnx, k) : make_float2(0.f, 0.f);
         float2 v[4];
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) {
             const int row = 4 * grp + vv;
             float2 r = scr[row * LT + ccol];
-            if (MODE == LEAF_DOWN && row < s && colok) {
-                if (row == 0) r = cfma(cbx, gval(fn1, k), r);
-                if (row == s - 1) r = cfma(cbx, gval(fn2, k), r);
-                if (k == 0) r = cfma(cby, gval(fn3, row), r);
-                if (k == K - 1) r = cfma(cby, gval(fn4, row), r);
+            if (MODE == LEAF_DOWN) {
+                if (vv == 0) r = xf1 ? cfma(cbx, gx, r) : r;
+                if (vv == ((s - 1) & 3)) r = xf2 ? cfma(cbx, gx, r) : r;
+                if (k == 0 || k == K - 1) {
+                    const float2 gy = gval(k == 0 ? fn3 : fn4, row < s ? row : s - 1);
+                    r = row < s ? cfma(cby, gy, r) : r;
+                }
             }
             if (k > 0) r = csub(r, lam_times(k, prev[vv]));
             v[vv] = r;
@@ -863,11 +877,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
         if (MODE == LEAF_UP) *slot = csub(cfma(sma, uB, gb), *slot);
         else *slot = uB;
     };
-    // x faces: rows 0, 1 (face 1) sit in the lanes of group 0 and rows s-1, s-2
-    // (face 2) in those of group (s-1)/4, a different group for s >= 8, so one
-    // emit per layer serves both faces
-    const bool xf1 = grp == 0, xf2 = grp == (s - 1) / 4;
-    const int fnx = xf1 ? fn1 : fn2;
+    // one emit per layer serves both x faces (a lane handles at most one)
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (!colok) break;
@@ -1196,16 +1206,16 @@ static bool mfma_leaf3_ok(const LeafGeom& G, const LeafArgs<float>& a) {
     return G.d == 3 && form && G.s <= 64 && (G.K == 4 || G.K == 6 || G.K == 8);
 }
 
-// The matrix-core path takes 2D leaves with 8 to 16 (even) points per layer and
-// 12, 14 or 16 layers.
+// The matrix-core path takes square 2D leaves of 12x12, 14x14 or 16x16 interior points.
 static bool mfma_leaf_ok(const LeafGeom& G, const LeafArgs<float>& a) {
+    // square leaves of 12, 14 or 16 points per axis (the kernel's layer size s = K)
     // forms with a matrix-core kernel: up with f in either precision; down with the
     // caller's f into u (complex64 or complex128), or with complex64 f as a correction
     const bool form = a.mode == LEAF_UP ? (a.f64 || a.f32)
                     : a.mode == LEAF_DOWN ? ((a.f64 && !a.ubase32 && !a.uacc) ||
                                              (a.f32 && a.u64 && (a.ubase32 || a.uacc)))
                     : false;
-    return G.d == 2 && form && G.s == G.n[0] && G.s >= 8 && G.s <= 16 && (G.s % 2) == 0 &&
+    return G.d == 2 && form && G.s == G.n[0] && G.s == G.K && G.nb == 4 * G.K && G.n_int == G.K * G.K &&
            (G.K == 12 || G.K == 14 || G.K == 16);
 }
 

@@ -32,6 +32,8 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("d,n,L,kb,n_refine", [(2, 64, 4, 10.0, 2), (2, 128, 6, 30.0, 2), (3, 16, 6, 4.0, 2),
                                                  (2, 64, 4, 10.0, 0),
+                                                 # 2D leaves of 8 x 16 points (the generic leaf kernel)
+                                                 (2, 64, 5, 10.0, 2),
                                                  # 3D leaves of 8^3 (64-point layers) and 6^3 (36, padded)
                                                  (3, 16, 3, 4.0, 2), (3, 12, 3, 4.0, 2), (3, 16, 3, 4.0, 0)])
 def test_mixed_matches_oracle(d, n, L, kb, n_refine, gpu):
