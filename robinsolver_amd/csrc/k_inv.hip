@@ -26,11 +26,7 @@ namespace {
 
 // block-wide argmax of |re|+|im|; ties go to the smallest index
 __device__ void block_argmax(double v, int i, double* rv, int* ri, double& best, int& bi) {
-    for (int off = 32; off > 0; off >>= 1) {
-        double ov = __shfl_xor(v, off);
-        int oi = __shfl_xor(i, off);
-        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
-    }
+    wave_argmax(v, i);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     if (lane == 0) { rv[wave] = v; ri[wave] = i; }
     __syncthreads();
@@ -145,12 +141,7 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(double2* __restrict__ mat
     if (tid == 0 && bad) atomicOr(info, 2);
 }
 
-// Panel elimination with the panel held in registers: thread t owns rows
-// t, t + nt, ... (RPT of them) of the n x w panel (w <= NB columns).  One
-// Gauss-Jordan step is a wave-shuffle argmax over the candidate rows, one
-// barrier to combine the waves' results, the pivot row (scaled) and the old
-// row j published through LDS, a second barrier, and the rank-1 update of each
-// thread's own rows.  Same pivot choice as gj_panel_kernel.
+// the RPT panel rows of one thread (register panel kernel below)
 template <int NB, int RPT>
 struct PanelRegs {
     double2 a[RPT][NB];
@@ -173,11 +164,7 @@ __device__ __forceinline__ void gj_reg_step(PanelRegs<NB, RPT>& P, int n, int j0
             if (x > v) { v = x; vi = r; }
         }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(v, off);
-        const int oi = __shfl_xor(vi, off);
-        if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
-    }
+    wave_argmax(v, vi);
     if (lane == 0) { rv[wave] = v; ri[wave] = vi; }
     __syncthreads();
     double best = rv[0];
@@ -229,7 +216,7 @@ __device__ __forceinline__ void gj_reg_steps(std::integer_sequence<int, JJ...>, 
 
 // Panel elimination with the panel held in registers: thread t owns rows
 // t, t + nt, ... (RPT of them) of the n x w panel (w <= NB columns).  One
-// Gauss-Jordan step is a wave-shuffle argmax over the candidate rows, one
+// Gauss-Jordan step is a wave argmax over the candidate rows (DPP), one
 // barrier to combine the waves' results, the pivot row (scaled) and the old
 // row j published through LDS, a second barrier, and the rank-1 update of each
 // thread's own rows.  Same pivot choice as gj_panel_kernel.

@@ -158,11 +158,7 @@ __device__ void gj_invert_lds(double2* M, int s, int* piv, double2* colj, int* b
                 double v = cabs1(M[i + j * s]);
                 if (v > best) { best = v; bi = i; }
             }
-            for (int off = 32; off > 0; off >>= 1) {
-                double ob = __shfl_xor(best, off);
-                int oi = __shfl_xor(bi, off);
-                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-            }
+            wave_argmax(best, bi);
             if (tid == 0) {
                 piv[j] = bi;
                 if (!(best > 0.0)) *bad = 1;

@@ -56,6 +56,46 @@ template <typename V> __device__ inline void gst(V* p, V v) {
 }
 __device__ inline int gldi(const int* p) { return *(gptr<const int>)p; }
 
+// Wave-wide reductions through DPP lane permutations (quad swaps, then the row
+// half-mirror and mirror: every lane of a 16-lane row holds the row's result),
+// the four rows combined through readlane.  Unlike a shuffle (ds_bpermute) no
+// step goes through the LDS unit.
+template <int CTRL> __device__ inline int dpp_i(int x) { return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xf, 0xf, false); }
+template <int CTRL> __device__ inline double dpp_d(double x) {
+    const long long b = __builtin_bit_cast(long long, x);
+    const int lo = dpp_i<CTRL>((int)(b & 0xffffffffLL)), hi = dpp_i<CTRL>((int)(b >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double readlane_d(double x, int l) {
+    const long long b = __builtin_bit_cast(long long, x);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+// maximum over the wave (NaN values are passed over by fmax)
+__device__ inline double wave_max(double v) {
+    v = fmax(v, dpp_d<0xB1>(v));    // quad_perm [1,0,3,2]
+    v = fmax(v, dpp_d<0x4E>(v));    // quad_perm [2,3,0,1]
+    v = fmax(v, dpp_d<0x141>(v));   // row_half_mirror
+    v = fmax(v, dpp_d<0x140>(v));   // row_mirror
+    return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)), fmax(readlane_d(v, 32), readlane_d(v, 48)));
+}
+__device__ inline int wave_min(int v) {
+    v = min(v, dpp_i<0xB1>(v));
+    v = min(v, dpp_i<0x4E>(v));
+    v = min(v, dpp_i<0x141>(v));
+    v = min(v, dpp_i<0x140>(v));
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+// argmax over the wave: the largest v, and the smallest index i among the lanes
+// holding it; every lane gets both
+__device__ inline void wave_argmax(double& v, int& i) {
+    const double m = wave_max(v);
+    i = wave_min(v == m ? i : 0x7fffffff);
+    v = m;
+}
+
 template <typename Dst, typename Src> __host__ __device__ inline Dst ccast(Src a) {
     Dst z; z.x = (decltype(z.x))a.x; z.y = (decltype(z.y))a.y; return z;
 }
