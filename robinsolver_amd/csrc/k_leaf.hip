@@ -421,12 +421,21 @@ __device__ inline const float2* ucolumn<float2>(double2* const*, const float2* u
 // One thread per (leaf, point, RES_CPT columns); the loads of a thread are issued
 // together.  More columns per thread share the geometry but lower occupancy:
 // at 1024^2 with 256 right-hand sides 1, 2, 4, 8 measured 2.82, 2.86, 3.11, 3.65 ms.
-constexpr int RES_CPT = 1;
+#ifndef RS_RES_CPT
+#define RS_RES_CPT 1
+#endif
+constexpr int RES_CPT = RS_RES_CPT;
 // Column groups in the grid: each thread loops over its share of the columns
 // with the point's geometry and k loaded once (k was otherwise fetched again
 // for every column).  At 1024^2 with 256 right-hand sides: 256 groups (one
-// column per thread) 2.85 ms, 16 groups 2.44, 4 groups 2.45, 1 group 2.99.
-constexpr int RES_GROUPS = 16;
+// column per thread) 2.85 ms, 32 groups 2.39, 16 groups 2.44, 8 groups 2.48,
+// 4 groups 2.45, 1 group 2.99; two or four columns per loop step (8 or 16
+// groups) 2.63-3.14.  (Both constants can be set when compiling, for such
+// comparisons.)
+#ifndef RS_RES_GROUPS
+#define RS_RES_GROUPS 32
+#endif
+constexpr int RES_GROUPS = RS_RES_GROUPS;
 
 template <typename U>
 __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, double2* const* __restrict__ u64,
