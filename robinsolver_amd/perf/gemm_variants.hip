@@ -4,8 +4,10 @@
 //   (no argument)  the 2D 1024^2 tree's shapes, N = 256 right-hand sides
 //   3d             the 3D 64^3 tree's top-level shapes, N = 16
 //
-//   hipcc -x hip --offload-arch=gfx950 -Xclang -target-feature -Xclang -packed-fp32-ops -O3 -std=c++17 \
+//   hipcc -x hip --offload-arch=gfx950 -Xclang -target-feature -Xclang -packed-fp32-ops \
+//         -mllvm -amdgpu-mfma-vgpr-form -O3 -std=c++17 \
 //         robinsolver_amd/perf/gemm_variants.hip -o gemm_variants && ./gemm_variants [3d]
+// (the library's own flags, robinsolver_amd/build.py)
 #include "../csrc/k_gemm.hip"
 
 #include <cstdio>
