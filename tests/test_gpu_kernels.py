@@ -156,6 +156,25 @@ def test_inverse_needs_pivoting(n, gpu):
     assert np.linalg.norm(inv @ M - np.eye(n)) / np.sqrt(n) < 1e-10
 
 
+@pytest.mark.parametrize("n", [64, 256, 1024])
+def test_inverse_with_tied_pivots(n, gpu):
+    """A Sylvester-Hadamard matrix times (1 + i): every candidate of the first
+    column has the same |re|+|im|, and ties recur in later columns, within a wave
+    and across the waves of the pivot search (the smallest row index wins, as
+    zgetrf's idamax-style choice)."""
+    print("case: inverse tied pivots", n, flush=True)
+    H = np.array([[1.0]])
+    while H.shape[0] < n:
+        H = np.block([[H, H], [H, -H]])
+    M = (1.0 + 1.0j) * H
+    flat = M.reshape(-1, order="F").copy()
+    assert _lib().rs_check_inverse(n, 1, _dp(flat.view(np.float64))) == 0
+    inv = flat.reshape(n, n, order="F")
+    assert np.linalg.norm(inv @ M - np.eye(n)) / np.sqrt(n) < 1e-12
+    # the exact inverse is H^T / (n (1 + i))
+    assert np.abs(inv - H.T / (n * (1.0 + 1.0j))).max() < 1e-12
+
+
 @pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED, rsa.RS_MIXED_MERGE])
 @pytest.mark.parametrize("M,N,K,scatter", [(48, 256, 16, True), (100, 70, 777, False), (200, 33, 1000, False),
                                            (16, 64, 32, True)])
