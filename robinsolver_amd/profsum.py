@@ -5,6 +5,12 @@
   python -m robinsolver_amd.profsum traffic <fetch_dir> <write_dir> <kernel> <category> "<command>"
       HBM bytes per launch of one kernel from a FETCH_SIZE run and a WRITE_SIZE
       run (rocprofv3 --pmc ... --output-format csv), as JSON
+  python -m robinsolver_amd.profsum mfma <pmc_dir> <after_kernel> <until_kernel|-> "<title>" "<command>"
+      matrix-core utilisation per kernel, as markdown, from a run with
+      --pmc MfmaUtil SQ_INSTS_VALU_MFMA_MOPS_F64 SQ_INSTS_VALU_MFMA_MOPS_F32
+      over a window of dispatches (from the last <after_kernel> to the next
+      <until_kernel>): MfmaUtil weighted by dispatch duration, and the MFMA
+      flops (MOPS * 512, rocprofv3's MfmaFlops)
 
 FETCH_SIZE and WRITE_SIZE are reported in KiB per dispatch.  Each is scaled by
 what it reads for a plain device copy of known size in the same process (the
@@ -77,6 +83,56 @@ def traffic(fetch_dir, write_dir, kernel, category, command, copy_bytes=1 << 28,
     return res
 
 
+def mfma_md(pmc_dir, after_kernel, until_kernel, title, command):
+    """Per kernel: rocprofv3's MfmaUtil (per dispatch, weighted by the dispatch
+    durations) and the MFMA flops of the MOPS counters, over the dispatches from
+    the last <after_kernel> up to the first <until_kernel> after it ("-": to the end)."""
+    rows = _rows(os.path.join(pmc_dir, "run_counter_collection.csv"))
+    per = defaultdict(dict)
+    names, dur = {}, {}
+    for r in rows:
+        d = int(r["Dispatch_Id"])
+        per[d][r["Counter_Name"]] = float(r["Counter_Value"])
+        names[d] = short_name(r["Kernel_Name"])
+        dur[d] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
+    ids = sorted(per)
+    first = max((d for d in ids if after_kernel in names[d]), default=ids[0])
+    last = ids[-1]
+    if until_kernel != "-":
+        last = min((d for d in ids if d > first and until_kernel in names[d]), default=ids[-1])
+    agg = defaultdict(lambda: defaultdict(float))
+    for d in ids:
+        if d < first or d > last:
+            continue
+        a = agg[names[d]]
+        c = per[d]
+        a["n"] += 1
+        a["t"] += dur[d]
+        a["ut"] += c.get("MfmaUtil", 0.0) * dur[d]
+        a["f64"] += c.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0) * 512
+        a["f32"] += c.get("SQ_INSTS_VALU_MFMA_MOPS_F32", 0.0) * 512
+    out = ["# %s" % title, "", "Command (MI355X):", "", "```", command, "```", "",
+           "Per kernel, over the dispatches from the last `%s`%s: rocprofv3's `MfmaUtil` (matrix-core busy"
+           " cycles over GPU busy cycles times the SIMD count) averaged with the dispatch durations as"
+           " weights, and the MFMA floating-point operations of the `SQ_INSTS_VALU_MFMA_MOPS_*` counters"
+           " (x 512; complex products run as Gauss's three real products, 6 real flops per complex"
+           " multiply-add). Durations are those of the counting run." % (
+               after_kernel, "" if until_kernel == "-" else " to the next `%s`" % until_kernel), "",
+           "| kernel | dispatches | total us | MfmaUtil % | MFMA TFLOP/s | FP64 MFMA GFLOP | FP32 MFMA GFLOP |",
+           "|---|---:|---:|---:|---:|---:|---:|"]
+    tot = defaultdict(float)
+    for k, a in sorted(agg.items(), key=lambda x: -x[1]["t"])[:20]:
+        out.append("| %s | %d | %.1f | %.1f | %.1f | %.1f | %.1f |" % (
+            k.replace("|", "/"), a["n"], a["t"] * 1e6, a["ut"] / a["t"] if a["t"] else 0.0,
+            (a["f64"] + a["f32"]) / a["t"] / 1e12 if a["t"] else 0.0, a["f64"] / 1e9, a["f32"] / 1e9))
+    for a in agg.values():
+        for k, v in a.items():
+            tot[k] += v
+    out += ["", "All of them: %.1f us, MfmaUtil %.1f%%, FP64 MFMA %.1f GFLOP, FP32 MFMA %.1f GFLOP." % (
+        tot["t"] * 1e6, tot["ut"] / tot["t"] if tot["t"] else 0.0, tot["f64"] / 1e9, tot["f32"] / 1e9)]
+    return "\n".join(out) + "\n"
+
+
 def main(argv):
     if argv[1] == "stats":
         sys.stdout.write(kernel_stats_md(argv[2], argv[3], argv[4]))
@@ -87,6 +143,8 @@ def main(argv):
         if "--last" in argv:
             last = int(argv[argv.index("--last") + 1])
         print(json.dumps(traffic(argv[2], argv[3], argv[4], argv[5], argv[6], select=sel, last=last), indent=1))
+    elif argv[1] == "mfma":
+        sys.stdout.write(mfma_md(argv[2], argv[3], argv[4], argv[5], argv[6]))
     else:
         raise SystemExit(__doc__)
 
