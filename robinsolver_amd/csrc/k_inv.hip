@@ -336,7 +336,10 @@ void launch_inverse(const InvPlan& ip, double2* mats, int* pivs, double2* W, con
                                pivs, info);
         hipLaunchKernelGGL(gj_swap_extract_kernel, dim3((n - nb + 255) / 256, ip.batch), dim3(256), 0, st, mats, n,
                            ip.stride, j0, nb, pivs, W, wstride);
-        launch_gemm<double>(d_descs + (size_t)pi * ip.batch, ip.batch, n, n - nb, nb, st);
+        // the rank-nb update: one 16-deep chunk per tile, so latency-bound; 32x32
+        // tiles (four times the workgroups of 64x64) measured fastest
+        // (factor at 1024^2: 64x64 38.4 ms, 32x64 38.0, 64x32 38.0, 16x64 38.0, 32x32 37.8)
+        launch_gemm<double>(d_descs + (size_t)pi * ip.batch, ip.batch, n, n - nb, nb, st, nullptr, GEMM_TILE_32x32);
     }
     hipLaunchKernelGGL(gj_unpermute_kernel, dim3((n + 255) / 256, ip.batch), dim3(256), 0, st, mats, n, ip.stride,
                        pivs);
