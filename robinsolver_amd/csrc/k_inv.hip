@@ -141,112 +141,122 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(double2* __restrict__ mat
     if (tid == 0 && bad) atomicOr(info, 2);
 }
 
-// the RPT panel rows of one thread (register panel kernel below)
-template <int NB, int RPT>
-struct PanelRegs {
-    double2 a[RPT][NB];
-};
-
-// Gauss-Jordan step JJ (a compile-time column, so the panel stays in registers)
-template <int JJ, int NB, int RPT>
-__device__ __forceinline__ void gj_reg_step(PanelRegs<NB, RPT>& P, int n, int j0, int w, int* piv, double* rv,
-                                            int* ri, double2* prow, double2* jrow, bool& bad) {
-    if (JJ >= w) return;   // w is uniform: every thread skips the same steps
-    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
-    const int j = j0 + JJ;
-    double v = -1.0;
-    int vi = n;
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const int r = tid + k * nt;
-        if (r >= j && r < n) {
-            const double x = cabs1(P.a[k][JJ]);
-            if (x > v) { v = x; vi = r; }
-        }
-    }
-    wave_argmax(v, vi);
-    if (lane == 0) { rv[wave] = v; ri[wave] = vi; }
-    __syncthreads();
-    double best = rv[0];
-    int p = ri[0];
-    for (int q = 1; q < nw; ++q)
-        if (rv[q] > best || (rv[q] == best && ri[q] < p)) { best = rv[q]; p = ri[q]; }
-    if (!(best > 0.0) || p >= n) { bad = true; p = j; }   // singular: no interchange, reported through info
-    // publish old row j and the scaled pivot row
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const int r = tid + k * nt;
-        if (r == j)
-#pragma unroll
-            for (int c = 0; c < NB; ++c) jrow[c] = P.a[k][c];
-        if (r == p) {
-            const double2 inv = cinv(P.a[k][JJ]);
-#pragma unroll
-            for (int c = 0; c < NB; ++c) prow[c] = (c == JJ) ? inv : cmul(P.a[k][c], inv);
-        }
-    }
-    if (tid == 0) piv[j] = p;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const int r = tid + k * nt;
-        if (r == j) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) P.a[k][c] = prow[c];
-            continue;
-        }
-        if (r == p)
-#pragma unroll
-            for (int c = 0; c < NB; ++c) P.a[k][c] = jrow[c];
-        const double2 m = P.a[k][JJ];
-#pragma unroll
-        for (int c = 0; c < NB; ++c) {
-            const double2 pc = prow[c];
-            P.a[k][c] = csub((c == JJ) ? make_double2(0.0, 0.0) : P.a[k][c], cmul(m, pc));
-        }
-    }
-}
-
-template <int NB, int RPT, int... JJ>
-__device__ __forceinline__ void gj_reg_steps(std::integer_sequence<int, JJ...>, PanelRegs<NB, RPT>& P, int n,
-                                             int j0, int w, int* piv, double* rv, int* ri, double2* prow,
-                                             double2* jrow, bool& bad) {
-    (gj_reg_step<JJ, NB, RPT>(P, n, j0, w, piv, rv, ri, prow, jrow, bad), ...);
-}
-
 // Panel elimination with the panel held in registers: thread t owns rows
-// t, t + nt, ... (RPT of them) of the n x w panel (w <= NB columns).  One
-// Gauss-Jordan step is a wave argmax over the candidate rows (DPP), one
-// barrier to combine the waves' results, the pivot row (scaled) and the old
-// row j published through LDS, a second barrier, and the rank-1 update of each
-// thread's own rows.  Same pivot choice as gj_panel_kernel.
+// t, t + nt, ... (RPT of them) of the n x w panel (w <= NB columns).  Same
+// pivot choice as gj_panel_kernel.  One Gauss-Jordan step has one workgroup
+// barrier: each wave finds its best row by a DPP argmax, and the lane holding
+// that row publishes it (with the old row j, from its owner); after the
+// barrier every thread picks the winning wave's row, inverts its pivot and
+// applies the rank-1 update to its own rows.  The published rows are
+// double-buffered by step parity: the writes of step s + 2 come after the
+// barrier of step s + 1, which every reader of step s has passed.
+//
+// The step loop stays rolled.  The kernel runs once per panel, so a fully
+// unrolled body is fetched from L2 instruction by instruction: unrolled over
+// the 16 steps (10k-17k instructions) it ran at about 3.5 ns per instruction.
+// To keep every register index static in a rolled loop, the columns rotate
+// left by one each step: the current column is always a[.][0], and the column
+// the step produces enters at a[.][NB-1]; after w steps original column c sits
+// at position (c - w) mod NB.
 template <int NB, int RPT, int LB = 1024>
 __global__ void __launch_bounds__(LB) gj_panel_reg_kernel(double2* __restrict__ mats, int n, size_t stride,
                                                             int j0, int w, int* __restrict__ pivs,
                                                             int* __restrict__ info) {
-    __shared__ double rv[16];
-    __shared__ int ri[16];
-    __shared__ double2 prow[NB];   // new row j: the pivot row, scaled
-    __shared__ double2 jrow[NB];   // old row j (becomes row p)
-    const int tid = threadIdx.x, nt = blockDim.x;
+    __shared__ double rv[2][16];          // each wave's best |re|+|im| ...
+    __shared__ int ri[2][16];             // ... and its row
+    __shared__ double2 cand[2][16][NB];   // each wave's best row (rotated like the registers)
+    __shared__ double2 jrow[2][NB];       // old row j (becomes row p)
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
     double2* G = mats + (size_t)blockIdx.x * stride + (size_t)j0 * n;
     int* piv = pivs + (size_t)blockIdx.x * n;
-    PanelRegs<NB, RPT> P;
+    double2 a[RPT][NB];
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const int r = tid + k * nt;
 #pragma unroll
-        for (int c = 0; c < NB; ++c) P.a[k][c] = (r < n && c < w) ? G[r + (size_t)c * n] : make_double2(0.0, 0.0);
+        for (int c = 0; c < NB; ++c) a[k][c] = (r < n && c < w) ? G[r + (size_t)c * n] : make_double2(0.0, 0.0);
     }
     bool bad = false;
-    gj_reg_steps<NB, RPT>(std::make_integer_sequence<int, NB>{}, P, n, j0, w, piv, rv, ri, prow, jrow, bad);
+#pragma unroll 1
+    for (int s = 0; s < w; ++s) {
+        const int j = j0 + s, b = s & 1;
+        double v = -1.0;
+        int vi = n;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int r = tid + k * nt;
+            if (r >= j && r < n) {
+                const double x = cabs1(a[k][0]);
+                if (x > v) { v = x; vi = r; }
+            }
+        }
+        wave_argmax(v, vi);
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int r = tid + k * nt;
+            if (r == vi)
+#pragma unroll
+                for (int c = 0; c < NB; ++c) cand[b][wave][c] = a[k][c];
+            if (r == j)
+#pragma unroll
+                for (int c = 0; c < NB; ++c) jrow[b][c] = a[k][c];
+        }
+        if (lane == 0) { rv[b][wave] = v; ri[b][wave] = vi; }
+        __syncthreads();
+        double best = rv[b][0];
+        int p = ri[b][0], qw = 0;
+        for (int q = 1; q < nw; ++q) {
+            const double x = rv[b][q];
+            const int y = ri[b][q];
+            if (x > best || (x == best && y < p)) { best = x; p = y; qw = q; }
+        }
+        if (!(best > 0.0) || p >= n) { bad = true; p = j; }   // singular: no interchange, reported through info
+        if (tid == 0) piv[j] = p;
+        const double2* prow = cand[b][qw];   // the pivot row
+        // 1 / pivot, scaled against overflow of |.|^2 (every thread: no divergent branch)
+        double2 inv;
+        {
+            const double2 d = prow[0];
+            const double sc = fmax(fabs(d.x), fabs(d.y)), rs = 1.0 / sc;
+            const double bx = d.x * rs, by = d.y * rs, rd = 1.0 / ((bx * bx + by * by) * sc);
+            inv = make_double2(bx * rd, -by * rd);
+        }
+        // the rank-1 update, rotated: a[c - 1] <- new column c, a[NB - 1] <- new column 0
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int r = tid + k * nt;
+            if (r == j) {
+#pragma unroll
+                for (int c = 1; c < NB; ++c) a[k][c - 1] = cmul(prow[c], inv);
+                a[k][NB - 1] = inv;
+                continue;
+            }
+            if (r == p)
+#pragma unroll
+                for (int c = 0; c < NB; ++c) a[k][c] = jrow[b][c];
+            const double2 m = cmul(a[k][0], inv);
+#pragma unroll
+            for (int c = 1; c < NB; ++c) a[k][c - 1] = cfms4(a[k][c], m, prow[c]);
+            a[k][NB - 1] = make_double2(-m.x, -m.y);
+        }
+    }
+    // (w == NB, the usual case, has the columns back in place: static addresses,
+    // stores issued back to back)
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const int r = tid + k * nt;
-        if (r < n)
+        if (r < n) {
+            if (w == NB) {
 #pragma unroll
-            for (int c = 0; c < NB; ++c)
-                if (c < w) G[r + (size_t)c * n] = P.a[k][c];
+                for (int c = 0; c < NB; ++c) G[r + (size_t)c * n] = a[k][c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) {
+                    const int oc = c + w < NB ? c + w : c + w - NB;   // original column at position c
+                    if (oc < w) G[r + (size_t)oc * n] = a[k][c];
+                }
+            }
+        }
     }
     if (bad && tid == 0) atomicOr(info, 2);
 }

@@ -24,6 +24,10 @@ template <typename V> __host__ __device__ inline V cmul(V a, V b) {
 template <typename V> __host__ __device__ inline V cfma(V b, V c, V a) {
     V z; z.x = a.x + b.x * c.x - b.y * c.y; z.y = a.y + b.x * c.y + b.y * c.x; return z;
 }
+// a - b*c as four fused multiply-adds
+template <typename V> __host__ __device__ inline V cfms4(V a, V b, V c) {
+    V z; z.x = fma(b.y, c.y, fma(-b.x, c.x, a.x)); z.y = fma(-b.y, c.x, fma(-b.x, c.y, a.y)); return z;
+}
 template <typename V, typename S> __host__ __device__ inline V cscale(V a, S s) { V z; z.x = a.x * s; z.y = a.y * s; return z; }
 template <typename V> __host__ __device__ inline V cinv(V a) {
     // 1/a with scaling to avoid overflow of |a|^2
