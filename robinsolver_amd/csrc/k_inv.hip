@@ -263,6 +263,46 @@ __global__ void __launch_bounds__(LB) gj_panel_reg_kernel(double2* __restrict__ 
 
 // Apply the panel's row interchanges to the columns outside the panel, move the
 // pivot rows of those columns to W (nb x (n-nb), compact) and zero them.
+// A group of nb lanes per column (nb <= 32 divides 256): the interchanges
+// touch the contiguous rows j0 .. j0+nb-1 and the rows piv[j], so lane i
+// reads the rows whose values end at j0 + i and at piv[j0 + i] (from the
+// composed permutation, built once per workgroup in LDS), moves the first to
+// W, zeroes row j0 + i and stores the second.  Every load of a column comes
+// before its stores in program order (the stores wait for the loaded values).
+__global__ void __launch_bounds__(256) gj_swap_extract_grp_kernel(double2* __restrict__ mats, int n, size_t stride,
+                                                                  int j0, int nb, const int* __restrict__ pivs,
+                                                                  double2* __restrict__ W, size_t wstride) {
+    extern __shared__ int rowat[];   // [n]: row whose value ends at each (touched) position
+    const int tid = threadIdx.x;
+    const int* piv = pivs + (size_t)blockIdx.y * n;
+    if (tid < nb) {
+        rowat[j0 + tid] = j0 + tid;
+        const int p = piv[j0 + tid];
+        rowat[p] = p;   // (a position in both sets gets the same value twice)
+    }
+    __syncthreads();
+    if (tid == 0)
+        for (int jj = 0; jj < nb; ++jj) {
+            const int j = j0 + jj, p = piv[j];
+            if (p != j) { const int t = rowat[j]; rowat[j] = rowat[p]; rowat[p] = t; }
+        }
+    __syncthreads();
+    const int g = tid / nb, i = tid - g * nb;
+    const int c2 = blockIdx.x * (256 / nb) + g;
+    if (c2 >= n - nb) return;
+    const int c = c2 < j0 ? c2 : c2 + nb;
+    double2* col = mats + (size_t)blockIdx.y * stride + (size_t)c * n;
+    const int j = j0 + i, p = piv[j];
+    const bool outside = p >= j0 + nb;   // (a pivot row inside the panel's rows is some lane's j)
+    const double2 v1 = col[rowat[j]];
+    const double2 v2 = outside ? col[rowat[p]] : make_double2(0.0, 0.0);
+    W[(size_t)blockIdx.y * wstride + (size_t)c2 * nb + i] = v1;
+    if (outside) col[p] = v2;
+    col[j] = make_double2(0.0, 0.0);
+}
+
+// (one thread per column, sequential interchanges: interfaces too large for the
+// LDS position table of the kernel above)
 __global__ void gj_swap_extract_kernel(double2* __restrict__ mats, int n, size_t stride, int j0, int nb,
                                        const int* __restrict__ pivs, double2* __restrict__ W, size_t wstride) {
     const int c2 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -344,8 +384,14 @@ void launch_inverse(const InvPlan& ip, double2* mats, int* pivs, double2* W, con
         else
             hipLaunchKernelGGL(gj_panel_kernel, dim3(ip.batch), dim3(256), plds, st, mats, n, ip.stride, j0, nb,
                                pivs, info);
-        hipLaunchKernelGGL(gj_swap_extract_kernel, dim3((n - nb + 255) / 256, ip.batch), dim3(256), 0, st, mats, n,
-                           ip.stride, j0, nb, pivs, W, wstride);
+        if ((size_t)n * sizeof(int) <= 32768 && nb <= 32 && (nb & (nb - 1)) == 0) {
+            const int cpb = 256 / nb;
+            hipLaunchKernelGGL(gj_swap_extract_grp_kernel, dim3((n - nb + cpb - 1) / cpb, ip.batch), dim3(256),
+                               (size_t)n * sizeof(int), st, mats, n, ip.stride, j0, nb, pivs, W, wstride);
+        } else {
+            hipLaunchKernelGGL(gj_swap_extract_kernel, dim3((n - nb + 255) / 256, ip.batch), dim3(256), 0, st, mats,
+                               n, ip.stride, j0, nb, pivs, W, wstride);
+        }
         // the rank-nb update: one 16-deep chunk per tile, so latency-bound; 32x32
         // tiles (four times the workgroups of 64x64) measured fastest
         // (factor at 1024^2: 64x64 38.4 ms, 32x64 38.0, 64x32 38.0, 16x64 38.0, 32x32 37.8)
