@@ -336,7 +336,10 @@ size_t inv_small_lds(int n) { return (size_t)n * n * sizeof(double2) + (size_t)n
 // rows per thread of the register panel kernel (0: use the LDS panel kernel).
 // Interfaces of 513..1024 points take two rows per thread on half as many
 // threads, still with 16-column panels (a third fewer waves to synchronise per
-// step: 54 -> 51 us per panel at 1024).
+// step).  A step costs about 2.3 us at 512 points whatever the panel width (its
+// latency chain: argmax, barrier, wave scan, pivot inverse, update), so
+// 32-column panels were slower (103 us per panel against 2 x 37.6 us; factor at
+// 1024^2 33.1 -> 34.7 ms) although they halve the launches.
 static int gj_reg_rows(int n) { return n <= 512 ? 1 : (n <= 2048 ? 2 : (n <= 4096 ? 4 : 0)); }
 
 int gj_panel_width(int n, size_t lds_limit) {
@@ -351,7 +354,10 @@ int gj_panel_width(int n, size_t lds_limit) {
 
 void InvPlan::build(int n_, int batch_, size_t stride_, size_t lds_limit_) {
     n = n_; batch = batch_; stride = stride_; lds_limit = lds_limit_;
-    small = inv_small_lds(n) + 512 <= lds_limit;
+    // the whole-matrix LDS kernel (five barriers per column) below 64 points; from
+    // 64 on the register panels are faster (64 points, 256 / 128 matrices:
+    // 252 / 246 us with the LDS kernel)
+    small = n < 64 && inv_small_lds(n) + 512 <= lds_limit;
     nb = small ? n : gj_panel_width(n, lds_limit);
 }
 
