@@ -142,12 +142,15 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(double2* __restrict__ mat
 }
 
 // Panel elimination with the panel held in registers: thread t owns rows
-// t, t + nt, ... (RPT of them) of the n x w panel (w <= NB columns).  Same
-// pivot choice as gj_panel_kernel.  One Gauss-Jordan step has one workgroup
-// barrier: each wave finds its best row by a DPP argmax, and the lane holding
-// that row publishes it (with the old row j, from its owner); after the
-// barrier every thread picks the winning wave's row, inverts its pivot and
-// applies the rank-1 update to its own rows.  The published rows are
+// t, t + nt, ... (RPT of them) of the n x w panel (w <= NB columns, n <= 4096).
+// One Gauss-Jordan step has one workgroup barrier: each wave finds its best
+// row by a DPP argmax, the lane holding that row publishes it (with the old
+// row j, from its owner), and the waves' results meet in one LDS atomic max;
+// after the barrier every thread reads the pivot row, inverts its pivot and
+// applies the rank-1 update to its own rows.  The pivot is the row of largest
+// |re|+|im| compared on its top 52 bits (relative resolution 2^-41; the
+// smallest row wins among equals), so it can differ from gj_panel_kernel's
+// only between candidates equal to 12 digits.  The published rows are
 // double-buffered by step parity: the writes of step s + 2 come after the
 // barrier of step s + 1, which every reader of step s has passed.
 //
@@ -162,8 +165,7 @@ template <int NB, int RPT, int LB = 1024>
 __global__ void __launch_bounds__(LB) gj_panel_reg_kernel(double2* __restrict__ mats, int n, size_t stride,
                                                             int j0, int w, int* __restrict__ pivs,
                                                             int* __restrict__ info) {
-    __shared__ double rv[2][16];          // each wave's best |re|+|im| ...
-    __shared__ int ri[2][16];             // ... and its row
+    __shared__ unsigned long long best3[3];   // the step's pivot key (three slots: see below)
     __shared__ double2 cand[2][16][NB];   // each wave's best row (rotated like the registers)
     __shared__ double2 jrow[2][NB];       // old row j (becomes row p)
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
@@ -176,10 +178,12 @@ __global__ void __launch_bounds__(LB) gj_panel_reg_kernel(double2* __restrict__ 
 #pragma unroll
         for (int c = 0; c < NB; ++c) a[k][c] = (r < n && c < w) ? G[r + (size_t)c * n] : make_double2(0.0, 0.0);
     }
+    if (tid < 3) best3[tid] = 0;
+    __syncthreads();
     bool bad = false;
 #pragma unroll 1
     for (int s = 0; s < w; ++s) {
-        const int j = j0 + s, b = s & 1;
+        const int j = j0 + s, b = s & 1, b3 = s % 3;
         double v = -1.0;
         int vi = n;
 #pragma unroll
@@ -201,16 +205,20 @@ __global__ void __launch_bounds__(LB) gj_panel_reg_kernel(double2* __restrict__ 
 #pragma unroll
                 for (int c = 0; c < NB; ++c) jrow[b][c] = a[k][c];
         }
-        if (lane == 0) { rv[b][wave] = v; ri[b][wave] = vi; }
+        // the waves' candidates meet in one LDS atomic max of a key: the bits of
+        // |re|+|im| (>= 0, so ordered like the values) above the lowest 11, then
+        // 4095 - row (the smaller row wins among equal values)
+        if (lane == 0 && v >= 0.0)
+            atomicMax(&best3[b3], ((unsigned long long)__double_as_longlong(v) >> 11 << 12) |
+                                      (unsigned long long)(4095 - vi));
         __syncthreads();
-        double best = rv[b][0];
-        int p = ri[b][0], qw = 0;
-        for (int q = 1; q < nw; ++q) {
-            const double x = rv[b][q];
-            const int y = ri[b][q];
-            if (x > best || (x == best && y < p)) { best = x; p = y; qw = q; }
-        }
-        if (!(best > 0.0) || p >= n) { bad = true; p = j; }   // singular: no interchange, reported through info
+        const unsigned long long key = best3[b3];
+        int p = 4095 - (int)(key & 4095);
+        // slot (s + 2) % 3 was read in step s - 1, before this step's barrier, and is
+        // next written in step s + 2, after the next one
+        if (tid == 0) best3[(s + 2) % 3] = 0;
+        if ((key >> 12) == 0 || p >= n) { bad = true; p = j; }   // singular: no interchange, reported through info
+        const int qw = (p % nt) >> 6;   // the wave holding row p
         if (tid == 0) piv[j] = p;
         const double2* prow = cand[b][qw];   // the pivot row
         // 1 / pivot, scaled against overflow of |.|^2 (every thread: no divergent branch)
