@@ -1189,7 +1189,11 @@ void launch_leaf_blocks(const LeafGeom& G, const double2* kint, double2* sinv, i
                         hipStream_t st) {
     LeafCoef C = make_coef(G);
     size_t lds = leaf_blocks_lds(G);
-    hipLaunchKernelGGL(leaf_blocks_kernel, dim3(n_leaves), dim3(256), lds, st, G, C, kint, sinv, info);
+    // one wave per leaf for blocks of at most 256 entries (2D leaves): the
+    // Gauss-Jordan steps synchronise one wave instead of four, and four times as
+    // many leaves fit a CU
+    const int nt = G.s * G.s <= 256 ? 64 : 256;
+    hipLaunchKernelGGL(leaf_blocks_kernel, dim3(n_leaves), dim3(nt), lds, st, G, C, kint, sinv, info);
 }
 
 template <typename R> int leaf_apply_chunk(const LeafGeom& G, int ncols, size_t lds_limit) {
