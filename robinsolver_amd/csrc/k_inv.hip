@@ -5,15 +5,18 @@
 // S^-1 = zeta I - T1_II - nu^2 D by zgetrf (robin_tree_module.f90:440-467;
 // factorization_module.f90:49-141).  Here both D and S are formed explicitly
 // (Gauss-Jordan, the pivot being the first entry of maximal |re|+|im| in the
-// column, as zgetrf chooses it), so that the merge and the solve passes are
+// column, as zgetrf chooses it; the register panels compare the candidates on
+// the top 52 bits of that value, so they can choose differently only between
+// candidates equal to 12 digits), so that the merge and the solve passes are
 // plain matrix products on the matrix cores.
 //
-// Small interfaces (the whole matrix fits in LDS) use one workgroup per matrix.
-// Larger ones use a blocked right-looking Gauss-Jordan: an n x nb panel is
-// eliminated in LDS, the row interchanges are applied to the other columns and
-// the pivot rows moved out (W), and the rest of the matrix gets the rank-nb
-// update A[:, R] += A[:, panel] W as a batched GEMM.  After the last panel the
-// interchanges are undone as column interchanges.
+// Interfaces below 64 points use one workgroup per matrix with the whole matrix
+// in LDS.  Larger ones use a blocked right-looking Gauss-Jordan: an n x nb
+// panel is eliminated (in registers up to 4096 points, in LDS above), the row
+// interchanges are applied to the other columns and the pivot rows moved out
+// (W), and the rest of the matrix gets the rank-nb update A[:, R] += A[:, panel] W
+// as a batched GEMM.  After the last panel the interchanges are undone as
+// column interchanges.
 #include <algorithm>
 #include <utility>
 
