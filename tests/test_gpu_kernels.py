@@ -121,8 +121,8 @@ def test_gemm_split_and_scatter(prec, gpu):
     assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < tol
 
 
-@pytest.mark.parametrize("n,batch", [(16, 5), (48, 3), (64, 2), (128, 2), (200, 1), (512, 2), (1024, 2),
-                                     (1536, 1), (2048, 1), (3072, 1)])
+@pytest.mark.parametrize("n,batch", [(16, 5), (48, 3), (64, 2), (128, 2), (200, 1), (202, 2), (512, 2), (1004, 1),
+                                     (1024, 2), (1536, 1), (2048, 1), (3072, 1)])
 def test_inverse_matches_numpy(n, batch, gpu):
     print("case: inverse", n, batch, flush=True)
     rng = np.random.default_rng(n)
