@@ -10,6 +10,7 @@
  * thread count: no value is accumulated across leaves or nodes.
  */
 #include "rs_oracle.h"
+#include "ro_dense.h"
 
 #include <complex.h>
 #include <math.h>
@@ -91,60 +92,14 @@ static void omp_setup(void) {
  * Dense complex linear algebra (restated LAPACK/BLAS semantics)
  * ========================================================================== */
 
-static double cabs1(cplx z) { return fabs(creal(z)) + fabs(cimag(z)); }
-
 /* zgetrf (factorization_module.f90:71): partial pivoting, pivot = first entry of
- * maximal |re|+|im| (izamax).  Returns info (0 ok, j+1 for a zero pivot). */
-static int lu_factor(int n, cplx *A, int *piv) {
-    int info = 0;
-    for (int j = 0; j < n; ++j) {
-        int p = j;
-        double best = cabs1(A[j + (size_t)j * n]);
-        for (int i = j + 1; i < n; ++i) {
-            double v = cabs1(A[i + (size_t)j * n]);
-            if (v > best) { best = v; p = i; }
-        }
-        piv[j] = p;
-        if (best == 0.0) { if (!info) info = j + 1; continue; }
-        if (p != j)
-            for (int c = 0; c < n; ++c) {
-                cplx tmp = A[j + (size_t)c * n];
-                A[j + (size_t)c * n] = A[p + (size_t)c * n];
-                A[p + (size_t)c * n] = tmp;
-            }
-        cplx r = 1.0 / A[j + (size_t)j * n];
-        for (int i = j + 1; i < n; ++i) A[i + (size_t)j * n] *= r;
-        for (int c = j + 1; c < n; ++c) {
-            cplx u = A[j + (size_t)c * n];
-            if (u == 0.0) continue;
-            cplx *col = A + (size_t)c * n;
-            const cplx *lj = A + (size_t)j * n;
-            for (int i = j + 1; i < n; ++i) col[i] -= lj[i] * u;
-        }
-    }
-    return info;
-}
+ * maximal |re|+|im| (izamax); blocked (ro_dense.c).  Returns info (0 ok, j+1 for
+ * a zero pivot). */
+static int lu_factor(int n, cplx *A, int *piv) { return ro_zgetrf(n, A, piv); }
 
 /* zgetrs('n') (factorization_module.f90:346-349): B <- A^-1 B, nrhs columns. */
 static void lu_solve(int n, const cplx *LU, const int *piv, int nrhs, cplx *B, int ldb) {
-    for (int c = 0; c < nrhs; ++c) {
-        cplx *b = B + (size_t)c * ldb;
-        for (int i = 0; i < n; ++i)
-            if (piv[i] != i) { cplx t = b[i]; b[i] = b[piv[i]]; b[piv[i]] = t; }
-        for (int j = 0; j < n; ++j) {
-            cplx v = b[j];
-            if (v == 0.0) continue;
-            const cplx *l = LU + (size_t)j * n;
-            for (int i = j + 1; i < n; ++i) b[i] -= l[i] * v;
-        }
-        for (int j = n - 1; j >= 0; --j) {
-            const cplx *u = LU + (size_t)j * n;
-            b[j] /= u[j];
-            cplx v = b[j];
-            if (v == 0.0) continue;
-            for (int i = 0; i < j; ++i) b[i] -= u[i] * v;
-        }
-    }
+    ro_zgetrs(n, LU, piv, nrhs, B, ldb);
 }
 
 /* matrixdata%invvec (factorization_module.f90:291-366), LU mode:
@@ -170,17 +125,7 @@ static void invvec(int n, const cplx *LU, const int *piv, int nrhs, cplx *b, cpl
 /* zgemm('n','n'): C(ldc) = alpha*A(m x k, lda)*B(k x n, ldb) + beta*C. */
 static void gemm(int m, int n, int k, cplx alpha, const cplx *A, int lda, const cplx *B,
                  int ldb, cplx beta, cplx *C, int ldc) {
-    for (int j = 0; j < n; ++j) {
-        cplx *c = C + (size_t)j * ldc;
-        if (beta == 0.0) for (int i = 0; i < m; ++i) c[i] = 0.0;
-        else if (beta != 1.0) for (int i = 0; i < m; ++i) c[i] *= beta;
-        for (int l = 0; l < k; ++l) {
-            cplx bv = alpha * B[l + (size_t)j * ldb];
-            if (bv == 0.0) continue;
-            const cplx *a = A + (size_t)l * lda;
-            for (int i = 0; i < m; ++i) c[i] += a[i] * bv;
-        }
-    }
+    ro_zgemm(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 /* zgemv('n') with alpha = beta = 1: y += A x. */
@@ -714,27 +659,58 @@ static int merge_node(ro_tree *t, ro_node *nd) {
     return 0;
 }
 
+/* one leaf (level L) or one merge (level < L) of ro_factor; 1 = singular leaf,
+ * 2 = merge failure, 0 = ok */
+static int factor_node(ro_tree *t, int l, int q, const cplx *k_int, int ni) {
+    if (l == t->L) {
+        ro_node *lf = t->leaves[q];
+        if (leaf_rtr(t, lf, k_int + (size_t)q * ni)) return 1;
+        snapshot_rtr(t, lf);
+        return 0;
+    }
+    ro_node *nd = t->lvl_nodes[l][q];
+    if (merge_node(t, nd)) return 2;
+    snapshot_rtr(t, nd);
+    return 0;
+}
+
+static int max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 int ro_factor(ro_tree *t, const double *k_int_d) {
     omp_setup();
     const cplx *k_int = (const cplx *)k_int_d;
     int ni = prod(t->leaves[0]->ptbox, t->d);
     int bad = 0;
+    /* Levels with at least as many nodes as threads run their nodes in parallel,
+     * each node on one thread.  The nodes of a level with fewer nodes (the top
+     * of the tree) run one after another, each with its dense operations spread
+     * over all threads (ro_dense.c).  Either way a node's arithmetic is the
+     * same. */
+    const int nth = max_threads();
+    for (int l = t->L; l >= 0; --l) {
+        const int cnt = l == t->L ? t->n_leaves : t->lvl_count[l];
+        const int outer = cnt >= nth;
+        ro_dense_set_inner(outer ? 1 : nth);
+        /* (the sequential loop is not an inactive parallel region: teams opened
+         * inside one would be nested teams, which the OpenMP runtime does not
+         * take from its thread pool) */
+        if (outer) {
 #pragma omp parallel for schedule(dynamic) reduction(| : bad)
-    for (int q = 0; q < t->n_leaves; ++q) {
-        ro_node *lf = t->leaves[q];
-        if (leaf_rtr(t, lf, k_int + (size_t)q * ni)) bad |= 1;
-        else snapshot_rtr(t, lf);
-    }
-    if (bad) { set_err(t, "Error: singular leaf matrix"); return 1; }
-    for (int l = t->L - 1; l >= 0; --l) {
-#pragma omp parallel for schedule(dynamic) reduction(| : bad)
-        for (int q = 0; q < t->lvl_count[l]; ++q) {
-            ro_node *nd = t->lvl_nodes[l][q];
-            if (merge_node(t, nd)) bad |= 1;
-            else snapshot_rtr(t, nd);
+            for (int q = 0; q < cnt; ++q) bad |= factor_node(t, l, q, k_int, ni);
+        } else {
+            for (int q = 0; q < cnt; ++q) bad |= factor_node(t, l, q, k_int, ni);
         }
-        if (bad) return 1;
+        if (bad) break;
     }
+    ro_dense_set_inner(0);
+    if (bad & 1) { set_err(t, "Error: singular leaf matrix"); return 1; }
+    if (bad) return 1;
     t->factored = 1;
     return 0;
 }
