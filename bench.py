@@ -10,7 +10,9 @@ the mixed path adds the complex64 conversion), the plain FP64 path (one RHS
 per call, the reference's semantics), the batched FP64 path, the rel. l2
 error of every mixed RHS against the FP64 solution, the roofline of the
 dominant kernel (HIP events inside the library, on the stream the kernels run
-on) and the CPU oracle timed on a bounded sample on this host.
+on) and the CPU oracle timed on this host on the same tree (its factor and two
+of the right-hand sides), whose solutions also give the GPU paths' rel. l2
+error against the oracle.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -26,61 +28,14 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
+from robinsolver_amd.workloads import (MIX_TREES, device_points, gaussian_sources, plane_wave_g_dev,  # noqa: E402
+                                      potential_dev)
+
 PEAK = {"hbm_GBs": 8000.0, "fp64_TF": 78.6, "fp32_TF": 157.3}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def potential(pts_int, kb, seed, nmodes=16, d=2):
-    """kb^2 (1 + 0.3 q) + i 0.01 kb^2, q = sum_m a_m cos(2 pi p_m.x + phi_m) / sum|a_m|,
-    a_m ~ U[-1,1], p_m in {1..8}^d, phi_m ~ U[0, 2 pi)."""
-    rng = np.random.default_rng(seed)
-    a = rng.uniform(-1, 1, nmodes)
-    p = rng.integers(1, 9, size=(nmodes, d))
-    ph = rng.uniform(0, 2 * np.pi, nmodes)
-    q = np.zeros(pts_int.shape[:-1])
-    for j in range(nmodes):
-        q += a[j] * np.cos(2 * np.pi * (pts_int[..., :d] @ p[j]) + ph[j])
-    q /= np.abs(a).sum()
-    return kb ** 2 * (1 + 0.3 * q) + 1j * 0.01 * kb ** 2
-
-
-def potential_dev(torch, pts_int_dev, kb, seed, nmodes=16, d=2):
-    """potential() evaluated on the GPU at device coordinates (the same seeded
-    modes; complex128 [n_leaves][n_int])."""
-    rng = np.random.default_rng(seed)
-    a = rng.uniform(-1, 1, nmodes)
-    p = rng.integers(1, 9, size=(nmodes, d))
-    ph = rng.uniform(0, 2 * np.pi, nmodes)
-    x = pts_int_dev[..., :d]
-    q = torch.zeros(x.shape[:-1], dtype=torch.float64, device=x.device)
-    for j in range(nmodes):
-        pj = torch.tensor(p[j], dtype=torch.float64, device=x.device)
-        q += float(a[j]) * torch.cos(2 * np.pi * (x @ pj) + float(ph[j]))
-    q /= float(np.abs(a).sum())
-    return (kb ** 2 * (1 + 0.3 * q)).to(torch.complex128) + 1j * 0.01 * kb ** 2
-
-
-def device_points(torch, tree):
-    """All points' coordinates [n_leaves][m][3] and face ids [n_leaves][m] on the GPU."""
-    xyz = torch.empty((tree.n_leaves, tree.m, 3), dtype=torch.float64, device="cuda:0")
-    face = torch.empty((tree.n_leaves, tree.m), dtype=torch.int32, device="cuda:0")
-    tree.points_device(xyz.data_ptr(), face.data_ptr())
-    torch.cuda.synchronize()
-    return xyz, face
-
-
-def gaussian_sources(torch, pts_int_dev, nrhs, seed, sigma=0.02, d=2):
-    rng = np.random.default_rng(seed + 1000)
-    c = torch.tensor(rng.uniform(0.1, 0.9, size=(nrhs, d)), device=pts_int_dev.device, dtype=torch.float64)
-    x = pts_int_dev[..., :d]  # (n_leaves, n_int, d)
-    f = torch.empty((nrhs,) + tuple(x.shape[:2]), dtype=torch.complex128, device=x.device)
-    for r in range(nrhs):
-        r2 = ((x - c[r]) ** 2).sum(-1)
-        f[r] = torch.exp(-r2 / (2 * sigma * sigma)).to(torch.complex128)
-    return f
 
 
 class URing:
@@ -206,36 +161,10 @@ def serve_cfg3(torch, rsa, t, f, g, u32, u64, args, sz_f, sz_g):
     return out
 
 
-MIX_TREES = [(128, 6, 10.0), (256, 8, 10.0), (384, 10, 20.0), (512, 10, 20.0), (640, 10, 40.0), (768, 12, 40.0),
-             (896, 12, 80.0), (1024, 12, 80.0)]
-
-
-def plane_wave_g_dev(torch, tree, xyz, face, kb, thetas):
-    """Impedance data of u = exp(i kb d.x), d = (cos t, sin t), on the physical
-    faces, g = a1 u + du/dn with a1 = -i kb, so g = i kb (d.n - 1) u; evaluated
-    on the GPU from device coordinates and face ids.  Returns
-    [len(thetas)][n_leaves][n_bnd] (zero away from the boundary)."""
-    nb = tree.nb
-    x = xyz[:, :nb, :2]
-    fc = face[:, :nb]
-    normal = torch.zeros_like(x)
-    phys = torch.zeros(x.shape[:2], dtype=torch.bool, device=x.device)
-    for fid, ax, sgn, edge in ((1, 0, -1.0, 0.0), (2, 0, 1.0, 1.0), (3, 1, -1.0, 0.0), (4, 1, 1.0, 1.0)):
-        on = (fc == fid) & ((x[..., ax] - edge).abs() < 1e-12)
-        phys |= on
-        normal[..., ax] = torch.where(fc == fid, torch.full_like(normal[..., ax], sgn), normal[..., ax])
-    out = torch.zeros((len(thetas),) + tuple(x.shape[:2]), dtype=torch.complex128, device=x.device)
-    for i, th in enumerate(thetas):
-        dv = torch.tensor([np.cos(th), np.sin(th)], dtype=torch.float64, device=x.device)
-        u = torch.exp(1j * kb * (x @ dv).to(torch.complex128))
-        val = 1j * kb * ((normal @ dv) - 1.0).to(torch.complex128) * u
-        out[i] = torch.where(phys, val, torch.zeros_like(val))
-    return out
-
-
 def serve_mix(args):
     """Eight resident trees (2D 128^2 .. 1024^2, leaves near 16^2, kb by grid),
-    requests of 1..32 plane-wave right-hand sides on a uniformly chosen tree,
+    requests of 1..32 right-hand sides (Gaussian sources plus the impedance data
+    of a random-angle plane wave) on a uniformly chosen tree,
     2% of requests refactorizations with a new potential; a saturated client
     for sustained solves/s, then Poisson arrivals at 70% of it for latency."""
     import torch
@@ -249,14 +178,16 @@ def serve_mix(args):
         t = rsa.GpuTree(2, L, unit, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=1)
         xyz, face = device_points(torch, t)
         ks = [potential_dev(torch, xyz[:, t.nb:], kb, args.seed + 100 * gi + j).contiguous() for j in range(2)]
+        torch.cuda.synchronize()
         t.factor_device(ks[0].data_ptr())
         gpool = plane_wave_g_dev(torch, t, xyz, face, kb, rng.uniform(0, 2 * np.pi, 32))
-        fpool = torch.zeros((32, t.n_leaves, t.n_int), dtype=torch.complex128, device=dev)
+        fpool = gaussian_sources(torch, xyz[:, t.nb:, :], 32, args.seed + 100 * gi).contiguous()
         # accuracy of one refinement step on this tree, against its FP64 solve
         t64 = rsa.GpuTree(2, L, unit, 1.0 / n, -kb, precision=rsa.RS_FP64)
         t64.factor_device(ks[0].data_ptr())
         ua = torch.zeros((4, t.n_leaves, t.m), dtype=torch.complex128, device=dev)
         ub = torch.zeros_like(ua)
+        torch.cuda.synchronize()
         t.solve_device(4, fpool.data_ptr(), gpool.data_ptr(), ua.data_ptr())
         t64.solve_device(4, fpool.data_ptr(), gpool.data_ptr(), ub.data_ptr())
         torch.cuda.synchronize()
@@ -324,46 +255,55 @@ def serve_mix(args):
     return out
 
 
-def cpu_baseline(args, kb_phys, kb_opt):
-    """The oracle (reference algorithm restated in C, OpenMP over leaves and
-    nodes) on a bounded sample of the workload: the [0,1/4]^2 corner at the
-    same grid spacing (256^2, L=8, same 16x16 leaves, kb, potential family),
-    factor + solves timed, scaled to the full problem by the ratio of the
-    reference algorithm's solve/factor flop counts."""
+def cpu_baseline(args, kb_opt, k_dev, f, u64, u32, ncols=2):
+    """The oracle (the reference algorithm restated in C: dense leaf LU, zgetrf /
+    zgetrs / zgemm semantics, blocked and threaded over this host's cores) on
+    the full configuration: factor, then `ncols` of the batch's right-hand
+    sides, one solve each (the reference's one-RHS solve), timed.  Their
+    solutions are the parity check of the GPU's FP64 and mixed solutions of the
+    same columns."""
     from oracle.pyoracle import OracleTree
-    import robinsolver_amd as rsa
-    threads = min(os.cpu_count() or 1, 16)
-    L = args.levels - 4
-    dom = [[0, 0.25], [0, 0.25], [0, 1]]
-    o = OracleTree(2, L, dom, 1.0 / args.grid, kb_opt, threads=threads)
-    pts, _ = o.all_points()
-    k = potential(pts[:, o.nb:], kb_phys, args.seed)
+    nproc = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = nproc
+    threads = max(1, min(avail, int(os.environ.get("OMP_NUM_THREADS", avail))))
+    o = OracleTree(args.dim, args.levels, [[0, 1], [0, 1], [0, 1]], 1.0 / args.grid, kb_opt, threads=threads)
     t0 = time.perf_counter()
-    o.factor(k)
+    o.factor(k_dev.cpu().numpy())
     tf = time.perf_counter() - t0
-    rng = np.random.default_rng(7)
-    f = np.exp(-((pts[:, o.nb:, 0] - 0.12) ** 2 + (pts[:, o.nb:, 1] - 0.13) ** 2) / (2 * 0.02 ** 2)) + 0j
-    g = np.zeros((o.n_leaves, o.nb), np.complex128)
-    ns = 0
-    t0 = time.perf_counter()
-    while ns < 3 or time.perf_counter() - t0 < 2.0:
-        o.solve(f, g)
-        ns += 1
-    ts = (time.perf_counter() - t0) / ns
-    # flop ratio sample -> full, reference-algorithm counts
-    full = rsa.GpuTree(2, args.levels, [[0, 1], [0, 1], [0, 1]], 1.0 / args.grid, kb_opt)
-    samp = rsa.GpuTree(2, L, dom, 1.0 / args.grid, kb_opt)
-    ff_full, fs_full = full.algorithmic_counts()
-    ff_s, fs_s = samp.algorithmic_counts()
+    fh = f[:ncols].cpu().numpy()
+    g0 = np.zeros((o.n_leaves, o.nb), np.complex128)
+    ts, uo = [], []
+    for r in range(ncols):
+        t0 = time.perf_counter()
+        uo.append(o.solve(fh[r], g0))
+        ts.append(time.perf_counter() - t0)
+    o.close()
+    uo = np.stack(uo).reshape(ncols, -1)
+
+    def rel(u):
+        a = u[:ncols].cpu().numpy().reshape(ncols, -1)
+        return (np.linalg.norm(a - uo, axis=1) / np.linalg.norm(uo, axis=1)).tolist()
+
+    r64, rmx = rel(u64), rel(u32)
+    log("oracle: factor %.2f s, solve %.3f s/RHS (%d threads); rel l2 vs oracle: fp64 %s, mixed %s"
+        % (tf, float(np.mean(ts)), threads, r64, rmx))
     return {
-        "value": (1.0 / ts) * fs_s / fs_full,
+        "value": ncols / sum(ts),
         "unit": "solves/s",
         "cores": threads,
         "kind": "port",
-        "sample": ("oracle on the [0,1/4]^2 corner at h=1/%d (%d^2, L=%d): factor %.2fs, solve %.3fs; "
-                   "scaled by the reference-algorithm flop ratio (solve x%.1f, factor x%.1f)"
-                   % (args.grid, args.grid // 4, L, tf, ts, fs_full / fs_s, ff_full / ff_s)),
-        "factor_s_est": tf * ff_full / ff_s,
+        "sample": ("the oracle on the full %d^2 L=%d tree: factor (%.2f s), then %d of the %d right-hand sides, "
+                   "one solve each (%s s); %d threads, nproc %d"
+                   % (args.grid, args.levels, tf, ncols, args.nrhs, ", ".join("%.3f" % x for x in ts), threads,
+                      nproc)),
+        "factor_s": tf,
+        "solve_s": ts,
+        "nproc": nproc,
+        "fp64_rel_l2_vs_oracle": r64,
+        "mixed_rel_l2_vs_oracle": rmx,
     }
 
 
@@ -402,6 +342,7 @@ def main():
         t = rsa.GpuTree(2, args.levels, [[0, 1], [0, 1], [0, 1]], 1.0 / args.grid, -args.kb, precision=rsa.RS_FP64)
         xyz, _ = device_points(torch, t)
         k_dev = potential_dev(torch, xyz[:, t.nb:], args.kb, args.seed).contiguous()
+        torch.cuda.synchronize()
         times = []
         for _ in range(args.factor_only):
             t.factor_device(k_dev.data_ptr())
@@ -420,7 +361,8 @@ def main():
             "metric": "served solves/s and latency, 8 resident 2D Helmholtz trees 128^2..1024^2 (mixed precision)",
             "value": sat["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
             "dtype": "c64", "data": "synthetic",
-            "config": {"workload": "8 trees (grid, L, kb) = %s; requests of 1..32 plane-wave RHS, 2%% refactors"
+            "config": {"workload": "8 trees (grid, L, kb) = %s; requests of 1..32 RHS (Gaussian sources + "
+                                   "plane-wave g), 2%% refactors"
                                    % [list(x) for x in MIX_TREES], "max_cols": args.nrhs, "lanes": 2,
                        "requests": args.mix_requests},
             "p50_ms": mix["poisson70"]["lat_p50_ms"], "p99_ms": mix["poisson70"]["lat_p99_ms"], "mix": mix}))
@@ -442,7 +384,8 @@ def main():
     # inputs sampled on the GPU at the device coordinates of the points
     xyz, _ = device_points(torch, t)
     pint_dev = xyz[:, t.nb:, :]
-    k_dev = potential_dev(torch, pint_dev, kb_phys, args.seed, nmodes=16 if d == 2 else 24, d=d).contiguous()
+    k_dev = potential_dev(torch, pint_dev, kb_phys, args.seed, nmodes=16 if d == 2 else 24, d=d,
+                          absorb=0.01 if d == 2 else 0.0).contiguous()
     f = gaussian_sources(torch, pint_dev, args.nrhs, args.seed, sigma=0.02 if d == 2 else 0.05, d=d)
     g = torch.zeros((args.nrhs, t.n_leaves, t.nb), dtype=torch.complex128, device=dev)
     u64 = torch.zeros((args.nrhs, t.n_leaves, t.m), dtype=torch.complex128, device=dev)
@@ -658,7 +601,10 @@ def main():
 
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(args, kb_phys, kb_opt)
+        cb = cpu_baseline(args, kb_opt, k_dev, f, u64, u32)
+        out["cpu_baseline"] = cb
+        out["mixed_rel_l2_vs_oracle"] = max(cb["mixed_rel_l2_vs_oracle"])
+        out["fp64_rel_l2_vs_oracle"] = max(cb["fp64_rel_l2_vs_oracle"])
     print(json.dumps(out))
 
 

@@ -96,7 +96,8 @@ int rs_leaf_points(const rs_tree *t, int leaf, double *xyz, int *face);
 int rs_points_device(rs_tree *t, double *xyz, int *face, void *stream);
 
 /* rs_factor with k at the interior points already in device memory
- * ([n_leaves][n_int] complex128, DFS leaf order), synchronous. */
+ * ([n_leaves][n_int] complex128, DFS leaf order), synchronous.  k is read
+ * after the work queued on the default stream before the call. */
 int rs_factor_device(rs_tree *t, const double *k_int);
 /* Leaf grid: points per axis (ptbox) and spacing h, the values the reference
  * stores in node%ptbox and T%opts%h (fd_module.f90:1060-1084,
@@ -122,7 +123,15 @@ int rs_get_solution(const rs_tree *t, int rhs, double *u);
 /* Device-resident variant of rs_solve (no reference counterpart: the caller's
  * inputs already live in HBM).  f_dev, g_dev, u_dev are device pointers with
  * the multi-RHS layouts above; stream is a hipStream_t (NULL = the tree's own
- * stream).  Asynchronous with respect to the host. */
+ * stream).  Asynchronous with respect to the host.
+ * Ordering: with stream = NULL the solve runs on the tree's own stream after
+ * the work the caller queued on the default stream (e.g. torch's default
+ * stream, hipMemset, hipMemcpy) before the call, and work queued on the default
+ * stream after the call waits for the solve.  A caller producing f/g on another
+ * stream passes that stream and orders its own reads of u.  Consecutive solves
+ * on one handle may use different streams: each waits for the previous one
+ * (they share one workspace).  Calls on one handle must not overlap in host
+ * threads. */
 int rs_solve_device(rs_tree *t, int nrhs, const void *f_dev, const void *g_dev, void *u_dev,
                     int precision, void *stream);
 

@@ -282,51 +282,65 @@ int rs_check_gemm(int precision, int M, int N, int K, const double* A, int lda, 
 int rs_check_inverse(int n, int batch, double* mats) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_err = "Error: no HIP device."; return 1; }
+    if (n < 1 || batch < 1 || !mats) { g_err = "Error: bad arguments."; return 1; }
     hipDeviceProp_t prop;
-    (void)hipGetDeviceProperties(&prop, 0);
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) { g_err = "Error: hipGetDeviceProperties failed."; return 1; }
     rs::InvPlan ip;
     size_t nn = (size_t)n * n;
     ip.build(n, batch, nn, prop.sharedMemPerBlock);
+    if (!ip.fits()) { g_err = "Error: matrix too large for the interface inverses."; return 1; }
     double2 *dm = nullptr, *W = nullptr;
     int *piv = nullptr, *info = nullptr, *cm = nullptr;
     rs::GemmDesc* dd = nullptr;
     std::vector<rs::GemmDesc> descs;
     std::vector<int> cmaps;
-    (void)hipMalloc(&dm, nn * batch * 16);
-    (void)hipMalloc(&info, sizeof(int));
-    (void)hipMemset(info, 0, sizeof(int));
-    (void)hipMemcpy(dm, mats, nn * batch * 16, hipMemcpyHostToDevice);
-    if (!ip.small) {
-        (void)hipMalloc(&W, ip.w_elems() * 16 + 16);
-        (void)hipMalloc(&piv, (size_t)n * batch * sizeof(int));
+    auto release = [&]() {
+        for (void* p : {(void*)dm, (void*)W, (void*)piv, (void*)info, (void*)cm, (void*)dd})
+            if (p) (void)hipFree(p);
+    };
+    // every allocation and copy is checked before any launch
+    hipError_t e = hipMalloc(&dm, nn * batch * 16);
+    if (e == hipSuccess) e = hipMalloc(&info, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(info, 0, sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(dm, mats, nn * batch * 16, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !ip.small) {
+        e = hipMalloc(&W, ip.w_elems() * 16 + 16);
+        if (e == hipSuccess) e = hipMalloc(&piv, (size_t)n * batch * sizeof(int));
         cmaps.resize((size_t)ip.panels() * (n - ip.nb));
         for (int pi = 0; pi < ip.panels(); ++pi) {
             int j0 = pi * ip.nb;
             for (int c2 = 0; c2 < n - ip.nb; ++c2) cmaps[(size_t)pi * (n - ip.nb) + c2] = c2 < j0 ? c2 : c2 + ip.nb;
         }
-        (void)hipMalloc(&cm, cmaps.size() * sizeof(int) + 4);
-        (void)hipMemcpy(cm, cmaps.data(), cmaps.size() * sizeof(int), hipMemcpyHostToDevice);
-        for (int pi = 0; pi < ip.panels(); ++pi)
-            for (int b = 0; b < batch; ++b) {
-                rs::GemmDesc g;
-                std::memset(&g, 0, sizeof g);
-                double2* Mb = dm + (size_t)b * nn;
-                g.A = Mb + (size_t)pi * ip.nb * n; g.lda = n;
-                g.B = W + (size_t)b * ip.nb * (n - ip.nb); g.ldb = ip.nb;
-                g.Out = Mb; g.ldo = n; g.cmap = cm + (size_t)pi * (n - ip.nb);
-                g.Din = Mb; g.ldd = n; g.dmap = 1; g.ar = 1.0; g.br = 1.0; g.ksplit = 1 << 30;
-                descs.push_back(g);
-            }
-        (void)hipMalloc(&dd, descs.size() * sizeof(rs::GemmDesc));
-        (void)hipMemcpy(dd, descs.data(), descs.size() * sizeof(rs::GemmDesc), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&cm, cmaps.size() * sizeof(int) + 4);
+        if (e == hipSuccess) e = hipMemcpy(cm, cmaps.data(), cmaps.size() * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            for (int pi = 0; pi < ip.panels(); ++pi)
+                for (int b = 0; b < batch; ++b) {
+                    rs::GemmDesc g;
+                    std::memset(&g, 0, sizeof g);
+                    double2* Mb = dm + (size_t)b * nn;
+                    g.A = Mb + (size_t)pi * ip.nb * n; g.lda = n;
+                    g.B = W + (size_t)b * ip.nb * (n - ip.nb); g.ldb = ip.nb;
+                    g.Out = Mb; g.ldo = n; g.cmap = cm + (size_t)pi * (n - ip.nb);
+                    g.Din = Mb; g.ldd = n; g.dmap = 1; g.ar = 1.0; g.br = 1.0; g.ksplit = 1 << 30;
+                    descs.push_back(g);
+                }
+            e = hipMalloc(&dd, descs.size() * sizeof(rs::GemmDesc));
+            if (e == hipSuccess)
+                e = hipMemcpy(dd, descs.data(), descs.size() * sizeof(rs::GemmDesc), hipMemcpyHostToDevice);
+        }
+    }
+    if (e != hipSuccess) {
+        release();
+        g_err = std::string("HIP error: ") + hipGetErrorString(e);
+        return 1;
     }
     rs::launch_inverse(ip, dm, piv, W, dd, info, nullptr);
-    hipError_t e = hipDeviceSynchronize();
+    e = hipDeviceSynchronize();
     int hinfo = 0;
     if (e == hipSuccess) e = hipMemcpy(mats, dm, nn * batch * 16, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost);
-    for (void* p : {(void*)dm, (void*)W, (void*)piv, (void*)info, (void*)cm, (void*)dd})
-        if (p) (void)hipFree(p);
+    release();
     if (e != hipSuccess) { g_err = hipGetErrorString(e); return 1; }
     if (hinfo) { g_err = "Error: singular matrix."; return 2; }
     return 0;

@@ -372,6 +372,11 @@ void InvPlan::build(int n_, int batch_, size_t stride_, size_t lds_limit_) {
     nb = small ? n : gj_panel_width(n, lds_limit);
 }
 
+bool InvPlan::fits() const {
+    if (small || gj_reg_rows(n)) return true;   // register panels: rows up to 4096 (12-bit row in the pivot key)
+    return nb >= 1 && ((size_t)n * nb + n) * sizeof(double2) + 1024 <= lds_limit;
+}
+
 void launch_inverse(const InvPlan& ip, double2* mats, int* pivs, double2* W, const GemmDesc* d_descs,
                     int* info, hipStream_t st) {
     if (ip.batch <= 0) return;

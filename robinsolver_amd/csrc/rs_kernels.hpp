@@ -95,6 +95,9 @@ struct InvPlan {
     size_t stride = 0, lds_limit = 0;
     bool small = true;
     void build(int n, int batch, size_t stride, size_t lds_limit);
+    // false when no kernel of launch_inverse takes an n-point matrix (above ~5k
+    // points the LDS panel of even one column exceeds the LDS)
+    bool fits() const;
     int panels() const { return small ? 0 : n / nb; }
     size_t w_elems() const { return small ? 0 : (size_t)nb * (n - nb) * batch; }
 };

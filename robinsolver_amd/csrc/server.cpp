@@ -365,7 +365,8 @@ bool rs_server::dispatch_solves(std::unique_lock<std::mutex>& lk) {
             rc_cols.f = (const double2* const*)ln.tab_dev;
             rc_cols.g = (const double2* const*)(ln.tab_dev + mc);
             rc_cols.u = (double2* const*)(ln.tab_dev + 2 * mc);
-            rc = t->solve_cols(cols, rc_cols, -1, ln.st, j);
+            rs::Slot& sl = t->slot(j);
+            rc = t->slot_enter(sl, ln.st) || t->solve_cols(cols, rc_cols, -1, ln.st, j) || t->slot_leave(sl, ln.st);
             if (rc) msg = t->err;
         }
         if (hipEventRecord(ln.done, ln.st) != hipSuccess && !rc) {

@@ -50,15 +50,77 @@ GemmDesc desc0() {
 }  // namespace
 
 template <typename R> void SolveWork<R>::clear() {
-    for (auto* v : {&hV, &gF, &G, &Z})
-        for (DevBuf* b : *v) delete b;
     hV.clear(); gF.clear(); G.clear(); Z.clear();
-    for (DevBuf* b : descs) delete b;
-    descs.clear();
     d_up_g.clear(); d_up_h.clear(); d_dn_t.clear(); d_dn_g.clear();
-    rf.release(); rg.release(); rj.release(); uA.release(); ws.release();
+    ws = nullptr; rf = rg = rj = uA = nullptr;
     for (auto& t : tile) t.clear();
+    arena.release();
     cap = 0;
+    refine = false;
+}
+
+template <typename R> void SolveWork<R>::bind(char* base, const WorkLayout& lay) {
+    using V_ = c_t<R>;
+    auto at = [&](size_t off) -> char* { return off == WorkLayout::NONE ? nullptr : base + off; };
+    const size_t L1 = lay.gF.size();
+    hV.assign(L1, nullptr); gF.assign(L1, nullptr);
+    for (size_t l = 0; l < L1; ++l) { hV[l] = (V_*)at(lay.hV[l]); gF[l] = (V_*)at(lay.gF[l]); }
+    const size_t L = lay.G.size();
+    G.assign(L, nullptr); Z.assign(L, nullptr);
+    d_up_g.assign(L, nullptr); d_up_h.assign(L, nullptr); d_dn_t.assign(L, nullptr); d_dn_g.assign(L, nullptr);
+    for (size_t l = 0; l < L; ++l) {
+        G[l] = (V_*)at(lay.G[l]); Z[l] = (V_*)at(lay.Z[l]);
+        d_up_g[l] = (GemmDesc*)at(lay.dug[l]); d_up_h[l] = (GemmDesc*)at(lay.duh[l]);
+        d_dn_t[l] = (GemmDesc*)at(lay.ddt[l]); d_dn_g[l] = (GemmDesc*)at(lay.ddg[l]);
+    }
+    ws = at(lay.ws);
+    rf = (float2*)at(lay.rf); rg = (float2*)at(lay.rg); rj = (float2*)at(lay.rj); uA = (float2*)at(lay.uA);
+    cap = lay.cap;
+    refine = lay.refine;
+}
+
+// The descriptors of node n's four solve products at one level:
+//   up_g  g_I = invM [h1_I; h2_I]                          (B from the two children)
+//   up_h  h_parent = P (h_N + T_NI g_I), per child c        (rows scattered by prow)
+//   dn_t  Z_c = T_IN g_N of child c
+//   dn_g  g_I += invM Z, into the children's incoming data  (rows through imap)
+__host__ __device__ inline void solve_descs(const SolveLevel& p, int n, GemmDesc* ug, GemmDesc* uh, GemmDesc* dt,
+                                            GemmDesc* dg) {
+    const size_t e = p.esz, i = p.i, b = p.b, bn = p.bn, ii = i * i, tstride = b * b;
+    const int par = (p.lvl == 0) ? 0 : (n & 1);
+    GemmDesc z = {};
+    z.ar = 1.0;
+    z.ksplit = 1 << 30;
+    GemmDesc g = z;
+    g.A = p.IM + (size_t)n * 4 * ii * e; g.lda = 2 * p.i;
+    g.B = p.hC + ((size_t)(2 * n) * b + bn) * e; g.B2 = p.hC + ((size_t)(2 * n + 1) * b + bn) * e;
+    g.ksplit = p.i; g.ldb = p.ldc;
+    g.Out = p.Gl + (size_t)n * 2 * i * e; g.ldo = p.ldg;
+    *ug = g;
+    GemmDesc q = z;
+    q.A = p.IM + (size_t)n * 4 * ii * e; q.lda = 2 * p.i;
+    q.B = p.Zl + (size_t)n * 2 * i * e; q.ldb = p.ldg;
+    q.Din = p.Gl + (size_t)n * 2 * i * e; q.ldd = p.ldg; q.br = 1.0;
+    q.Out = p.gC + (size_t)(2 * n) * b * e; q.ldo = p.ldc;
+    q.rmap = p.imap;
+    *dg = q;
+    for (int c = 0; c < 2; ++c) {
+        const char* Tcn = p.Tc + (size_t)(2 * n + c) * tstride * e;
+        if (p.lvl > 0) {
+            GemmDesc u = z;
+            u.A = Tcn + bn * b * e; u.lda = p.b;
+            u.B = p.Gl + ((size_t)n * 2 * i + (size_t)c * i) * e; u.ldb = p.ldg;
+            u.Out = p.hP + (size_t)n * p.Pp * e; u.ldo = p.ldp;
+            u.rmap = p.prow + (size_t)(par * 2 + c) * bn;
+            u.Din = p.hC + (size_t)(2 * n + c) * b * e; u.ldd = p.ldc; u.br = 1.0;
+            uh[c] = u;
+        }
+        GemmDesc t = z;
+        t.A = Tcn + bn * e; t.lda = p.b;
+        t.B = p.gC + (size_t)(2 * n + c) * b * e; t.ldb = p.ldc;
+        t.Out = p.Zl + ((size_t)n * 2 * i + (size_t)c * i) * e; t.ldo = p.ldg;
+        dt[c] = t;
+    }
 }
 
 Tree::Tree() { rs_opts_default(&opts); }
@@ -83,7 +145,13 @@ int Tree::ensure_device() {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     lds_limit = prop.sharedMemPerBlock;
+    // The tree's own stream does not synchronise implicitly with the default
+    // stream; calls that run on it order themselves against the default stream
+    // with two events instead (join_default_in / join_default_out), which also
+    // holds for the launch of a captured graph.
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    if (!ev_in) HIPCHK(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
+    if (!ev_out) HIPCHK(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
     device_ready = true;
     if (leaf_blocks_lds(G) > lds_limit) return fail("Error: leaf layer too large for LDS (reduce the leaf size).");
     const Plan& P = plan;
@@ -100,8 +168,26 @@ int Tree::ensure_device() {
     return 0;
 }
 
+// Work the caller queued on the default stream (torch's default stream,
+// hipMemset / hipMemcpy without a stream) before the call runs before the
+// call's work on st.
+int Tree::join_default_in(hipStream_t st) {
+    HIPCHK(hipEventRecord(ev_in, nullptr));
+    HIPCHK(hipStreamWaitEvent(st, ev_in, 0));
+    return 0;
+}
+
+// Work queued on the default stream after the call runs after the call's work on st.
+int Tree::join_default_out(hipStream_t st) {
+    HIPCHK(hipEventRecord(ev_out, st));
+    HIPCHK(hipStreamWaitEvent(nullptr, ev_out, 0));
+    return 0;
+}
+
 Tree::~Tree() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
     clear_slots();
     free_factor();
     for (DevBuf* b : d_prow) delete b;
@@ -189,10 +275,13 @@ int Tree::points_device(double* xyz, int* face, hipStream_t st) {
             }
         if (upload(d_pts_box, box) || upload(d_pts_pat, P.point_pattern())) return 1;
     }
+    const bool own = !st;
+    if (own) st = stream;
+    if (own && join_default_in(st)) return 1;
     launch_leaf_points(d_pts_box.as<double>(), d_pts_pat.as<int>(), P.h, P.lv[P.L].ptbox, P.d, P.m, P.n_leaves, xyz,
-                       face, st ? st : stream);
+                       face, st);
     HIPCHK(hipGetLastError());
-    return 0;
+    return own ? join_default_out(st) : 0;
 }
 
 int Tree::factor(const double* k_host) {
@@ -266,6 +355,9 @@ int Tree::factor_device(const double2* k) {
     for (int l = L - 1; l >= 0; --l) {
         const Level& V = P.lv[l];
         ml[l] = merge_layout(V, lds_limit);
+        if (!ml[l].ip.fits())
+            return fail("Error: interface of " + std::to_string(V.isz) +
+                        " points is larger than the interface inverses support (LDS panel).");
         arena = std::max(arena, ml[l].total);
         if (!invM[l]) invM[l] = new DevBuf;
         if (need(*invM[l], (size_t)V.nodes * 4 * V.isz * V.isz * 16)) return 1;
@@ -387,7 +479,8 @@ int Tree::factor_device(const double2* k) {
         }
     }
 
-    // leaves: block inverses and RtR maps
+    // leaves: block inverses and RtR maps (k may come from the default stream)
+    if (join_default_in(stream)) return 1;
     HIPCHK(hipMemcpyAsync(kint.p, k, (size_t)nl * P.n_int * sizeof(double2), hipMemcpyDeviceToDevice, stream));
     HIPCHK(hipMemsetAsync(info.p, 0, sizeof(int), stream));
     launch_leaf_blocks(G, kint.as<double2>(), sinv.as<double2>(), info.as<int>(), nl, stream);
@@ -504,96 +597,110 @@ int Tree::make_c64() {
     return 0;
 }
 
-template <typename R> int Tree::ensure_work(SolveWork<R>& w, int nrhs) {
-    if (w.cap >= nrhs) return 0;
-    w.clear();
-    using V = c_t<R>;
+WorkLayout Tree::work_layout(int cap, int esz, bool refine, bool tabs) const {
     const Plan& P = plan;
     const int L = P.L;
-    const std::vector<DevBuf*>& Tt = std::is_same<R, double>::value ? T : T32;
-    const std::vector<DevBuf*>& IMt = std::is_same<R, double>::value ? invM : invM32;
-    const int cap = nrhs < 1 ? 1 : nrhs;
-    w.hV.assign(L + 1, nullptr); w.gF.assign(L + 1, nullptr);
-    w.G.assign(L, nullptr); w.Z.assign(L, nullptr);
+    WorkLayout lay;
+    lay.cap = cap < 1 ? 1 : cap;
+    lay.esz = esz;
+    lay.refine = refine;
+    cap = lay.cap;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { const size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+    lay.hV.assign(L + 1, WorkLayout::NONE); lay.gF.assign(L + 1, WorkLayout::NONE);
     for (int l = 0; l <= L; ++l) {
-        size_t n = (size_t)cap * P.lv[l].nodes * P.lv[l].P * sizeof(V);
-        w.gF[l] = new DevBuf;
-        if (alloc(*w.gF[l], n)) return 1;
-        if (l > 0 || L == 0) { w.hV[l] = new DevBuf; if (alloc(*w.hV[l], n)) return 1; }
+        const size_t n = (size_t)cap * P.lv[l].nodes * P.lv[l].P * esz;
+        lay.gF[l] = take(n);
+        if (l > 0 || L == 0) lay.hV[l] = take(n);
     }
-    for (int l = 0; l < L; ++l) {
-        size_t n = (size_t)cap * P.lv[l].nodes * 2 * P.lv[l].isz * sizeof(V);
-        w.G[l] = new DevBuf; w.Z[l] = new DevBuf;
-        if (alloc(*w.G[l], n) || alloc(*w.Z[l], n)) return 1;
-    }
-    w.d_up_g.assign(L, nullptr); w.d_up_h.assign(L, nullptr); w.d_dn_t.assign(L, nullptr); w.d_dn_g.assign(L, nullptr);
+    lay.G.assign(L, WorkLayout::NONE); lay.Z.assign(L, WorkLayout::NONE);
+    lay.dug.assign(L, WorkLayout::NONE); lay.duh.assign(L, WorkLayout::NONE);
+    lay.ddt.assign(L, WorkLayout::NONE); lay.ddg.assign(L, WorkLayout::NONE);
     size_t wse = 0;
     for (int l = 0; l < L; ++l) {
         const Level& Vl = P.lv[l];
+        const size_t n = (size_t)cap * Vl.nodes * 2 * Vl.isz * esz;
+        lay.G[l] = take(n);
+        lay.Z[l] = take(n);
         wse = std::max(wse, gemm_ws_elems(Vl.nodes, 2 * Vl.isz, cap, 2 * Vl.isz,
                                           solve_ks_target(Vl.nodes, 2 * Vl.isz, 2 * Vl.isz)));
         wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.bn, cap, Vl.isz, solve_ks_target(2 * Vl.nodes, Vl.bn, Vl.isz)));
         wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.isz, cap, Vl.bn, solve_ks_target(2 * Vl.nodes, Vl.isz, Vl.bn)));
     }
-    if (wse && alloc(w.ws, wse * sizeof(V))) return 1;
+    if (wse) lay.ws = take(wse * esz);
     for (int l = 0; l < L; ++l) {
-        const Level& Vl = P.lv[l];
-        const Level& C = P.lv[l + 1];
-        const int nodes = Vl.nodes, i = Vl.isz, b = Vl.bc, bn = Vl.bn, Pp = Vl.P;
-        const size_t ii = (size_t)i * i, tstride = (size_t)b * b;
-        const V* IM = IMt[l]->template as<V>();
-        const V* Tc = Tt[l + 1]->template as<V>();
-        V* hC = w.hV[l + 1]->template as<V>();
-        V* hP = (l > 0) ? w.hV[l]->template as<V>() : nullptr;
-        V* Gl = w.G[l]->template as<V>();
-        V* Zl = w.Z[l]->template as<V>();
-        V* gC = w.gF[l + 1]->template as<V>();
-        const int ldc = C.nodes * b, ldg = nodes * 2 * i, ldp = nodes * Pp;
-        std::vector<GemmDesc> ug, uh, dt, dg;
-        for (int n = 0; n < nodes; ++n) {
-            int par = (l == 0) ? 0 : (n & 1);
-            GemmDesc g = desc0();
-            g.A = IM + n * 4 * ii; g.lda = 2 * i;
-            g.B = hC + (size_t)(2 * n) * b + bn; g.B2 = hC + (size_t)(2 * n + 1) * b + bn; g.ksplit = i; g.ldb = ldc;
-            g.Out = Gl + (size_t)n * 2 * i; g.ldo = ldg;
-            ug.push_back(g);
-            GemmDesc q = desc0();
-            q.A = IM + n * 4 * ii; q.lda = 2 * i;
-            q.B = Zl + (size_t)n * 2 * i; q.ldb = ldg;
-            q.Din = Gl + (size_t)n * 2 * i; q.ldd = ldg; q.br = 1.0;
-            q.Out = gC + (size_t)(2 * n) * b; q.ldo = ldc;
-            q.rmap = d_imap[l]->as<int>();
-            dg.push_back(q);
-            for (int c = 0; c < 2; ++c) {
-                const V* Tcn = Tc + (size_t)(2 * n + c) * tstride;
-                if (l > 0) {
-                    GemmDesc u = desc0();
-                    u.A = Tcn + (size_t)bn * b; u.lda = b;
-                    u.B = Gl + (size_t)n * 2 * i + c * i; u.ldb = ldg;
-                    u.Out = hP + (size_t)n * Pp; u.ldo = ldp;
-                    u.rmap = d_prow[l]->as<int>() + (size_t)(par * 2 + c) * bn;
-                    u.Din = hC + (size_t)(2 * n + c) * b; u.ldd = ldc; u.br = 1.0;
-                    uh.push_back(u);
-                }
-                GemmDesc t = desc0();
-                t.A = Tcn + bn; t.lda = b;
-                t.B = gC + (size_t)(2 * n + c) * b; t.ldb = ldc;
-                t.Out = Zl + (size_t)n * 2 * i + c * i; t.ldo = ldg;
-                dt.push_back(t);
+        const size_t nd = P.lv[l].nodes, gd = sizeof(GemmDesc);
+        lay.dug[l] = take(nd * gd);
+        if (l > 0) lay.duh[l] = take(2 * nd * gd);
+        lay.ddt[l] = take(2 * nd * gd);
+        lay.ddg[l] = take(nd * gd);
+    }
+    if (refine) {
+        const size_t nl = P.n_leaves;
+        lay.rf = take((size_t)cap * nl * P.n_int * 8);
+        lay.rg = take((size_t)cap * nl * P.nb * 8);
+        lay.rj = take((size_t)cap * nl * P.nb * 8);
+        lay.uA = take((size_t)cap * nl * P.m * 8);
+    }
+    if (tabs) lay.tabs = take((size_t)3 * cap * sizeof(void*));
+    lay.total = off;
+    return lay;
+}
+
+template <typename R> SolveLevel Tree::solve_level(const SolveWork<R>& w, int l) const {
+    const Plan& P = plan;
+    const Level& Vl = P.lv[l];
+    const Level& C = P.lv[l + 1];
+    const std::vector<DevBuf*>& Tt = std::is_same<R, double>::value ? T : T32;
+    const std::vector<DevBuf*>& IMt = std::is_same<R, double>::value ? invM : invM32;
+    SolveLevel p;
+    p.IM = (const char*)IMt[l]->p;
+    p.Tc = (const char*)Tt[l + 1]->p;
+    p.hC = (char*)w.hV[l + 1];
+    p.hP = l > 0 ? (char*)w.hV[l] : nullptr;
+    p.Gl = (char*)w.G[l];
+    p.Zl = (char*)w.Z[l];
+    p.gC = (char*)w.gF[l + 1];
+    p.prow = d_prow[l]->as<int>();
+    p.imap = d_imap[l]->as<int>();
+    p.nodes = Vl.nodes; p.i = Vl.isz; p.b = Vl.bc; p.bn = Vl.bn; p.Pp = Vl.P;
+    p.ldc = C.nodes * Vl.bc; p.ldg = Vl.nodes * 2 * Vl.isz; p.ldp = Vl.nodes * Vl.P;
+    p.lvl = l;
+    p.esz = (int)sizeof(c_t<R>);
+    return p;
+}
+
+template <typename R> int Tree::ensure_work(SolveWork<R>& w, int nrhs, bool refine) {
+    if (w.cap >= nrhs && (w.refine || !refine)) return 0;
+    refine = refine || w.refine;
+    w.clear();
+    const Plan& P = plan;
+    const int L = P.L;
+    const WorkLayout lay = work_layout(nrhs, (int)sizeof(c_t<R>), refine, false);
+    if (alloc(w.arena, lay.total)) return 1;
+    w.bind((char*)w.arena.p, lay);
+    // descriptors: built on the host, one upload (they are contiguous in the layout)
+    if (L > 0) {
+        const size_t lo = lay.dug[0];
+        const size_t hi = lay.ddg[L - 1] + (size_t)P.lv[L - 1].nodes * sizeof(GemmDesc);
+        std::vector<GemmDesc> hd((hi - lo + sizeof(GemmDesc) - 1) / sizeof(GemmDesc));
+        char* hbase = (char*)hd.data() - lo;   // host image of the arena's descriptor range
+        for (int l = 0; l < L; ++l) {
+            const SolveLevel p = solve_level(w, l);
+            GemmDesc* ug = (GemmDesc*)(hbase + lay.dug[l]);
+            GemmDesc* dt = (GemmDesc*)(hbase + lay.ddt[l]);
+            GemmDesc* dg = (GemmDesc*)(hbase + lay.ddg[l]);
+            GemmDesc* uh = l > 0 ? (GemmDesc*)(hbase + lay.duh[l]) : nullptr;
+            for (int n = 0; n < p.nodes; ++n) {
+                GemmDesc h[2];
+                solve_descs(p, n, ug + n, h, dt + 2 * n, dg + n);
+                if (uh) { uh[2 * n] = h[0]; uh[2 * n + 1] = h[1]; }
             }
         }
-        for (auto* pr : {&ug, &uh, &dt, &dg}) {
-            DevBuf* db = new DevBuf;
-            w.descs.push_back(db);
-            if (!pr->empty() && upload(*db, *pr)) return 1;
-        }
-        size_t k0 = w.descs.size() - 4;
-        w.d_up_g[l] = w.descs[k0]->template as<GemmDesc>();
-        w.d_up_h[l] = w.descs[k0 + 1]->template as<GemmDesc>();
-        w.d_dn_t[l] = w.descs[k0 + 2]->template as<GemmDesc>();
-        w.d_dn_g[l] = w.descs[k0 + 3]->template as<GemmDesc>();
+        HIPCHK(hipMemcpyAsync((char*)w.arena.p + lo, hd.data(), hi - lo, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
     }
-    w.cap = cap;
+    const int cap = lay.cap;
     const long long key = 2LL * cap + (std::is_same<R, double>::value ? 1 : 0);
     auto it = tile_cache.find(key);
     if (it != tile_cache.end()) {
@@ -653,9 +760,9 @@ template <typename R> int Tree::tune_work(SolveWork<R>& w) {
             int bt = -1;
             for (int t = 0; t < GEMM_TILES; ++t) {
                 const int kt = solve_ks_target(st_.np, st_.M, st_.K);
-                launch_gemm<R>(st_.d, st_.np, st_.M, N, st_.K, stream, w.ws.p, t, kt);
+                launch_gemm<R>(st_.d, st_.np, st_.M, N, st_.K, stream, w.ws, t, kt);
                 (void)hipEventRecord(e0, stream);
-                for (int r = 0; r < 3; ++r) launch_gemm<R>(st_.d, st_.np, st_.M, N, st_.K, stream, w.ws.p, t, kt);
+                for (int r = 0; r < 3; ++r) launch_gemm<R>(st_.d, st_.np, st_.M, N, st_.K, stream, w.ws, t, kt);
                 (void)hipEventRecord(e1, stream);
                 if (hipEventSynchronize(e1) != hipSuccess) { rc = fail("Error: GEMM autotuning failed."); break; }
                 float ms = 0.f;
@@ -672,31 +779,10 @@ template <typename R> int Tree::tune_work(SolveWork<R>& w) {
 }
 
 long long Tree::work_bytes(int cap, int prec) const {
-    const Plan& P = plan;
-    const int L = P.L;
-    const double e = (prec == RS_MIXED) ? 8.0 : 16.0;
-    long long b = 0;
-    for (int l = 0; l <= L; ++l) {
-        const long long n = (long long)((double)cap * P.lv[l].nodes * P.lv[l].P * e);
-        b += (l > 0 || L == 0) ? 2 * n : n;
-    }
-    size_t wse = 0;
-    for (int l = 0; l < L; ++l) {
-        const Level& Vl = P.lv[l];
-        b += 2 * (long long)((double)cap * Vl.nodes * 2 * Vl.isz * e);
-        wse = std::max(wse, gemm_ws_elems(Vl.nodes, 2 * Vl.isz, cap, 2 * Vl.isz,
-                                          solve_ks_target(Vl.nodes, 2 * Vl.isz, 2 * Vl.isz)));
-        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.bn, cap, Vl.isz, solve_ks_target(2 * Vl.nodes, Vl.bn, Vl.isz)));
-        wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.isz, cap, Vl.bn, solve_ks_target(2 * Vl.nodes, Vl.isz, Vl.bn)));
-        const long long nd = (long long)Vl.nodes * (l > 0 ? 6 : 4);
-        b += nd * (long long)sizeof(GemmDesc);
-    }
-    if (wse) b += (long long)(wse * e);
-    const long long nl = P.n_leaves;
-    if (prec == RS_MIXED && opts.n_refine > 0)
-        b += (long long)cap * nl * (P.n_int * 8 + P.nb * 16 + P.m * 8);
-    b += 3LL * cap * (long long)sizeof(void*);
-    return b;
+    // the workspace arena, plus the column tables of a contiguous-array call
+    const bool refine = prec == RS_MIXED && opts.n_refine > 0;
+    return (long long)work_layout(cap, prec == RS_MIXED ? 8 : 16, refine, false).total +
+           3LL * cap * (long long)sizeof(void*);
 }
 
 void Tree::factor_bytes(long long* resident_out, long long* peak_out) const {
@@ -788,14 +874,14 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     const double leaf_fl = (double)nl * nr * (2.0 * G.K * G.s * G.s * 8.0);
     LeafArgs<R> up;
     up.mode = LEAF_UP; up.ncols = nrhs; up.n_leaves = nl; up.sinv = sv;
-    up.f64 = io.f64; up.f32 = io.f32; up.jump = io.jump; up.hout = w.hV[L]->template as<V>();
+    up.f64 = io.f64; up.f32 = io.f32; up.jump = io.jump; up.hout = w.hV[L];
     pbeg(P_LEAF_UP, st);
     launch_leaf_apply<R>(G, up, lds_limit, st);
     pend(sinv_b + nr * nl * (ef * P.n_int + e * P.nb + (io.jump ? 8.0 * P.nb : 0.0)), leaf_fl, st);
     auto gemm = [&](int cat, int lvl, const GemmDesc* d, int np, int M, int N, int K, bool din) {
         const std::vector<int>& tiles = w.tile[cat - P_GEMM_UP_G];   // up_g, up_h, dn_t, dn_g
         pbeg(cat, st);
-        launch_gemm<R>(d, np, M, N, K, st, w.ws.p, lvl < (int)tiles.size() ? tiles[lvl] : -1, solve_ks_target(np, M, K));
+        launch_gemm<R>(d, np, M, N, K, st, w.ws, lvl < (int)tiles.size() ? tiles[lvl] : -1, solve_ks_target(np, M, K));
         pend((double)np * e * ((double)M * K + (double)K * N + (double)M * N * (din ? 2 : 1)),
              8.0 * np * (double)M * N * K, st);
     };
@@ -806,16 +892,16 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     }
     pbeg(P_GATHER, st);
     if (io.g32)
-        launch_root_gather32<V>(w.gF[0]->template as<V>(), io.g32, d_root_leaf.as<int>(), d_root_loc.as<int>(),
+        launch_root_gather32<V>(w.gF[0], io.g32, d_root_leaf.as<int>(), d_root_loc.as<int>(),
                                 P.lv[0].P, nl, P.nb, nrhs, st);
     else
-        launch_root_gather<V>(w.gF[0]->template as<V>(), io.g, d_root_leaf.as<int>(), d_root_loc.as<int>(),
+        launch_root_gather<V>(w.gF[0], io.g, d_root_leaf.as<int>(), d_root_loc.as<int>(),
                               P.lv[0].P, P.nb, nrhs, st);
     pend(nr * P.lv[0].P * ((io.g32 ? 8.0 : 16.0) + e), 0, st);
     for (int l = 0; l < L; ++l) {
         const Level& Vl = P.lv[l];
         pbeg(P_GATHER, st);
-        launch_child_gather<V>(w.gF[l + 1]->template as<V>(), w.gF[l]->template as<V>(), d_prow[l]->as<int>(),
+        launch_child_gather<V>(w.gF[l + 1], w.gF[l], d_prow[l]->as<int>(),
                                Vl.bn, Vl.bc, Vl.P, Vl.nodes, l == 0, nrhs, st);
         pend(nr * 2.0 * Vl.nodes * Vl.bn * 2 * e, 0, st);
         gemm(P_GEMM_DN_T, l, w.d_dn_t[l], 2 * Vl.nodes, Vl.isz, nrhs, Vl.bn, false);
@@ -823,7 +909,7 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     }
     LeafArgs<R> dn;
     dn.mode = LEAF_DOWN; dn.ncols = nrhs; dn.n_leaves = nl; dn.sinv = sv;
-    dn.f64 = io.f64; dn.f32 = io.f32; dn.gin = w.gF[L]->template as<V>();
+    dn.f64 = io.f64; dn.f32 = io.f32; dn.gin = w.gF[L];
     dn.u64 = io.u64; dn.u32 = io.u32; dn.ubase32 = io.ubase32; dn.uacc = io.uacc;
     pbeg(P_LEAF_DOWN, st);
     launch_leaf_apply<R>(G, dn, lds_limit, st);
@@ -832,12 +918,29 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     return 0;
 }
 
+int Tree::slot_enter(Slot& sl, hipStream_t st) {
+    // the slot's tables and workspaces may still be in use by a solve queued on
+    // another stream: this stream waits for it
+    if (sl.last_stream && sl.last_stream != st) HIPCHK(hipStreamWaitEvent(st, sl.done, 0));
+    return 0;
+}
+
+int Tree::slot_leave(Slot& sl, hipStream_t st) {
+    if (!sl.done) HIPCHK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(sl.done, st));
+    sl.last_stream = st;
+    return 0;
+}
+
 int Tree::solve_device(int nrhs, const double2* f, const double2* g, double2* u, int prec, hipStream_t st) {
     if (!factored) return fail("Error: Robin Tree must be factored first.");
     if (nrhs < 1) return fail("Error: nrhs must be positive.");
-    if (!st) st = stream;
+    const bool own = !st;
+    if (own) st = stream;
     // column tables of the contiguous arrays, built on the stream
     Slot& sl = slot(0);
+    if (own && join_default_in(st)) return 1;
+    if (slot_enter(sl, st)) return 1;
     if (sl.tabs_cap < nrhs) {
         if (alloc(sl.tabs, (size_t)3 * nrhs * sizeof(void*))) return 1;
         sl.tabs_cap = nrhs;
@@ -850,7 +953,9 @@ int Tree::solve_device(int nrhs, const double2* f, const double2* g, double2* u,
     launch_col_table(tab + 2 * nrhs, u, nl * P.m, nrhs, st);
     RhsCols c;
     c.f = tab; c.g = tab + nrhs; c.u = tab + 2 * nrhs;
-    return solve_cols(nrhs, c, prec, st, 0, true);
+    if (solve_cols(nrhs, c, prec, st, 0, true)) return 1;
+    if (slot_leave(sl, st)) return 1;
+    return own ? join_default_out(st) : 0;
 }
 
 int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, int slot_id, bool graph) {
@@ -866,18 +971,15 @@ int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, in
     // that moves invalidates the slot's captured solves
     if (prec == RS_MIXED) {
         if (!have32 && make_c64()) return 1;
-        const void* before = sl.w32.ws.p;
-        const int cap = sl.w32.cap;
-        if (ensure_work(sl.w32, nrhs)) return 1;
-        if (opts.n_refine > 0 && ensure_refine(sl.w32)) return 1;
-        if (sl.w32.cap != cap || sl.w32.ws.p != before) sl.drop_graphs();
+        const void* before = sl.w32.arena.p;
+        if (ensure_work(sl.w32, nrhs, opts.n_refine > 0)) return 1;
+        if (sl.w32.arena.p != before) sl.drop_graphs();
     } else {
-        const void* before = sl.w64.ws.p;
-        const int cap = sl.w64.cap;
+        const void* before = sl.w64.arena.p;
         if (ensure_work(sl.w64, nrhs)) return 1;
-        if (sl.w64.cap != cap || sl.w64.ws.p != before) sl.drop_graphs();
+        if (sl.w64.arena.p != before) sl.drop_graphs();
     }
-    if (!graph || !opts.graphs || prof_on) return solve_launches(nrhs, cols, prec, st, sl);
+    if (!graph || !opts.graphs || prof_on) return solve_launches(nrhs, cols, prec, st, sl.w64, sl.w32);
     const int nref = prec == RS_MIXED ? opts.n_refine : 0;
     GraphEnt* hit = nullptr;
     for (GraphEnt& g : sl.graphs)
@@ -885,7 +987,7 @@ int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, in
     if (!hit) {
         // capture this shape's launch sequence once
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        const int rc = solve_launches(nrhs, cols, prec, st, sl);
+        const int rc = solve_launches(nrhs, cols, prec, st, sl.w64, sl.w32);
         hipGraph_t gr = nullptr;
         const hipError_t e = hipStreamEndCapture(st, &gr);
         if (rc || e != hipSuccess) {
@@ -914,9 +1016,9 @@ int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, in
 
 // The launch sequence of one solve (no allocation, no synchronisation): the
 // workspaces of `sl` are ready.
-int Tree::solve_launches(int nrhs, const RhsCols& cols, int prec, hipStream_t st, Slot& sl) {
+int Tree::solve_launches(int nrhs, const RhsCols& cols, int prec, hipStream_t st, SolveWork<double>& w64,
+                         SolveWork<float>& w32) {
     if (prec == RS_MIXED) {
-        SolveWork<float>& w32 = sl.w32;
         PassIO io;
         io.f64 = cols.f; io.g = cols.g;
         if (opts.n_refine <= 0) {
@@ -927,18 +1029,18 @@ int Tree::solve_launches(int nrhs, const RhsCols& cols, int prec, hipStream_t st
         // the rounded iterate, and the correction restores full accuracy.  Then
         // u = u_{i} + K32^-1 (b - K u_{i}) for a fixed number of steps, so that a
         // right-hand side gets the same bits alone or batched.
-        float2* uA = w32.uA.as<float2>();
+        float2* uA = w32.uA;
         io.u32 = uA;
         if (solve_pass(w32, nrhs, io, st)) return 1;
         for (int it = 0; it < opts.n_refine; ++it) {
             if (residual(nrhs, it == 0 ? nullptr : cols.u, it == 0 ? uA : nullptr, cols.f, cols.g,
-                         w32.rf.as<float2>(), w32.rg.as<float2>(), w32.rj.as<float2>(), st))
+                         w32.rf, w32.rg, w32.rj, st))
                 return 1;
             // the correction pass writes u = u_i + K32^-1 r directly (its leaf stage adds
             // the correction to the current iterate as it stores the solution)
             PassIO c;
-            c.f32 = w32.rf.as<float2>(); c.jump = w32.rj.as<float2>();
-            c.g32 = w32.rg.as<float2>();
+            c.f32 = w32.rf; c.jump = w32.rj;
+            c.g32 = w32.rg;
             c.u64 = cols.u;
             if (it == 0) c.ubase32 = uA;
             else c.uacc = 1;
@@ -948,16 +1050,7 @@ int Tree::solve_launches(int nrhs, const RhsCols& cols, int prec, hipStream_t st
     }
     PassIO io;
     io.f64 = cols.f; io.g = cols.g; io.u64 = cols.u;
-    return solve_pass(sl.w64, nrhs, io, st);
-}
-
-int Tree::ensure_refine(SolveWork<float>& w) {
-    if (w.rf.p) return 0;
-    const size_t nl = plan.n_leaves, cap = w.cap;
-    if (alloc(w.rf, cap * nl * plan.n_int * 8) || alloc(w.rg, cap * nl * plan.nb * 8) ||
-        alloc(w.rj, cap * nl * plan.nb * 8) || alloc(w.uA, cap * nl * plan.m * 8))
-        return 1;
-    return 0;
+    return solve_pass(w64, nrhs, io, st);
 }
 
 int Tree::reserve_slot(int slot_id, int cap) {
@@ -965,8 +1058,7 @@ int Tree::reserve_slot(int slot_id, int cap) {
     Slot& sl = slot(slot_id);
     if (opts.precision == RS_MIXED) {
         if (!have32 && make_c64()) return 1;
-        if (ensure_work(sl.w32, cap)) return 1;
-        return opts.n_refine > 0 ? ensure_refine(sl.w32) : 0;
+        return ensure_work(sl.w32, cap, opts.n_refine > 0);
     }
     return ensure_work(sl.w64, cap);
 }
@@ -995,6 +1087,7 @@ int Tree::residual_norms(int nrhs, const double2* f, const double2* g, const dou
         alloc(tabs, (size_t)3 * nrhs * sizeof(void*)) || alloc(nrm, (size_t)2 * nrhs * sizeof(double)))
         return 1;
     double2** tab = tabs.as<double2*>();
+    if (join_default_in(stream)) return 1;
     launch_col_table(tab, const_cast<double2*>(f), nl * P.n_int, nrhs, stream);
     launch_col_table(tab + nrhs, const_cast<double2*>(g), nl * P.nb, nrhs, stream);
     launch_col_table(tab + 2 * nrhs, const_cast<double2*>(u), nl * P.m, nrhs, stream);
@@ -1060,9 +1153,9 @@ int Tree::node_rtr(int lvl, int idx, double* out, int* Pout) const {
     return 0;
 }
 
-template int Tree::ensure_work<double>(SolveWork<double>&, int);
+template int Tree::ensure_work<double>(SolveWork<double>&, int, bool);
 template int Tree::tune_work<double>(SolveWork<double>&);
 template int Tree::tune_work<float>(SolveWork<float>&);
-template int Tree::ensure_work<float>(SolveWork<float>&, int);
+template int Tree::ensure_work<float>(SolveWork<float>&, int, bool);
 
 }  // namespace rs

@@ -31,18 +31,47 @@ struct DevBuf {
     template <typename T> T* as() const { return (T*)p; }
 };
 
-// Solve workspace for one precision and a capacity of RHS columns.
+// Byte offsets of a solve workspace of `cap` columns inside one arena (256-byte
+// aligned): per-level vectors, the K-split GEMM partials, the refinement buffers
+// of the mixed path, optional column tables, and the products' descriptors.
+struct WorkLayout {
+    static constexpr size_t NONE = ~(size_t)0;
+    int cap = 0, esz = 0;
+    bool refine = false;
+    std::vector<size_t> hV, gF, G, Z;          // per level (NONE where absent)
+    std::vector<size_t> dug, duh, ddt, ddg;    // descriptor arrays per level
+    size_t ws = NONE, rf = NONE, rg = NONE, rj = NONE, uA = NONE, tabs = NONE;
+    size_t total = 0;
+};
+
+// Solve workspace for one precision and a capacity of RHS columns.  Buffers are
+// raw device pointers: into the workspace's own arena, or into a caller-side
+// arena (the allocation-free entry point).
 template <typename R> struct SolveWork {
     using V = c_t<R>;
     int cap = 0;
-    std::vector<DevBuf*> hV, gF, G, Z;   // per level
-    DevBuf ws;                           // K-split GEMM workspace
-    DevBuf rf, rg, rj, uA;               // refinement: residuals and the first iterate (complex64)
-    std::vector<DevBuf*> descs;          // per (level, kind)
+    bool refine = false;
+    std::vector<V*> hV, gF, G, Z;        // per level
+    void* ws = nullptr;                  // K-split GEMM workspace
+    float2 *rf = nullptr, *rg = nullptr, *rj = nullptr, *uA = nullptr;   // refinement (complex64)
     std::vector<GemmDesc*> d_up_g, d_up_h, d_dn_t, d_dn_g;
     std::vector<int> tile[4];            // per product (up_g, up_h, dn_t, dn_g) and level: GEMM tile, -1 = by shape
+    DevBuf arena;                        // the owned memory (empty for a caller-side arena)
     ~SolveWork() { clear(); }
     void clear();
+    // point every buffer into base according to the layout
+    void bind(char* base, const WorkLayout& lay);
+};
+
+// One level's solve products, for building their descriptors (pointers as
+// bytes; esz = bytes per complex element of the pass).
+struct SolveLevel {
+    const char* IM;     // invM of the level's nodes, [nodes][2i][2i]
+    const char* Tc;     // children's RtR maps, [2 nodes][b][b]
+    char *hC, *hP;      // outgoing data of the children / of the nodes (NULL at the root)
+    char *Gl, *Zl, *gC; // interface data, down-pass products, children's incoming data
+    const int *prow, *imap;
+    int nodes, i, b, bn, Pp, ldc, ldg, ldp, lvl, esz;
 };
 
 enum ProfCat { P_LEAF_UP = 0, P_LEAF_DOWN, P_GEMM_UP_G, P_GEMM_UP_H, P_GEMM_DN_T, P_GEMM_DN_G, P_GATHER,
@@ -94,6 +123,9 @@ struct Slot {
     int tabs_cap = 0;
     std::vector<GraphEnt> graphs;  // at most kMaxGraphs, least recently used replaced
     unsigned long long tick = 0;
+    // completion of the last solve queued on this slot, and its stream
+    hipEvent_t done = nullptr;
+    hipStream_t last_stream = nullptr;
     static constexpr int kMaxGraphs = 8;
     void drop_graphs() {
         for (GraphEnt& g : graphs)
@@ -101,7 +133,7 @@ struct Slot {
         graphs.clear();
     }
     void clear() { drop_graphs(); w64.clear(); w32.clear(); tabs.release(); tabs_cap = 0; }
-    ~Slot() { drop_graphs(); }
+    ~Slot() { drop_graphs(); if (done) (void)hipEventDestroy(done); }
 };
 
 struct Tree {
@@ -111,6 +143,7 @@ struct Tree {
     double f_time = 0, s_time = 0;
     std::string err;
     hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;   // ordering against the default stream
     size_t lds_limit = 65536;
     LeafGeom G{};
     long long dev_bytes = 0;
@@ -165,7 +198,11 @@ struct Tree {
     int factor(const double* k_int_host);
     int factor_device(const double2* k_int_dev);
     int make_c64();
-    template <typename R> int ensure_work(SolveWork<R>& w, int nrhs);
+    template <typename R> int ensure_work(SolveWork<R>& w, int nrhs, bool refine = false);
+    // the arena layout of a workspace of cap columns (esz 8: complex64 passes,
+    // 16: complex128), with refinement buffers and/or column tables
+    WorkLayout work_layout(int cap, int esz, bool refine, bool tabs) const;
+    template <typename R> SolveLevel solve_level(const SolveWork<R>& w, int l) const;
     // time the tile shapes of every product of the workspace (at its capacity) and keep the fastest
     template <typename R> int tune_work(SolveWork<R>& w);
     // GEMM tile choices by (capacity, precision): they depend on the plan's
@@ -184,9 +221,14 @@ struct Tree {
     int residual(int nrhs, double2* const* u64, const float2* u32, const double2* const* f,
                  const double2* const* g, float2* rf, float2* rg, float2* rj, hipStream_t st);
     int solve_device(int nrhs, const double2* f, const double2* g, double2* u, int prec, hipStream_t st);
+    // order a solve on stream st after the slot's previous solve (queued on any stream)
+    int slot_enter(Slot& sl, hipStream_t st);
+    int join_default_in(hipStream_t st);
+    int join_default_out(hipStream_t st);
+    int slot_leave(Slot& sl, hipStream_t st);
     int solve_cols(int nrhs, const RhsCols& c, int prec, hipStream_t st, int slot_id, bool graph = false);
-    int solve_launches(int nrhs, const RhsCols& c, int prec, hipStream_t st, Slot& sl);
-    int ensure_refine(SolveWork<float>& w);
+    int solve_launches(int nrhs, const RhsCols& c, int prec, hipStream_t st, SolveWork<double>& w64,
+                       SolveWork<float>& w32);
     // allocate slot slot_id's workspace for cap columns at the tree's precision
     int reserve_slot(int slot_id, int cap);
     // device bytes: a solve workspace of cap columns at precision prec (with the

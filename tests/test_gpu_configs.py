@@ -1,0 +1,166 @@
+"""The benchmark configurations at full size, on the GPU, through the C ABI.
+
+cfg2 (2D 512^2, L=10, kb=40, seed 2, 64 Gaussian sources) and cfg3 (2D 1024^2,
+L=12, kb=80, seed 3, 256 sources): the plain FP64 GPU path and the mixed path
+(complex64 passes, one FP64 refinement step) against the CPU oracle on the
+first right-hand sides of the seeded batch -- rel. l2 <= 1e-10 (FP64) and
+<= 1e-8 (mixed) per RHS -- and the FD residual of the global system
+(robin_tree_module.f90:1045-1113 restated as one sparse system) on every RHS of
+the batch.  The inputs are sampled on the GPU (robinsolver_amd.workloads) and
+the oracle gets the same arrays.
+
+cfg4 (3D 64^3, L=9, kb=8, 16 sources): the oracle would need about 53 TFLOP,
+so the full size is checked through size-independent properties: the FD
+residual of the FP64 solution, the mixed solution against the FP64 one, and
+bitwise batch invariance.  A 3D 32^3 tree (interfaces up to 1024 points, 8^3
+leaves) is compared with the oracle directly.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import robinsolver_amd as rsa
+from robinsolver_amd import workloads as wl
+from oracle.pyoracle import OracleTree
+
+pytestmark = pytest.mark.gpu
+
+UNIT = [[0, 1], [0, 1], [0, 1]]
+TOL64 = 1e-10       # FP64 GPU path against the oracle
+TOLMIX = 1e-8       # mixed path (after refinement) against the oracle
+RES64 = 1e-11       # FD residual of the FP64 path, relative to ||(f, g)||
+RESMIX = 1e-8       # FD residual of the mixed path
+
+
+def _case(tag):
+    print("case:", tag, flush=True)
+    sys.stdout.flush()
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _rel_cols(u, ref):
+    """rel. l2 per right-hand side over all leaf vectors (get_solution order)."""
+    n = u.shape[0]
+    a, b = u.reshape(n, -1), ref.reshape(n, -1)
+    return np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+
+
+def _setup(cfg, prec, n_refine=1):
+    torch = _torch()
+    kb = cfg["kb"]
+    t = rsa.GpuTree(cfg["d"], cfg["L"], UNIT, 1.0 / cfg["grid"], -kb, precision=prec, n_refine=n_refine)
+    xyz, face = wl.device_points(torch, t)
+    pint = xyz[:, t.nb:, :]
+    k = wl.potential_dev(torch, pint, kb, cfg["seed"], nmodes=cfg["nmodes"], d=cfg["d"],
+                         absorb=cfg["absorb"]).contiguous()
+    return t, xyz, face, k
+
+
+def _sources(t, xyz, cfg, nrhs):
+    torch = _torch()
+    f = wl.gaussian_sources(torch, xyz[:, t.nb:, :], nrhs, cfg["seed"], sigma=cfg["sigma"], d=cfg["d"])
+    g = torch.zeros((nrhs, t.n_leaves, t.nb), dtype=torch.complex128, device="cuda:0")
+    return f.contiguous(), g
+
+
+def _solve(t, f, g, prec):
+    torch = _torch()
+    n = f.shape[0]
+    u = torch.zeros((n, t.n_leaves, t.m), dtype=torch.complex128, device="cuda:0")
+    torch.cuda.synchronize()
+    t.solve_device(n, f.data_ptr(), g.data_ptr(), u.data_ptr(), precision=prec)
+    torch.cuda.synchronize()
+    return u
+
+
+def _residuals(t, f, g, u):
+    return np.array(t.residual_norms(f.shape[0], f.data_ptr(), g.data_ptr(), u.data_ptr()))
+
+
+@pytest.mark.parametrize("cfg,n_oracle", [(wl.CFG2, 4), (wl.CFG3, 2)], ids=["cfg2_512", "cfg3_1024"])
+def test_2d_config_against_oracle(cfg, n_oracle, gpu):
+    torch = _torch()
+    _case(("config", cfg["grid"], cfg["L"], cfg["kb"], cfg["nrhs"]))
+    t64, xyz, face, k = _setup(cfg, rsa.RS_FP64)
+    f, g = _sources(t64, xyz, cfg, cfg["nrhs"])
+    t64.factor_device(k.data_ptr())
+    u64 = _solve(t64, f, g, rsa.RS_FP64)
+    tm = rsa.GpuTree(cfg["d"], cfg["L"], UNIT, 1.0 / cfg["grid"], -cfg["kb"], precision=rsa.RS_MIXED, n_refine=1)
+    tm.factor_device(k.data_ptr())
+    um = _solve(tm, f, g, rsa.RS_MIXED)
+    res64 = _residuals(t64, f, g, u64)
+    resm = _residuals(tm, f, g, um)
+    print("  FD residual: FP64 max %.2e, mixed max %.2e" % (res64.max(), resm.max()), flush=True)
+    assert res64.max() <= RES64, res64.max()
+    assert resm.max() <= RESMIX, resm.max()
+    # the oracle on the same inputs (host copies of the device arrays)
+    _case(("oracle", cfg["grid"], cfg["L"]))
+    o = OracleTree(cfg["d"], cfg["L"], UNIT, 1.0 / cfg["grid"], -cfg["kb"])
+    o.factor(k.cpu().numpy())
+    fh = f[:n_oracle].cpu().numpy()
+    gh = g[:n_oracle].cpu().numpy()
+    uo = np.stack([o.solve(fh[r], gh[r]) for r in range(n_oracle)])
+    o.close()
+    r64 = _rel_cols(u64[:n_oracle].cpu().numpy(), uo)
+    rm = _rel_cols(um[:n_oracle].cpu().numpy(), uo)
+    print("  rel l2 vs oracle: FP64 %s, mixed %s" % (np.array2string(r64, precision=2),
+                                                      np.array2string(rm, precision=2)), flush=True)
+    assert r64.max() <= TOL64, r64
+    assert rm.max() <= TOLMIX, rm
+    del u64, um, f, g
+    torch.cuda.empty_cache()
+
+
+def test_3d_64_full_size(gpu):
+    """cfg4 at full size: FP64 FD residual, mixed against FP64, and a batch
+    whose columns give the same bits as the same columns solved in smaller
+    batches (both precisions)."""
+    torch = _torch()
+    cfg = wl.CFG4
+    _case(("config 3D", cfg["grid"], cfg["L"], cfg["kb"], cfg["nrhs"]))
+    t64, xyz, face, k = _setup(cfg, rsa.RS_FP64)
+    f, g = _sources(t64, xyz, cfg, cfg["nrhs"])
+    t64.factor_device(k.data_ptr())
+    u64 = _solve(t64, f, g, rsa.RS_FP64)
+    res64 = _residuals(t64, f, g, u64)
+    tm = rsa.GpuTree(3, cfg["L"], UNIT, 1.0 / cfg["grid"], -cfg["kb"], precision=rsa.RS_MIXED, n_refine=1)
+    tm.factor_device(k.data_ptr())
+    um = _solve(tm, f, g, rsa.RS_MIXED)
+    rel = _rel_cols(um.cpu().numpy(), u64.cpu().numpy())
+    print("  FD residual FP64 max %.2e; mixed vs FP64 max %.2e" % (res64.max(), rel.max()), flush=True)
+    assert res64.max() <= RES64, res64.max()
+    assert rel.max() <= TOLMIX, rel
+    for tree, prec, full in ((t64, rsa.RS_FP64, u64), (tm, rsa.RS_MIXED, um)):
+        for a, b in ((3, 7), (9, 10)):
+            part = _solve(tree, f[a:b].contiguous(), g[a:b].contiguous(), prec)
+            assert torch.equal(part, full[a:b]), (prec, a, b)
+
+
+def test_3d_32_against_oracle(gpu):
+    """3D 32^3, L=6 (8^3 leaves, interfaces of 64..1024 points): FP64 and mixed
+    against the oracle."""
+    torch = _torch()
+    cfg = dict(wl.CFG4, grid=32, L=6, nrhs=3)
+    _case(("3D 32^3 vs oracle",))
+    t64, xyz, face, k = _setup(cfg, rsa.RS_FP64)
+    f, g = _sources(t64, xyz, cfg, cfg["nrhs"])
+    t64.factor_device(k.data_ptr())
+    u64 = _solve(t64, f, g, rsa.RS_FP64)
+    tm = rsa.GpuTree(3, cfg["L"], UNIT, 1.0 / cfg["grid"], -cfg["kb"], precision=rsa.RS_MIXED, n_refine=1)
+    tm.factor_device(k.data_ptr())
+    um = _solve(tm, f, g, rsa.RS_MIXED)
+    o = OracleTree(3, cfg["L"], UNIT, 1.0 / cfg["grid"], -cfg["kb"])
+    o.factor(k.cpu().numpy())
+    fh, gh = f.cpu().numpy(), g.cpu().numpy()
+    uo = np.stack([o.solve(fh[r], gh[r]) for r in range(cfg["nrhs"])])
+    o.close()
+    r64 = _rel_cols(u64.cpu().numpy(), uo)
+    rm = _rel_cols(um.cpu().numpy(), uo)
+    print("  rel l2 vs oracle: FP64 %s, mixed %s" % (r64, rm), flush=True)
+    assert r64.max() <= TOL64, r64
+    assert rm.max() <= TOLMIX, rm
