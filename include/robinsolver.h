@@ -135,6 +135,24 @@ int rs_get_solution(const rs_tree *t, int rhs, double *u);
 int rs_solve_device(rs_tree *t, int nrhs, const void *f_dev, const void *g_dev, void *u_dev,
                     int precision, void *stream);
 
+/* Allocation-free variant of rs_solve_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, ordering and results (bit for bit) as rs_solve_device, but
+ * the pass's whole workspace -- level vectors, K-split partial products,
+ * refinement buffers, column tables and product descriptors -- lives in one
+ * fixed device array of RS_NOALLOC_SCRATCH_MIB MiB that every GPU the library
+ * runs on holds until the process ends.  A solve needing more fails with a
+ * message that states the bytes needed (fewer right-hand sides per call fit:
+ * about 60 MB per RHS at 1024^2 in RS_MIXED).  The tree must be factored, and
+ * for RS_MIXED hold its complex64 data (factored with RS_MIXED).  Every call on
+ * one GPU uses that GPU's one array: calls that overlap on the device (other
+ * streams, trees or threads) race; calls queued on one stream are ordered.  No
+ * hipMalloc, hipFree, graph or profiling event is used. */
+#define RS_NOALLOC_SCRATCH_MIB 256
+int rs_solve_device_noalloc(rs_tree *t, int nrhs, const void *f_dev, const void *g_dev, void *u_dev,
+                            int precision, void *stream);
+
 /* Number of FP64 refinement steps of the mixed path for later solves
  * (rs_opts.n_refine; no reference counterpart). */
 int rs_set_refine(rs_tree *t, int n_refine);

@@ -1,5 +1,6 @@
 // capi.cpp -- the extern "C" boundary of include/robinsolver.h.
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -9,6 +10,34 @@
 namespace {
 std::string g_err;
 thread_local std::string g_tls;
+
+// Device scratch of rs_solve_device_noalloc: one fixed array per GPU the
+// library runs on, held until the process ends.  In host code `scratch` is only
+// a stand-in for the symbol; its device address (one per device) comes from
+// hipGetSymbolAddress and is the only form passed to kernels or copies.
+constexpr size_t kNoallocScratchBytes = (size_t)RS_NOALLOC_SCRATCH_MIB << 20;
+__device__ __align__(256) char scratch[kNoallocScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// the address of the current device's scratch array (looked up once per device)
+char* noalloc_scratch(std::string& err) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) { err = "Error: no HIP device."; return nullptr; }
+    if (dev >= kMaxDevices) { err = "Error: device index beyond the scratch table."; return nullptr; }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("HIP error: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = (char*)p;
+    }
+    return g_scratch_ptr[dev];
+}
 }  // namespace
 
 extern "C" {
@@ -114,6 +143,15 @@ int rs_solve_device(rs_tree* t, int nrhs, const void* f_dev, const void* g_dev, 
     if (!t || !f_dev || !g_dev || !u_dev) { g_err = "Error: null argument."; return 1; }
     return t->t.solve_device(nrhs, (const double2*)f_dev, (const double2*)g_dev, (double2*)u_dev, precision,
                              (hipStream_t)stream);
+}
+
+int rs_solve_device_noalloc(rs_tree* t, int nrhs, const void* f_dev, const void* g_dev, void* u_dev, int precision,
+                            void* stream) {
+    if (!t || !f_dev || !g_dev || !u_dev) { g_err = "Error: null argument."; return 1; }
+    char* sc = noalloc_scratch(t->t.err);
+    if (!sc) return 1;
+    return t->t.solve_device_noalloc(nrhs, (const double2*)f_dev, (const double2*)g_dev, (double2*)u_dev, precision,
+                                     (hipStream_t)stream, sc, kNoallocScratchBytes);
 }
 
 int rs_set_refine(rs_tree* t, int n_refine) {

@@ -79,7 +79,9 @@ template <typename R> void SolveWork<R>::bind(char* base, const WorkLayout& lay)
     refine = lay.refine;
 }
 
-// The descriptors of node n's four solve products at one level:
+// The descriptors of node n's four solve products at one level (written by the
+// host for a workspace of its own, and by solve_desc_kernel for one in the
+// device scratch array):
 //   up_g  g_I = invM [h1_I; h2_I]                          (B from the two children)
 //   up_h  h_parent = P (h_N + T_NI g_I), per child c        (rows scattered by prow)
 //   dn_t  Z_c = T_IN g_N of child c
@@ -121,6 +123,14 @@ __host__ __device__ inline void solve_descs(const SolveLevel& p, int n, GemmDesc
         t.Out = p.Zl + ((size_t)n * 2 * i + (size_t)c * i) * e; t.ldo = p.ldg;
         dt[c] = t;
     }
+}
+
+__global__ void solve_desc_kernel(SolveLevel p, GemmDesc* ug, GemmDesc* uh, GemmDesc* dt, GemmDesc* dg) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.nodes) return;
+    GemmDesc h[2];
+    solve_descs(p, n, ug + n, h, dt + 2 * n, dg + n);
+    if (p.lvl > 0) { uh[2 * n] = h[0]; uh[2 * n + 1] = h[1]; }
 }
 
 Tree::Tree() { rs_opts_default(&opts); }
@@ -955,6 +965,66 @@ int Tree::solve_device(int nrhs, const double2* f, const double2* g, double2* u,
     c.f = tab; c.g = tab + nrhs; c.u = tab + 2 * nrhs;
     if (solve_cols(nrhs, c, prec, st, 0, true)) return 1;
     if (slot_leave(sl, st)) return 1;
+    return own ? join_default_out(st) : 0;
+}
+
+// rs_solve_device with the whole pass workspace inside a caller-side device
+// arena (the library's fixed scratch array): it allocates nothing.  The
+// workspace is cleared first (the arena keeps whatever the previous call left),
+// the column tables and the products' descriptors are written on the device,
+// and the launch sequence is the one rs_solve_device runs, so the results are
+// the same bits.
+int Tree::solve_device_noalloc(int nrhs, const double2* f, const double2* g, double2* u, int prec, hipStream_t st,
+                               char* scratch, size_t scratch_bytes) {
+    if (!factored) return fail("Error: Robin Tree must be factored first.");
+    if (nrhs < 1) return fail("Error: nrhs must be positive.");
+    if (!f || !g || !u || !scratch) return fail("Error: null argument.");
+    if (prec < 0) prec = opts.precision;
+    if (prec != RS_FP64 && prec != RS_MIXED) return fail("Error: precision must be RS_FP64 or RS_MIXED.");
+    if (prec == RS_FP64 && merge_c64)
+        return fail("Error: an FP64 solve needs a tree factored in FP64 (this tree's merge products are complex64).");
+    if (prec == RS_MIXED && !have32)
+        return fail("Error: the allocation-free solve needs the tree's complex64 solve data: factor it with "
+                    "RS_MIXED (or run one RS_MIXED rs_solve_device first).");
+    const bool mixed = prec == RS_MIXED;
+    const WorkLayout lay = work_layout(nrhs, mixed ? 8 : 16, mixed && opts.n_refine > 0, true);
+    if (lay.total > scratch_bytes)
+        return fail("Error: this solve needs " + std::to_string(lay.total) + " bytes of device scratch; the "
+                    "allocation-free entry point has " + std::to_string(scratch_bytes) +
+                    " (pass fewer right-hand sides per call, or use rs_solve_device).");
+    const bool own = !st;
+    if (own) st = stream;
+    if (own && join_default_in(st)) return 1;
+    launch_zero((double2*)scratch, lay.total / sizeof(double2), st);
+    const Plan& P = plan;
+    const size_t nl = P.n_leaves;
+    double2** tab = (double2**)(scratch + lay.tabs);
+    launch_col_table(tab, const_cast<double2*>(f), nl * P.n_int, nrhs, st);
+    launch_col_table(tab + nrhs, const_cast<double2*>(g), nl * P.nb, nrhs, st);
+    launch_col_table(tab + 2 * nrhs, u, nl * P.m, nrhs, st);
+    RhsCols cols;
+    cols.f = tab; cols.g = tab + nrhs; cols.u = tab + 2 * nrhs;
+    SolveWork<double> w64;
+    SolveWork<float> w32;
+    auto prepare = [&](auto& w) {
+        w.bind(scratch, lay);
+        for (int l = 0; l < P.L; ++l) {
+            const SolveLevel p = solve_level(w, l);
+            hipLaunchKernelGGL(solve_desc_kernel, dim3((p.nodes + 255) / 256), dim3(256), 0, st, p, w.d_up_g[l],
+                               w.d_up_h[l], w.d_dn_t[l], w.d_dn_g[l]);
+        }
+        // tile choices tuned for this width earlier, else the shape rule (the bits do not depend on them)
+        auto it = tile_cache.find(2LL * lay.cap + (mixed ? 0 : 1));
+        for (int c = 0; c < 4; ++c) w.tile[c] = it != tile_cache.end() ? it->second[c] : std::vector<int>(P.L, -1);
+    };
+    if (mixed) prepare(w32);
+    else prepare(w64);
+    HIPCHK(hipGetLastError());
+    const bool was = prof_on;   // (the profiling events would be created on demand)
+    prof_on = false;
+    const int rc = solve_launches(nrhs, cols, prec, st, w64, w32);
+    prof_on = was;
+    if (rc) return 1;
     return own ? join_default_out(st) : 0;
 }
 

@@ -229,6 +229,10 @@ struct Tree {
     int solve_cols(int nrhs, const RhsCols& c, int prec, hipStream_t st, int slot_id, bool graph = false);
     int solve_launches(int nrhs, const RhsCols& c, int prec, hipStream_t st, SolveWork<double>& w64,
                        SolveWork<float>& w32);
+    // rs_solve_device with its workspace in the library's fixed device scratch
+    // array: no device allocation (see capi)
+    int solve_device_noalloc(int nrhs, const double2* f, const double2* g, double2* u, int prec, hipStream_t st,
+                             char* scratch, size_t scratch_bytes);
     // allocate slot slot_id's workspace for cap columns at the tree's precision
     int reserve_slot(int slot_id, int cap);
     // device bytes: a solve workspace of cap columns at precision prec (with the
