@@ -375,14 +375,19 @@ class robin_tree:
         t0 = time.perf_counter()
         t, pts, face = self._t, self._pts, self._face
         f = np.array([[rhs.f(pts[q, p]) for p in range(t.nb, t.m)] for q in range(t.n_leaves)], np.complex128)
+        # g_face is called at the physical boundary points only, as the reference
+        # does (apply_bc, robin_tree_module.f90:1186-1237)
         g = np.zeros((t.n_leaves, t.nb), np.complex128)
+        dom = np.asarray(self.ell_op.domain, np.float64)
         for p in range(t.nb):
             fc = int(face[p])
             gf = rhs.g[fc - 1]
             if gf is None:
                 continue
+            ax, side = (fc - 1) // 2, (fc - 1) % 2
             for q in range(t.n_leaves):
-                g[q, p] = gf(_packed(pts[q, p], fc, t.d))
+                if pts[q, p, ax] == dom[side, ax]:
+                    g[q, p] = gf(_packed(pts[q, p], fc, t.d))
         self._u = t.solve(f, g)
         self.s_time = time.perf_counter() - t0
 

@@ -1,7 +1,9 @@
 ! Driver written against robinsolver's public Fortran API (the usage of
 ! example.f90:1-89 and README.md:36-126): 2D unit square, k = kb = 1,
 ! manufactured solution cos(x/sqrt 2) cos(y/sqrt 2) with its impedance data,
-! f = 0.  Prints the relative l2 error of the leaf solutions.
+! f = 0.  Prints the relative l2 error of the leaf solutions; with a fourth
+! argument, writes every leaf's solution vector (DFS leaf order, complex128,
+! stream access) to that file.
 program driver_2d
     use derived_type_module
     use local_solver_module
@@ -16,7 +18,8 @@ program driver_2d
     type(leaf_solution), dimension(:), allocatable :: u_out, zero_sol
     real(dp), dimension(:), allocatable :: errs, nrm
     character(len=32) :: arg
-    integer :: n, L, q
+    character(len=512) :: outpath
+    integer :: n, L, q, unit_out
     real(dp) :: num, den
 
     n = 64
@@ -64,6 +67,14 @@ program driver_2d
     num = sqrt(sum(errs**2))
     den = sqrt(sum(nrm**2))
     print '(A,ES12.4)', 'RELL2 ', num / den
+    if (command_argument_count() >= 4) then
+        call get_command_argument(4, outpath)
+        open(newunit=unit_out, file=trim(outpath), access='stream', form='unformatted', status='replace')
+        do q = 1, size(u_out)
+            write(unit_out) u_out(q)%u
+        end do
+        close(unit_out)
+    end if
     call tree%inspect()
 
 contains
