@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 from robinsolver_amd.workloads import (MIX_TREES, device_points, gaussian_sources, plane_wave_g_dev,  # noqa: E402
-                                      potential_dev)
+                                      potential_dev, rel_l2_cols, same_bits)
 
 PEAK = {"hbm_GBs": 8000.0, "fp64_TF": 78.6, "fp32_TF": 157.3}
 
@@ -141,10 +141,9 @@ def serve_cfg3(torch, rsa, t, f, g, u32, u64, args, sz_f, sz_g):
                 us = ring.u[j["ucol"]:j["ucol"] + j["nrhs"]]
                 ub = u32[j["off"]:j["off"] + j["nrhs"]]
                 tot += 1
-                same += int(torch.equal(us, ub))
+                same += int(same_bits(us, ub))
                 ref = u64[j["off"]:j["off"] + j["nrhs"]]
-                d = torch.linalg.vector_norm((us - ref).reshape(j["nrhs"], -1), dim=1)
-                worst = max(worst, float((d / torch.linalg.vector_norm(ref.reshape(j["nrhs"], -1), dim=1)).max()))
+                worst = max(worst, float(rel_l2_cols(us, ref).max()))
             out[name] = {"requests": len(js), "rhs": rhs, "wall_s": wall, "solves_per_s": rhs / wall,
                          "offered_requests_per_s": rate, "lat_p50_ms": 1e3 * st["lat_p50_s"],
                          "lat_p99_ms": 1e3 * st["lat_p99_s"], "lat_max_ms": 1e3 * st["lat_max_s"],
@@ -191,8 +190,7 @@ def serve_mix(args):
         t.solve_device(4, fpool.data_ptr(), gpool.data_ptr(), ua.data_ptr())
         t64.solve_device(4, fpool.data_ptr(), gpool.data_ptr(), ub.data_ptr())
         torch.cuda.synchronize()
-        rel = float((torch.linalg.vector_norm((ua - ub).reshape(4, -1), dim=1) /
-                     torch.linalg.vector_norm(ub.reshape(4, -1), dim=1)).max())
+        rel = float(rel_l2_cols(ua, ub).max())
         if rel > 1e-8:
             t.set_refine(2)
         del t64, ua, ub
@@ -439,9 +437,7 @@ def main():
     # fewest refinement steps that bring every RHS to rel. l2 <= 1e-8 against
     # the FP64 GPU solution (checked on the whole batch)
     def max_rel(ua, ub):
-        d = (ua - ub).reshape(args.nrhs, -1)
-        r = torch.linalg.vector_norm(d, dim=1) / torch.linalg.vector_norm(ub.reshape(args.nrhs, -1), dim=1)
-        return r.cpu().numpy()
+        return rel_l2_cols(ua, ub)
 
     refine_err = {}
     n_ref = 2
@@ -499,9 +495,7 @@ def main():
     log("kernel time per step (events): %.2f ms" % prof_ms)
 
     # accuracy of every mixed RHS against the FP64 GPU solution
-    diff = (u32 - u64).reshape(args.nrhs, -1)
-    rel = (torch.linalg.vector_norm(diff, dim=1) / torch.linalg.vector_norm(u64.reshape(args.nrhs, -1), dim=1))
-    rel = rel.cpu().numpy()
+    rel = rel_l2_cols(u32, u64)
 
     # roofline of the dominant kernel category
     cat = max(prof, key=lambda c: prof[c][0])

@@ -63,10 +63,24 @@ typedef struct rs_opts {
                              products at the top k levels (root first) and
                              runs the levels below in complex64; -1 (or k >= L)
                              keeps the whole factorization FP64               */
+    int retain;           /* what a factored tree keeps beyond its solve data
+                             (RS_RETAIN_* flags, 0 = the solve data only)     */
 } rs_opts;
 
+/* rs_opts.retain.  The solve data a tree keeps after rs_factor, as the
+ * reference keeps D/S, nu, the children's T_NI and T_IN and the leaf factors
+ * and frees T_II and T_NN (robin_tree_module.f90:472-475, 617-645): the leaf
+ * block inverses, inv(M) of every merge and the children's T_NI / T_IN, in the
+ * tree's precision (complex64 for RS_MIXED, complex128 for RS_FP64). */
+enum {
+    RS_RETAIN_MAPS = 1,     /* the full RtR map of every node (rs_node_rtr)            */
+    RS_RETAIN_FP64 = 2,     /* RS_MIXED: also the complex128 solve data (FP64 solves) */
+    RS_RETAIN_SCRATCH = 4   /* the factorization's working memory, so that a refactor
+                               of the same plan allocates nothing                    */
+};
+
 /* Defaults: precision RS_FP64, skip_root_rtr 1, n_refine 2, verbose 0, no cap,
- * autotune 1, graphs 1, merge_fp64_levels -1. */
+ * autotune 1, graphs 1, merge_fp64_levels -1, retain 0. */
 void rs_opts_default(rs_opts *o);
 
 /* Create / destroy a tree handle (type robin_tree, robin_tree_module.f90:55-77;

@@ -26,6 +26,7 @@ _lib = None
 RS_FP64 = 0
 RS_MIXED = 1
 RS_MIXED_MERGE = 2   # rs_check_gemm: complex64 products of complex128 data
+RS_RETAIN_MAPS, RS_RETAIN_FP64, RS_RETAIN_SCRATCH = 1, 2, 4   # rs_opts.retain
 
 
 class RobinError(RuntimeError):
@@ -35,7 +36,7 @@ class RobinError(RuntimeError):
 class RsOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("skip_root_rtr", ctypes.c_int), ("n_refine", ctypes.c_int),
                 ("verbose", ctypes.c_int), ("hbm_budget", ctypes.c_longlong), ("autotune", ctypes.c_int),
-                ("graphs", ctypes.c_int), ("merge_fp64_levels", ctypes.c_int)]
+                ("graphs", ctypes.c_int), ("merge_fp64_levels", ctypes.c_int), ("retain", ctypes.c_int)]
 
 
 def lib():
@@ -96,13 +97,14 @@ class GpuTree:
     """One factored tree resident on the GPU (C ABI handle)."""
 
     def __init__(self, d, L, domain, h_tgt, kb, bndry_params=None, precision=RS_FP64, skip_root_rtr=True,
-                 n_refine=2, hbm_budget=0, autotune=True, graphs=True, merge_fp64_levels=None):
+                 n_refine=2, hbm_budget=0, autotune=True, graphs=True, merge_fp64_levels=None, retain=0):
         L_ = lib()
         o = RsOpts()
         L_.rs_opts_default(ctypes.byref(o))
         o.precision, o.skip_root_rtr, o.n_refine, o.hbm_budget = precision, int(skip_root_rtr), n_refine, hbm_budget
         o.autotune = int(autotune)
         o.graphs = int(graphs)
+        o.retain = int(retain)
         if merge_fp64_levels is not None:
             o.merge_fp64_levels = int(merge_fp64_levels)
         h = ctypes.c_void_p()

@@ -88,6 +88,30 @@ __global__ void to_c64_kernel(float2* __restrict__ dst, const double2* __restric
     if (e < n) dst[e] = make_float2((float)src[e].x, (float)src[e].y);
 }
 
+// The blocks of the node maps that the solve reads, out of full b x b maps (node
+// order: the non-interface faces first, the interface last): per node
+// [T_NI (bn x i, leading dimension bn) | T_IN (i x bn, leading dimension i)],
+// into complex128 and/or complex64.  One thread per compact element, reads and
+// writes both running along columns.
+__global__ void extract_ni_in_kernel(const double2* __restrict__ T, size_t tstride, int b, int bn, int isz,
+                                     size_t total, double2* __restrict__ d64, float2* __restrict__ d32) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const size_t per = 2 * (size_t)bn * isz, ni = (size_t)bn * isz;
+    const size_t n = e / per, r = e - n * per;
+    const double2* Tn = T + n * tstride;
+    double2 v;
+    if (r < ni) {
+        const size_t c = r / bn, q = r - c * bn;
+        v = Tn[q + (bn + c) * (size_t)b];
+    } else {
+        const size_t r2 = r - ni, c = r2 / isz, q = r2 - c * isz;
+        v = Tn[(bn + q) + c * (size_t)b];
+    }
+    if (d64) d64[e] = v;
+    if (d32) d32[e] = make_float2((float)v.x, (float)v.y);
+}
+
 __global__ void zero_kernel(double2* __restrict__ p, size_t n) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < n) p[e] = make_double2(0.0, 0.0);
@@ -143,6 +167,13 @@ void launch_child_gather(V* child, const V* parent, const int* prow, int bn, int
 
 void launch_to_c64(float2* dst, const double2* src, size_t n, hipStream_t st) {
     if (n) hipLaunchKernelGGL(to_c64_kernel, dim3(nblk(n)), dim3(256), 0, st, dst, src, n);
+}
+
+void launch_extract_ni_in(const double2* T, size_t tstride, int b, int bn, int isz, int nodes, double2* d64,
+                          float2* d32, hipStream_t st) {
+    const size_t total = (size_t)nodes * 2 * bn * isz;
+    if (total) hipLaunchKernelGGL(extract_ni_in_kernel, dim3(nblk(total)), dim3(256), 0, st, T, tstride, b, bn, isz,
+                                  total, d64, d32);
 }
 
 void launch_zero(double2* p, size_t n, hipStream_t st) {

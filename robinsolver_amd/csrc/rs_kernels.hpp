@@ -121,6 +121,10 @@ template <typename V>
 void launch_child_gather(V* child, const V* parent, const int* prow, int bn, int bc, int P, int nodes, int root,
                          int ncols, hipStream_t st);
 void launch_to_c64(float2* dst, const double2* src, size_t n, hipStream_t st);
+// per node [T_NI | T_IN] (bn x i and i x bn, column-major) out of full b x b
+// node maps of stride tstride, into d64 and/or d32 (either may be NULL)
+void launch_extract_ni_in(const double2* T, size_t tstride, int b, int bn, int isz, int nodes, double2* d64,
+                          float2* d32, hipStream_t st);
 void launch_zero(double2* p, size_t n, hipStream_t st);
 void launch_axpy(double2* y, const double2* x, size_t n, hipStream_t st);
 // tab[c] = base + c * stride for c < ncols (column table of a contiguous array)

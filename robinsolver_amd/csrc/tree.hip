@@ -88,7 +88,8 @@ template <typename R> void SolveWork<R>::bind(char* base, const WorkLayout& lay)
 //   dn_g  g_I += invM Z, into the children's incoming data  (rows through imap)
 __host__ __device__ inline void solve_descs(const SolveLevel& p, int n, GemmDesc* ug, GemmDesc* uh, GemmDesc* dt,
                                             GemmDesc* dg) {
-    const size_t e = p.esz, i = p.i, b = p.b, bn = p.bn, ii = i * i, tstride = b * b;
+    // children's maps: compact [T_NI (bn x i) | T_IN (i x bn)] per child
+    const size_t e = p.esz, i = p.i, b = p.b, bn = p.bn, ii = i * i, tstride = 2 * bn * i;
     const int par = (p.lvl == 0) ? 0 : (n & 1);
     GemmDesc z = {};
     z.ar = 1.0;
@@ -110,7 +111,7 @@ __host__ __device__ inline void solve_descs(const SolveLevel& p, int n, GemmDesc
         const char* Tcn = p.Tc + (size_t)(2 * n + c) * tstride * e;
         if (p.lvl > 0) {
             GemmDesc u = z;
-            u.A = Tcn + bn * b * e; u.lda = p.b;
+            u.A = Tcn; u.lda = p.bn;
             u.B = p.Gl + ((size_t)n * 2 * i + (size_t)c * i) * e; u.ldb = p.ldg;
             u.Out = p.hP + (size_t)n * p.Pp * e; u.ldo = p.ldp;
             u.rmap = p.prow + (size_t)(par * 2 + c) * bn;
@@ -118,7 +119,7 @@ __host__ __device__ inline void solve_descs(const SolveLevel& p, int n, GemmDesc
             uh[c] = u;
         }
         GemmDesc t = z;
-        t.A = Tcn + bn * e; t.lda = p.b;
+        t.A = Tcn + bn * i * e; t.lda = p.i;
         t.B = p.gC + (size_t)(2 * n + c) * b * e; t.ldb = p.ldc;
         t.Out = p.Zl + ((size_t)n * 2 * i + (size_t)c * i) * e; t.ldo = p.ldg;
         dt[c] = t;
@@ -231,10 +232,10 @@ void Tree::free_factor() {
         for (DevBuf* b : v) delete b;
         v.clear();
     };
-    drop(T); drop(invM); drop(T32); drop(invM32); drop(fdesc);
+    drop(T); drop(Tn); drop(Tn32); drop(invM); drop(invM32); drop(fdesc);
     ftile.clear();
-    for (DevBuf* b : {&sinv, &sinv32, &info, &fscratch}) b->release();
-    have32 = false;
+    for (DevBuf* b : {&sinv, &sinv32, &info, &fscratch, &invM_t, &tmap[0], &tmap[1]}) b->release();
+    have32 = have64 = false;
     factored = false;
 }
 
@@ -334,55 +335,122 @@ static MergeLayout merge_layout(const Level& V, size_t lds_limit) {
     return m;
 }
 
+// What factor_device allocates, and which of it the tree keeps (bytes).
+struct FactorSizes {
+    size_t sinv64 = 0, sinv32 = 0, maps_full = 0, tmap[2] = {0, 0}, invm_t = 0, arena = 0;
+    std::vector<size_t> Tn64, Tn32, invm64, invm32, Tfull;   // per level
+    long long fdesc = 0;
+};
+
+static FactorSizes factor_sizes(const Plan& P, const LeafGeom& G, const rs_opts& o, size_t lds) {
+    FactorSizes z;
+    const int L = P.L;
+    const bool mixed = o.precision == RS_MIXED;
+    const bool keep64 = !mixed || (o.retain & RS_RETAIN_FP64);
+    const size_t nl = P.n_leaves;
+    z.sinv64 = nl * (size_t)G.K * G.s * G.s * 16;
+    z.sinv32 = mixed ? z.sinv64 / 2 : 0;
+    z.Tn64.assign(L + 1, 0); z.Tn32.assign(L + 1, 0); z.Tfull.assign(L + 1, 0);
+    z.invm64.assign(L, 0); z.invm32.assign(L, 0);
+    z.Tfull[L] = nl * (size_t)P.nb * P.nb * 16;
+    const long long gd = (long long)sizeof(GemmDesc);
+    for (int l = L - 1; l >= 0; --l) {
+        const Level& V = P.lv[l];
+        const size_t nodes = V.nodes, i = V.isz, bn = V.bn;
+        const size_t compact = 2 * nodes * 2 * bn * i;   // the level's children
+        if (keep64) z.Tn64[l + 1] = compact * 16;
+        if (mixed) z.Tn32[l + 1] = compact * 8;
+        const size_t im = nodes * 4 * i * i * 16;
+        if (keep64) z.invm64[l] = im;
+        else z.invm_t = std::max(z.invm_t, im);
+        if (mixed) z.invm32[l] = im / 2;
+        if (l > 0 || !o.skip_root_rtr) z.Tfull[l] = nodes * (size_t)V.P * V.P * 16;
+        const MergeLayout m = merge_layout(V, lds);
+        z.arena = std::max(z.arena, m.total);
+        if (!m.ip.small) z.fdesc += 2 * (long long)m.ip.panels() * nodes * gd + (long long)m.ip.panels() * (i - m.ip.nb) * 4;
+        z.fdesc += 3 * nodes * gd + 2 * nodes * gd + ((l > 0 || !o.skip_root_rtr) ? 4 * nodes * gd : 0);
+    }
+    if (!z.arena) z.arena = 256;
+    for (int l = 0; l <= L; ++l) {
+        z.maps_full += z.Tfull[l];
+        z.tmap[l & 1] = std::max(z.tmap[l & 1], z.Tfull[l]);
+    }
+    return z;
+}
+
 int Tree::factor_device(const double2* k) {
     if (!planned) return fail("Error: rs_plan must be called before rs_factor.");
     if (ensure_device()) return 1;
     const double t0 = now_s();
-    // A refactor of the same plan overwrites the factor data in place, with the
-    // merge scratch and the product descriptors kept from the first factor: no
-    // allocation, no host synchronisation until the end, and the solve
-    // workspaces (whose descriptors point into the factor data) stay valid.
+    // A refactor of the same plan overwrites the factor data in place.  When the
+    // working memory was kept (RS_RETAIN_SCRATCH) the merge scratch and the
+    // product descriptors of the first factorization are reused: no allocation,
+    // and no host synchronisation until the end.  The solve workspaces' product
+    // descriptors point into the retained solve data, which keeps its addresses.
     factored = false;
-    have32 = false;
+    have32 = have64 = false;
     const Plan& P = plan;
     const int L = P.L, nl = P.n_leaves;
+    const bool mixed = opts.precision == RS_MIXED;
+    const bool keep64 = !mixed || (opts.retain & RS_RETAIN_FP64);
+    const bool keep_maps = (opts.retain & RS_RETAIN_MAPS) != 0;
+    const bool keep_scratch = (opts.retain & RS_RETAIN_SCRATCH) != 0;
+    for (int l = L - 1; l >= 0; --l)
+        if (!merge_layout(P.lv[l], lds_limit).ip.fits())
+            return fail("Error: interface of " + std::to_string(P.lv[l].isz) +
+                        " points is larger than the interface inverses support (LDS panel).");
+    const FactorSizes z = factor_sizes(P, G, opts, lds_limit);
     bool moved = false;
     auto need = [&](DevBuf& b, size_t bytes) -> int {
+        if (!bytes) { if (b.p) { b.release(); moved = true; } return 0; }
         if (b.p && b.bytes == bytes) return 0;
         moved = true;
         return alloc(b, bytes);
     };
-    // keep the potential resident (the FP64 residual of the mixed path reads it)
-    if (need(kint, (size_t)nl * P.n_int * sizeof(double2)) ||
-        need(sinv, (size_t)nl * G.K * G.s * G.s * sizeof(double2)) || need(info, sizeof(int)))
+    auto needv = [&](std::vector<DevBuf*>& v, int l, size_t bytes) -> int {
+        if (!v[l]) v[l] = new DevBuf;
+        return need(*v[l], bytes);
+    };
+    T.resize(L + 1, nullptr); Tn.resize(L + 1, nullptr); Tn32.resize(L + 1, nullptr);
+    invM.resize(L, nullptr); invM32.resize(L, nullptr);
+    // addresses of the solve data before: the solve workspaces' descriptors and
+    // captured graphs point into it
+    auto solve_ptrs = [&]() {
+        std::vector<const void*> v = {keep64 ? sinv.p : nullptr, sinv32.p};
+        for (DevBuf* b : Tn) v.push_back(b ? b->p : nullptr);
+        for (DevBuf* b : Tn32) v.push_back(b ? b->p : nullptr);
+        for (DevBuf* b : invM) v.push_back(b ? b->p : nullptr);
+        for (DevBuf* b : invM32) v.push_back(b ? b->p : nullptr);
+        return v;
+    };
+    const std::vector<const void*> before = solve_ptrs();
+    // the potential stays resident (the FP64 residual of the mixed path reads it).
+    // An FP64 tree's complex64 copies (made for mixed solves) keep their buffers:
+    // make_c64 refills them in place.
+    if (need(kint, (size_t)nl * P.n_int * sizeof(double2)) || need(info, sizeof(int)) || need(sinv, z.sinv64) ||
+        (mixed && need(sinv32, z.sinv32)) || need(invM_t, z.invm_t) || need(fscratch, z.arena))
         return 1;
-    T.resize(L + 1, nullptr);
-    invM.resize(L, nullptr);
-    if (!T[L]) T[L] = new DevBuf;
-    if (need(*T[L], (size_t)nl * P.nb * P.nb * sizeof(double2))) return 1;
-    std::vector<MergeLayout> ml(L);
-    size_t arena = 0;
-    for (int l = L - 1; l >= 0; --l) {
-        const Level& V = P.lv[l];
-        ml[l] = merge_layout(V, lds_limit);
-        if (!ml[l].ip.fits())
-            return fail("Error: interface of " + std::to_string(V.isz) +
-                        " points is larger than the interface inverses support (LDS panel).");
-        arena = std::max(arena, ml[l].total);
-        if (!invM[l]) invM[l] = new DevBuf;
-        if (need(*invM[l], (size_t)V.nodes * 4 * V.isz * V.isz * 16)) return 1;
-        if (l > 0 || !opts.skip_root_rtr) {
-            if (!T[l]) T[l] = new DevBuf;
-            if (need(*T[l], (size_t)V.nodes * V.P * V.P * 16)) return 1;
-        }
+    for (int l = 0; l <= L; ++l) {
+        if (needv(Tn, l, z.Tn64[l]) || (mixed && needv(Tn32, l, z.Tn32[l])) ||
+            needv(T, l, keep_maps ? z.Tfull[l] : 0))
+            return 1;
+        if (l < L && (needv(invM, l, z.invm64[l]) || (mixed && needv(invM32, l, z.invm32[l])))) return 1;
     }
-    if (need(fscratch, arena ? arena : 256)) return 1;
+    for (int q = 0; q < 2; ++q)
+        if (need(tmap[q], keep_maps ? 0 : z.tmap[q])) return 1;
+    if (solve_ptrs() != before) clear_slots();
+    std::vector<MergeLayout> ml(L);
+    for (int l = 0; l < L; ++l) ml[l] = merge_layout(P.lv[l], lds_limit);
     char* ar = fscratch.as<char>();
+    // full node maps of level l, and the FP64 inv(M) of level l
+    auto Tfull = [&](int l) -> double2* { return keep_maps ? T[l]->as<double2>() : tmap[l & 1].as<double2>(); };
+    auto IMf = [&](int l) -> double2* { return keep64 ? invM[l]->as<double2>() : invM_t.as<double2>(); };
 
     const double2 zeta = make_double2(P.zeta[0], P.zeta[1]);
     const double2 nu = make_double2(P.nu[0], P.nu[1]);
     const double2 nu2 = cmul(nu, nu);
-    // product descriptors of every level (kept for refactors)
+    // product descriptors of every level (kept for refactors while the buffers they
+    // point to are)
     if (moved || (int)fdesc.size() != 6 * L) {
         for (DevBuf* b : fdesc) delete b;
         fdesc.assign(6 * L, nullptr);
@@ -392,13 +460,13 @@ int Tree::factor_device(const double2* k) {
             const MergeLayout& m = ml[l];
             const int nodes = V.nodes, i = V.isz, b = V.bc, bn = V.bn, Pp = V.P;
             const size_t ii = (size_t)i * i, tstride = (size_t)b * b;
-            const double2* Tc = T[l + 1]->as<double2>();
+            const double2* Tc = Tfull(l + 1);
             double2* Dm = (double2*)(ar + m.D);
             double2* Sm = (double2*)(ar + m.S);
             double2* SD = (double2*)(ar + m.SD);
             double2* Y = (double2*)(ar + m.Y);
             double2* W = (double2*)(ar + m.W);
-            double2* IM = invM[l]->as<double2>();
+            double2* IM = IMf(l);
             DevBuf** dl = &fdesc[6 * l];   // inv D, inv S, S D / B21 / B22, Y, parent T, GJ column maps
             if (!m.ip.small) {
                 const int n = i, nb_ = m.ip.nb;
@@ -477,7 +545,7 @@ int Tree::factor_device(const double2* k) {
                             g.A = Tcn + (size_t)bn * b; g.lda = b;
                             g.B = Y + (size_t)n * 2 * i * 2 * bn + (size_t)c * i + (size_t)c2 * bn * 2 * i;
                             g.ldb = 2 * i;
-                            g.Out = T[l]->as<double2>() + (size_t)n * Pp * Pp; g.ldo = Pp;
+                            g.Out = Tfull(l) + (size_t)n * Pp * Pp; g.ldo = Pp;
                             g.rmap = prow + (size_t)(par * 2 + c) * bn;
                             g.cmap = prow + (size_t)(par * 2 + c2) * bn;
                             if (c == c2) { g.Din = Tcn; g.ldd = b; g.br = 1.0; }
@@ -497,9 +565,10 @@ int Tree::factor_device(const double2* k) {
     {
         LeafArgs<double> a;
         a.mode = LEAF_FACTOR_T; a.ncols = P.nb; a.n_leaves = nl;
-        a.sinv = sinv.as<double2>(); a.T = T[L]->as<double2>();
+        a.sinv = sinv.as<double2>(); a.T = Tfull(L);
         launch_leaf_apply<double>(G, a, lds_limit, stream);
     }
+    if (mixed) launch_to_c64(sinv32.as<float2>(), sinv.as<double2>(), z.sinv64 / 16, stream);
     HIPCHK(hipGetLastError());
     // merges, bottom-up; every level of one launch sequence on the tree's stream.
     // The merge products are idempotent (each writes a buffer it does not read),
@@ -516,7 +585,7 @@ int Tree::factor_device(const double2* k) {
     // products of the levels below the top k run in complex64 on the matrix
     // cores (FP32 accumulation, complex128 operands and results); the interface
     // inverses around them stay FP64 with pivoting.
-    const int c64_from = (opts.precision == RS_MIXED && opts.merge_fp64_levels >= 0) ? opts.merge_fp64_levels : L;
+    const int c64_from = (mixed && opts.merge_fp64_levels >= 0) ? opts.merge_fp64_levels : L;
     merge_c64 = c64_from < L;
     auto gemm = [&](int lvl, const GemmDesc* d, int np, int M, int N, int K, void* gws, int t) {
         if (lvl >= c64_from)
@@ -547,14 +616,14 @@ int Tree::factor_device(const double2* k) {
         const MergeLayout& m = ml[l];
         const int nodes = V.nodes, i = V.isz, b = V.bc, bn = V.bn;
         const size_t ii = (size_t)i * i, tstride = (size_t)b * b;
-        const double2* Tc = T[l + 1]->as<double2>();
+        const double2* Tc = Tfull(l + 1);
         double2* Dm = (double2*)(ar + m.D);
         double2* Sm = (double2*)(ar + m.S);
         double2* SD = (double2*)(ar + m.SD);
         double2* W = (double2*)(ar + m.W);
         int* piv = (int*)(ar + m.piv);
         void* gws = ar + m.ws;
-        double2* IM = invM[l]->as<double2>();
+        double2* IM = IMf(l);
         DevBuf** dl = &fdesc[6 * l];
         // D = (zeta I - T2_II)^-1
         launch_form_iface(Dm, Tc, tstride, 2, 1, b, bn, i, nodes, zeta, make_double2(0, 0), nullptr, stream);
@@ -573,6 +642,11 @@ int Tree::factor_device(const double2* k) {
         if (fgemm(4 * l + 2, dl[3]->as<GemmDesc>(), 2 * nodes, 2 * i, bn, i, gws)) return 1;
         if ((l > 0 || !opts.skip_root_rtr) && fgemm(4 * l + 3, dl[4]->as<GemmDesc>(), 4 * nodes, bn, bn, i, gws))
             return 1;
+        // the solve data of this merge: inv(M), and the children's T_NI / T_IN,
+        // before the next level reuses the buffers they come from
+        if (mixed) launch_to_c64(invM32[l]->as<float2>(), IM, (size_t)nodes * 4 * ii, stream);
+        launch_extract_ni_in(Tc, tstride, b, bn, i, 2 * nodes, keep64 ? Tn[l + 1]->as<double2>() : nullptr,
+                             mixed ? Tn32[l + 1]->as<float2>() : nullptr, stream);
         HIPCHK(hipGetLastError());
     }
     if (te0) (void)hipEventDestroy(te0);
@@ -582,25 +656,35 @@ int Tree::factor_device(const double2* k) {
     HIPCHK(hipStreamSynchronize(stream));
     if (hinfo & 1) return fail("Error: singular leaf block in the leaf factorization.");
     if (hinfo & 2) return fail("Error: singular interface matrix in the merge.");
+    // the working memory goes unless it is to be kept for refactors
+    if (!keep_scratch) {
+        fscratch.release(); invM_t.release(); tmap[0].release(); tmap[1].release();
+        if (!keep64) sinv.release();
+        for (DevBuf* b : fdesc) delete b;
+        fdesc.clear();
+    }
     factored = true;
-    if (opts.precision == RS_MIXED && make_c64()) return 1;
+    have64 = keep64;
+    have32 = mixed;
     f_time = now_s() - t0;
     return 0;
 }
 
+// The complex64 solve data of a tree factored in FP64 (made on its first mixed
+// solve): the same blocks rounded, in place when the buffers exist.
 int Tree::make_c64() {
     const Plan& P = plan;
-    // in place when the buffers exist (a refactor), so descriptors stay valid
+    if (!have64) return fail("Error: the tree holds no FP64 solve data to round.");
     auto conv = [&](DevBuf& dst, const DevBuf& src) -> int {
         size_t n = src.bytes / sizeof(double2);
         if (ensure_buf(dst, n * sizeof(float2))) return 1;
         launch_to_c64(dst.as<float2>(), src.as<double2>(), n, stream);
         return 0;
     };
-    T32.resize(P.L + 1, nullptr);
+    Tn32.resize(P.L + 1, nullptr);
     invM32.resize(P.L, nullptr);
     if (conv(sinv32, sinv)) return 1;
-    for (int l = 1; l <= P.L; ++l) { if (!T32[l]) T32[l] = new DevBuf; if (conv(*T32[l], *T[l])) return 1; }
+    for (int l = 1; l <= P.L; ++l) { if (!Tn32[l]) Tn32[l] = new DevBuf; if (conv(*Tn32[l], *Tn[l])) return 1; }
     for (int l = 0; l < P.L; ++l) { if (!invM32[l]) invM32[l] = new DevBuf; if (conv(*invM32[l], *invM[l])) return 1; }
     HIPCHK(hipStreamSynchronize(stream));
     have32 = true;
@@ -661,7 +745,7 @@ template <typename R> SolveLevel Tree::solve_level(const SolveWork<R>& w, int l)
     const Plan& P = plan;
     const Level& Vl = P.lv[l];
     const Level& C = P.lv[l + 1];
-    const std::vector<DevBuf*>& Tt = std::is_same<R, double>::value ? T : T32;
+    const std::vector<DevBuf*>& Tt = std::is_same<R, double>::value ? Tn : Tn32;
     const std::vector<DevBuf*>& IMt = std::is_same<R, double>::value ? invM : invM32;
     SolveLevel p;
     p.IM = (const char*)IMt[l]->p;
@@ -684,6 +768,7 @@ template <typename R> int Tree::ensure_work(SolveWork<R>& w, int nrhs, bool refi
     if (w.cap >= nrhs && (w.refine || !refine)) return 0;
     refine = refine || w.refine;
     w.clear();
+    ++w.gen;
     const Plan& P = plan;
     const int L = P.L;
     const WorkLayout lay = work_layout(nrhs, (int)sizeof(c_t<R>), refine, false);
@@ -796,40 +881,27 @@ long long Tree::work_bytes(int cap, int prec) const {
 }
 
 void Tree::factor_bytes(long long* resident_out, long long* peak_out) const {
-    // mirrors factor_device: everything is allocated up front and kept, so the
-    // peak equals the resident total
+    // mirrors factor_device: everything is allocated up front; the working memory
+    // is released at the end unless RS_RETAIN_SCRATCH keeps it
     const Plan& P = plan;
     const int L = P.L;
-    const long long nl = P.n_leaves;
+    const bool mixed = opts.precision == RS_MIXED;
+    const bool keep64 = !mixed || (opts.retain & RS_RETAIN_FP64);
+    const bool keep_maps = (opts.retain & RS_RETAIN_MAPS) != 0;
+    const FactorSizes z = factor_sizes(P, G, opts, device_ready ? lds_limit : 163840);
     long long res = 0;
     for (int l = 0; l < L; ++l) res += (long long)(P.lv[l].prow.size() + P.lv[l].imap.size()) * 4;
     res += (long long)(P.root_leaf.size() + P.root_loc.size() + P.leaf_nbr.size()) * 4;
-    const long long leafblk = nl * (long long)G.K * G.s * G.s, tleaf = nl * (long long)P.nb * P.nb;
-    res += nl * P.n_int * 16 + leafblk * 16 + 4 + tleaf * 16;
-    long long c64 = (leafblk + tleaf) * 8;
-    const size_t lds = device_ready ? lds_limit : 163840;
-    const long long gd = (long long)sizeof(GemmDesc);
-    size_t arena = 0;
-    for (int l = L - 1; l >= 0; --l) {
-        const Level& V = P.lv[l];
-        const long long nodes = V.nodes, i = V.isz, Pp = V.P;
-        const long long ii = i * i;
-        const bool formT = l > 0 || !opts.skip_root_rtr;
-        res += nodes * 4 * ii * 16;
-        c64 += nodes * 4 * ii * 8;
-        if (formT) {
-            res += nodes * Pp * Pp * 16;
-            if (l > 0) c64 += nodes * Pp * Pp * 8;
-        }
-        const MergeLayout m = merge_layout(V, lds);
-        arena = std::max(arena, m.total);
-        if (!m.ip.small) res += 2 * (long long)m.ip.panels() * nodes * gd + (long long)m.ip.panels() * (i - m.ip.nb) * 4;
-        res += 3 * nodes * gd + 2 * nodes * gd + (formT ? 4 * nodes * gd : 0);
-    }
-    res += (long long)(arena ? arena : 256);
-    if (opts.precision == RS_MIXED) res += c64;
+    res += (long long)P.n_leaves * P.n_int * 16 + 4;   // k, info
+    res += (long long)z.sinv32 + (keep64 ? (long long)z.sinv64 : 0);
+    for (int l = 0; l <= L; ++l) res += (long long)(z.Tn64[l] + z.Tn32[l]);
+    for (int l = 0; l < L; ++l) res += (long long)(z.invm64[l] + z.invm32[l]);
+    if (keep_maps) res += (long long)z.maps_full;
+    long long work = (long long)z.arena + (long long)z.invm_t + z.fdesc + (keep64 ? 0 : (long long)z.sinv64) +
+                     (keep_maps ? 0 : (long long)(z.tmap[0] + z.tmap[1]));
+    if (opts.retain & RS_RETAIN_SCRATCH) { res += work; work = 0; }
     if (resident_out) *resident_out = res;
-    if (peak_out) *peak_out = res;
+    if (peak_out) *peak_out = res + work;
 }
 
 void Tree::pbeg(int cat, hipStream_t st) {
@@ -983,6 +1055,9 @@ int Tree::solve_device_noalloc(int nrhs, const double2* f, const double2* g, dou
     if (prec != RS_FP64 && prec != RS_MIXED) return fail("Error: precision must be RS_FP64 or RS_MIXED.");
     if (prec == RS_FP64 && merge_c64)
         return fail("Error: an FP64 solve needs a tree factored in FP64 (this tree's merge products are complex64).");
+    if (prec == RS_FP64 && !have64)
+        return fail("Error: this tree keeps complex64 solve data only (rs_opts.retain RS_RETAIN_FP64 keeps the "
+                    "FP64 data for FP64 solves).");
     if (prec == RS_MIXED && !have32)
         return fail("Error: the allocation-free solve needs the tree's complex64 solve data: factor it with "
                     "RS_MIXED (or run one RS_MIXED rs_solve_device first).");
@@ -1034,20 +1109,25 @@ int Tree::solve_cols(int nrhs, const RhsCols& cols, int prec, hipStream_t st, in
     if (prec < 0) prec = opts.precision;
     if (prec == RS_FP64 && merge_c64)
         return fail("Error: an FP64 solve needs a tree factored in FP64 (this tree's merge products are complex64).");
+    if (prec == RS_FP64 && !have64)
+        return fail("Error: this tree keeps complex64 solve data only (rs_opts.retain RS_RETAIN_FP64 keeps the "
+                    "FP64 data for FP64 solves).");
     if (prec != RS_FP64 && prec != RS_MIXED) return fail("Error: precision must be RS_FP64 or RS_MIXED.");
     if (!st) st = stream;
     Slot& sl = slot(slot_id);
     // workspaces first: allocation and autotuning cannot be captured; a workspace
     // that moves invalidates the slot's captured solves
+    // (a rebuilt workspace has a new layout even when the allocator returns the
+    // same base address: the captured graphs go whenever it is rebuilt)
     if (prec == RS_MIXED) {
         if (!have32 && make_c64()) return 1;
-        const void* before = sl.w32.arena.p;
+        const unsigned long long gen = sl.w32.gen;
         if (ensure_work(sl.w32, nrhs, opts.n_refine > 0)) return 1;
-        if (sl.w32.arena.p != before) sl.drop_graphs();
+        if (sl.w32.gen != gen) sl.drop_graphs();
     } else {
-        const void* before = sl.w64.arena.p;
+        const unsigned long long gen = sl.w64.gen;
         if (ensure_work(sl.w64, nrhs)) return 1;
-        if (sl.w64.arena.p != before) sl.drop_graphs();
+        if (sl.w64.gen != gen) sl.drop_graphs();
     }
     if (!graph || !opts.graphs || prof_on) return solve_launches(nrhs, cols, prec, st, sl.w64, sl.w32);
     const int nref = prec == RS_MIXED ? opts.n_refine : 0;
@@ -1208,7 +1288,9 @@ int Tree::node_rtr(int lvl, int idx, double* out, int* Pout) const {
     Tree* self = const_cast<Tree*>(this);
     if (!factored) return self->fail("Error: not factored.");
     if (lvl < 0 || lvl > plan.L || idx < 0 || idx >= plan.lv[lvl].nodes) return self->fail("Error: no such node.");
-    if (!T[lvl] || !T[lvl]->p) return self->fail("Error: RtR of this node is not formed.");
+    if (lvl >= (int)T.size() || !T[lvl] || !T[lvl]->p)
+        return self->fail("Error: RtR maps are kept only with rs_opts.retain RS_RETAIN_MAPS (and the root's "
+                          "only with skip_root_rtr = 0).");
     const int Pp = plan.lv[lvl].P;
     if (Pout) *Pout = Pp;
     if (!out) return 0;

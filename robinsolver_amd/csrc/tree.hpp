@@ -51,6 +51,7 @@ template <typename R> struct SolveWork {
     using V = c_t<R>;
     int cap = 0;
     bool refine = false;
+    unsigned long long gen = 0;          // bumped on every rebuild (captured graphs depend on the layout)
     std::vector<V*> hV, gF, G, Z;        // per level
     void* ws = nullptr;                  // K-split GEMM workspace
     float2 *rf = nullptr, *rg = nullptr, *rj = nullptr, *uA = nullptr;   // refinement (complex64)
@@ -148,16 +149,25 @@ struct Tree {
     LeafGeom G{};
     long long dev_bytes = 0;
 
-    // factor data (FP64)
-    DevBuf kint, sinv, info;
-    std::vector<DevBuf*> T;      // per level: [nodes][P][P] node order
-    std::vector<DevBuf*> invM;   // per level < L: [nodes][2i][2i]
-    DevBuf fscratch;              // merge scratch of the largest level (kept for refactors)
+    // Retained solve data: the leaf block inverses, inv(M) of every merge level
+    // and, per level l = 1..L, the children's [T_NI | T_IN] in a compact layout
+    // (the only blocks of the node maps the solve reads), in complex128 (FP64
+    // trees, or RS_RETAIN_FP64) and/or complex64 (mixed trees, or made on demand
+    // for a mixed solve of an FP64 tree).
+    DevBuf kint, info;
+    DevBuf sinv, sinv32;
+    std::vector<DevBuf*> Tn, Tn32;   // index l = 1..L
+    std::vector<DevBuf*> invM, invM32;
+    bool have64 = false;
+    // Full node maps: per level when retained (RS_RETAIN_MAPS, for rs_node_rtr);
+    // otherwise the factorization keeps two consecutive levels in tmap[l & 1].
+    std::vector<DevBuf*> T;
+    // the factorization's working memory (released after it unless RS_RETAIN_SCRATCH)
+    DevBuf tmap[2];
+    DevBuf invM_t;                // FP64 inv(M) of one level (mixed trees without FP64 solve data)
+    DevBuf fscratch;              // merge scratch of the largest level
     std::vector<int> ftile;       // per level, 4 merge products: GEMM tile chosen by the first factor's timing
-    std::vector<DevBuf*> fdesc;   // per level: the merge's product descriptors (kept for refactors)
-    // complex64 copies for the mixed solve
-    DevBuf sinv32;
-    std::vector<DevBuf*> T32, invM32;
+    std::vector<DevBuf*> fdesc;   // per level: the merge's product descriptors
     bool have32 = false;
     bool merge_c64 = false;      // some merge products of the factorization ran in complex64
     // maps

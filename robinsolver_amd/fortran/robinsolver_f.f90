@@ -134,6 +134,7 @@ module rs_capi
         integer(c_int) :: autotune
         integer(c_int) :: graphs
         integer(c_int) :: merge_fp64_levels
+        integer(c_int) :: retain
     end type rs_opts_t
 
     interface

@@ -100,16 +100,22 @@ def test_complex64_merges_and_precision_rules(fp64_levels, gpu):
     complex64, so the RtR maps differ from an FP64 tree's by complex64 rounding
     (not more); such a tree refuses FP64 solves and its refined mixed solves
     reach the mixed-path bound.  The default mixed tree factors in FP64: its
-    RtR maps are bitwise an FP64 tree's and it takes FP64 solves."""
+    RtR maps are bitwise an FP64 tree's; it keeps complex64 solve data only and
+    refuses FP64 solves, unless it keeps the FP64 data too (RS_RETAIN_FP64),
+    and then its FP64 solves are bitwise an FP64 tree's."""
     print("case: complex64 merges, fp64 levels", fp64_levels, flush=True)
     kb, n, L = 12.0, 64, 4
     o = OracleTree(2, L, UNIT, 1.0 / n, -kb)
-    tm = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=2, merge_fp64_levels=fp64_levels)
-    tdef = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=1)
-    td = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_FP64)
+    maps = rsa.RS_RETAIN_MAPS
+    tm = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=2, merge_fp64_levels=fp64_levels,
+                     retain=maps)
+    tdef = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=1,
+                       retain=maps | rsa.RS_RETAIN_FP64)
+    tplain = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=1)
+    td = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_FP64, retain=maps)
     pts, _ = o.all_points()
     k = _k(pts[:, o.nb:], kb)
-    for t in (o, tm, tdef, td):
+    for t in (o, tm, tdef, tplain, td):
         t.factor(k)
     for lvl, idx in ((L - 1, 0), (1, 1)):
         a, b = tm.node_rtr(lvl, idx), td.node_rtr(lvl, idx)
@@ -122,6 +128,9 @@ def test_complex64_merges_and_precision_rules(fp64_levels, gpu):
     g = _rand_c(rng, (2, o.n_leaves, o.nb))
     with pytest.raises(rsa.RobinError, match="FP64 solve"):
         tm.solve(f, g, precision=rsa.RS_FP64)
+    with pytest.raises(rsa.RobinError, match="complex64 solve data only"):
+        tplain.solve(f, g, precision=rsa.RS_FP64)
+    assert np.array_equal(tplain.solve(f, g), tdef.solve(f, g))
     assert np.array_equal(tdef.solve(f, g, precision=rsa.RS_FP64), td.solve(f, g))
     for u in (tm.solve(f, g), td.solve(f, g, precision=rsa.RS_MIXED)):
         for r in range(2):

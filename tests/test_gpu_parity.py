@@ -55,7 +55,7 @@ CASES = [
 
 def _pair(d, n, L, kb, bp, skip_root=True):
     o = OracleTree(d, L, UNIT, 1.0 / n, kb, bndry_params=bp)
-    t = rsa.GpuTree(d, L, UNIT, 1.0 / n, kb, bndry_params=bp, skip_root_rtr=skip_root)
+    t = rsa.GpuTree(d, L, UNIT, 1.0 / n, kb, bndry_params=bp, skip_root_rtr=skip_root, retain=rsa.RS_RETAIN_MAPS)
     pts, face = o.all_points()
     k = _k(pts[:, o.nb:], kb)
     o.factor(k)

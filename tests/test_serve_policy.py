@@ -110,10 +110,13 @@ def test_lanes_overlap_across_trees():
 
 
 def test_admission_evicts_idle_workspaces():
-    t1, t2 = _tree(), _tree()
+    # trees that keep their factorization's working memory (resident = peak)
+    t1 = rsa.GpuTree(2, 4, UNIT, 1.0 / 64, -10.0, precision=rsa.RS_MIXED, n_refine=1, retain=rsa.RS_RETAIN_SCRATCH)
+    t2 = rsa.GpuTree(2, 4, UNIT, 1.0 / 64, -10.0, precision=rsa.RS_MIXED, n_refine=1, retain=rsa.RS_RETAIN_SCRATCH)
     r1, p1, w1 = t1.memory_estimate(16)
+    assert p1 == r1
     # room for both factorizations and one solve workspace at a time
-    limit = 2 * r1 + max(p1 - r1, w1) + w1 // 2
+    limit = 2 * r1 + w1 + w1 // 2
     with _server(max_cols=16, lanes=1, hbm_limit=limit) as srv:
         a, b = srv.add_tree(t1), srv.add_tree(t2)
         _factor(srv, a)
@@ -130,9 +133,10 @@ def test_admission_evicts_idle_workspaces():
 
 
 def test_admission_refuses_what_can_never_fit():
-    t = _tree()
+    t = rsa.GpuTree(2, 4, UNIT, 1.0 / 64, -10.0, precision=rsa.RS_MIXED, n_refine=1, retain=rsa.RS_RETAIN_SCRATCH)
     r, p, w = t.memory_estimate(16)
-    with _server(max_cols=16, lanes=1, hbm_limit=r + (p - r) + w // 2) as srv:
+    assert p == r
+    with _server(max_cols=16, lanes=1, hbm_limit=r + w // 2) as srv:
         tid = srv.add_tree(t)
         _factor(srv, tid)
         q = srv.solve(tid, 4, 0, 0, 0)
@@ -155,7 +159,7 @@ def test_memory_estimates_scale():
     assert 1.9 < w32 / w16 < 2.1         # the workspace is per right-hand side
     t64 = rsa.GpuTree(2, 6, UNIT, 1.0 / 128, -10.0, precision=rsa.RS_FP64)
     r64, _, w64 = t64.memory_estimate(16)
-    assert r64 < r                        # no complex64 copies
+    assert r64 > r                        # complex128 solve data (a mixed tree keeps complex64 only)
     assert w64 < w16 * 2.0                # no refinement buffers in FP64
 
 
