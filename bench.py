@@ -168,7 +168,7 @@ def serve_mix(args):
     for sustained solves/s, then Poisson arrivals at 70% of it for latency."""
     import torch
     import robinsolver_amd as rsa
-    from robinsolver_amd.serve import Server
+    from robinsolver_amd.serve import Server, lane_bytes
     dev = torch.device("cuda:0")
     unit = [[0, 1], [0, 1], [0, 1]]
     rng = np.random.default_rng(args.seed + 5)
@@ -195,8 +195,10 @@ def serve_mix(args):
             t.set_refine(2)
         del t64, ua, ub
         r, p, w = t.memory_estimate(args.nrhs)
+        arena, descs = lane_bytes(t, args.nrhs)
         info.append({"grid": n, "L": L, "kb": kb, "leaf": int(round((t.n_int) ** 0.5)), "rel_l2_1step": rel,
-                     "factor_s": t.timings()[0], "resident_GB": r / 1e9, "workspace_GB_per_lane": w / 1e9})
+                     "factor_s": t.timings()[0], "resident_GB": r / 1e9, "lane_arena_GB": arena / 1e9,
+                     "descriptors_MB_per_lane": descs / 1e6})
         trees.append((t, ks, fpool, gpool))
         log("tree %d^2 L=%d: factor %.3fs, rel l2 (1 step) %.2e" % (n, L, t.timings()[0], rel))
     out = {"trees": info}

@@ -261,9 +261,12 @@ int rs_profile_read(rs_tree *t, int cat, double *ms, int *launches, double *byte
  * free it takes the tree whose oldest queued solve is oldest and coalesces that
  * tree's queued solves (up to max_cols right-hand sides) into one multi-RHS pass.
  * Requests that arrive while passes run join the next pass on a free lane.
+ * Each lane owns one solve workspace arena, shared by every tree it runs and
+ * grown to the largest tree's workspace at max_cols; a tree keeps per lane only
+ * the descriptors of its products there (rs_server_lane_bytes gives both).
  * Passes and factorizations are admitted by free device memory (hipMemGetInfo
- * less hbm_reserve, or hbm_limit less what the server holds); idle solve
- * workspaces are evicted to make room.  A factor request waits for the tree's
+ * less hbm_reserve, or hbm_limit less what the server holds); idle lane arenas
+ * and descriptors are evicted to make room.  A factor request waits for the tree's
  * earlier solves and holds back its later ones.  Results do not depend on how
  * requests are batched: every kernel computes each right-hand side the same
  * way whatever the batch, and the refinement count is fixed per tree.
@@ -288,12 +291,13 @@ typedef struct rs_server_stats {
     long long passes;          /* multi-RHS passes run                             */
     long long factors_done;    /* factorizations completed                         */
     long long admission_waits; /* times a request was held back for device memory */
-    long long evictions;       /* solve workspaces freed to admit other work       */
+    long long evictions;       /* lane arenas and descriptors freed to admit work  */
     long long failed;          /* requests that failed                             */
     double busy_s;             /* wall seconds with at least one pass in flight    */
     double lat_p50_s, lat_p99_s, lat_max_s;  /* solve latency, submission to completion */
     double mean_cols;          /* mean right-hand sides per pass                   */
-    long long held_bytes;      /* device bytes the server accounts to its trees    */
+    long long held_bytes;      /* device bytes the server accounts to its trees and
+                                  lanes                                            */
 } rs_server_stats;
 
 enum { RS_REQ_QUEUED = 0, RS_REQ_RUNNING = 1, RS_REQ_DONE = 2, RS_REQ_FAILED = 3 };
@@ -329,6 +333,10 @@ int rs_server_get_stats(rs_server *s, rs_server_stats *st);
 int rs_server_reset_stats(rs_server *s);
 /* Message of a failed request, or of the last failed call when req < 0. */
 const char *rs_server_error(rs_server *s, long long req);
+/* What a server with this max_cols holds for tree t (planned): the lane arena
+ * the tree needs (a lane's arena is the largest of these over its trees) and
+ * the tree's descriptors on one lane.  Host arithmetic, no device needed. */
+int rs_server_lane_bytes(const rs_tree *t, int max_cols, long long *arena, long long *descs);
 
 #ifdef __cplusplus
 }

@@ -2,17 +2,21 @@
 // the reference solves one right-hand side per call, robin_tree_module.f90:198-233).
 //
 // One scheduler thread owns all device submissions.  Requests are queued per
-// tree in arrival order.  A lane is one stream with its own column tables; a
-// tree run on lane j uses the tree's solve slot j, so passes of different
-// lanes never share a workspace and overlap on the device.  Whenever a lane is
+// tree in arrival order.  A lane is one stream with its own column tables and
+// one workspace arena, shared by every tree the lane runs and sized to the
+// largest of them; a tree run on lane j keeps in its solve slot 1 + j only the
+// descriptors of its products in that arena (slot 0 stays the tree's own entry
+// points').  Passes of one lane are ordered by its stream; passes of different
+// lanes never share memory and overlap on the device.  Whenever a lane is
 // free, the tree whose oldest queued solve is oldest gets it, and that tree's
 // queued solves up to max_cols right-hand sides form one pass: the callers'
 // arrays are reached through the column tables, so nothing is copied.
 // Factorizations run on a worker thread on the tree's own stream, once the
 // tree's earlier passes have completed; its later solves wait for it.
-// Admission: a pass needs the tree's slot workspace, a factorization its
-// resident and transient bytes; both are admitted against free device memory
-// (or a byte limit), evicting idle workspaces first.
+// Admission: a pass needs its lane's arena to be large enough for the tree and
+// the tree's descriptors on that lane, a factorization its resident and
+// transient bytes; both are admitted against free device memory (or a byte
+// limit), evicting idle lane arenas and descriptors first.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -54,8 +58,8 @@ struct TreeRec {
     bool factored = false, factoring = false;
     long long fbytes = 0;        // resident factor bytes accounted to the tree
     long long ftrans = 0;        // transient bytes of a running factorization
-    std::vector<long long> ws;   // per lane: bytes of the tree's slot workspace (0 = none)
-    std::vector<double> ws_used; // per lane: last use of that workspace (eviction order)
+    std::vector<long long> ws;   // per lane: bytes of the tree's descriptors there (0 = none)
+    std::vector<double> ws_used; // per lane: last use of them (eviction order)
 };
 
 struct Lane {
@@ -63,6 +67,10 @@ struct Lane {
     hipEvent_t done = nullptr;
     void** tab_dev = nullptr;    // [3][max_cols]: f, g, u column pointers
     void** tab_host = nullptr;   // pinned staging of the same
+    char* arena = nullptr;       // the lane's solve workspace (every tree's passes on this lane)
+    long long arena_bytes = 0;   // its size (accounted also when simulated)
+    unsigned long long arena_gen = 0;   // changes with every allocation (the trees rebind)
+    double arena_used = 0;       // last use (eviction order)
     bool busy = false;
     int tree = -1, cols = 0;
     std::vector<long long> reqs;
@@ -83,6 +91,7 @@ struct rs_server {
     std::vector<std::unique_ptr<TreeRec>> trees;
     std::unordered_map<long long, Req> reqs;
     long long next_id = 1, next_pass = 0;
+    unsigned long long next_gen = 0;
     int queued = 0;               // requests not yet completed
     std::vector<Lane> lanes;
     std::vector<std::thread> fworkers;
@@ -101,6 +110,7 @@ struct rs_server {
             b += tr->fbytes + tr->ftrans;
             for (long long w : tr->ws) b += w;
         }
+        for (const Lane& l : lanes) b += l.arena_bytes;
         return b;
     }
     long long avail() const {
@@ -130,6 +140,11 @@ struct rs_server {
         --queued;
     }
     bool evict_for(long long need, const TreeRec* keep_tr, int keep_lane);
+    void free_arena(Lane& l) {
+        if (l.arena) (void)hipFree(l.arena);
+        l.arena = nullptr;
+        l.arena_bytes = 0;
+    }
     bool pass_done(int j);
     void finish_pass(int j);
     bool dispatch_factors();
@@ -137,12 +152,16 @@ struct rs_server {
     void loop();
 };
 
-// Free idle solve workspaces, least recently used first, until `need` bytes are
-// available.  A workspace is idle when its lane is not running its tree.
+// Free idle lane arenas and descriptors, least recently used first, until
+// `need` bytes are available.  A lane's arena is idle when the lane has no pass
+// in flight; a tree's descriptors on a lane, when the lane is not running it.
+// The lane keep_lane (about to run keep_tr) keeps both.
 bool rs_server::evict_for(long long need, const TreeRec* keep_tr, int keep_lane) {
     if (avail() >= need) return true;
-    struct Cand { TreeRec* tr; int lane; double used; };
+    struct Cand { TreeRec* tr; int lane; double used; };   // tr == nullptr: the lane's arena
     std::vector<Cand> c;
+    for (int j = 0; j < (int)lanes.size(); ++j)
+        if (lanes[j].arena_bytes && !lanes[j].busy && j != keep_lane) c.push_back({nullptr, j, lanes[j].arena_used});
     for (auto& tr : trees) {
         if (tr->factoring) continue;
         for (int j = 0; j < (int)lanes.size(); ++j) {
@@ -154,8 +173,13 @@ bool rs_server::evict_for(long long need, const TreeRec* keep_tr, int keep_lane)
     std::sort(c.begin(), c.end(), [](const Cand& a, const Cand& b) { return a.used < b.used; });
     for (const Cand& e : c) {
         if (avail() >= need) break;
-        if (!o.simulate) e.tr->t->slot(e.lane).clear();
-        e.tr->ws[e.lane] = 0;
+        if (!e.tr) {
+            if (!o.simulate) free_arena(lanes[e.lane]);
+            lanes[e.lane].arena_bytes = 0;
+        } else {
+            if (!o.simulate) e.tr->t->slot(1 + e.lane).clear();
+            e.tr->ws[e.lane] = 0;
+        }
         ++st_evict;
     }
     return avail() >= need;
@@ -177,6 +201,7 @@ void rs_server::finish_pass(int j) {
     TreeRec& tr = *trees[ln.tree];
     --tr.inflight;
     tr.ws_used[j] = t;
+    ln.arena_used = t;
     ln.busy = false;
     ln.reqs.clear();
     ln.failed = false;
@@ -195,7 +220,7 @@ bool rs_server::dispatch_factors() {
         if (r.kind != RQ_FACTOR) continue;
         long long resident = 0, peak = 0;
         tr.t->factor_bytes(&resident, &peak);
-        // the tile tuning after a first factorization holds a solve workspace
+        // the tile tuning after a first factorization holds a workspace of its own
         if (!o.simulate && tr.t->tiles_pending(o.max_cols))
             peak = std::max(peak, resident + tr.t->work_bytes(o.max_cols, tr.t->opts.precision));
         const long long need = std::max(0LL, resident - tr.fbytes) + (peak - resident);
@@ -279,8 +304,14 @@ bool rs_server::dispatch_solves(std::unique_lock<std::mutex>& lk) {
         }
         if (best < 0) break;
         TreeRec& tr = *trees[best];
-        if (!tr.ws[j]) {
-            const long long need = tr.t->work_bytes(o.max_cols, tr.t->opts.precision);
+        Lane& ln = lanes[j];
+        // the lane's arena, grown to this tree's workspace when it is smaller
+        // (the old one is freed first), and the tree's descriptors on the lane
+        const long long wneed = tr.t->lane_bytes(o.max_cols);
+        const long long grow = wneed > ln.arena_bytes ? wneed : 0;
+        const long long dneed = tr.ws[j] ? 0 : tr.t->lane_desc_bytes();
+        const long long need = (grow ? grow - ln.arena_bytes : 0) + dneed;
+        if (need > 0) {
             if (!evict_for(need, &tr, j)) {
                 if (!any_busy() && factors_running == 0) {
                     while (!tr.q.empty() && reqs[tr.q.front()].kind == RQ_SOLVE) {
@@ -295,21 +326,34 @@ bool rs_server::dispatch_solves(std::unique_lock<std::mutex>& lk) {
                 }
                 break;   // the oldest work waits for memory; later work does not overtake it
             }
-            if (!o.simulate && tr.t->reserve_slot(j, o.max_cols)) {
-                const std::string m = tr.t->err;
-                tr.t->slot(j).clear();
-                while (!tr.q.empty() && reqs[tr.q.front()].kind == RQ_SOLVE) {
-                    complete(tr.q.front(), false, m, now_s());
-                    tr.q.pop_front();
+            if (grow) {
+                if (!o.simulate) {
+                    free_arena(ln);
+                    if (hipMalloc((void**)&ln.arena, (size_t)grow) != hipSuccess) {
+                        (void)hipGetLastError();
+                        ln.arena = nullptr;
+                    }
                 }
-                cv_done.notify_all();
-                any = true;
-                continue;
+                ln.arena_bytes = (o.simulate || ln.arena) ? grow : 0;
+                ln.arena_gen = ++next_gen;
             }
-            tr.ws[j] = need;
+            tr.ws[j] = tr.t->lane_desc_bytes();
+        }
+        if (!o.simulate && (!ln.arena_bytes ||
+                            tr.t->bind_slot(1 + j, ln.arena, (size_t)ln.arena_bytes, ln.arena_gen, o.max_cols, ln.st))) {
+            const std::string m = ln.arena_bytes ? tr.t->err : std::string("Error: could not allocate the lane's "
+                                                                           "solve workspace.");
+            tr.t->slot(1 + j).clear();
+            tr.ws[j] = 0;
+            while (!tr.q.empty() && reqs[tr.q.front()].kind == RQ_SOLVE) {
+                complete(tr.q.front(), false, m, now_s());
+                tr.q.pop_front();
+            }
+            cv_done.notify_all();
+            any = true;
+            continue;
         }
         // coalesce the tree's queued solves
-        Lane& ln = lanes[j];
         ln.reqs.clear();
         int cols = 0;
         const double t0 = now_s();
@@ -351,8 +395,9 @@ bool rs_server::dispatch_solves(std::unique_lock<std::mutex>& lk) {
                 ln.tab_host[2 * mc + c] = (void*)(r->u + q * nu);
             }
         rs::Tree* t = tr.t;
-        // launch without the lock: submitters may queue meanwhile (the lane and
-        // the tree's slot j belong to this thread until the pass completes)
+        // launch without the lock: submitters may queue meanwhile (the lane, its
+        // arena and the tree's slot 1 + j belong to this thread until the pass
+        // completes)
         lk.unlock();
         std::string msg;
         int rc = 0;
@@ -365,8 +410,9 @@ bool rs_server::dispatch_solves(std::unique_lock<std::mutex>& lk) {
             rc_cols.f = (const double2* const*)ln.tab_dev;
             rc_cols.g = (const double2* const*)(ln.tab_dev + mc);
             rc_cols.u = (double2* const*)(ln.tab_dev + 2 * mc);
-            rs::Slot& sl = t->slot(j);
-            rc = t->slot_enter(sl, ln.st) || t->solve_cols(cols, rc_cols, -1, ln.st, j) || t->slot_leave(sl, ln.st);
+            rs::Slot& sl = t->slot(1 + j);
+            rc = t->slot_enter(sl, ln.st) || t->solve_cols(cols, rc_cols, -1, ln.st, 1 + j) ||
+                 t->slot_leave(sl, ln.st);
             if (rc) msg = t->err;
         }
         if (hipEventRecord(ln.done, ln.st) != hipSuccess && !rc) {
@@ -488,6 +534,7 @@ int rs_server_destroy(rs_server* s) {
         if (l.done) (void)hipEventDestroy(l.done);
         if (l.tab_dev) (void)hipFree(l.tab_dev);
         if (l.tab_host) (void)hipHostFree(l.tab_host);
+        s->free_arena(l);
     }
     delete s;
     return 0;
@@ -650,6 +697,13 @@ int rs_server_reset_stats(rs_server* s) {
     s->st_busy = 0;
     s->st_busy_since = s->any_busy() ? now_s() : -1;
     s->lat.clear();
+    return 0;
+}
+
+int rs_server_lane_bytes(const rs_tree* t, int max_cols, long long* arena, long long* descs) {
+    if (!t || !t->t.planned || max_cols < 1) return 1;
+    if (arena) *arena = t->t.lane_bytes(max_cols);
+    if (descs) *descs = t->t.lane_desc_bytes();
     return 0;
 }
 

@@ -691,7 +691,7 @@ int Tree::make_c64() {
     return 0;
 }
 
-WorkLayout Tree::work_layout(int cap, int esz, bool refine, bool tabs) const {
+WorkLayout Tree::work_layout(int cap, int esz, bool refine, bool tabs, bool descs) const {
     const Plan& P = plan;
     const int L = P.L;
     WorkLayout lay;
@@ -722,7 +722,7 @@ WorkLayout Tree::work_layout(int cap, int esz, bool refine, bool tabs) const {
         wse = std::max(wse, gemm_ws_elems(2 * Vl.nodes, Vl.isz, cap, Vl.bn, solve_ks_target(2 * Vl.nodes, Vl.isz, Vl.bn)));
     }
     if (wse) lay.ws = take(wse * esz);
-    for (int l = 0; l < L; ++l) {
+    for (int l = 0; l < L && descs; ++l) {
         const size_t nd = P.lv[l].nodes, gd = sizeof(GemmDesc);
         lay.dug[l] = take(nd * gd);
         if (l > 0) lay.duh[l] = take(2 * nd * gd);
@@ -762,6 +762,72 @@ template <typename R> SolveLevel Tree::solve_level(const SolveWork<R>& w, int l)
     p.lvl = l;
     p.esz = (int)sizeof(c_t<R>);
     return p;
+}
+
+long long Tree::lane_bytes(int cap) const {
+    const bool mixed = opts.precision == RS_MIXED;
+    return (long long)work_layout(cap, mixed ? 8 : 16, mixed && opts.n_refine > 0, false, false).total;
+}
+
+long long Tree::lane_desc_bytes() const {
+    const WorkLayout d = work_layout(1, 8, false, false, true);
+    const WorkLayout n = work_layout(1, 8, false, false, false);
+    return d.total > n.total ? (long long)(d.total - n.total) : 256;   // (a tree without levels: one small buffer)
+}
+
+// Slot slot_id's workspace (at the tree's precision, cap columns) inside an
+// arena the caller owns -- a server lane's, shared by every tree the lane runs.
+// The product descriptors, which point into the arena and into this tree's
+// solve data, live in the slot's own small buffer; they are written on stream
+// st (before the pass that reads them) whenever the arena or its generation
+// changed since the last binding.
+int Tree::bind_slot(int slot_id, char* base, size_t bytes, unsigned long long gen, int cap, hipStream_t st) {
+    if (!factored) return fail("Error: Robin Tree must be factored first.");
+    const bool mixed = opts.precision == RS_MIXED;
+    if (mixed && !have32 && make_c64()) return 1;
+    const bool refine = mixed && opts.n_refine > 0;
+    const WorkLayout lay = work_layout(cap, mixed ? 8 : 16, refine, false, false);
+    if (lay.total > bytes) return fail("Error: the lane's workspace is smaller than this tree needs.");
+    Slot& sl = slot(slot_id);
+    auto bind = [&](auto& w) -> int {
+        if (sl.ext_base == base && sl.ext_gen == gen && w.cap == lay.cap && w.refine == refine && !w.arena.p)
+            return 0;
+        w.clear();
+        ++w.gen;
+        sl.drop_graphs();
+        w.bind(base, lay);
+        // descriptor arrays of every level, consecutive in the slot's buffer
+        // (256-byte aligned, as in a workspace of its own: lane_desc_bytes bytes)
+        const Plan& P = plan;
+        std::vector<size_t> off(4 * P.L, 0);
+        size_t tot = 0;
+        auto take = [&](size_t bytes) { const size_t o = tot; tot += (bytes + 255) & ~(size_t)255; return o; };
+        for (int l = 0; l < P.L; ++l) {
+            const size_t nd = P.lv[l].nodes * sizeof(GemmDesc);
+            off[4 * l] = take(nd);
+            off[4 * l + 1] = l > 0 ? take(2 * nd) : 0;
+            off[4 * l + 2] = take(2 * nd);
+            off[4 * l + 3] = take(nd);
+        }
+        if (ensure_buf(sl.descs, tot ? tot : 256)) return 1;
+        char* db = sl.descs.as<char>();
+        for (int l = 0; l < P.L; ++l) {
+            w.d_up_g[l] = (GemmDesc*)(db + off[4 * l]);
+            w.d_up_h[l] = l > 0 ? (GemmDesc*)(db + off[4 * l + 1]) : nullptr;
+            w.d_dn_t[l] = (GemmDesc*)(db + off[4 * l + 2]);
+            w.d_dn_g[l] = (GemmDesc*)(db + off[4 * l + 3]);
+            const SolveLevel p = solve_level(w, l);
+            hipLaunchKernelGGL(solve_desc_kernel, dim3((p.nodes + 255) / 256), dim3(256), 0, st, p, w.d_up_g[l],
+                               w.d_up_h[l], w.d_dn_t[l], w.d_dn_g[l]);
+        }
+        HIPCHK(hipGetLastError());
+        auto it = tile_cache.find(2LL * lay.cap + (mixed ? 0 : 1));
+        for (int c = 0; c < 4; ++c) w.tile[c] = it != tile_cache.end() ? it->second[c] : std::vector<int>(P.L, -1);
+        sl.ext_base = base;
+        sl.ext_gen = gen;
+        return 0;
+    };
+    return mixed ? bind(sl.w32) : bind(sl.w64);
 }
 
 template <typename R> int Tree::ensure_work(SolveWork<R>& w, int nrhs, bool refine) {

@@ -127,13 +127,21 @@ struct Slot {
     // completion of the last solve queued on this slot, and its stream
     hipEvent_t done = nullptr;
     hipStream_t last_stream = nullptr;
+    // a workspace in an arena the slot does not own (a server lane's): its base
+    // and generation, and the slot's own product descriptors
+    const char* ext_base = nullptr;
+    unsigned long long ext_gen = 0;
+    DevBuf descs;
     static constexpr int kMaxGraphs = 8;
     void drop_graphs() {
         for (GraphEnt& g : graphs)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
         graphs.clear();
     }
-    void clear() { drop_graphs(); w64.clear(); w32.clear(); tabs.release(); tabs_cap = 0; }
+    void clear() {
+        drop_graphs(); w64.clear(); w32.clear(); tabs.release(); tabs_cap = 0;
+        descs.release(); ext_base = nullptr; ext_gen = 0;
+    }
     ~Slot() { drop_graphs(); if (done) (void)hipEventDestroy(done); }
 };
 
@@ -211,7 +219,12 @@ struct Tree {
     template <typename R> int ensure_work(SolveWork<R>& w, int nrhs, bool refine = false);
     // the arena layout of a workspace of cap columns (esz 8: complex64 passes,
     // 16: complex128), with refinement buffers and/or column tables
-    WorkLayout work_layout(int cap, int esz, bool refine, bool tabs) const;
+    WorkLayout work_layout(int cap, int esz, bool refine, bool tabs, bool descs = true) const;
+    // a server lane's arena: its bytes for cap columns, the per-tree descriptor
+    // bytes, and the binding of slot slot_id to it
+    long long lane_bytes(int cap) const;
+    long long lane_desc_bytes() const;
+    int bind_slot(int slot_id, char* base, size_t bytes, unsigned long long gen, int cap, hipStream_t st);
     template <typename R> SolveLevel solve_level(const SolveWork<R>& w, int l) const;
     // time the tile shapes of every product of the workspace (at its capacity) and keep the fastest
     template <typename R> int tune_work(SolveWork<R>& w);

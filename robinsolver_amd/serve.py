@@ -51,6 +51,17 @@ def bind(L):
     L.rs_server_reset_stats.argtypes = [vp]
     L.rs_server_error.argtypes = [vp, ll]
     L.rs_server_error.restype = ctypes.c_char_p
+    L.rs_server_lane_bytes.argtypes = [vp, ctypes.c_int, llp, llp]
+
+
+def lane_bytes(tree, max_cols):
+    """(lane arena bytes tree needs, the tree's descriptor bytes per lane) for a
+    server with this max_cols: a lane's arena is the largest over its trees."""
+    from . import lib, RobinError
+    a, d = ctypes.c_longlong(), ctypes.c_longlong()
+    if lib().rs_server_lane_bytes(tree.h, int(max_cols), ctypes.byref(a), ctypes.byref(d)):
+        raise RobinError("rs_server_lane_bytes: the tree is not planned, or max_cols < 1.")
+    return a.value, d.value
 
 
 class Server:

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import robinsolver_amd as rsa
-from robinsolver_amd.serve import Server, RS_REQ_DONE
+from robinsolver_amd.serve import Server, RS_REQ_DONE, lane_bytes
 from oracle.pyoracle import OracleTree
 
 pytestmark = pytest.mark.gpu
@@ -160,5 +160,10 @@ def test_memory_estimates_match_allocations(prec, gpu):
         u = torch.zeros((2, t.n_leaves, t.m), dtype=torch.complex128, device="cuda:0")
         srv.wait(srv.solve(tid, 2, f.data_ptr(), g.data_ptr(), u.data_ptr()))
         held2, _ = t.memory_used()
-        # the server's slot holds no column tables of its own (the lane has them)
-        assert abs((held2 - held) - (work - 3 * 24 * 8)) <= 4096, (held2 - held, work)
+        # the tree holds only its descriptors on the lane; the lane holds the
+        # workspace arena (the tree's own workspace without its column tables
+        # and descriptors)
+        arena, descs = lane_bytes(t, 24)
+        assert held2 - held == descs, (held2 - held, descs)
+        assert srv.stats()["held_bytes"] == res + arena + descs
+        assert arena + descs <= work - 3 * 24 * 8 <= arena + descs + 4096

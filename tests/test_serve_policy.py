@@ -1,7 +1,8 @@
 """The serving loop's scheduling policy on the host (simulated device): per-tree
 arrival order, coalescing of queued right-hand sides into passes of at most
 max_cols, factor requests ordered against the tree's solves, admission by
-memory with eviction of idle workspaces, and the latency statistics.  Trees
+memory (one workspace arena per lane, shared by the trees) with eviction of
+idle arenas, and the latency statistics.  Trees
 are really planned (the memory figures come from their plans); passes and
 factorizations take modelled times instead of running on a GPU."""
 import time
@@ -9,7 +10,7 @@ import time
 import pytest
 
 import robinsolver_amd as rsa
-from robinsolver_amd.serve import Server, RS_REQ_DONE, RS_REQ_FAILED
+from robinsolver_amd.serve import Server, RS_REQ_DONE, RS_REQ_FAILED, lane_bytes
 
 UNIT = [[0, 1], [0, 1], [0, 1]]
 
@@ -109,34 +110,61 @@ def test_lanes_overlap_across_trees():
         assert s2 < d1 and s1 < d2          # both passes were in flight together
 
 
-def test_admission_evicts_idle_workspaces():
-    # trees that keep their factorization's working memory (resident = peak)
-    t1 = rsa.GpuTree(2, 4, UNIT, 1.0 / 64, -10.0, precision=rsa.RS_MIXED, n_refine=1, retain=rsa.RS_RETAIN_SCRATCH)
-    t2 = rsa.GpuTree(2, 4, UNIT, 1.0 / 64, -10.0, precision=rsa.RS_MIXED, n_refine=1, retain=rsa.RS_RETAIN_SCRATCH)
-    r1, p1, w1 = t1.memory_estimate(16)
-    assert p1 == r1
-    # room for both factorizations and one solve workspace at a time
-    limit = 2 * r1 + w1 + w1 // 2
+def test_lane_arena_is_shared_across_trees():
+    # three trees of different sizes: a lane holds one arena, the size of the
+    # largest tree it ran, and each tree its descriptors on the lanes it ran on
+    ts = [_tree(64, 4), _tree(128, 6), _tree(256, 8)]
+    res = sum(t.memory_estimate(16)[0] for t in ts)
+    lb = [lane_bytes(t, 16) for t in ts]
+    arena = max(a for a, _ in lb)
+    descs = sum(d for _, d in lb)
+    own = [t.memory_estimate(16)[2] for t in ts]
+    assert arena <= max(own) and sum(own) > 1.2 * arena    # not the sum over trees
+    for lanes in (1, 2):
+        with _server(max_cols=16, lanes=lanes, sim_pass_s=0.02) as srv:
+            ids = [srv.add_tree(t) for t in ts]
+            for i in ids:
+                _factor(srv, i)
+            for _ in range(3):
+                for i in ids:
+                    srv.solve(i, 4, 0, 0, 0)
+                time.sleep(0.005)
+            srv.drain()
+            st = srv.stats()
+        assert st["evictions"] == 0
+        if lanes == 1:
+            assert st["held_bytes"] == res + arena + descs, (st["held_bytes"], res, arena, descs)
+        else:
+            assert res + arena + descs <= st["held_bytes"] <= res + 2 * arena + 2 * descs
+
+
+def test_admission_evicts_idle_lane_arena():
+    # a refactorization that needs the memory of the idle lane arena
+    t = _tree()
+    r, p, _ = t.memory_estimate(16)
+    a, d = lane_bytes(t, 16)
+    x, y = p - r, a + d
+    assert x > 0
+    limit = r + max(x, y) + min(x, a) // 2
     with _server(max_cols=16, lanes=1, hbm_limit=limit) as srv:
-        a, b = srv.add_tree(t1), srv.add_tree(t2)
-        _factor(srv, a)
-        _factor(srv, b)
-        reqs = []
-        for i in range(6):
-            reqs.append(srv.solve(a if i % 2 == 0 else b, 4, 0, 0, 0))
-            time.sleep(0.015)
-        srv.drain()
-        assert all(srv.state(r) == RS_REQ_DONE for r in reqs)
+        tid = srv.add_tree(t)
+        _factor(srv, tid)
+        srv.wait(srv.solve(tid, 4, 0, 0, 0))
+        assert srv.stats()["held_bytes"] == r + a + d
+        _factor(srv, tid)                       # needs x more: the arena goes (the descriptors may stay)
         st = srv.stats()
-        assert st["evictions"] >= 1
-        assert st["held_bytes"] <= limit
+        assert st["evictions"] >= 1 and st["held_bytes"] in (r, r + d)
+        q = srv.solve(tid, 4, 0, 0, 0)          # and come back
+        srv.wait(q)
+        assert srv.stats()["held_bytes"] == r + a + d <= limit
 
 
 def test_admission_refuses_what_can_never_fit():
     t = rsa.GpuTree(2, 4, UNIT, 1.0 / 64, -10.0, precision=rsa.RS_MIXED, n_refine=1, retain=rsa.RS_RETAIN_SCRATCH)
     r, p, w = t.memory_estimate(16)
-    assert p == r
-    with _server(max_cols=16, lanes=1, hbm_limit=r + w // 2) as srv:
+    a, d = lane_bytes(t, 16)
+    assert p == r and 0 < d < a < w
+    with _server(max_cols=16, lanes=1, hbm_limit=r + a // 2) as srv:
         tid = srv.add_tree(t)
         _factor(srv, tid)
         q = srv.solve(tid, 4, 0, 0, 0)
