@@ -283,6 +283,11 @@ typedef struct rs_server_opts {
                               right-hand side and a factorization sim_factor_s (tests of
                               the scheduling policy on a host without a GPU)              */
     double sim_pass_s, sim_col_s, sim_factor_s;
+    int group;             /* trees per pass (default 1): above 1, a pass also takes the
+                              queued solves of further trees of the same precision (oldest
+                              first, within max_cols and the lane's arena) and runs every
+                              level's products as one grouped launch across them; the
+                              lane's arena then also holds the descriptors              */
 } rs_server_opts;
 
 typedef struct rs_server_stats {
@@ -298,6 +303,7 @@ typedef struct rs_server_stats {
     double mean_cols;          /* mean right-hand sides per pass                   */
     long long held_bytes;      /* device bytes the server accounts to its trees and
                                   lanes                                            */
+    double mean_trees;         /* mean trees per pass (above 1 only with group > 1) */
 } rs_server_stats;
 
 enum { RS_REQ_QUEUED = 0, RS_REQ_RUNNING = 1, RS_REQ_DONE = 2, RS_REQ_FAILED = 3 };
@@ -308,7 +314,10 @@ int rs_server_create(rs_server **s, const rs_server_opts *o);
  * streams and tables (the trees stay with their owner; destroy them after). */
 int rs_server_destroy(rs_server *s);
 /* Register a planned (and possibly factored) tree; its id is returned.  While
- * registered, the tree must be used only through the server. */
+ * registered, the tree must be used only through the server.  The server sets
+ * RS_RETAIN_SCRATCH in the tree's options: its factorization's working memory
+ * stays allocated, so that a refactor allocates and frees nothing while passes
+ * of other trees run. */
 int rs_server_add_tree(rs_server *s, rs_tree *t, int *tree_id);
 /* Queue a (re)factorization of tree tree_id with the potential k_dev: device
  * memory [n_leaves][n_int] complex128, valid until the request completes. */
@@ -335,7 +344,8 @@ int rs_server_reset_stats(rs_server *s);
 const char *rs_server_error(rs_server *s, long long req);
 /* What a server with this max_cols holds for tree t (planned): the lane arena
  * the tree needs (a lane's arena is the largest of these over its trees) and
- * the tree's descriptors on one lane.  Host arithmetic, no device needed. */
+ * the tree's descriptors on one lane (with group > 1 the arena holds them:
+ * arena + descs per tree).  Host arithmetic, no device needed. */
 int rs_server_lane_bytes(const rs_tree *t, int max_cols, long long *arena, long long *descs);
 
 #ifdef __cplusplus

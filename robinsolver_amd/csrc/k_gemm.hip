@@ -23,6 +23,8 @@
 // maps (the face permutation between a child's and its parent's boundary
 // ordering), read B from two row ranges (the two children's interface
 // vectors), and add beta*Din in the epilogue.
+#include <algorithm>
+
 #include "rs_common.hpp"
 #include "rs_kernels.hpp"
 
@@ -71,10 +73,13 @@ constexpr int KALIGN = 32; // K-split slices start at multiples of this (whateve
 // few-RHS matrix streams, do best with BK = 32 and one buffer, 12% ahead of
 // BK = 16 with two on the 3D top levels at N = 16.)  The MFMA steps of a chunk
 // are always fully unrolled.
+//
+// One launch may hold several groups of problems (GemmLaunch): each group has
+// its own shape, descriptors, K split and workspace, and owns a consecutive
+// range of workgroups.  A problem runs the same instruction sequence whether
+// its group is launched alone or with others.
 template <typename S, typename R, int BM, int BN, int BK = (BN == 16 ? 32 : 16), bool DB = (BN != 16)>
-__global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ descs, int M, int N, int K,
-                                                  int tilesM, int tilesN, int kslices, int kchunk,
-                                                  c_t<R>* __restrict__ ws) {
+__global__ void __launch_bounds__(256) gemm_kernel(GemmLaunch GL) {
     using V = c_t<R>;
     using VS = c_t<S>;
     using A4 = typename acc_t<R>::T;
@@ -93,10 +98,22 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmDesc* __restrict__ 
     constexpr int SMEM = ((DB ? 2 : 1) * CHUNK > BN * (BM + 1)) ? (DB ? 2 : 1) * CHUNK : BN * (BM + 1);
     __shared__ V smem[SMEM];
 
+    // this workgroup's group (static indices only: the groups stay in kernel arguments)
+    const GemmDesc* descs = GL.g[0].descs;
+    V* ws = (V*)GL.g[0].ws;
+    int M = GL.g[0].M, N = GL.g[0].N, K = GL.g[0].K, tilesM = GL.g[0].tilesM, tilesN = GL.g[0].tilesN,
+        kslices = GL.g[0].ks, kchunk = GL.g[0].kchunk, wg = blockIdx.x;
+#pragma unroll
+    for (int q = 1; q < GEMM_MAX_GROUPS; ++q)
+        if (q < GL.n && (int)blockIdx.x >= GL.g[q].wg0) {
+            descs = GL.g[q].descs; ws = (V*)GL.g[q].ws;
+            M = GL.g[q].M; N = GL.g[q].N; K = GL.g[q].K; tilesM = GL.g[q].tilesM; tilesN = GL.g[q].tilesN;
+            kslices = GL.g[q].ks; kchunk = GL.g[q].kchunk; wg = blockIdx.x - GL.g[q].wg0;
+        }
     const int tiles = tilesM * tilesN;
     const int per = tiles * kslices;
-    const int p = blockIdx.x / per;
-    const int rem = blockIdx.x - p * per;
+    const int p = wg / per;
+    const int rem = wg - p * per;
     const int ks = rem / tiles;
     const int t = rem - ks * tiles;
     const int tm = t % tilesM, tn = t / tilesM;
@@ -348,41 +365,76 @@ int gemm_tile_rule(int M, int N) { return N <= 16 ? GEMM_TILE_64x16 : (M <= 16 ?
 
 namespace {
 
+// One launch over up to GEMM_MAX_GROUPS groups of problems, all with one tile
+// shape; then the K-slice sums of the groups that were split.
 template <typename S, typename R>
-void launch_gemm_sr(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile,
-                    int ks_target) {
-    if (nprob <= 0 || M <= 0 || N <= 0) return;
-    int ks = gemm_kslices(nprob, M, K, ks_target);
-    if (!ws) ks = 1;
-    const int kchunk = ((K + ks - 1) / ks + KALIGN - 1) / KALIGN * KALIGN;
-    ks = (K + kchunk - 1) / kchunk;
-    c_t<R>* W = (c_t<R>*)ws;
+void launch_gemm_groups(const GemmGroup* grp, int ngroups, hipStream_t st, int tile) {
+    GemmLaunch GL = {};
+    int tile_M = 0, tile_N = 0;
+    for (int q = 0; q < ngroups; ++q)
+        if (grp[q].nprob > 0 && grp[q].M > 0 && grp[q].N > 0) {
+            tile_M = std::max(tile_M, grp[q].M);
+            tile_N = std::max(tile_N, grp[q].N);
+        }
+    if (!tile_M) return;
     // Every 16x16 output block runs the same MFMA sequence over K whatever the
     // tile, so the tile shape (by rule from M and N, or chosen by the caller's
     // autotuning) never changes the results.
-    if (tile < 0 || tile >= GEMM_TILES) tile = gemm_tile_rule(M, N);
-    auto run = [&](auto kern, int bm, int bn) {
-        const int tilesM = (M + bm - 1) / bm, tilesN = (N + bn - 1) / bn;
-        hipLaunchKernelGGL(kern, dim3(nprob * tilesM * tilesN * ks), dim3(256), 0, st, d_descs, M, N, K, tilesM, tilesN,
-                           ks, kchunk, W);
-    };
-    switch (tile) {
-        case GEMM_TILE_64x16: run(gemm_kernel<S, R, 64, 16>, 64, 16); break;
-        case GEMM_TILE_16x64: run(gemm_kernel<S, R, 16, 64>, 16, 64); break;
-        case GEMM_TILE_32x64: run(gemm_kernel<S, R, 32, 64>, 32, 64); break;
-        case GEMM_TILE_32x32: run(gemm_kernel<S, R, 32, 32>, 32, 32); break;
-        case GEMM_TILE_64x32: run(gemm_kernel<S, R, 64, 32>, 64, 32); break;
-        case GEMM_TILE_128x16: run(gemm_kernel<S, R, 128, 16>, 128, 16); break;
-        default: run(gemm_kernel<S, R, 64, 64>, 64, 64); break;
+    if (tile < 0 || tile >= GEMM_TILES) tile = gemm_tile_rule(tile_M, tile_N);
+    static const int TBM[GEMM_TILES] = {64, 32, 16, 64, 32, 64, 128}, TBN[GEMM_TILES] = {64, 64, 64, 16, 32, 32, 16};
+    const int bm = TBM[tile], bn = TBN[tile];
+    int wg = 0;
+    for (int q = 0; q < ngroups; ++q) {
+        const GemmGroup& g = grp[q];
+        if (g.nprob <= 0 || g.M <= 0 || g.N <= 0) continue;
+        int ks = gemm_kslices(g.nprob, g.M, g.K, g.ks_target);
+        if (!g.ws) ks = 1;
+        const int kchunk = ((g.K + ks - 1) / ks + KALIGN - 1) / KALIGN * KALIGN;
+        ks = (g.K + kchunk - 1) / kchunk;
+        GemmLaunch::Grp& e = GL.g[GL.n++];
+        e.descs = g.descs; e.ws = g.ws;
+        e.M = g.M; e.N = g.N; e.K = g.K;
+        e.tilesM = (g.M + bm - 1) / bm; e.tilesN = (g.N + bn - 1) / bn;
+        e.ks = ks; e.kchunk = kchunk; e.wg0 = wg;
+        wg += g.nprob * e.tilesM * e.tilesN * ks;
     }
-    if (ks > 1) {
-        const size_t mn = (size_t)M * N;
+    auto run = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(wg), dim3(256), 0, st, GL); };
+    switch (tile) {
+        case GEMM_TILE_64x16: run(gemm_kernel<S, R, 64, 16>); break;
+        case GEMM_TILE_16x64: run(gemm_kernel<S, R, 16, 64>); break;
+        case GEMM_TILE_32x64: run(gemm_kernel<S, R, 32, 64>); break;
+        case GEMM_TILE_32x32: run(gemm_kernel<S, R, 32, 32>); break;
+        case GEMM_TILE_64x32: run(gemm_kernel<S, R, 64, 32>); break;
+        case GEMM_TILE_128x16: run(gemm_kernel<S, R, 128, 16>); break;
+        default: run(gemm_kernel<S, R, 64, 64>); break;
+    }
+    for (int q = 0; q < GL.n; ++q) {
+        const GemmLaunch::Grp& e = GL.g[q];
+        if (e.ks <= 1) continue;
+        const size_t mn = (size_t)e.M * e.N;
+        const int nprob = ((q + 1 < GL.n ? GL.g[q + 1].wg0 : wg) - e.wg0) / (e.tilesM * e.tilesN * e.ks);
         hipLaunchKernelGGL((gemm_reduce_kernel<S, R>), dim3((unsigned)((mn + 255) / 256), nprob), dim3(256), 0, st,
-                           d_descs, M, N, ks, W);
+                           e.descs, e.M, e.N, e.ks, (const c_t<R>*)e.ws);
     }
 }
 
+template <typename S, typename R>
+void launch_gemm_sr(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile,
+                    int ks_target) {
+    GemmGroup g;
+    g.descs = d_descs; g.nprob = nprob; g.M = M; g.N = N; g.K = K; g.ws = ws; g.ks_target = ks_target;
+    launch_gemm_groups<S, R>(&g, 1, st, tile);
+}
+
 }  // namespace
+
+template <typename R>
+void launch_gemm_grouped(const GemmGroup* groups, int ngroups, hipStream_t st, int tile) {
+    for (int q0 = 0; q0 < ngroups; q0 += GEMM_MAX_GROUPS)
+        launch_gemm_groups<R, R>(groups + q0, std::min(GEMM_MAX_GROUPS, ngroups - q0), st, tile);
+}
+template void launch_gemm_grouped<double>(const GemmGroup*, int, hipStream_t, int);
+template void launch_gemm_grouped<float>(const GemmGroup*, int, hipStream_t, int);
 
 template <typename R>
 void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile,

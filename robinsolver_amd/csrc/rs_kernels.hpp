@@ -42,6 +42,29 @@ void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStr
 // factorization).  ws as for launch_gemm<float>.
 void launch_gemm_c64(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws = nullptr,
                      int tile = -1);
+// Several groups of problems of different shapes in one launch (problems of
+// several trees at one level): group q's problems are products of its own M, N
+// and K with its own K split (gemm_kslices of its own nprob, M, K and
+// ks_target, exactly as in a launch of that group alone) into its own
+// workspace, so a group's results are the same bits in either form.
+constexpr int GEMM_MAX_GROUPS = 8;
+struct GemmGroup {
+    const GemmDesc* descs = nullptr;
+    int nprob = 0, M = 0, N = 0, K = 0;
+    void* ws = nullptr;
+    int ks_target = 256;
+};
+// the kernel-side form of a launch (kernel argument)
+struct GemmLaunch {
+    struct Grp {
+        const GemmDesc* descs;
+        void* ws;
+        int M, N, K, tilesM, tilesN, ks, kchunk, wg0;
+    } g[GEMM_MAX_GROUPS];
+    int n;
+};
+template <typename R>
+void launch_gemm_grouped(const GemmGroup* groups, int ngroups, hipStream_t st, int tile = -1);
 int gemm_kslices(int nprob, int M, int K, int ks_target = 256);
 size_t gemm_ws_elems(int nprob, int M, int N, int K, int ks_target = 256);
 // K-split target of the solve passes: with few right-hand sides a long-K product

@@ -72,7 +72,9 @@ struct Lane {
     unsigned long long arena_gen = 0;   // changes with every allocation (the trees rebind)
     double arena_used = 0;       // last use (eviction order)
     bool busy = false;
-    int tree = -1, cols = 0;
+    std::vector<int> trees;      // the trees of the pass in flight (one, or the members of a grouped pass)
+    std::vector<rs::GroupMember> members;
+    int cols = 0;
     std::vector<long long> reqs;
     long long pass = -1;
     double t_sim_end = 0;
@@ -99,7 +101,8 @@ struct rs_server {
     int factors_running = 0;
     std::string err;
     // statistics
-    long long st_done = 0, st_rhs = 0, st_passes = 0, st_factors = 0, st_waits = 0, st_evict = 0, st_failed = 0;
+    long long st_done = 0, st_rhs = 0, st_passes = 0, st_factors = 0, st_waits = 0, st_evict = 0, st_failed = 0,
+              st_members = 0;
     double st_busy = 0, st_busy_since = -1;
     std::vector<double> lat;
 
@@ -162,12 +165,14 @@ bool rs_server::evict_for(long long need, const TreeRec* keep_tr, int keep_lane)
     std::vector<Cand> c;
     for (int j = 0; j < (int)lanes.size(); ++j)
         if (lanes[j].arena_bytes && !lanes[j].busy && j != keep_lane) c.push_back({nullptr, j, lanes[j].arena_used});
-    for (auto& tr : trees) {
+    for (int ti = 0; ti < (int)trees.size(); ++ti) {
+        TreeRec* tr = trees[ti].get();
         if (tr->factoring) continue;
         for (int j = 0; j < (int)lanes.size(); ++j) {
-            if (!tr->ws[j] || (tr.get() == keep_tr && j == keep_lane)) continue;
-            if (lanes[j].busy && trees[lanes[j].tree].get() == tr.get()) continue;
-            c.push_back({tr.get(), j, tr->ws_used[j]});
+            if (!tr->ws[j] || (tr == keep_tr && j == keep_lane)) continue;
+            const std::vector<int>& run = lanes[j].trees;
+            if (lanes[j].busy && std::find(run.begin(), run.end(), ti) != run.end()) continue;
+            c.push_back({tr, j, tr->ws_used[j]});
         }
     }
     std::sort(c.begin(), c.end(), [](const Cand& a, const Cand& b) { return a.used < b.used; });
@@ -198,9 +203,12 @@ void rs_server::finish_pass(int j) {
     Lane& ln = lanes[j];
     const double t = now_s();
     for (long long id : ln.reqs) complete(id, !ln.failed, ln.err, t);
-    TreeRec& tr = *trees[ln.tree];
-    --tr.inflight;
-    tr.ws_used[j] = t;
+    for (int ti : ln.trees) {
+        TreeRec& tr = *trees[ti];
+        --tr.inflight;
+        tr.ws_used[j] = t;
+    }
+    ln.trees.clear();
     ln.arena_used = t;
     ln.busy = false;
     ln.reqs.clear();
@@ -282,6 +290,7 @@ bool rs_server::dispatch_factors() {
 // Fill free lanes with coalesced passes.
 bool rs_server::dispatch_solves(std::unique_lock<std::mutex>& lk) {
     bool any = false;
+    const bool group = o.group > 1;
     for (int j = 0; j < (int)lanes.size(); ++j) {
         if (lanes[j].busy) continue;
         // the tree whose oldest queued solve is oldest
@@ -307,9 +316,15 @@ bool rs_server::dispatch_solves(std::unique_lock<std::mutex>& lk) {
         Lane& ln = lanes[j];
         // the lane's arena, grown to this tree's workspace when it is smaller
         // (the old one is freed first), and the tree's descriptors on the lane
-        const long long wneed = tr.t->lane_bytes(o.max_cols);
+        // (a grouped pass keeps them in the arena)
+        // (grouped: the largest factored tree's, so that any tree's pass fits
+        // and smaller trees' passes can join it)
+        long long wneed = group ? 0 : tr.t->lane_bytes(o.max_cols);
+        if (group)
+            for (const auto& m : trees)
+                if (m->factored && !m->factoring) wneed = std::max(wneed, m->t->group_bytes(o.max_cols));
         const long long grow = wneed > ln.arena_bytes ? wneed : 0;
-        const long long dneed = tr.ws[j] ? 0 : tr.t->lane_desc_bytes();
+        const long long dneed = (group || tr.ws[j]) ? 0 : tr.t->lane_desc_bytes();
         const long long need = (grow ? grow - ln.arena_bytes : 0) + dneed;
         if (need > 0) {
             if (!evict_for(need, &tr, j)) {
@@ -337,10 +352,11 @@ bool rs_server::dispatch_solves(std::unique_lock<std::mutex>& lk) {
                 ln.arena_bytes = (o.simulate || ln.arena) ? grow : 0;
                 ln.arena_gen = ++next_gen;
             }
-            tr.ws[j] = tr.t->lane_desc_bytes();
+            if (!group) tr.ws[j] = tr.t->lane_desc_bytes();
         }
         if (!o.simulate && (!ln.arena_bytes ||
-                            tr.t->bind_slot(1 + j, ln.arena, (size_t)ln.arena_bytes, ln.arena_gen, o.max_cols, ln.st))) {
+                            (!group && tr.t->bind_slot(1 + j, ln.arena, (size_t)ln.arena_bytes, ln.arena_gen,
+                                                       o.max_cols, ln.st)))) {
             const std::string m = ln.arena_bytes ? tr.t->err : std::string("Error: could not allocate the lane's "
                                                                            "solve workspace.");
             tr.t->slot(1 + j).clear();
@@ -353,50 +369,107 @@ bool rs_server::dispatch_solves(std::unique_lock<std::mutex>& lk) {
             any = true;
             continue;
         }
-        // coalesce the tree's queued solves
+        // coalesce the tree's queued solves; a grouped pass then takes the
+        // queued solves of further trees (oldest first, the same precision)
+        // while the pass stays within max_cols and the lane's arena
         ln.reqs.clear();
+        ln.trees.clear();
+        ln.members.clear();
         int cols = 0;
+        size_t used = 0;
         const double t0 = now_s();
         const long long pass = next_pass++;
-        std::vector<const Req*> batch;
-        while (!tr.q.empty()) {
-            Req& r = reqs[tr.q.front()];
-            if (r.kind != RQ_SOLVE || cols + r.nrhs > o.max_cols) break;
-            r.state = RS_REQ_RUNNING;
-            r.t_start = t0;
-            r.pass = pass;
-            cols += r.nrhs;
-            ln.reqs.push_back(tr.q.front());
-            batch.push_back(&r);
-            tr.q.pop_front();
+        std::vector<std::vector<const Req*>> batch;
+        auto take = [&](int ti, int room) {
+            TreeRec& m = *trees[ti];
+            std::vector<const Req*> b;
+            int c = 0;
+            while (!m.q.empty()) {
+                Req& r = reqs[m.q.front()];
+                if (r.kind != RQ_SOLVE || c + r.nrhs > room) break;
+                r.state = RS_REQ_RUNNING;
+                r.t_start = t0;
+                r.pass = pass;
+                c += r.nrhs;
+                ln.reqs.push_back(m.q.front());
+                b.push_back(&r);
+                m.q.pop_front();
+            }
+            ln.trees.push_back(ti);
+            ++m.inflight;
+            rs::GroupMember gm;
+            gm.t = m.t;
+            gm.nrhs = c;
+            if (group) {
+                gm.bytes = (size_t)((m.t->group_bytes(c) + 255) / 256 * 256);
+                gm.base = ln.arena ? ln.arena + used : nullptr;
+                used += gm.bytes;
+            }
+            ln.members.push_back(gm);
+            batch.push_back(b);
+            cols += c;
+        };
+        take(best, o.max_cols);
+        if (group) {
+            std::vector<std::pair<double, int>> cand;
+            for (int ti = 0; ti < (int)trees.size(); ++ti) {
+                TreeRec& m = *trees[ti];
+                if (ti == best || m.factoring || !m.factored || m.q.empty()) continue;
+                if (m.t->opts.precision != tr.t->opts.precision) continue;
+                const Req& r = reqs[m.q.front()];
+                if (r.kind != RQ_SOLVE) continue;
+                cand.push_back({r.t_submit, ti});
+            }
+            std::sort(cand.begin(), cand.end());
+            for (const auto& cd : cand) {
+                if ((int)ln.trees.size() >= o.group || cols >= o.max_cols) break;
+                TreeRec& m = *trees[cd.second];
+                const int first = reqs[m.q.front()].nrhs;
+                if (cols + first > o.max_cols) continue;
+                // the columns that fit, and the workspace they need
+                int c = 0;
+                for (long long id : m.q) {
+                    const Req& r = reqs[id];
+                    if (r.kind != RQ_SOLVE || cols + c + r.nrhs > o.max_cols) break;
+                    c += r.nrhs;
+                }
+                const size_t bytes = (size_t)((m.t->group_bytes(c) + 255) / 256 * 256);
+                if (used + bytes > (size_t)ln.arena_bytes) continue;
+                take(cd.second, o.max_cols - cols);
+            }
         }
         ln.busy = true;
-        ln.tree = best;
         ln.cols = cols;
         ln.pass = pass;
-        ++tr.inflight;
         ++st_passes;
+        st_members += (long long)ln.trees.size();
         if (st_busy_since < 0) st_busy_since = t0;
         any = true;
         if (o.simulate) {
             ln.t_sim_end = t0 + o.sim_pass_s + o.sim_col_s * cols;
             continue;
         }
-        // column tables of the pass
-        const rs::Plan& P = tr.t->plan;
-        const size_t nf = (size_t)P.n_leaves * P.n_int, ng = (size_t)P.n_leaves * P.nb,
-                     nu = (size_t)P.n_leaves * P.m;
+        // column tables of the pass: the members' columns one after another
         const int mc = o.max_cols;
         int c = 0;
-        for (const Req* r : batch)
-            for (int q = 0; q < r->nrhs; ++q, ++c) {
-                ln.tab_host[c] = (void*)(r->f + q * nf);
-                ln.tab_host[mc + c] = (void*)(r->g + q * ng);
-                ln.tab_host[2 * mc + c] = (void*)(r->u + q * nu);
-            }
+        for (size_t q = 0; q < ln.members.size(); ++q) {
+            const rs::Plan& P = ln.members[q].t->plan;
+            const size_t nf = (size_t)P.n_leaves * P.n_int, ng = (size_t)P.n_leaves * P.nb,
+                         nu = (size_t)P.n_leaves * P.m;
+            rs::RhsCols& rc_cols = ln.members[q].cols;
+            rc_cols.f = (const double2* const*)(ln.tab_dev + c);
+            rc_cols.g = (const double2* const*)(ln.tab_dev + mc + c);
+            rc_cols.u = (double2* const*)(ln.tab_dev + 2 * mc + c);
+            for (const Req* r : batch[q])
+                for (int k = 0; k < r->nrhs; ++k, ++c) {
+                    ln.tab_host[c] = (void*)(r->f + k * nf);
+                    ln.tab_host[mc + c] = (void*)(r->g + k * ng);
+                    ln.tab_host[2 * mc + c] = (void*)(r->u + k * nu);
+                }
+        }
         rs::Tree* t = tr.t;
         // launch without the lock: submitters may queue meanwhile (the lane, its
-        // arena and the tree's slot 1 + j belong to this thread until the pass
+        // arena and the members' slots 1 + j belong to this thread until the pass
         // completes)
         lk.unlock();
         std::string msg;
@@ -405,13 +478,11 @@ bool rs_server::dispatch_solves(std::unique_lock<std::mutex>& lk) {
                            ln.st) != hipSuccess) {
             rc = 1;
             msg = "HIP error: column table upload failed.";
+        } else if (group) {
+            rc = rs::Tree::solve_grouped(ln.members, ln.st, msg);
         } else {
-            rs::RhsCols rc_cols;
-            rc_cols.f = (const double2* const*)ln.tab_dev;
-            rc_cols.g = (const double2* const*)(ln.tab_dev + mc);
-            rc_cols.u = (double2* const*)(ln.tab_dev + 2 * mc);
             rs::Slot& sl = t->slot(1 + j);
-            rc = t->slot_enter(sl, ln.st) || t->solve_cols(cols, rc_cols, -1, ln.st, 1 + j) ||
+            rc = t->slot_enter(sl, ln.st) || t->solve_cols(cols, ln.members[0].cols, -1, ln.st, 1 + j) ||
                  t->slot_leave(sl, ln.st);
             if (rc) msg = t->err;
         }
@@ -471,6 +542,7 @@ void rs_server_opts_default(rs_server_opts* o) {
     o->sim_pass_s = 1e-3;
     o->sim_col_s = 1e-5;
     o->sim_factor_s = 5e-3;
+    o->group = 1;
 }
 
 int rs_server_create(rs_server** out, const rs_server_opts* o) {
@@ -481,6 +553,10 @@ int rs_server_create(rs_server** out, const rs_server_opts* o) {
     if (o) oo = *o;
     if (oo.max_cols < 1 || oo.lanes < 1 || oo.lanes > 8) {
         g_srv_err = "Error: max_cols must be positive and lanes in 1..8.";
+        return 1;
+    }
+    if (oo.group < 1 || oo.group > rs::GEMM_MAX_GROUPS) {
+        g_srv_err = "Error: group must be in 1..8 (trees per pass).";
         return 1;
     }
     if (oo.simulate && oo.hbm_limit <= 0) {
@@ -549,6 +625,10 @@ int rs_server_add_tree(rs_server* s, rs_tree* t, int* tree_id) {
     if (!t->t.planned) return s->fail("Error: rs_plan must be called before rs_server_add_tree.");
     for (const auto& tr : s->trees)
         if (tr->h == t) return s->fail("Error: tree already registered with this server.");
+    // A served tree keeps its factorization's working memory: a refactor then
+    // allocates and frees nothing (a device free waits for the device to drain,
+    // which stalls every lane's passes meanwhile).
+    t->t.opts.retain |= RS_RETAIN_SCRATCH;
     std::unique_ptr<TreeRec> tr(new TreeRec);
     tr->h = t;
     tr->t = &t->t;
@@ -686,6 +766,7 @@ int rs_server_get_stats(rs_server* s, rs_server_stats* st) {
     st->lat_p99_s = q(0.99);
     st->lat_max_s = l.empty() ? 0.0 : l.back();
     st->mean_cols = s->st_passes ? (double)s->st_rhs / (double)s->st_passes : 0.0;
+    st->mean_trees = s->st_passes ? (double)s->st_members / (double)s->st_passes : 0.0;
     st->held_bytes = s->held();
     return 0;
 }
@@ -694,6 +775,7 @@ int rs_server_reset_stats(rs_server* s) {
     if (!s) return 1;
     std::lock_guard<std::mutex> g(s->mu);
     s->st_done = s->st_rhs = s->st_passes = s->st_factors = s->st_waits = s->st_evict = s->st_failed = 0;
+    s->st_members = 0;
     s->st_busy = 0;
     s->st_busy_since = s->any_busy() ? now_s() : -1;
     s->lat.clear();

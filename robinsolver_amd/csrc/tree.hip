@@ -1269,6 +1269,185 @@ int Tree::solve_launches(int nrhs, const RhsCols& cols, int prec, hipStream_t st
     return solve_pass(w64, nrhs, io, st);
 }
 
+long long Tree::group_bytes(int nrhs) const {
+    const bool mixed = opts.precision == RS_MIXED;
+    return (long long)work_layout(nrhs, mixed ? 8 : 16, mixed && opts.n_refine > 0, false, true).total;
+}
+
+// The launches of solve_pass for several trees at once: the leaf stages and
+// gathers per tree, each level's products as one grouped launch.  Up pass:
+// level L_t - 1 - h of every tree t at height h; down pass: level d of every
+// tree at depth d.  Within a tree the launch order is the one of solve_pass.
+template <typename R>
+int Tree::pass_grouped(const std::vector<Tree*>& ts, const std::vector<SolveWork<R>*>& ws,
+                       const std::vector<int>& nrhs, const std::vector<PassIO>& io, hipStream_t st) {
+    using V = c_t<R>;
+    const int n = (int)ts.size();
+    int maxL = 0;
+    for (Tree* t : ts) maxL = std::max(maxL, t->plan.L);
+    auto sinv_of = [&](Tree* t) {
+        return std::is_same<R, double>::value ? (const V*)t->sinv.p : (const V*)t->sinv32.p;
+    };
+    for (int q = 0; q < n; ++q) {
+        Tree* t = ts[q];
+        LeafArgs<R> up;
+        up.mode = LEAF_UP; up.ncols = nrhs[q]; up.n_leaves = t->plan.n_leaves; up.sinv = sinv_of(t);
+        up.f64 = io[q].f64; up.f32 = io[q].f32; up.jump = io[q].jump; up.hout = ws[q]->hV[t->plan.L];
+        launch_leaf_apply<R>(t->G, up, t->lds_limit, st);
+    }
+    std::vector<GemmGroup> g;
+    auto add = [&](const GemmDesc* d, int np, int M, int N, int K, void* wsp) {
+        GemmGroup e;
+        e.descs = d; e.nprob = np; e.M = M; e.N = N; e.K = K; e.ws = wsp; e.ks_target = solve_ks_target(np, M, K);
+        g.push_back(e);
+    };
+    for (int h = 0; h < maxL; ++h) {
+        g.clear();
+        for (int q = 0; q < n; ++q) {
+            const int l = ts[q]->plan.L - 1 - h;
+            if (l < 0) continue;
+            const Level& Vl = ts[q]->plan.lv[l];
+            add(ws[q]->d_up_g[l], Vl.nodes, 2 * Vl.isz, nrhs[q], 2 * Vl.isz, ws[q]->ws);
+        }
+        launch_gemm_grouped<R>(g.data(), (int)g.size(), st);
+        g.clear();
+        for (int q = 0; q < n; ++q) {
+            const int l = ts[q]->plan.L - 1 - h;
+            if (l <= 0) continue;
+            const Level& Vl = ts[q]->plan.lv[l];
+            add(ws[q]->d_up_h[l], 2 * Vl.nodes, Vl.bn, nrhs[q], Vl.isz, ws[q]->ws);
+        }
+        if (!g.empty()) launch_gemm_grouped<R>(g.data(), (int)g.size(), st);
+    }
+    for (int q = 0; q < n; ++q) {
+        Tree* t = ts[q];
+        const Plan& P = t->plan;
+        if (io[q].g32)
+            launch_root_gather32<V>(ws[q]->gF[0], io[q].g32, t->d_root_leaf.as<int>(), t->d_root_loc.as<int>(),
+                                    P.lv[0].P, P.n_leaves, P.nb, nrhs[q], st);
+        else
+            launch_root_gather<V>(ws[q]->gF[0], io[q].g, t->d_root_leaf.as<int>(), t->d_root_loc.as<int>(),
+                                  P.lv[0].P, P.nb, nrhs[q], st);
+    }
+    for (int d = 0; d < maxL; ++d) {
+        for (int q = 0; q < n; ++q) {
+            Tree* t = ts[q];
+            if (d >= t->plan.L) continue;
+            const Level& Vl = t->plan.lv[d];
+            launch_child_gather<V>(ws[q]->gF[d + 1], ws[q]->gF[d], t->d_prow[d]->as<int>(), Vl.bn, Vl.bc, Vl.P,
+                                   Vl.nodes, d == 0, nrhs[q], st);
+        }
+        g.clear();
+        for (int q = 0; q < n; ++q) {
+            if (d >= ts[q]->plan.L) continue;
+            const Level& Vl = ts[q]->plan.lv[d];
+            add(ws[q]->d_dn_t[d], 2 * Vl.nodes, Vl.isz, nrhs[q], Vl.bn, ws[q]->ws);
+        }
+        launch_gemm_grouped<R>(g.data(), (int)g.size(), st);
+        g.clear();
+        for (int q = 0; q < n; ++q) {
+            if (d >= ts[q]->plan.L) continue;
+            const Level& Vl = ts[q]->plan.lv[d];
+            add(ws[q]->d_dn_g[d], Vl.nodes, 2 * Vl.isz, nrhs[q], 2 * Vl.isz, ws[q]->ws);
+        }
+        launch_gemm_grouped<R>(g.data(), (int)g.size(), st);
+    }
+    for (int q = 0; q < n; ++q) {
+        Tree* t = ts[q];
+        LeafArgs<R> dn;
+        dn.mode = LEAF_DOWN; dn.ncols = nrhs[q]; dn.n_leaves = t->plan.n_leaves; dn.sinv = sinv_of(t);
+        dn.f64 = io[q].f64; dn.f32 = io[q].f32; dn.gin = ws[q]->gF[t->plan.L];
+        dn.u64 = io[q].u64; dn.u32 = io[q].u32; dn.ubase32 = io[q].ubase32; dn.uacc = io[q].uacc;
+        launch_leaf_apply<R>(t->G, dn, t->lds_limit, st);
+    }
+    return 0;
+}
+
+int Tree::solve_grouped(std::vector<GroupMember>& ms, hipStream_t st, std::string& err) {
+    auto bad = [&](const std::string& m) { err = m; return 1; };
+    if (ms.empty()) return 0;
+    const int prec = ms[0].t->opts.precision;
+    const bool mixed = prec == RS_MIXED;
+    for (GroupMember& m : ms) {
+        Tree* t = m.t;
+        if (!t || !t->factored) return bad("Error: Robin Tree must be factored first.");
+        if (t->opts.precision != prec) return bad("Error: a grouped pass needs trees of one precision.");
+        if (m.nrhs < 1) return bad("Error: nrhs must be positive.");
+        if (!mixed && t->merge_c64)
+            return bad("Error: an FP64 solve needs a tree factored in FP64 (this tree's merge products are "
+                       "complex64).");
+        if (mixed && !t->have32 && t->make_c64()) return bad(t->err);
+        if ((long long)m.bytes < t->group_bytes(m.nrhs)) return bad("Error: a grouped member's workspace is too small.");
+    }
+    const int n = (int)ms.size();
+    std::vector<Tree*> ts(n);
+    std::vector<int> nr(n);
+    for (int q = 0; q < n; ++q) { ts[q] = ms[q].t; nr[q] = ms[q].nrhs; }
+    // the members' workspaces and product descriptors, written on the stream
+    auto prepare = [&](auto& w, GroupMember& m) {
+        Tree* t = m.t;
+        const WorkLayout lay =
+            t->work_layout(m.nrhs, mixed ? 8 : 16, mixed && t->opts.n_refine > 0, false, true);
+        w.bind(m.base, lay);
+        for (int l = 0; l < t->plan.L; ++l) {
+            const SolveLevel p = t->solve_level(w, l);
+            hipLaunchKernelGGL(solve_desc_kernel, dim3((p.nodes + 255) / 256), dim3(256), 0, st, p, w.d_up_g[l],
+                               w.d_up_h[l], w.d_dn_t[l], w.d_dn_g[l]);
+        }
+    };
+    if (!mixed) {
+        std::vector<SolveWork<double>> w(n);
+        std::vector<SolveWork<double>*> wp(n);
+        std::vector<PassIO> io(n);
+        for (int q = 0; q < n; ++q) {
+            prepare(w[q], ms[q]);
+            wp[q] = &w[q];
+            io[q].f64 = ms[q].cols.f; io[q].g = ms[q].cols.g; io[q].u64 = ms[q].cols.u;
+        }
+        if (hipGetLastError() != hipSuccess) return bad("HIP error: descriptor kernels of a grouped pass.");
+        pass_grouped<double>(ts, wp, nr, io, st);
+    } else {
+        std::vector<SolveWork<float>> w(n);
+        std::vector<SolveWork<float>*> wp(n);
+        std::vector<PassIO> io(n);
+        int maxref = 0;
+        for (int q = 0; q < n; ++q) {
+            prepare(w[q], ms[q]);
+            wp[q] = &w[q];
+            const int nref = ts[q]->opts.n_refine;
+            maxref = std::max(maxref, nref);
+            io[q].f64 = ms[q].cols.f; io[q].g = ms[q].cols.g;
+            // the complex64 first iterate (refined below), or the answer itself
+            if (nref <= 0) io[q].u64 = ms[q].cols.u;
+            else io[q].u32 = w[q].uA;
+        }
+        if (hipGetLastError() != hipSuccess) return bad("HIP error: descriptor kernels of a grouped pass.");
+        pass_grouped<float>(ts, wp, nr, io, st);
+        // u = u_i + K32^-1 (b - K u_i) for each tree's fixed number of steps
+        for (int it = 0; it < maxref; ++it) {
+            std::vector<Tree*> st_t;
+            std::vector<SolveWork<float>*> st_w;
+            std::vector<int> st_n;
+            std::vector<PassIO> st_io;
+            for (int q = 0; q < n; ++q) {
+                if (ts[q]->opts.n_refine <= it) continue;
+                const RhsCols& c = ms[q].cols;
+                if (ts[q]->residual(nr[q], it == 0 ? nullptr : c.u, it == 0 ? w[q].uA : nullptr, c.f, c.g, w[q].rf,
+                                    w[q].rg, w[q].rj, st))
+                    return bad(ts[q]->err);
+                PassIO ci;
+                ci.f32 = w[q].rf; ci.jump = w[q].rj; ci.g32 = w[q].rg; ci.u64 = c.u;
+                if (it == 0) ci.ubase32 = w[q].uA;
+                else ci.uacc = 1;
+                st_t.push_back(ts[q]); st_w.push_back(wp[q]); st_n.push_back(nr[q]); st_io.push_back(ci);
+            }
+            pass_grouped<float>(st_t, st_w, st_n, st_io, st);
+        }
+    }
+    if (hipGetLastError() != hipSuccess) return bad("HIP error: a grouped pass's launches.");
+    return 0;
+}
+
 int Tree::reserve_slot(int slot_id, int cap) {
     if (!factored) return fail("Error: Robin Tree must be factored first.");
     Slot& sl = slot(slot_id);

@@ -145,6 +145,17 @@ struct Slot {
     ~Slot() { drop_graphs(); if (done) (void)hipEventDestroy(done); }
 };
 
+struct Tree;
+// A tree's share of a grouped pass: its right-hand sides (column tables) and
+// its workspace inside the caller's memory.
+struct GroupMember {
+    Tree* t = nullptr;
+    int nrhs = 0;
+    RhsCols cols;
+    char* base = nullptr;
+    size_t bytes = 0;
+};
+
 struct Tree {
     rs_opts opts;
     Plan plan;
@@ -258,6 +269,17 @@ struct Tree {
                              char* scratch, size_t scratch_bytes);
     // allocate slot slot_id's workspace for cap columns at the tree's precision
     int reserve_slot(int slot_id, int cap);
+    // One pass over several trees' right-hand sides (the same precision): each
+    // tree's leaf stages and gathers launched per tree, every level's four
+    // products launched once for all trees (grouped by height in the up pass,
+    // by depth in the down pass).  A member's workspace (group_bytes of its
+    // column count, product descriptors included) lies in the caller's memory.
+    // Every right-hand side gets the bits of the tree's own solve.
+    long long group_bytes(int nrhs) const;
+    static int solve_grouped(std::vector<GroupMember>& ms, hipStream_t st, std::string& err);
+    template <typename R>
+    static int pass_grouped(const std::vector<Tree*>& ts, const std::vector<SolveWork<R>*>& ws,
+                            const std::vector<int>& nrhs, const std::vector<PassIO>& io, hipStream_t st);
     // device bytes: a solve workspace of cap columns at precision prec (with the
     // refinement buffers of the mixed path), and the factor's resident and peak bytes
     long long work_bytes(int cap, int prec) const;

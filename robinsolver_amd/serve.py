@@ -17,7 +17,7 @@ RS_REQ_QUEUED, RS_REQ_RUNNING, RS_REQ_DONE, RS_REQ_FAILED = 0, 1, 2, 3
 class ServerOpts(ctypes.Structure):
     _fields_ = [("max_cols", ctypes.c_int), ("lanes", ctypes.c_int), ("hbm_reserve", ctypes.c_longlong),
                 ("hbm_limit", ctypes.c_longlong), ("simulate", ctypes.c_int), ("sim_pass_s", ctypes.c_double),
-                ("sim_col_s", ctypes.c_double), ("sim_factor_s", ctypes.c_double)]
+                ("sim_col_s", ctypes.c_double), ("sim_factor_s", ctypes.c_double), ("group", ctypes.c_int)]
 
 
 class ServerStats(ctypes.Structure):
@@ -26,7 +26,7 @@ class ServerStats(ctypes.Structure):
                 ("admission_waits", ctypes.c_longlong), ("evictions", ctypes.c_longlong),
                 ("failed", ctypes.c_longlong), ("busy_s", ctypes.c_double), ("lat_p50_s", ctypes.c_double),
                 ("lat_p99_s", ctypes.c_double), ("lat_max_s", ctypes.c_double), ("mean_cols", ctypes.c_double),
-                ("held_bytes", ctypes.c_longlong)]
+                ("held_bytes", ctypes.c_longlong), ("mean_trees", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -68,7 +68,7 @@ class Server:
     """Request loop over resident trees (see include/robinsolver.h, rs_server_*)."""
 
     def __init__(self, max_cols=256, lanes=2, hbm_reserve=2 << 30, hbm_limit=0, simulate=False,
-                 sim_pass_s=1e-3, sim_col_s=1e-5, sim_factor_s=5e-3):
+                 sim_pass_s=1e-3, sim_col_s=1e-5, sim_factor_s=5e-3, group=1):
         from . import lib, RobinError
         self._L = lib()
         self._err = RobinError
@@ -76,6 +76,7 @@ class Server:
         self._L.rs_server_opts_default(ctypes.byref(o))
         o.max_cols, o.lanes, o.hbm_reserve, o.hbm_limit = max_cols, lanes, hbm_reserve, hbm_limit
         o.simulate, o.sim_pass_s, o.sim_col_s, o.sim_factor_s = int(simulate), sim_pass_s, sim_col_s, sim_factor_s
+        o.group = group
         h = ctypes.c_void_p()
         if self._L.rs_server_create(ctypes.byref(h), ctypes.byref(o)):
             raise RobinError(self._L.rs_server_error(None, -1).decode())

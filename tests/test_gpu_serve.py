@@ -1,6 +1,7 @@
 """GPU: the serving loop.  Every request's result is bitwise the same when it
 is solved alone (rs_solve_device, one request per call) and when the server
-coalesces it with other requests of several trees on several streams; a
+coalesces it with other requests of several trees on several streams, or
+runs it in a pass grouped across trees; a
 factorization queued through the server (including a refactor in place with a
 new potential) gives the same results as a fresh tree; mixed-path results stay
 within rel. l2 1e-8 of the FP64 oracle; the host-side memory estimates match
@@ -51,9 +52,11 @@ def _alone(t, f, g):
     return u.cpu().numpy()
 
 
-@pytest.mark.parametrize("prec", [rsa.RS_MIXED, rsa.RS_FP64])
-def test_results_same_alone_and_in_the_mix(prec, gpu):
-    print("case: serve mix, precision", prec, flush=True)
+@pytest.mark.parametrize("prec,group", [(rsa.RS_MIXED, 1), (rsa.RS_FP64, 1), (rsa.RS_MIXED, 4), (rsa.RS_FP64, 4)])
+def test_results_same_alone_and_in_the_mix(prec, group, gpu):
+    """group 4: passes that hold several trees' right-hand sides, every level's
+    products launched once across them, give every request its own bits."""
+    print("case: serve mix, precision", prec, "group", group, flush=True)
     torch = _torch()
     specs = [(2, 64, 4, 10.0), (2, 128, 6, 20.0), (3, 16, 6, 4.0)]
     trees = []
@@ -71,7 +74,7 @@ def test_results_same_alone_and_in_the_mix(prec, gpu):
         g = _dev(_rand_c(rng, (nr, t.n_leaves, t.nb)))
         reqs.append((ti, f, g))
     alone = [_alone(trees[ti], f, g) for ti, f, g in reqs]
-    with Server(max_cols=16, lanes=2) as srv:
+    with Server(max_cols=16, lanes=2 if group == 1 else 1, group=group) as srv:
         ids = [srv.add_tree(t) for t in trees]
         outs, rids = [], []
         for ti, f, g in reqs:
@@ -85,6 +88,8 @@ def test_results_same_alone_and_in_the_mix(prec, gpu):
         st = srv.stats()
         print("stats:", st, flush=True)
         assert st["passes"] < len(reqs)          # some requests were coalesced
+        if group > 1:
+            assert st["mean_trees"] > 1.0        # and some passes held several trees
     for (ti, f, g), u, a in zip(reqs, outs, alone):
         assert np.array_equal(u.cpu().numpy(), a)
 
@@ -147,10 +152,12 @@ def test_memory_estimates_match_allocations(prec, gpu):
     print("case: memory estimates, precision", prec, flush=True)
     torch = _torch()
     t, k = _make(2, 128, 6, 12.0, prec, autotune=False)   # no tuning workspace after the factor
-    res, peak, work = t.memory_estimate(24)
     kd = _dev(k)
     with Server(max_cols=24, lanes=1) as srv:
         tid = srv.add_tree(t)
+        # a served tree keeps its factorization's working memory
+        res, peak, work = t.memory_estimate(24)
+        assert res == peak
         srv.wait(srv.factor(tid, kd.data_ptr()))
         held, most = t.memory_used()
         assert held == res

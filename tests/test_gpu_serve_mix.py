@@ -29,9 +29,10 @@ def _rel(a, b):
     return float(wl.rel_l2_cols(a, b).max())
 
 
-def test_serving_mix_of_eight_trees(gpu):
+@pytest.mark.parametrize("group", [1, 8])
+def test_serving_mix_of_eight_trees(group, gpu):
     import torch
-    print("case: serving mix, 8 trees", flush=True)
+    print("case: serving mix, 8 trees, group", group, flush=True)
     rng = np.random.default_rng(55)
     trees = []
     for gi, (n, L, kb) in enumerate(wl.MIX_TREES):
@@ -66,7 +67,7 @@ def test_serving_mix_of_eight_trees(gpu):
     print("  %d requests, %d refactorizations" % (nreq, sum(j["kind"] == "factor" for j in jobs)), flush=True)
 
     arrivals = np.cumsum(rng.exponential(1.0 / 1500.0, nreq))
-    with Server(max_cols=256, lanes=2) as srv:
+    with Server(max_cols=256, lanes=2, group=group) as srv:
         ids = [srv.add_tree(T["t"]) for T in trees]
         rids = []
         t0 = time.perf_counter()

@@ -114,7 +114,7 @@ def test_lane_arena_is_shared_across_trees():
     # three trees of different sizes: a lane holds one arena, the size of the
     # largest tree it ran, and each tree its descriptors on the lanes it ran on
     ts = [_tree(64, 4), _tree(128, 6), _tree(256, 8)]
-    res = sum(t.memory_estimate(16)[0] for t in ts)
+    res = sum(t.memory_estimate(16)[1] for t in ts)   # served: resident = peak
     lb = [lane_bytes(t, 16) for t in ts]
     arena = max(a for a, _ in lb)
     descs = sum(d for _, d in lb)
@@ -138,25 +138,66 @@ def test_lane_arena_is_shared_across_trees():
             assert res + arena + descs <= st["held_bytes"] <= res + 2 * arena + 2 * descs
 
 
-def test_admission_evicts_idle_lane_arena():
-    # a refactorization that needs the memory of the idle lane arena
-    t = _tree()
-    r, p, _ = t.memory_estimate(16)
-    a, d = lane_bytes(t, 16)
-    x, y = p - r, a + d
-    assert x > 0
-    limit = r + max(x, y) + min(x, a) // 2
-    with _server(max_cols=16, lanes=1, hbm_limit=limit) as srv:
-        tid = srv.add_tree(t)
-        _factor(srv, tid)
-        srv.wait(srv.solve(tid, 4, 0, 0, 0))
-        assert srv.stats()["held_bytes"] == r + a + d
-        _factor(srv, tid)                       # needs x more: the arena goes (the descriptors may stay)
+def test_grouped_passes_hold_several_trees():
+    ts = [_tree(64, 4), _tree(128, 6), _tree(256, 8)]
+    sizes = [3, 4, 5, 2, 6, 1, 7, 3, 2]
+    with _server(max_cols=16, lanes=1, group=3, sim_pass_s=0.03) as srv:
+        ids = [srv.add_tree(t) for t in ts]
+        for i in ids:
+            _factor(srv, i)
+        first = srv.solve(ids[0], 2, 0, 0, 0)        # occupies the lane meanwhile
+        time.sleep(0.005)
+        reqs = [(k % 3, srv.solve(ids[k % 3], n, 0, 0, 0)) for k, n in enumerate(sizes)]
+        srv.drain()
         st = srv.stats()
-        assert st["evictions"] >= 1 and st["held_bytes"] in (r, r + d)
-        q = srv.solve(tid, 4, 0, 0, 0)          # and come back
-        srv.wait(q)
-        assert srv.stats()["held_bytes"] == r + a + d <= limit
+        assert srv.state(first) == RS_REQ_DONE
+        pid = [srv.info(r)[3] for _, r in reqs]
+    per_pass = {}
+    for (ti, _), n, p in zip(reqs, sizes, pid):
+        per_pass.setdefault(p, []).append((ti, n))
+    for ti in range(3):                               # per tree, arrival order
+        mine = [p for (t, _), p in zip(reqs, pid) if t == ti]
+        assert mine == sorted(mine)
+    assert all(sum(n for _, n in v) <= 16 for v in per_pass.values())
+    assert max(len({ti for ti, _ in v}) for v in per_pass.values()) > 1
+    assert st["mean_trees"] > 1.0
+    res = sum(t.memory_estimate(16)[1] for t in ts)   # served: resident = peak
+    arena = max(sum(lane_bytes(t, 16)) for t in ts)   # the arena holds the descriptors too
+    assert st["held_bytes"] == res + arena
+
+
+def test_admission_evicts_idle_lane_arena():
+    # a second tree's factorization that needs the memory of the idle lane arena
+    t1, t2 = _tree(), _tree()
+    with _server(max_cols=16, lanes=1, hbm_limit=1 << 40) as srv:
+        srv.add_tree(t1)                        # (registration keeps the factor's working memory)
+    r, p, _ = t1.memory_estimate(16)
+    assert r == p
+    a, d = lane_bytes(t1, 16)
+    limit = 2 * p + a + d - a // 2
+    with _server(max_cols=16, lanes=1, hbm_limit=limit) as srv:
+        i1, i2 = srv.add_tree(t1), srv.add_tree(t2)
+        _factor(srv, i1)
+        srv.wait(srv.solve(i1, 4, 0, 0, 0))
+        assert srv.stats()["held_bytes"] == p + a + d
+        _factor(srv, i2)                        # needs p: the arena goes (the descriptors may stay)
+        st = srv.stats()
+        assert st["evictions"] >= 1 and st["held_bytes"] in (2 * p, 2 * p + d)
+        assert st["held_bytes"] <= limit
+
+
+def test_served_tree_refactors_in_place():
+    # registered trees keep their factorization's working memory: resident = peak
+    t = _tree()
+    r0, p0, _ = t.memory_estimate(16)
+    assert r0 < p0
+    with _server(max_cols=16, lanes=1) as srv:
+        tid = srv.add_tree(t)
+        r, p, _ = t.memory_estimate(16)
+        assert r == p == p0
+        _factor(srv, tid)
+        _factor(srv, tid)
+        assert srv.stats()["held_bytes"] == p
 
 
 def test_admission_refuses_what_can_never_fit():
