@@ -444,6 +444,25 @@ def main():
     torch.cuda.synchronize()
     r1_rate = nr1 / (time.perf_counter() - t0)
     log("fp64 R=1: %.1f solves/s" % r1_rate)
+    # the same one-RHS solves with HIP events around every launch: algorithmic
+    # bytes per solve (leaf blocks twice, inv(M) twice, T_NI and T_IN once per
+    # level, the vectors) over the measured time per solve
+    t64.profile(True)
+    t64.profile_read()
+    for r in range(nr1):
+        solve(t64, 1, f.data_ptr() + r * sz_f * 16, g.data_ptr() + r * sz_g * 16, u64.data_ptr() + r * sz_u * 16,
+              rsa.RS_FP64)
+    torch.cuda.synchronize()
+    prof1 = t64.profile_read()
+    t64.profile(False)
+    r1_bytes = sum(v[2] for v in prof1.values()) / nr1
+    r1_launches = sum(v[1] for v in prof1.values()) / nr1
+    r1_roof = {"bound": "hbm", "achieved": r1_bytes * r1_rate / 1e9, "peak": PEAK["hbm_GBs"], "unit": "GB/s",
+               "algorithmic_bytes_per_solve": r1_bytes, "launches_per_solve": r1_launches,
+               "kernel_ms_per_solve": {c: v[0] / nr1 for c, v in prof1.items() if v[1]}}
+    r1_roof["frac"] = r1_roof["achieved"] / r1_roof["peak"]
+    log("fp64 R=1 roofline: %.3f GB/solve, %.0f GB/s, frac %.3f, %.0f launches" %
+        (r1_bytes / 1e9, r1_roof["achieved"], r1_roof["frac"], r1_launches))
 
     # batched FP64
     solve(t64, args.nrhs, f.data_ptr(), g.data_ptr(), u64.data_ptr(), rsa.RS_FP64)
@@ -583,6 +602,7 @@ def main():
         "factor_s_fp64": f_time_64,
         "factor_s_mixed": f_time_mixed,
         "fp64_r1_solves_per_s": r1_rate,
+        "fp64_r1_roofline": r1_roof,
         "fp64_batched_solves_per_s": b64_rate,
         "mixed_over_fp64_r1": mixed_rate / r1_rate,
         "mixed_rel_l2_vs_fp64": {"max": float(rel.max()), "median": float(np.median(rel))},

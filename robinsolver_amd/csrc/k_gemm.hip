@@ -24,6 +24,7 @@
 // ordering), read B from two row ranges (the two children's interface
 // vectors), and add beta*Din in the epilogue.
 #include <algorithm>
+#include <type_traits>
 
 #include "rs_common.hpp"
 #include "rs_kernels.hpp"
@@ -316,6 +317,117 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmLaunch GL) {
     }
 }
 
+// Products of at most 16 columns per tile (one-RHS and few-RHS solve passes),
+// which stream A once: each wave computes a 16-row block of a 64-row tile and
+// reads its A operand straight from memory in the MFMA register layout (lane l
+// holds A[row l % 16][k l / 16], 16 consecutive rows of one column per 16
+// lanes), with SK k-steps of operands in flight in each of two register
+// stages; no LDS, no barrier.  The MFMA sequence of every 16 x 16 output block
+// (k ascending in steps of four, three real products) is that of gemm_kernel,
+// so the results are the same bits as with any other tile; the K split, the
+// workspace and the epilogue are the same too.
+template <typename R, int SK>
+__global__ void __launch_bounds__(256) gemm_skinny_kernel(GemmLaunch GL) {
+    using V = c_t<R>;
+    using A4 = typename acc_t<R>::T;
+    const GemmDesc* descs = GL.g[0].descs;
+    V* ws = (V*)GL.g[0].ws;
+    int M = GL.g[0].M, N = GL.g[0].N, K = GL.g[0].K, tilesM = GL.g[0].tilesM, tilesN = GL.g[0].tilesN,
+        kslices = GL.g[0].ks, kchunk = GL.g[0].kchunk, wg = blockIdx.x;
+#pragma unroll
+    for (int q = 1; q < GEMM_MAX_GROUPS; ++q)
+        if (q < GL.n && (int)blockIdx.x >= GL.g[q].wg0) {
+            descs = GL.g[q].descs; ws = (V*)GL.g[q].ws;
+            M = GL.g[q].M; N = GL.g[q].N; K = GL.g[q].K; tilesM = GL.g[q].tilesM; tilesN = GL.g[q].tilesN;
+            kslices = GL.g[q].ks; kchunk = GL.g[q].kchunk; wg = blockIdx.x - GL.g[q].wg0;
+        }
+    const int tiles = tilesM * tilesN;
+    const int per = tiles * kslices;
+    const int p = wg / per;
+    const int rem = wg - p * per;
+    const int ks = rem / tiles;
+    const int t = rem - ks * tiles;
+    const int tm = t % tilesM, tn = t / tilesM;
+    const GemmDesc& D = descs[p];
+    const V* const A = (const V*)D.A;
+    const V* const B = (const V*)D.B;
+    const V* const B2 = (const V*)D.B2;
+    const int lda = D.lda, ldb = D.ldb, ksplit = D.ksplit;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, kq = lane >> 4;
+    const int m0 = tm * 64 + wave * 16, n0 = tn * 16;
+    const int kbeg = ks * kchunk, kend = min(K, kbeg + kchunk);
+    const int arow = m0 + li, bcol = n0 + li;
+    const bool rowok = arow < M, colok = bcol < N;
+    const V* const Ar = A + (size_t)(rowok ? arow : M - 1);
+    const size_t boff = (size_t)(colok ? bcol : N - 1) * ldb;
+
+    A4 p1, p2, p3;
+    for (int v = 0; v < 4; ++v) { p1[v] = R(0); p2[v] = R(0); p3[v] = R(0); }
+    // two stages of SK k-steps (4 k each); every load is issued unconditionally
+    // from a clamped index and zeroed outside the matrix or the K range
+    V a0[SK], b0[SK], a1[SK], b1[SK];
+    auto fetch = [&](int k0, V (&a)[SK], V (&b)[SK]) {
+#pragma unroll
+        for (int u = 0; u < SK; ++u) {
+            const int k = k0 + 4 * u + kq;
+            const int kc = k < kend ? k : kend - 1;
+            a[u] = gld(Ar + (size_t)kc * lda);
+            b[u] = gld(kc < ksplit ? B + boff + kc : B2 + boff + (kc - ksplit));
+        }
+    };
+    auto step = [&](int k0, V (&a)[SK], V (&b)[SK]) {
+#pragma unroll
+        for (int u = 0; u < SK; ++u) {
+            const int k = k0 + 4 * u;
+            if (k >= kend) break;
+            const bool ok = k + kq < kend;
+            const V av = (ok && rowok) ? a[u] : cz<R>();
+            const V bv = (ok && colok) ? b[u] : cz<R>();
+            p1 = mfma(av.x, bv.x, p1);
+            p2 = mfma(av.y, bv.y, p2);
+            p3 = mfma(av.x + av.y, bv.x + bv.y, p3);
+        }
+    };
+    constexpr int KS = 4 * SK;
+    if (kbeg < kend) fetch(kbeg, a0, b0);
+    for (int k0 = kbeg; k0 < kend; k0 += 2 * KS) {
+        if (k0 + KS < kend) fetch(k0 + KS, a1, b1);
+        step(k0, a0, b0);
+        if (k0 + KS >= kend) break;
+        if (k0 + 2 * KS < kend) fetch(k0 + 2 * KS, a0, b0);
+        step(k0 + KS, a1, b1);
+    }
+
+    // epilogue: lane holds rows out_row(lane, v) of column li of the wave's block
+    if (kslices > 1) {
+        V* W = ws + ((size_t)p * kslices + ks) * (size_t)M * N;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = m0 + out_row<R>(lane, v), col = bcol;
+            if (row < M && colok) W[(size_t)row + (size_t)col * M] = mk<R>(p1[v] - p2[v], (p3[v] - p1[v]) - p2[v]);
+        }
+        return;
+    }
+    V* const Out = (V*)D.Out;
+    const V* const Din = (const V*)D.Din;
+    const V alpha = mk<R>((R)D.ar, (R)D.ai), beta = mk<R>((R)D.br, (R)D.bi);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int row = m0 + out_row<R>(lane, v);
+        const bool ok = row < M && colok;
+        const int rowc = row < M ? row : M - 1, colc = colok ? bcol : N - 1;
+        const int orow = D.rmap ? gldi(D.rmap + rowc) : rowc;
+        const int ocol = D.cmap ? gldi(D.cmap + colc) : colc;
+        V val = cmul(alpha, mk<R>(p1[v] - p2[v], (p3[v] - p1[v]) - p2[v]));
+        if (Din) {
+            const size_t di = D.dmap ? (size_t)orow + (size_t)ocol * D.ldd : (size_t)rowc + (size_t)colc * D.ldd;
+            val = cfma(beta, gld(Din + di), val);
+        }
+        if (ok) gst(Out + ((size_t)orow + (size_t)ocol * D.ldo), val);
+    }
+}
+
 // Sum of the K slices in slice order, then the epilogue of gemm_kernel.
 // (The slices are summed in precision R, the epilogue runs in S.)
 template <typename S, typename R>
@@ -361,7 +473,12 @@ size_t gemm_ws_elems(int nprob, int M, int N, int K, int ks_target) {
     return ks > 1 ? (size_t)nprob * ks * M * N : 0;
 }
 
-int gemm_tile_rule(int M, int N) { return N <= 16 ? GEMM_TILE_64x16 : (M <= 16 ? GEMM_TILE_16x64 : (M <= 32 ? GEMM_TILE_32x64 : GEMM_TILE_64x64)); }
+// (few columns: the skinny tiles stream the tall products of the top levels
+// fastest, the LDS-staged 64 x 16 tiles the many small ones below)
+int gemm_tile_rule(int M, int N) {
+    if (N <= 16) return M >= 512 ? GEMM_TILE_SKINNY : GEMM_TILE_64x16;
+    return M <= 16 ? GEMM_TILE_16x64 : (M <= 32 ? GEMM_TILE_32x64 : GEMM_TILE_64x64);
+}
 
 namespace {
 
@@ -381,7 +498,10 @@ void launch_gemm_groups(const GemmGroup* grp, int ngroups, hipStream_t st, int t
     // tile, so the tile shape (by rule from M and N, or chosen by the caller's
     // autotuning) never changes the results.
     if (tile < 0 || tile >= GEMM_TILES) tile = gemm_tile_rule(tile_M, tile_N);
-    static const int TBM[GEMM_TILES] = {64, 32, 16, 64, 32, 64, 128}, TBN[GEMM_TILES] = {64, 64, 64, 16, 32, 32, 16};
+    // the skinny kernel runs the products of the mixed-precision merge (S != R) as 64x16
+    if (tile == GEMM_TILE_SKINNY && !std::is_same<S, R>::value) tile = GEMM_TILE_64x16;
+    static const int TBM[GEMM_TILES] = {64, 32, 16, 64, 32, 64, 128, 64},
+                     TBN[GEMM_TILES] = {64, 64, 64, 16, 32, 32, 16, 16};
     const int bm = TBM[tile], bn = TBN[tile];
     int wg = 0;
     for (int q = 0; q < ngroups; ++q) {
@@ -400,6 +520,9 @@ void launch_gemm_groups(const GemmGroup* grp, int ngroups, hipStream_t st, int t
     }
     auto run = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(wg), dim3(256), 0, st, GL); };
     switch (tile) {
+        case GEMM_TILE_SKINNY:
+            if constexpr (std::is_same<S, R>::value) run(gemm_skinny_kernel<R, 8>);
+            break;
         case GEMM_TILE_64x16: run(gemm_kernel<S, R, 64, 16>); break;
         case GEMM_TILE_16x64: run(gemm_kernel<S, R, 16, 64>); break;
         case GEMM_TILE_32x64: run(gemm_kernel<S, R, 32, 64>); break;

@@ -29,8 +29,10 @@ struct GemmDesc {
 // ws: workspace of gemm_ws_elems(...) complex elements of precision R for the
 // K-split form (NULL: no split).  tile: one of GEMM_TILE_* (output tile rows x
 // columns), or -1 for the shape rule gemm_tile_rule; results do not depend on it.
+// GEMM_TILE_SKINNY: 64 x 16 tiles whose waves read A straight from memory
+// (products of few columns).
 enum { GEMM_TILE_64x64 = 0, GEMM_TILE_32x64 = 1, GEMM_TILE_16x64 = 2, GEMM_TILE_64x16 = 3, GEMM_TILE_32x32 = 4,
-       GEMM_TILE_64x32 = 5, GEMM_TILE_128x16 = 6, GEMM_TILES = 7 };
+       GEMM_TILE_64x32 = 5, GEMM_TILE_128x16 = 6, GEMM_TILE_SKINNY = 7, GEMM_TILES = 8 };
 int gemm_tile_rule(int M, int N);
 // ks_target: the K split aims at this many (row tile x slice) workgroups per
 // launch; it never depends on N.

@@ -105,10 +105,11 @@ def test_mixed_merge_gemm_is_complex64_products_with_complex128_result(M, N, K, 
 
 
 @pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED])
-def test_gemm_split_and_scatter(prec, gpu):
-    print("case: gemm split/scatter", prec, flush=True)
+@pytest.mark.parametrize("N", [20, 3])
+def test_gemm_split_and_scatter(prec, N, gpu):
+    print("case: gemm split/scatter", prec, N, flush=True)
     rng = np.random.default_rng(1)
-    M, N, K, ks = 40, 20, 30, 12
+    M, K, ks = 40, 30, 12
     A = _rc(rng, M, K)
     B1, B2 = _rc(rng, K, N), _rc(rng, K, N)
     Bfull = np.vstack([B1[:ks], B2[:K - ks]])
@@ -196,7 +197,8 @@ def test_inverse_with_tied_pivots(n, gpu):
 
 @pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED, rsa.RS_MIXED_MERGE])
 @pytest.mark.parametrize("M,N,K,scatter", [(48, 256, 16, True), (100, 70, 777, False), (200, 33, 1000, False),
-                                           (16, 64, 32, True)])
+                                           (16, 64, 32, True), (2048, 1, 2048, False), (130, 5, 77, True),
+                                           (64, 16, 4096, True)])
 def test_gemm_tile_shapes_give_identical_bits(prec, M, N, K, scatter, gpu):
     """The autotuner may pick any tile shape: every shape must give the same bits."""
     print("case: gemm tiles", prec, M, N, K, scatter, flush=True)
@@ -209,7 +211,7 @@ def test_gemm_tile_shapes_give_identical_bits(prec, M, N, K, scatter, gpu):
     C0 = _rc(rng, ldc, N)
     rmap = np.ascontiguousarray(rng.permutation(ldc)[:M].astype(np.int32)) if scatter else None
     outs = []
-    for tile in (-1, 0, 1, 2, 3, 4, 5, 6):
+    for tile in (-1, 0, 1, 2, 3, 4, 5, 6, 7):
         C = np.asfortranarray(C0.copy())
         a = np.asfortranarray(A)
         b = np.asfortranarray(B)
