@@ -22,7 +22,8 @@
 // Each problem of a batch may scatter its output rows/columns through index
 // maps (the face permutation between a child's and its parent's boundary
 // ordering), read B from two row ranges (the two children's interface
-// vectors), and add beta*Din in the epilogue.
+// vectors) or gather its rows through a row map (a node's incoming data, held
+// where the levels above wrote it), and add beta*Din in the epilogue.
 #include <algorithm>
 #include <type_traits>
 
@@ -123,6 +124,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmLaunch GL) {
     const VS* const A = (const VS*)D.A;
     const VS* const B = (const VS*)D.B;
     const VS* const B2 = (const VS*)D.B2;
+    const int* const bmap = D.bmap;
     const int lda = D.lda, ldb = D.ldb, ksplit = D.ksplit;
     const int m0 = tm * BM, n0 = tn * BN;
     const int kbeg = ks * kchunk, kend = min(K, kbeg + kchunk);
@@ -140,6 +142,11 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmLaunch GL) {
     // or the K range are replaced by zero when staged.
     VS ra[NA], rb[NB];
     bool oka[NA], okb[NB];
+    // a row-mapped B: this thread's B row is the same in every column (k =
+    // tid % BK); its index is loaded one chunk ahead, so that the data loads do
+    // not wait for it
+    int bix = 0;
+    if (bmap && kbeg < kend) bix = gldi(bmap + min(kbeg + (int)(tid % BK), kend - 1));
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
@@ -155,10 +162,12 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmLaunch GL) {
             const int k = e % BK, c = e / BK;
             const int gk = min(k0 + k, kend - 1), gc = min(n0 + c, N - 1);
             okb[q] = k0 + k < kend && n0 + c < N;
-            const VS* src = (gk < ksplit) ? B + ((size_t)gk + (size_t)gc * ldb)
-                                          : B2 + ((size_t)(gk - ksplit) + (size_t)gc * ldb);
+            const VS* src = bmap ? B + ((size_t)bix + (size_t)gc * ldb)
+                                 : (gk < ksplit) ? B + ((size_t)gk + (size_t)gc * ldb)
+                                                 : B2 + ((size_t)(gk - ksplit) + (size_t)gc * ldb);
             rb[q] = gld(src);
         }
+        if (bmap) bix = gldi(bmap + min(k0 + BK + (int)(tid % BK), kend - 1));
     };
     // chunk buffer b: As [BK][BM + 1], then Bs [BK][BN + 1]
     auto stash = [&](int b) {
@@ -352,6 +361,7 @@ __global__ void __launch_bounds__(256) gemm_skinny_kernel(GemmLaunch GL) {
     const V* const A = (const V*)D.A;
     const V* const B = (const V*)D.B;
     const V* const B2 = (const V*)D.B2;
+    const int* const bmap = D.bmap;
     const int lda = D.lda, ldb = D.ldb, ksplit = D.ksplit;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, kq = lane >> 4;
@@ -367,14 +377,26 @@ __global__ void __launch_bounds__(256) gemm_skinny_kernel(GemmLaunch GL) {
     // two stages of SK k-steps (4 k each); every load is issued unconditionally
     // from a clamped index and zeroed outside the matrix or the K range
     V a0[SK], b0[SK], a1[SK], b1[SK];
+    // a row-mapped B: the indices of a stage are loaded one stage ahead (the
+    // stages are fetched in k order)
+    int bix[SK];
+    auto bidx = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < SK; ++u) {
+            const int k = k0 + 4 * u + kq;
+            bix[u] = gldi(bmap + (k < kend ? k : kend - 1));
+        }
+    };
+    if (bmap && kbeg < kend) bidx(kbeg);
     auto fetch = [&](int k0, V (&a)[SK], V (&b)[SK]) {
 #pragma unroll
         for (int u = 0; u < SK; ++u) {
             const int k = k0 + 4 * u + kq;
             const int kc = k < kend ? k : kend - 1;
             a[u] = gld(Ar + (size_t)kc * lda);
-            b[u] = gld(kc < ksplit ? B + boff + kc : B2 + boff + (kc - ksplit));
+            b[u] = gld(bmap ? B + boff + bix[u] : (kc < ksplit ? B + boff + kc : B2 + boff + (kc - ksplit)));
         }
+        if (bmap) bidx(k0 + 4 * SK);
     };
     auto step = [&](int k0, V (&a)[SK], V (&b)[SK]) {
 #pragma unroll

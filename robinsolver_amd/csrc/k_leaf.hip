@@ -253,7 +253,9 @@ template <typename R> struct LeafIO {
     const double2* const* f64;  // interior data f, complex128: column c at f64[c], [leaf][n_int] ...
     const float2* f32;        //   ... or complex64 [col][leaf][n_int] (at most one set; none = 0)
     const float2* jump;       // UP: subtracted from the outgoing data h: [col][leaf][nb] local order
-    const c_t<R>* gin;        // DOWN: incoming data g: [col][leaf][nb] node order
+    const c_t<R>* gin;        // DOWN: incoming data g: row b (node order) of leaf q in column c at
+    const int* gsrc;          //   gin[gsrc[q * nb + b] + c * gld]
+    size_t gld;
     c_t<R>* hout;             // UP: outgoing data h: [col][leaf][nb] node order
     double2* const* u64;      // DOWN: solution, complex128: column c at u64[c], [leaf][m] ...
     float2* u32;              //   ... or complex64 [col][leaf][m] (one set); written, never read
@@ -324,7 +326,7 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
         if (io.mode == LEAF_FACTOR_T) {
             return (node_to_local(G, parity, col) == b) ? mk<R>(R(1), R(0)) : cz<R>();
         } else if (io.mode == LEAF_DOWN) {
-            return io.gin[((size_t)col * io.n_leaves + leaf) * nb + local_to_node(G, parity, b)];
+            return io.gin[(size_t)io.gsrc[(size_t)leaf * nb + local_to_node(G, parity, b)] + (size_t)col * io.gld];
         }
         return cz<R>();
     };
@@ -721,12 +723,12 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
         if (MODE == LEAF_DOWN || io.jump) {
             // stored in node order (g already is; the residual is in local order)
             float2 t[16];
+            const size_t gs = (MODE == LEAF_DOWN) ? (size_t)io.gsrc[(size_t)leaf * nb + bi] : 0;
 #pragma unroll
             for (int c = 0; c < 16; ++c) {
                 const int col = (wcol0 + c) < io.ncols ? wcol0 + c : 0;
-                const size_t gi = ((size_t)col * nl + leaf) * nb + bi;
-                if (MODE == LEAF_DOWN) t[c] = io.gin[gi];
-                else t[c] = ccast<float2>(io.jump[gi]);
+                if (MODE == LEAF_DOWN) t[c] = io.gin[gs + (size_t)col * io.gld];
+                else t[c] = ccast<float2>(io.jump[((size_t)col * nl + leaf) * nb + bi]);
             }
             const int pos = (MODE == LEAF_DOWN) ? bi : l2n(bi);
 #pragma unroll
@@ -982,9 +984,10 @@ __global__ void __launch_bounds__(256) leaf3_mfma_kernel(LeafGeom G, LeafCoef C,
                 for (int j = 0; j < 3; ++j) {
                     const AxisCoef& a = C.ax[j];
                     const float2 w = ccast<float2>(cscale(cinv(a.dd), -a.cib));
-                    const size_t gb = ((size_t)(c0 + c) * nl + leaf) * nb;
-                    if (g[j] == 1) v = cfma(w, io.gin[gb + local_to_node(G, parity, face_point(G, 2 * j + 1, g))], v);
-                    if (g[j] == G.n[j]) v = cfma(w, io.gin[gb + local_to_node(G, parity, face_point(G, 2 * j + 2, g))], v);
+                    const size_t cb = (size_t)(c0 + c) * io.gld;
+                    const int* gs = io.gsrc + (size_t)leaf * nb;
+                    if (g[j] == 1) v = cfma(w, io.gin[cb + gs[local_to_node(G, parity, face_point(G, 2 * j + 1, g))]], v);
+                    if (g[j] == G.n[j]) v = cfma(w, io.gin[cb + gs[local_to_node(G, parity, face_point(G, 2 * j + 2, g))]], v);
                 }
             }
         }
@@ -1116,9 +1119,10 @@ __global__ void __launch_bounds__(256) leaf3_mfma_kernel(LeafGeom G, LeafCoef C,
         const double a1 = j == 0 ? C.ax[0].a1 : (j == 1 ? C.ax[1].a1 : C.ax[2].a1);
         const double a2 = j == 0 ? C.ax[0].a2 : (j == 1 ? C.ax[1].a2 : C.ax[2].a2);
         const int col = c0 + c;
-        const size_t gi = ((size_t)col * nl + leaf) * nb;
+        const size_t gi = ((size_t)col * nl + leaf) * nb;   // (the jump and the outgoing data)
         const int pos = local_to_node(G, parity, b);
-        const float2 gb = (MODE == LEAF_DOWN) ? io.gin[gi + pos] : make_float2(0.f, 0.f);
+        const float2 gb = (MODE == LEAF_DOWN) ? io.gin[(size_t)io.gsrc[(size_t)leaf * nb + pos] + (size_t)col * io.gld]
+                                              : make_float2(0.f, 0.f);
         const float2 x1 = X[c * XS + q1], x2 = X[c * XS + q1 + step];
         const float2 num = csub(csub(gb, cscale(x1, (float)a1)), cscale(x2, (float)a2));
         const float2 uB = cmul(num, ccast<float2>(cinv(dd)));
@@ -1252,6 +1256,7 @@ void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit
     io.mode = a.mode; io.ncols = a.ncols; io.n_leaves = a.n_leaves;
     io.CH = leaf_apply_chunk<R>(G, a.ncols, lds_limit);
     io.sinv = a.sinv; io.f64 = a.f64; io.f32 = a.f32; io.jump = a.jump; io.gin = a.gin; io.hout = a.hout;
+    io.gsrc = a.gsrc; io.gld = a.gld;
     io.u64 = a.u64; io.u32 = a.u32; io.T = a.T;
     io.ubase32 = a.ubase32; io.uacc = a.uacc;
     const int mode = a.mode, ncols = a.ncols, n_leaves = a.n_leaves;

@@ -39,48 +39,22 @@ __global__ void copy_blocks_kernel(double2* __restrict__ dst, size_t dstride, in
 
 // root boundary data from the physical data g (column table, [leaf][nbl] per column)
 template <typename V>
-__global__ void root_gather_kernel(V* __restrict__ out, const double2* const* __restrict__ gin,
+__global__ void root_gather_kernel(V* __restrict__ out, size_t ldo, const double2* const* __restrict__ gin,
                                    const int* __restrict__ leaf_of, const int* __restrict__ loc_of, int P, int nbl) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int col = blockIdx.y;
     if (p >= P) return;
-    out[(size_t)col * P + p] = ccast<V>(gld(gin[col] + ((size_t)leaf_of[p] * nbl + loc_of[p])));
+    out[(size_t)col * ldo + p] = ccast<V>(gld(gin[col] + ((size_t)leaf_of[p] * nbl + loc_of[p])));
 }
 
 template <typename V>
-__global__ void root_gather32_kernel(V* __restrict__ out, const float2* __restrict__ gin,
+__global__ void root_gather32_kernel(V* __restrict__ out, size_t ldo, const float2* __restrict__ gin,
                                      const int* __restrict__ leaf_of, const int* __restrict__ loc_of, int P,
                                      int n_leaves, int nbl) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int col = blockIdx.y;
     if (p >= P) return;
-    out[(size_t)col * P + p] = ccast<V>(gin[((size_t)col * n_leaves + leaf_of[p]) * nbl + loc_of[p]]);
-}
-
-// child N-part gather: child (2n+c) rows [0, bn) <- parent n rows prow[parity(n)][c][j];
-// each thread moves one row for GATHER_CPT columns (the map is read once, the
-// column loads are independent)
-constexpr int GATHER_CPT = 8;
-template <typename V>
-__global__ void child_gather_kernel(V* __restrict__ child, const V* __restrict__ parent, const int* __restrict__ prow,
-                                    int bn, int bc, int P, int nodes, int root, int ncols) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= bn) return;
-    const int node = blockIdx.y >> 1, c = blockIdx.y & 1;
-    const int col0 = blockIdx.z * GATHER_CPT;
-    const int par = root ? 0 : (node & 1);
-    const int src = prow[(size_t)(par * 2 + c) * bn + j];
-    V v[GATHER_CPT];
-#pragma unroll
-    for (int q = 0; q < GATHER_CPT; ++q) {
-        const int col = min(col0 + q, ncols - 1);
-        v[q] = parent[(size_t)col * nodes * P + (size_t)node * P + src];
-    }
-#pragma unroll
-    for (int q = 0; q < GATHER_CPT; ++q) {
-        const int col = col0 + q;
-        if (col < ncols) child[(size_t)col * (2 * nodes) * bc + (size_t)(2 * node + c) * bc + j] = v[q];
-    }
+    out[(size_t)col * ldo + p] = ccast<V>(gin[((size_t)col * n_leaves + leaf_of[p]) * nbl + loc_of[p]]);
 }
 
 __global__ void to_c64_kernel(float2* __restrict__ dst, const double2* __restrict__ src, size_t n) {
@@ -144,25 +118,17 @@ void launch_copy_blocks(double2* dst, size_t dstride, int ldd, const double2* sr
 }
 
 template <typename V>
-void launch_root_gather(V* out, const double2* const* gin, const int* leaf_of, const int* loc_of, int P, int nbl,
-                        int ncols, hipStream_t st) {
-    hipLaunchKernelGGL(root_gather_kernel<V>, dim3(nblk(P), ncols), dim3(256), 0, st, out, gin, leaf_of, loc_of, P,
-                       nbl);
+void launch_root_gather(V* out, size_t ldo, const double2* const* gin, const int* leaf_of, const int* loc_of, int P,
+                        int nbl, int ncols, hipStream_t st) {
+    hipLaunchKernelGGL(root_gather_kernel<V>, dim3(nblk(P), ncols), dim3(256), 0, st, out, ldo, gin, leaf_of, loc_of,
+                       P, nbl);
 }
 
 template <typename V>
-void launch_root_gather32(V* out, const float2* gin, const int* leaf_of, const int* loc_of, int P, int n_leaves,
-                          int nbl, int ncols, hipStream_t st) {
-    hipLaunchKernelGGL(root_gather32_kernel<V>, dim3(nblk(P), ncols), dim3(256), 0, st, out, gin, leaf_of, loc_of, P,
-                       n_leaves, nbl);
-}
-
-template <typename V>
-void launch_child_gather(V* child, const V* parent, const int* prow, int bn, int bc, int P, int nodes, int root,
-                         int ncols, hipStream_t st) {
-    const int bs = bn >= 256 ? 256 : (bn + 63) / 64 * 64;
-    hipLaunchKernelGGL(child_gather_kernel<V>, dim3((bn + bs - 1) / bs, 2 * nodes, (ncols + GATHER_CPT - 1) / GATHER_CPT),
-                       dim3(bs), 0, st, child, parent, prow, bn, bc, P, nodes, root, ncols);
+void launch_root_gather32(V* out, size_t ldo, const float2* gin, const int* leaf_of, const int* loc_of, int P,
+                          int n_leaves, int nbl, int ncols, hipStream_t st) {
+    hipLaunchKernelGGL(root_gather32_kernel<V>, dim3(nblk(P), ncols), dim3(256), 0, st, out, ldo, gin, leaf_of,
+                       loc_of, P, n_leaves, nbl);
 }
 
 void launch_to_c64(float2* dst, const double2* src, size_t n, hipStream_t st) {
@@ -222,17 +188,13 @@ void launch_col_table(double2** tab, double2* base, size_t stride, int ncols, hi
     if (ncols > 0) hipLaunchKernelGGL(col_table_kernel, dim3(nblk(ncols)), dim3(256), 0, st, tab, base, stride, ncols);
 }
 
-template void launch_root_gather<double2>(double2*, const double2* const*, const int*, const int*, int, int, int,
-                                          hipStream_t);
-template void launch_root_gather<float2>(float2*, const double2* const*, const int*, const int*, int, int, int,
+template void launch_root_gather<double2>(double2*, size_t, const double2* const*, const int*, const int*, int, int,
+                                          int, hipStream_t);
+template void launch_root_gather<float2>(float2*, size_t, const double2* const*, const int*, const int*, int, int, int,
                                          hipStream_t);
-template void launch_root_gather32<double2>(double2*, const float2*, const int*, const int*, int, int, int, int,
-                                            hipStream_t);
-template void launch_root_gather32<float2>(float2*, const float2*, const int*, const int*, int, int, int, int,
+template void launch_root_gather32<double2>(double2*, size_t, const float2*, const int*, const int*, int, int, int,
+                                            int, hipStream_t);
+template void launch_root_gather32<float2>(float2*, size_t, const float2*, const int*, const int*, int, int, int, int,
                                            hipStream_t);
-template void launch_child_gather<double2>(double2*, const double2*, const int*, int, int, int, int, int, int,
-                                           hipStream_t);
-template void launch_child_gather<float2>(float2*, const float2*, const int*, int, int, int, int, int, int,
-                                          hipStream_t);
 
 }  // namespace rs

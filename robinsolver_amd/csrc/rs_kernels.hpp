@@ -21,6 +21,7 @@ struct GemmDesc {
     const void* Din;
     const int* rmap;
     const int* cmap;
+    const int* bmap;     // B row k at B + bmap[k] (elements) instead of B + k (no B2 then)
     int lda, ldb, ldo, ldd;
     int ksplit, dmap;
     double ar, ai, br, bi;
@@ -92,7 +93,9 @@ template <typename R> struct LeafArgs {
     const double2* const* f64 = nullptr;
     const float2* f32 = nullptr;
     const float2* jump = nullptr;
-    const c_t<R>* gin = nullptr;
+    const c_t<R>* gin = nullptr;       // DOWN: incoming data, row b of leaf q in column c at
+    const int* gsrc = nullptr;         //   gin[gsrc[q * nb + b] + c * gld]
+    size_t gld = 0;
     c_t<R>* hout = nullptr;
     double2* const* u64 = nullptr;
     float2* u32 = nullptr;
@@ -136,15 +139,12 @@ void launch_form_iface(double2* out, const double2* T, size_t tstride, int child
 void launch_copy_blocks(double2* dst, size_t dstride, int ldd, const double2* src, size_t sstride, int lds, int rows,
                         int cols, int batch, double2 scale, hipStream_t st);
 template <typename V>
-void launch_root_gather(V* out, const double2* const* gin, const int* leaf_of, const int* loc_of, int P,
+void launch_root_gather(V* out, size_t ldo, const double2* const* gin, const int* leaf_of, const int* loc_of, int P,
                         int nbl, int ncols, hipStream_t st);
 // the same from complex64 data, contiguous [col][leaf][nbl]
 template <typename V>
-void launch_root_gather32(V* out, const float2* gin, const int* leaf_of, const int* loc_of, int P, int n_leaves,
-                          int nbl, int ncols, hipStream_t st);
-template <typename V>
-void launch_child_gather(V* child, const V* parent, const int* prow, int bn, int bc, int P, int nodes, int root,
-                         int ncols, hipStream_t st);
+void launch_root_gather32(V* out, size_t ldo, const float2* gin, const int* leaf_of, const int* loc_of, int P,
+                          int n_leaves, int nbl, int ncols, hipStream_t st);
 void launch_to_c64(float2* dst, const double2* src, size_t n, hipStream_t st);
 // per node [T_NI | T_IN] (bn x i and i x bn, column-major) out of full b x b
 // node maps of stride tstride, into d64 and/or d32 (either may be NULL)

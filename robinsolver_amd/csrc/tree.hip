@@ -84,8 +84,8 @@ template <typename R> void SolveWork<R>::bind(char* base, const WorkLayout& lay)
 // device scratch array):
 //   up_g  g_I = invM [h1_I; h2_I]                          (B from the two children)
 //   up_h  h_parent = P (h_N + T_NI g_I), per child c        (rows scattered by prow)
-//   dn_t  Z_c = T_IN g_N of child c
-//   dn_g  g_I += invM Z, into the children's incoming data  (rows through imap)
+//   dn_t  Z_c = T_IN g_N of child c                         (g_N rows read through dsrc)
+//   dn_g  g_I += invM Z, in place in U                      (the children's incoming interface rows)
 __host__ __device__ inline void solve_descs(const SolveLevel& p, int n, GemmDesc* ug, GemmDesc* uh, GemmDesc* dt,
                                             GemmDesc* dg) {
     // children's maps: compact [T_NI (bn x i) | T_IN (i x bn)] per child
@@ -100,12 +100,13 @@ __host__ __device__ inline void solve_descs(const SolveLevel& p, int n, GemmDesc
     g.ksplit = p.i; g.ldb = p.ldc;
     g.Out = p.Gl + (size_t)n * 2 * i * e; g.ldo = p.ldg;
     *ug = g;
+    // (in place: the level's interface data becomes the children's incoming
+    // interface rows, where the levels below and the leaves read it)
     GemmDesc q = z;
     q.A = p.IM + (size_t)n * 4 * ii * e; q.lda = 2 * p.i;
-    q.B = p.Zl + (size_t)n * 2 * i * e; q.ldb = p.ldg;
+    q.B = p.Zl + (size_t)n * 2 * i * e; q.ldb = p.ldz;
     q.Din = p.Gl + (size_t)n * 2 * i * e; q.ldd = p.ldg; q.br = 1.0;
-    q.Out = p.gC + (size_t)(2 * n) * b * e; q.ldo = p.ldc;
-    q.rmap = p.imap;
+    q.Out = p.Gl + (size_t)n * 2 * i * e; q.ldo = p.ldg;
     *dg = q;
     for (int c = 0; c < 2; ++c) {
         const char* Tcn = p.Tc + (size_t)(2 * n + c) * tstride * e;
@@ -118,10 +119,11 @@ __host__ __device__ inline void solve_descs(const SolveLevel& p, int n, GemmDesc
             u.Din = p.hC + (size_t)(2 * n + c) * b * e; u.ldd = p.ldc; u.br = 1.0;
             uh[c] = u;
         }
+        // B: the child's non-interface incoming rows, read where they were written
         GemmDesc t = z;
         t.A = Tcn + bn * i * e; t.lda = p.i;
-        t.B = p.gC + (size_t)(2 * n + c) * b * e; t.ldb = p.ldc;
-        t.Out = p.Zl + ((size_t)n * 2 * i + (size_t)c * i) * e; t.ldo = p.ldg;
+        t.B = p.U; t.bmap = p.dsrc + (size_t)(2 * n + c) * bn; t.ldb = p.ldg;
+        t.Out = p.Zl + ((size_t)n * 2 * i + (size_t)c * i) * e; t.ldo = p.ldz;
         dt[c] = t;
     }
 }
@@ -167,15 +169,20 @@ int Tree::ensure_device() {
     if (leaf_blocks_lds(G) > lds_limit) return fail("Error: leaf layer too large for LDS (reduce the leaf size).");
     const Plan& P = plan;
     for (DevBuf* b : d_prow) delete b;
-    for (DevBuf* b : d_imap) delete b;
     d_prow.assign(P.L, nullptr);
-    d_imap.assign(P.L, nullptr);
     for (int l = 0; l < P.L; ++l) {
-        d_prow[l] = new DevBuf; d_imap[l] = new DevBuf;
-        if (upload(*d_prow[l], P.lv[l].prow) || upload(*d_imap[l], P.lv[l].imap)) return 1;
+        d_prow[l] = new DevBuf;
+        if (upload(*d_prow[l], P.lv[l].prow)) return 1;
     }
     if (upload(d_root_leaf, P.root_leaf) || upload(d_root_loc, P.root_loc) || upload(d_leaf_nbr, P.leaf_nbr))
         return 1;
+    for (DevBuf* b : d_src_dn) delete b;
+    d_src_dn.assign(P.L, nullptr);
+    for (int l = 0; l < P.L; ++l) {
+        d_src_dn[l] = new DevBuf;
+        if (upload(*d_src_dn[l], src_dn[l])) return 1;
+    }
+    if (upload(d_src_leaf, src_leaf)) return 1;
     return 0;
 }
 
@@ -202,7 +209,7 @@ Tree::~Tree() {
     clear_slots();
     free_factor();
     for (DevBuf* b : d_prow) delete b;
-    for (DevBuf* b : d_imap) delete b;
+    for (DevBuf* b : d_src_dn) delete b;
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -268,7 +275,50 @@ int Tree::plan_tree(int d, int L, const double* domain, double h_tgt, double kb,
         device_ready = false;
         if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
     }
+    if (build_sources()) return 1;
     planned = true;
+    return 0;
+}
+
+// The composition of the down pass's copies: a child's non-interface row j is
+// its parent's row prow[parity][c][j], and its interface rows are the rows of
+// the parent level's interface data that imap sends to them.  Followed from the
+// root down, every node row ends at the root boundary data or at one interface
+// value (the element that the level's dn_g product writes).
+int Tree::build_sources() {
+    const Plan& P = plan;
+    const int L = P.L;
+    iface_off.assign(L, 0);
+    int off = P.lv[0].P;
+    for (int l = 0; l < L; ++l) {
+        iface_off[l] = off;
+        off += P.lv[l].nodes * 2 * P.lv[l].isz;
+    }
+    iface_len = off;
+    std::vector<int> S(P.lv[0].P);
+    for (int r = 0; r < P.lv[0].P; ++r) S[r] = r;
+    src_dn.assign(L, {});
+    for (int l = 0; l < L; ++l) {
+        const Level& V = P.lv[l];
+        const int b = V.bc, bn = V.bn, i2 = 2 * V.isz;
+        std::vector<int> Sn((size_t)2 * V.nodes * b, -1);
+        std::vector<int>& dn = src_dn[l];
+        dn.assign((size_t)2 * V.nodes * bn, 0);
+        for (int n = 0; n < V.nodes; ++n) {
+            const int par = (l == 0) ? 0 : (n & 1);
+            for (int c = 0; c < 2; ++c)
+                for (int j = 0; j < bn; ++j) {
+                    const int v = S[(size_t)n * V.P + V.prow[(size_t)(par * 2 + c) * bn + j]];
+                    Sn[(size_t)(2 * n + c) * b + j] = v;
+                    dn[(size_t)(2 * n + c) * bn + j] = v;
+                }
+            for (int q = 0; q < i2; ++q) Sn[(size_t)2 * n * b + V.imap[q]] = iface_off[l] + n * i2 + q;
+        }
+        for (int v : Sn)
+            if (v < 0) return fail("Error: internal: a node row without a source in the down pass.");
+        S.swap(Sn);
+    }
+    src_leaf.swap(S);
     return 0;
 }
 
@@ -701,12 +751,12 @@ WorkLayout Tree::work_layout(int cap, int esz, bool refine, bool tabs, bool desc
     cap = lay.cap;
     size_t off = 0;
     auto take = [&](size_t bytes) { const size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+    // U: the root boundary data and every level's interface data, one column
+    // (iface_len elements) per right-hand side; G[l] is level l's slice of it
     lay.hV.assign(L + 1, WorkLayout::NONE); lay.gF.assign(L + 1, WorkLayout::NONE);
-    for (int l = 0; l <= L; ++l) {
-        const size_t n = (size_t)cap * P.lv[l].nodes * P.lv[l].P * esz;
-        lay.gF[l] = take(n);
-        if (l > 0 || L == 0) lay.hV[l] = take(n);
-    }
+    lay.gF[0] = take((size_t)cap * iface_len * esz);
+    for (int l = 0; l <= L; ++l)
+        if (l > 0 || L == 0) lay.hV[l] = take((size_t)cap * P.lv[l].nodes * P.lv[l].P * esz);
     lay.G.assign(L, WorkLayout::NONE); lay.Z.assign(L, WorkLayout::NONE);
     lay.dug.assign(L, WorkLayout::NONE); lay.duh.assign(L, WorkLayout::NONE);
     lay.ddt.assign(L, WorkLayout::NONE); lay.ddg.assign(L, WorkLayout::NONE);
@@ -714,7 +764,7 @@ WorkLayout Tree::work_layout(int cap, int esz, bool refine, bool tabs, bool desc
     for (int l = 0; l < L; ++l) {
         const Level& Vl = P.lv[l];
         const size_t n = (size_t)cap * Vl.nodes * 2 * Vl.isz * esz;
-        lay.G[l] = take(n);
+        lay.G[l] = lay.gF[0] + (size_t)iface_off[l] * esz;
         lay.Z[l] = take(n);
         wse = std::max(wse, gemm_ws_elems(Vl.nodes, 2 * Vl.isz, cap, 2 * Vl.isz,
                                           solve_ks_target(Vl.nodes, 2 * Vl.isz, 2 * Vl.isz)));
@@ -754,11 +804,11 @@ template <typename R> SolveLevel Tree::solve_level(const SolveWork<R>& w, int l)
     p.hP = l > 0 ? (char*)w.hV[l] : nullptr;
     p.Gl = (char*)w.G[l];
     p.Zl = (char*)w.Z[l];
-    p.gC = (char*)w.gF[l + 1];
+    p.U = (char*)w.gF[0];
     p.prow = d_prow[l]->as<int>();
-    p.imap = d_imap[l]->as<int>();
+    p.dsrc = d_src_dn[l]->as<int>();
     p.nodes = Vl.nodes; p.i = Vl.isz; p.b = Vl.bc; p.bn = Vl.bn; p.Pp = Vl.P;
-    p.ldc = C.nodes * Vl.bc; p.ldg = Vl.nodes * 2 * Vl.isz; p.ldp = Vl.nodes * Vl.P;
+    p.ldc = C.nodes * Vl.bc; p.ldg = iface_len; p.ldz = Vl.nodes * 2 * Vl.isz; p.ldp = Vl.nodes * Vl.P;
     p.lvl = l;
     p.esz = (int)sizeof(c_t<R>);
     return p;
@@ -956,7 +1006,8 @@ void Tree::factor_bytes(long long* resident_out, long long* peak_out) const {
     const bool keep_maps = (opts.retain & RS_RETAIN_MAPS) != 0;
     const FactorSizes z = factor_sizes(P, G, opts, device_ready ? lds_limit : 163840);
     long long res = 0;
-    for (int l = 0; l < L; ++l) res += (long long)(P.lv[l].prow.size() + P.lv[l].imap.size()) * 4;
+    for (int l = 0; l < L; ++l) res += (long long)(P.lv[l].prow.size() + src_dn[l].size()) * 4;
+    res += (long long)src_leaf.size() * 4;
     res += (long long)(P.root_leaf.size() + P.root_loc.size() + P.leaf_nbr.size()) * 4;
     res += (long long)P.n_leaves * P.n_int * 16 + 4;   // k, info
     res += (long long)z.sinv32 + (keep64 ? (long long)z.sinv64 : 0);
@@ -1040,24 +1091,23 @@ int Tree::solve_pass(SolveWork<R>& w, int nrhs, const PassIO& io, hipStream_t st
     }
     pbeg(P_GATHER, st);
     if (io.g32)
-        launch_root_gather32<V>(w.gF[0], io.g32, d_root_leaf.as<int>(), d_root_loc.as<int>(),
+        launch_root_gather32<V>(w.gF[0], (size_t)iface_len, io.g32, d_root_leaf.as<int>(), d_root_loc.as<int>(),
                                 P.lv[0].P, nl, P.nb, nrhs, st);
     else
-        launch_root_gather<V>(w.gF[0], io.g, d_root_leaf.as<int>(), d_root_loc.as<int>(),
+        launch_root_gather<V>(w.gF[0], (size_t)iface_len, io.g, d_root_leaf.as<int>(), d_root_loc.as<int>(),
                               P.lv[0].P, P.nb, nrhs, st);
     pend(nr * P.lv[0].P * ((io.g32 ? 8.0 : 16.0) + e), 0, st);
+    // (no copies down the tree: dn_t and the leaves read every incoming row
+    // where the root gather or a level's dn_g wrote it)
     for (int l = 0; l < L; ++l) {
         const Level& Vl = P.lv[l];
-        pbeg(P_GATHER, st);
-        launch_child_gather<V>(w.gF[l + 1], w.gF[l], d_prow[l]->as<int>(),
-                               Vl.bn, Vl.bc, Vl.P, Vl.nodes, l == 0, nrhs, st);
-        pend(nr * 2.0 * Vl.nodes * Vl.bn * 2 * e, 0, st);
         gemm(P_GEMM_DN_T, l, w.d_dn_t[l], 2 * Vl.nodes, Vl.isz, nrhs, Vl.bn, false);
         gemm(P_GEMM_DN_G, l, w.d_dn_g[l], Vl.nodes, 2 * Vl.isz, nrhs, 2 * Vl.isz, true);
     }
     LeafArgs<R> dn;
     dn.mode = LEAF_DOWN; dn.ncols = nrhs; dn.n_leaves = nl; dn.sinv = sv;
-    dn.f64 = io.f64; dn.f32 = io.f32; dn.gin = w.gF[L];
+    dn.f64 = io.f64; dn.f32 = io.f32; dn.gin = w.gF[0];
+    dn.gsrc = d_src_leaf.as<int>(); dn.gld = (size_t)iface_len;
     dn.u64 = io.u64; dn.u32 = io.u32; dn.ubase32 = io.ubase32; dn.uacc = io.uacc;
     pbeg(P_LEAF_DOWN, st);
     launch_leaf_apply<R>(G, dn, lds_limit, st);
@@ -1323,20 +1373,13 @@ int Tree::pass_grouped(const std::vector<Tree*>& ts, const std::vector<SolveWork
         Tree* t = ts[q];
         const Plan& P = t->plan;
         if (io[q].g32)
-            launch_root_gather32<V>(ws[q]->gF[0], io[q].g32, t->d_root_leaf.as<int>(), t->d_root_loc.as<int>(),
-                                    P.lv[0].P, P.n_leaves, P.nb, nrhs[q], st);
+            launch_root_gather32<V>(ws[q]->gF[0], (size_t)t->iface_len, io[q].g32, t->d_root_leaf.as<int>(),
+                                    t->d_root_loc.as<int>(), P.lv[0].P, P.n_leaves, P.nb, nrhs[q], st);
         else
-            launch_root_gather<V>(ws[q]->gF[0], io[q].g, t->d_root_leaf.as<int>(), t->d_root_loc.as<int>(),
-                                  P.lv[0].P, P.nb, nrhs[q], st);
+            launch_root_gather<V>(ws[q]->gF[0], (size_t)t->iface_len, io[q].g, t->d_root_leaf.as<int>(),
+                                  t->d_root_loc.as<int>(), P.lv[0].P, P.nb, nrhs[q], st);
     }
     for (int d = 0; d < maxL; ++d) {
-        for (int q = 0; q < n; ++q) {
-            Tree* t = ts[q];
-            if (d >= t->plan.L) continue;
-            const Level& Vl = t->plan.lv[d];
-            launch_child_gather<V>(ws[q]->gF[d + 1], ws[q]->gF[d], t->d_prow[d]->as<int>(), Vl.bn, Vl.bc, Vl.P,
-                                   Vl.nodes, d == 0, nrhs[q], st);
-        }
         g.clear();
         for (int q = 0; q < n; ++q) {
             if (d >= ts[q]->plan.L) continue;
@@ -1356,7 +1399,8 @@ int Tree::pass_grouped(const std::vector<Tree*>& ts, const std::vector<SolveWork
         Tree* t = ts[q];
         LeafArgs<R> dn;
         dn.mode = LEAF_DOWN; dn.ncols = nrhs[q]; dn.n_leaves = t->plan.n_leaves; dn.sinv = sinv_of(t);
-        dn.f64 = io[q].f64; dn.f32 = io[q].f32; dn.gin = ws[q]->gF[t->plan.L];
+        dn.f64 = io[q].f64; dn.f32 = io[q].f32; dn.gin = ws[q]->gF[0];
+        dn.gsrc = t->d_src_leaf.as<int>(); dn.gld = (size_t)t->iface_len;
         dn.u64 = io[q].u64; dn.u32 = io[q].u32; dn.ubase32 = io[q].ubase32; dn.uacc = io[q].uacc;
         launch_leaf_apply<R>(t->G, dn, t->lds_limit, st);
     }

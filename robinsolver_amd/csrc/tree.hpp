@@ -38,7 +38,7 @@ struct WorkLayout {
     static constexpr size_t NONE = ~(size_t)0;
     int cap = 0, esz = 0;
     bool refine = false;
-    std::vector<size_t> hV, gF, G, Z;          // per level (NONE where absent)
+    std::vector<size_t> hV, gF, G, Z;          // per level (NONE where absent; gF[0] is U, G[l] slices of it)
     std::vector<size_t> dug, duh, ddt, ddg;    // descriptor arrays per level
     size_t ws = NONE, rf = NONE, rg = NONE, rj = NONE, uA = NONE, tabs = NONE;
     size_t total = 0;
@@ -68,11 +68,12 @@ template <typename R> struct SolveWork {
 // bytes; esz = bytes per complex element of the pass).
 struct SolveLevel {
     const char* IM;     // invM of the level's nodes, [nodes][2i][2i]
-    const char* Tc;     // children's RtR maps, [2 nodes][b][b]
+    const char* Tc;     // children's RtR maps, compact [2 nodes][T_NI | T_IN]
     char *hC, *hP;      // outgoing data of the children / of the nodes (NULL at the root)
-    char *Gl, *Zl, *gC; // interface data, down-pass products, children's incoming data
-    const int *prow, *imap;
-    int nodes, i, b, bn, Pp, ldc, ldg, ldp, lvl, esz;
+    char *Gl, *Zl;      // the level's interface data (a slice of U), down-pass products
+    char* U;            // every level's interface data and the root boundary data (column stride ldg)
+    const int *prow, *dsrc;   // up-pass row scatter; the children's non-interface rows in U
+    int nodes, i, b, bn, Pp, ldc, ldg, ldz, ldp, lvl, esz;
 };
 
 enum ProfCat { P_LEAF_UP = 0, P_LEAF_DOWN, P_GEMM_UP_G, P_GEMM_UP_H, P_GEMM_DN_T, P_GEMM_DN_G, P_GATHER,
@@ -190,8 +191,21 @@ struct Tree {
     bool have32 = false;
     bool merge_c64 = false;      // some merge products of the factorization ran in complex64
     // maps
-    std::vector<DevBuf*> d_prow, d_imap;
+    std::vector<DevBuf*> d_prow;
     DevBuf d_root_leaf, d_root_loc, d_leaf_nbr;
+    // Where the down pass finds every node's incoming data: all of it is held in
+    // one array per right-hand side, [root boundary | interface data of level 0 |
+    // ... | of level L-1] (iface_len elements; level l's slice at iface_off[l],
+    // [nodes][2 isz]), and a node's incoming row is an element of it.  src_dn[l]:
+    // per child of a level-l node, its bn non-interface rows ([2 nodes][bn]);
+    // src_leaf: every leaf boundary row ([n_leaves][nb], node order).
+    int iface_len = 0;
+    std::vector<int> iface_off;
+    std::vector<std::vector<int>> src_dn;
+    std::vector<int> src_leaf;
+    std::vector<DevBuf*> d_src_dn;
+    DevBuf d_src_leaf;
+    int build_sources();
 
     // kernel timing
     bool prof_on = false;
