@@ -423,7 +423,8 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
 // of coincident leaf boundary points once the tree has solved.  A correction
 // solve takes (rf, rg) as (f, g) and subtracts rj from the leaf outgoing data.
 // Evaluated in FP64; stored in complex64, the precision the correction pass
-// computes in (rf [col][leaf][n_int], rg and rj [col][leaf][n_bnd]).
+// computes in (rf [col][leaf][n_int], rg and rj [col][leaf][n_bnd]; rg is
+// written on the physical boundary only).
 template <typename U>
 __device__ inline const U* ucolumn(double2* const* u64, const float2* u32, int col, int nl, int m);
 template <>
@@ -504,7 +505,9 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
             n0[c] = gld(uc + (un0 + b2));
             n1[c] = gld(uc + (un0 + nb + q1n));
             n2[c] = gld(uc + (un0 + nb + q1n + stn));
-            gv[c] = gld(g[cok[c] ? col0 + c : ncols - 1] + ((size_t)leaf * nb + p));
+            // g only matters on the physical boundary; elsewhere every lane loads the
+            // column's first element (one cache line) and the value is unused
+            gv[c] = gld(g[cok[c] ? col0 + c : ncols - 1] + (nl2 < 0 ? (size_t)leaf * nb + p : 0));
         }
 #pragma unroll
         for (int c = 0; c < RES_CPT; ++c) {
@@ -514,6 +517,7 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
             double2 gA = cmul(A.dd, up);
             gA = cadd(gA, cscale(ccast<double2>(a1[c]), A.a1));
             gA = cadd(gA, cscale(ccast<double2>(a2[c]), A.a2));
+            // (rg is read on the physical boundary only: the root gather's points)
             if (nl2 < 0) {
                 rg[o] = ccast<float2>(csub(gv[c], gA));
                 rj[o] = make_float2(0.f, 0.f);
@@ -523,7 +527,6 @@ __global__ void __launch_bounds__(256) residual_kernel(LeafGeom G, LeafCoef C, d
                 gN = cadd(gN, cscale(ccast<double2>(n2[c]), B.a2));
                 const double2 hA = cfma(sma, up, gA);
                 rj[o] = ccast<float2>(csub(cadd(cmul(zeta, gA), cmul(nu, gN)), hA));
-                rg[o] = make_float2(0.f, 0.f);
             }
         }
         }
@@ -1174,12 +1177,14 @@ __global__ void __launch_bounds__(256) fd_norms_kernel(LeafGeom G, const float2*
         d += b.x * b.x + b.y * b.y;
     }
     for (size_t e = tid; e < nbt; e += 256) {
-        const float2 a = rg[(size_t)col * nbt + e], c = rj[(size_t)col * nbt + e];
-        r += (double)a.x * a.x + (double)a.y * a.y + (double)c.x * c.x + (double)c.y * c.y;
+        const float2 c = rj[(size_t)col * nbt + e];
+        r += (double)c.x * c.x + (double)c.y * c.y;
         const int leaf = (int)(e / G.nb), p = (int)(e % G.nb);
         int fc = 1;
         while (fc < 2 * G.d && p >= G.foff[fc] + G.pf[fc]) ++fc;
-        if (nbr[(size_t)leaf * 7 + fc] < 0) {
+        if (nbr[(size_t)leaf * 7 + fc] < 0) {   // physical rows: rg is defined there only
+            const float2 a = rg[(size_t)col * nbt + e];
+            r += (double)a.x * a.x + (double)a.y * a.y;
             const double2 b = gld(g[col] + e);
             d += b.x * b.x + b.y * b.y;
         }
