@@ -224,12 +224,15 @@ __global__ void __launch_bounds__(LB) gj_panel_reg_kernel(double2* __restrict__ 
         const int qw = (p % nt) >> 6;   // the wave holding row p
         if (tid == 0) piv[j] = p;
         const double2* prow = cand[b][qw];   // the pivot row
-        // 1 / pivot, scaled against overflow of |.|^2 (every thread: no divergent branch)
+        // 1 / pivot = conj(d) / |d|^2, with d scaled by a power of two (exact)
+        // against overflow of |d|^2: one division on the step's critical path
+        // (every thread: no divergent branch)
         double2 inv;
         {
             const double2 d = prow[0];
-            const double sc = fmax(fabs(d.x), fabs(d.y)), rs = 1.0 / sc;
-            const double bx = d.x * rs, by = d.y * rs, rd = 1.0 / ((bx * bx + by * by) * sc);
+            const int ex = ilogb(fmax(fmax(fabs(d.x), fabs(d.y)), 1e-300));
+            const double bx = scalbn(d.x, -ex), by = scalbn(d.y, -ex);
+            const double rd = scalbn(1.0 / (bx * bx + by * by), -ex);
             inv = make_double2(bx * rd, -by * rd);
         }
         // the rank-1 update, rotated: a[c - 1] <- new column c, a[NB - 1] <- new column 0
@@ -350,7 +353,11 @@ size_t inv_small_lds(int n) { return (size_t)n * n * sizeof(double2) + (size_t)n
 // step).  A step costs about 2.3 us at 512 points whatever the panel width (its
 // latency chain: argmax, barrier, wave scan, pivot inverse, update), so
 // 32-column panels were slower (103 us per panel against 2 x 37.6 us; factor at
-// 1024^2 33.1 -> 34.7 ms) although they halve the launches.
+// 1024^2 33.1 -> 34.7 ms) although they halve the launches.  Timed alone (one
+// 16-column panel, events around the launch): at 1024 points about 15 us go to
+// the launch and to one CU loading and storing the panel, 14 us to the pivot
+// searches and barriers, 12 us to the updates; two rows per thread at 512 points
+// (four waves instead of eight) measured the same factor time.
 static int gj_reg_rows(int n) { return n <= 512 ? 1 : (n <= 2048 ? 2 : (n <= 4096 ? 4 : 0)); }
 
 int gj_panel_width(int n, size_t lds_limit) {
