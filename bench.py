@@ -346,6 +346,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-mix", action="store_true",
                     help="only the multi-tree serving scenario (8 resident trees 128^2..1024^2)")
+    ap.add_argument("--r1-only", action="store_true",
+                    help="only the FP64 one-RHS solves (--r1-solves of them) of the 2D configuration")
     ap.add_argument("--factor-only", type=int, default=0, metavar="REPS",
                     help="only time REPS factorizations of the workload tree (FP64), one JSON line")
     args = ap.parse_args()
@@ -370,6 +372,39 @@ def main():
         print(json.dumps({"metric": "factor seconds, 2D %d^2 Helmholtz (FP64)" % args.grid, "value": min(times),
                           "unit": "s", "higher_is_better": False, "times": times,
                           "reference_factor_GF": ff / 1e9}))
+        return
+
+    if args.r1_only:
+        # the reference's call shape alone: an FP64 tree, one right-hand side per
+        # solve (for kernel traces of that path)
+        import robinsolver_amd as rsa
+        import torch
+        t = rsa.GpuTree(2, args.levels, [[0, 1], [0, 1], [0, 1]], 1.0 / args.grid, -args.kb, precision=rsa.RS_FP64)
+        xyz, _ = device_points(torch, t)
+        k_dev = potential_dev(torch, xyz[:, t.nb:], args.kb, args.seed).contiguous()
+        n = args.r1_solves
+        f = gaussian_sources(torch, xyz[:, t.nb:, :], n, args.seed).contiguous()
+        g = torch.zeros((n, t.n_leaves, t.nb), dtype=torch.complex128, device="cuda:0")
+        u = torch.zeros((n, t.n_leaves, t.m), dtype=torch.complex128, device="cuda:0")
+        t.factor_device(k_dev.data_ptr())
+        sf, sg, su = t.n_leaves * t.n_int * 16, t.n_leaves * t.nb * 16, t.n_leaves * t.m * 16
+        for _ in range(2):
+            t.solve_device(1, f.data_ptr(), g.data_ptr(), u.data_ptr(), precision=rsa.RS_FP64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in range(n):
+            t.solve_device(1, f.data_ptr() + r * sf, g.data_ptr() + r * sg, u.data_ptr() + r * su,
+                           precision=rsa.RS_FP64)
+        torch.cuda.synchronize()
+        rate = n / (time.perf_counter() - t0)
+        log("fp64 R=1: %.1f solves/s" % rate)
+        # a plain copy of known size, the calibration point for hardware byte counters
+        cal = torch.empty(1 << 25, dtype=torch.float64, device="cuda:0")   # 256 MiB
+        torch.empty_like(cal).copy_(cal)
+        torch.cuda.synchronize()
+        print(json.dumps({"metric": "FP64 one-RHS solves/s, 2D %d^2 Helmholtz" % args.grid, "value": rate,
+                          "unit": "solves/s", "higher_is_better": True, "solves": n,
+                          "counter_calibration_copy_bytes": cal.numel() * 8}))
         return
 
     if args.serve_mix:
@@ -474,6 +509,20 @@ def main():
     torch.cuda.synchronize()
     b64_rate = nb64 * args.nrhs / (time.perf_counter() - t0)
     log("fp64 batched: %.1f solves/s" % b64_rate)
+
+    # the drop-in's call shape with host buffers (rs_solve on host f and g, then
+    # rs_get_solution per right-hand side): the mixed path including the PCIe
+    # copies of f, g and u (pageable host memory); reported beside the headline,
+    # which times device-resident inputs
+    nh = min(32, args.nrhs)
+    fh = f[:nh].cpu().numpy()
+    gh = g[:nh].cpu().numpy()
+    t.solve(fh, gh)
+    t0 = time.perf_counter()
+    uh = t.solve(fh, gh)
+    host_rate = nh / (time.perf_counter() - t0)
+    del uh
+    log("mixed, host buffers (%d RHS per call): %.1f solves/s" % (nh, host_rate))
 
     # fewest refinement steps that bring every RHS to rel. l2 <= 1e-8 against
     # the FP64 GPU solution (checked on the whole batch)
@@ -604,6 +653,9 @@ def main():
         "fp64_r1_solves_per_s": r1_rate,
         "fp64_r1_roofline": r1_roof,
         "fp64_batched_solves_per_s": b64_rate,
+        "host_buffers_solves_per_s": host_rate,
+        "host_buffers_note": "mixed path, %d RHS per rs_solve call on pageable host arrays, u read back with "
+                             "rs_get_solution: includes the PCIe copies of f, g and u" % nh,
         "mixed_over_fp64_r1": mixed_rate / r1_rate,
         "mixed_rel_l2_vs_fp64": {"max": float(rel.max()), "median": float(np.median(rel))},
         "refinement_steps": n_ref,
