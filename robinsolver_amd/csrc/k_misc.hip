@@ -37,6 +37,14 @@ __global__ void copy_blocks_kernel(double2* __restrict__ dst, size_t dstride, in
     dst[bat * dstride + r + (size_t)c * ldd] = cmul(scale, src[bat * sstride + r + (size_t)c * lds]);
 }
 
+// M[bat] (n x n, ld) += s I
+__global__ void add_diag_kernel(double2* __restrict__ M, size_t stride, int ld, int n, double2 s) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double2* p = M + blockIdx.y * stride + (size_t)r * (ld + 1);
+    *p = cadd(*p, s);
+}
+
 // root boundary data from the physical data g (column table, [leaf][nbl] per column)
 template <typename V>
 __global__ void root_gather_kernel(V* __restrict__ out, size_t ldo, const double2* const* __restrict__ gin,
@@ -115,6 +123,11 @@ void launch_copy_blocks(double2* dst, size_t dstride, int ldd, const double2* sr
                         int cols, int batch, double2 scale, hipStream_t st) {
     hipLaunchKernelGGL(copy_blocks_kernel, dim3(nblk((size_t)rows * cols), batch), dim3(256), 0, st, dst, dstride,
                        ldd, src, sstride, lds, rows, cols, scale);
+}
+
+void launch_add_diag(double2* M, size_t stride, int ld, int n, int batch, double2 s, hipStream_t st) {
+    if (n > 0 && batch > 0)
+        hipLaunchKernelGGL(add_diag_kernel, dim3(nblk((size_t)n), batch), dim3(256), 0, st, M, stride, ld, n, s);
 }
 
 template <typename V>

@@ -138,6 +138,8 @@ void launch_form_iface(double2* out, const double2* T, size_t tstride, int child
                        int isz, int nodes, double2 zeta, double2 coef, const double2* D, hipStream_t st);
 void launch_copy_blocks(double2* dst, size_t dstride, int ldd, const double2* src, size_t sstride, int lds, int rows,
                         int cols, int batch, double2 scale, hipStream_t st);
+// M[bat] (n x n, leading dimension ld, batch matrices stride elements apart) += s I
+void launch_add_diag(double2* M, size_t stride, int ld, int n, int batch, double2 s, hipStream_t st);
 template <typename V>
 void launch_root_gather(V* out, size_t ldo, const double2* const* gin, const int* leaf_of, const int* loc_of, int P,
                         int nbl, int ncols, hipStream_t st);

@@ -3,15 +3,21 @@
 // Factor (robin_tree_factor / compute_RTR, robin_tree_module.f90:96-185, 375-650):
 //   1. one launch factors every leaf (block-tridiagonal LU, k_leaf.hip) and one
 //      launch builds every leaf RtR map;
-//   2. per level, bottom-up, all nodes of the level at once:
-//        D = (zeta I - T2_II)^-1, S = (zeta I - T1_II - nu^2 D)^-1   (FP64, pivoted)
-//        invM = [S, -nu S D; -nu D S, D + nu^2 D S D]                 (3 GEMMs + copies)
+//   2. per level, bottom-up, all nodes of the level at once, with
+//      a = zeta I - T1_II, c = zeta I - T2_II and M = [a, nu I; nu I, c]:
+//        Wi   = (nu^2 I - a c)^-1                                     (GEMM, FP64 pivoted inverse)
+//        invM = [-c Wi, (I - (-c Wi) a) / nu; nu Wi, -Wi a]           (3 GEMMs + copy)
 //        Y    = invM diag(T1_IN, T2_IN)                               (GEMM)
 //        T    = diag(T1_NN, T2_NN) + diag(T1_NI, T2_NI) Y, scattered into the
 //               parent's node order                                  (GEMM)
 //      which is the reference's C11..C22 / zgemm block algebra written with
-//      explicit inverses.  The root RtR, which no solve reads, is skipped by
-//      default.
+//      an explicit inv(M).  The reference inverts two matrices one after the
+//      other, D = c^-1 and S = (a - nu^2 D)^-1 (robin_tree_module.f90:440-467);
+//      the same inv(M) follows from one inverse of the i x i matrix
+//      nu^2 I - a c (nu is nonzero whenever bndry_params passes its
+//      determinant check), which halves the sequential pivoted eliminations
+//      on the factor's critical path.  The root RtR, which no solve reads, is
+//      skipped by default.
 // Solve (forward_sub / backward_sub, robin_tree_module.f90:813-1146):
 //   up:   leaf particular solves -> per level g_I = invM [h1_I; h2_I],
 //         h_parent = P (h_N + T_NI g_I)
@@ -239,9 +245,10 @@ void Tree::free_factor() {
         for (DevBuf* b : v) delete b;
         v.clear();
     };
-    drop(T); drop(Tn); drop(Tn32); drop(invM); drop(invM32); drop(fdesc);
+    drop(T); drop(Tn); drop(Tn32); drop(invM); drop(invM32);
     ftile.clear();
-    for (DevBuf* b : {&sinv, &sinv32, &info, &fscratch, &invM_t, &tmap[0], &tmap[1]}) b->release();
+    fdoff.clear();
+    for (DevBuf* b : {&sinv, &sinv32, &info, &fscratch, &invM_t, &tmap[0], &tmap[1], &fdesc, &fcmap}) b->release();
     have32 = have64 = false;
     factored = false;
 }
@@ -356,10 +363,11 @@ int Tree::factor(const double* k_host) {
 }
 
 // Scratch of one level's merge, carved from the tree's merge arena (offsets in
-// bytes, 256-aligned): D, S, S D, Y, the K-split workspace of the level's
-// products, and the Gauss-Jordan panel and pivots of large interfaces.
+// bytes, 256-aligned): a = zeta I - T1_II, c = zeta I - T2_II, W (inverted in
+// place), Y, the K-split workspace of the level's products, and the
+// Gauss-Jordan panel and pivots of large interfaces.
 struct MergeLayout {
-    size_t D = 0, S = 0, SD = 0, Y = 0, ws = 0, W = 0, piv = 0, total = 0;
+    size_t A = 0, C = 0, W = 0, Y = 0, ws = 0, GW = 0, piv = 0, total = 0;
     InvPlan ip;
 };
 
@@ -368,9 +376,9 @@ static MergeLayout merge_layout(const Level& V, size_t lds_limit) {
     const size_t nodes = V.nodes, i = V.isz, bn = V.bn, ii = i * i;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
-    m.D = take(nodes * ii * 16);
-    m.S = take(nodes * ii * 16);
-    m.SD = take(nodes * ii * 16);
+    m.A = take(nodes * ii * 16);
+    m.C = take(nodes * ii * 16);
+    m.W = take(nodes * ii * 16);
     m.Y = take(nodes * 2 * i * 2 * bn * 16);
     size_t wse = 0;
     wse = std::max(wse, gemm_ws_elems((int)nodes, (int)i, (int)i, (int)i));
@@ -379,7 +387,7 @@ static MergeLayout merge_layout(const Level& V, size_t lds_limit) {
     wse = std::max(wse, gemm_ws_elems(4 * (int)nodes, (int)bn, (int)bn, (int)i));
     m.ws = take(wse * 16);
     m.ip.build((int)i, (int)nodes, ii, lds_limit);
-    m.W = take(m.ip.w_elems() * 16);
+    m.GW = take(m.ip.w_elems() * 16);
     m.piv = take(m.ip.small ? 0 : nodes * i * sizeof(int));
     m.total = off;
     return m;
@@ -417,8 +425,8 @@ static FactorSizes factor_sizes(const Plan& P, const LeafGeom& G, const rs_opts&
         if (l > 0 || !o.skip_root_rtr) z.Tfull[l] = nodes * (size_t)V.P * V.P * 16;
         const MergeLayout m = merge_layout(V, lds);
         z.arena = std::max(z.arena, m.total);
-        if (!m.ip.small) z.fdesc += 2 * (long long)m.ip.panels() * nodes * gd + (long long)m.ip.panels() * (i - m.ip.nb) * 4;
-        z.fdesc += 3 * nodes * gd + 2 * nodes * gd + ((l > 0 || !o.skip_root_rtr) ? 4 * nodes * gd : 0);
+        if (!m.ip.small) z.fdesc += (long long)m.ip.panels() * nodes * gd + (long long)m.ip.panels() * (i - m.ip.nb) * 4;
+        z.fdesc += 4 * nodes * gd + 2 * nodes * gd + ((l > 0 || !o.skip_root_rtr) ? 4 * nodes * gd : 0);
     }
     if (!z.arena) z.arena = 256;
     for (int l = 0; l <= L; ++l) {
@@ -499,92 +507,101 @@ int Tree::factor_device(const double2* k) {
     const double2 zeta = make_double2(P.zeta[0], P.zeta[1]);
     const double2 nu = make_double2(P.nu[0], P.nu[1]);
     const double2 nu2 = cmul(nu, nu);
-    // product descriptors of every level (kept for refactors while the buffers they
-    // point to are)
-    if (moved || (int)fdesc.size() != 6 * L) {
-        for (DevBuf* b : fdesc) delete b;
-        fdesc.assign(6 * L, nullptr);
-        for (DevBuf*& b : fdesc) b = new DevBuf;
+    const double nn = nu.x * nu.x + nu.y * nu.y;
+    const double2 rnu = make_double2(nu.x / nn, -nu.y / nn);
+    // product descriptors of every level, in one array (kept for refactors while
+    // the buffers they point to are)
+    if (moved || fdoff.size() != (size_t)L || !fdesc.p) {
+        fdoff.assign(L, MergeDescs{});
+        size_t ncm = 0;
+        for (int l = L - 1; l >= 0; --l)
+            if (!ml[l].ip.small) {
+                fdoff[l].cm = ncm;
+                ncm += (size_t)ml[l].ip.panels() * (P.lv[l].isz - ml[l].ip.nb);
+            }
+        if (need(fcmap, ncm * sizeof(int))) return 1;
+        std::vector<int> colmaps(ncm);
+        std::vector<GemmDesc> v;
         for (int l = L - 1; l >= 0; --l) {
             const Level& V = P.lv[l];
             const MergeLayout& m = ml[l];
+            MergeDescs& o = fdoff[l];
             const int nodes = V.nodes, i = V.isz, b = V.bc, bn = V.bn, Pp = V.P;
             const size_t ii = (size_t)i * i, tstride = (size_t)b * b;
             const double2* Tc = Tfull(l + 1);
-            double2* Dm = (double2*)(ar + m.D);
-            double2* Sm = (double2*)(ar + m.S);
-            double2* SD = (double2*)(ar + m.SD);
+            double2* Am = (double2*)(ar + m.A);
+            double2* Cm = (double2*)(ar + m.C);
+            double2* Wm = (double2*)(ar + m.W);
             double2* Y = (double2*)(ar + m.Y);
-            double2* W = (double2*)(ar + m.W);
+            double2* GW = (double2*)(ar + m.GW);
             double2* IM = IMf(l);
-            DevBuf** dl = &fdesc[6 * l];   // inv D, inv S, S D / B21 / B22, Y, parent T, GJ column maps
-            if (!m.ip.small) {
+            if (!m.ip.small) {  // Gauss-Jordan rank-nb updates of W, one per panel
                 const int n = i, nb_ = m.ip.nb;
-                std::vector<int> colmaps((size_t)m.ip.panels() * (n - nb_));
+                const int* cm = fcmap.as<int>() + o.cm;
                 for (int pi = 0; pi < m.ip.panels(); ++pi) {
                     const int j0 = pi * nb_;
-                    for (int c2 = 0; c2 < n - nb_; ++c2) colmaps[(size_t)pi * (n - nb_) + c2] = c2 < j0 ? c2 : c2 + nb_;
+                    for (int c2 = 0; c2 < n - nb_; ++c2)
+                        colmaps[o.cm + (size_t)pi * (n - nb_) + c2] = c2 < j0 ? c2 : c2 + nb_;
                 }
-                if (upload(*dl[5], colmaps)) return 1;
-                for (int which = 0; which < 2; ++which) {
-                    double2* mats = which == 0 ? Dm : Sm;
-                    std::vector<GemmDesc> v;
-                    for (int pi = 0; pi < m.ip.panels(); ++pi)
-                        for (int bt = 0; bt < nodes; ++bt) {
-                            GemmDesc g = desc0();
-                            double2* Mb = mats + (size_t)bt * ii;
-                            g.A = Mb + (size_t)pi * nb_ * n; g.lda = n;
-                            g.B = W + (size_t)bt * nb_ * (n - nb_); g.ldb = nb_;
-                            g.Out = Mb; g.ldo = n;
-                            g.cmap = dl[5]->as<int>() + (size_t)pi * (n - nb_);
-                            g.Din = Mb; g.ldd = n; g.dmap = 1; g.br = 1.0;
-                            v.push_back(g);
-                        }
-                    if (upload(*dl[which], v)) return 1;
-                }
-            }
-            {
-                std::vector<GemmDesc> v;
-                for (int n = 0; n < nodes; ++n) {  // SD = S D
-                    GemmDesc g = desc0();
-                    g.A = Sm + n * ii; g.lda = i;
-                    g.B = Dm + n * ii; g.ldb = i;
-                    g.Out = SD + n * ii; g.ldo = i;
-                    v.push_back(g);
-                }
-                for (int n = 0; n < nodes; ++n) {  // B21 = -nu D S
-                    GemmDesc g = desc0();
-                    g.A = Dm + n * ii; g.lda = i;
-                    g.B = Sm + n * ii; g.ldb = i;
-                    g.Out = IM + n * 4 * ii + i; g.ldo = 2 * i;
-                    g.ar = -nu.x; g.ai = -nu.y;
-                    v.push_back(g);
-                }
-                for (int n = 0; n < nodes; ++n) {  // B22 = D + nu^2 D (S D)
-                    GemmDesc g = desc0();
-                    g.A = Dm + n * ii; g.lda = i;
-                    g.B = SD + n * ii; g.ldb = i;
-                    g.Out = IM + n * 4 * ii + i + (size_t)i * 2 * i; g.ldo = 2 * i;
-                    g.ar = nu2.x; g.ai = nu2.y;
-                    g.Din = Dm + n * ii; g.ldd = i; g.br = 1.0;
-                    v.push_back(g);
-                }
-                if (upload(*dl[2], v)) return 1;
-            }
-            {  // Y = invM diag(T1_IN, T2_IN)
-                std::vector<GemmDesc> v;
-                for (int n = 0; n < nodes; ++n)
-                    for (int c = 0; c < 2; ++c) {
+                o.gj = v.size();
+                for (int pi = 0; pi < m.ip.panels(); ++pi)
+                    for (int bt = 0; bt < nodes; ++bt) {
                         GemmDesc g = desc0();
-                        g.A = IM + n * 4 * ii + (size_t)c * i * 2 * i; g.lda = 2 * i;
-                        g.B = Tc + (size_t)(2 * n + c) * tstride + bn; g.ldb = b;
-                        g.Out = Y + (size_t)n * 2 * i * 2 * bn + (size_t)c * bn * 2 * i; g.ldo = 2 * i;
+                        double2* Mb = Wm + (size_t)bt * ii;
+                        g.A = Mb + (size_t)pi * nb_ * n; g.lda = n;
+                        g.B = GW + (size_t)bt * nb_ * (n - nb_); g.ldb = nb_;
+                        g.Out = Mb; g.ldo = n;
+                        g.cmap = cm + (size_t)pi * (n - nb_);
+                        g.Din = Mb; g.ldd = n; g.dmap = 1; g.br = 1.0;
                         v.push_back(g);
                     }
-                if (upload(*dl[3], v)) return 1;
             }
+            o.w = v.size();
+            for (int n = 0; n < nodes; ++n) {  // W = -a c (nu^2 on the diagonal added after)
+                GemmDesc g = desc0();
+                g.A = Am + n * ii; g.lda = i;
+                g.B = Cm + n * ii; g.ldb = i;
+                g.Out = Wm + n * ii; g.ldo = i;
+                g.ar = -1.0;
+                v.push_back(g);
+            }
+            o.mb = v.size();
+            for (int n = 0; n < nodes; ++n) {  // invM_22 = -Wi a
+                GemmDesc g = desc0();
+                g.A = Wm + n * ii; g.lda = i;
+                g.B = Am + n * ii; g.ldb = i;
+                g.Out = IM + n * 4 * ii + i + (size_t)i * 2 * i; g.ldo = 2 * i;
+                g.ar = -1.0;
+                v.push_back(g);
+            }
+            for (int n = 0; n < nodes; ++n) {  // invM_11 = -c Wi
+                GemmDesc g = desc0();
+                g.A = Cm + n * ii; g.lda = i;
+                g.B = Wm + n * ii; g.ldb = i;
+                g.Out = IM + n * 4 * ii; g.ldo = 2 * i;
+                g.ar = -1.0;
+                v.push_back(g);
+            }
+            o.m12 = v.size();
+            for (int n = 0; n < nodes; ++n) {  // invM_12 = -(1 / nu) invM_11 a (I / nu added after)
+                GemmDesc g = desc0();
+                g.A = IM + n * 4 * ii; g.lda = 2 * i;
+                g.B = Am + n * ii; g.ldb = i;
+                g.Out = IM + n * 4 * ii + (size_t)i * 2 * i; g.ldo = 2 * i;
+                g.ar = -rnu.x; g.ai = -rnu.y;
+                v.push_back(g);
+            }
+            o.y = v.size();
+            for (int n = 0; n < nodes; ++n)  // Y = invM diag(T1_IN, T2_IN)
+                for (int c = 0; c < 2; ++c) {
+                    GemmDesc g = desc0();
+                    g.A = IM + n * 4 * ii + (size_t)c * i * 2 * i; g.lda = 2 * i;
+                    g.B = Tc + (size_t)(2 * n + c) * tstride + bn; g.ldb = b;
+                    g.Out = Y + (size_t)n * 2 * i * 2 * bn + (size_t)c * bn * 2 * i; g.ldo = 2 * i;
+                    v.push_back(g);
+                }
+            o.t = v.size();
             if (l > 0 || !opts.skip_root_rtr) {  // parent RtR, scattered into the parent's node order
-                std::vector<GemmDesc> v;
                 const int* prow = d_prow[l]->as<int>();
                 for (int n = 0; n < nodes; ++n) {
                     const int par = (l == 0) ? 0 : (n & 1);
@@ -602,11 +619,13 @@ int Tree::factor_device(const double2* k) {
                             v.push_back(g);
                         }
                 }
-                if (upload(*dl[4], v)) return 1;
             }
         }
+        // the column maps go into the buffer the descriptors point to; upload()
+        // waits for both copies
+        if (ncm) HIPCHK(hipMemcpyAsync(fcmap.p, colmaps.data(), ncm * sizeof(int), hipMemcpyHostToDevice, stream));
+        if (upload(fdesc, v)) return 1;
     }
-
     // leaves: block inverses and RtR maps (k may come from the default stream)
     if (join_default_in(stream)) return 1;
     HIPCHK(hipMemcpyAsync(kint.p, k, (size_t)nl * P.n_int * sizeof(double2), hipMemcpyDeviceToDevice, stream));
@@ -621,11 +640,12 @@ int Tree::factor_device(const double2* k) {
     if (mixed) launch_to_c64(sinv32.as<float2>(), sinv.as<double2>(), z.sinv64 / 16, stream);
     HIPCHK(hipGetLastError());
     // merges, bottom-up; every level of one launch sequence on the tree's stream.
-    // The merge products are idempotent (each writes a buffer it does not read),
+    // The merge products are idempotent (each writes a buffer it does not read;
+    // the diagonal terms of W and invM_12 are added by kernels of their own),
     // so the first factorization of a plan times their tile shapes and keeps the
     // fastest; later factorizations reuse the choice (results do not depend on it).
-    const bool tune = opts.autotune && (int)ftile.size() != 4 * L;
-    if (tune) ftile.assign(4 * L, -1);
+    const bool tune = opts.autotune && (int)ftile.size() != 5 * L;
+    if (tune) ftile.assign(5 * L, -1);
     hipEvent_t te0 = nullptr, te1 = nullptr;
     if (tune) {
         HIPCHK(hipEventCreate(&te0));
@@ -649,7 +669,7 @@ int Tree::factor_device(const double2* k) {
             float best = 0.f;
             for (int t = 0; t < GEMM_TILES; ++t) {
                 (void)hipEventRecord(te0, stream);
-                gemm(site / 4, d, np, M, N, K, gws, t);
+                gemm(site / 5, d, np, M, N, K, gws, t);
                 (void)hipEventRecord(te1, stream);
                 if (hipEventSynchronize(te1) != hipSuccess) return fail("Error: GEMM autotuning failed.");
                 float ms = 0.f;
@@ -658,7 +678,7 @@ int Tree::factor_device(const double2* k) {
             }
             ftile[site] = tile;
         }
-        gemm(site / 4, d, np, M, N, K, gws, tile);
+        gemm(site / 5, d, np, M, N, K, gws, tile);
         return 0;
     };
     for (int l = L - 1; l >= 0; --l) {
@@ -667,30 +687,30 @@ int Tree::factor_device(const double2* k) {
         const int nodes = V.nodes, i = V.isz, b = V.bc, bn = V.bn;
         const size_t ii = (size_t)i * i, tstride = (size_t)b * b;
         const double2* Tc = Tfull(l + 1);
-        double2* Dm = (double2*)(ar + m.D);
-        double2* Sm = (double2*)(ar + m.S);
-        double2* SD = (double2*)(ar + m.SD);
-        double2* W = (double2*)(ar + m.W);
+        double2* Am = (double2*)(ar + m.A);
+        double2* Cm = (double2*)(ar + m.C);
+        double2* Wm = (double2*)(ar + m.W);
+        double2* GW = (double2*)(ar + m.GW);
         int* piv = (int*)(ar + m.piv);
         void* gws = ar + m.ws;
         double2* IM = IMf(l);
-        DevBuf** dl = &fdesc[6 * l];
-        // D = (zeta I - T2_II)^-1
-        launch_form_iface(Dm, Tc, tstride, 2, 1, b, bn, i, nodes, zeta, make_double2(0, 0), nullptr, stream);
-        launch_inverse(m.ip, Dm, piv, W, dl[0]->as<GemmDesc>(), info.as<int>(), stream);
-        // S = (zeta I - T1_II - nu^2 D)^-1
-        launch_form_iface(Sm, Tc, tstride, 2, 0, b, bn, i, nodes, zeta, make_double2(-nu2.x, -nu2.y), Dm, stream);
-        launch_inverse(m.ip, Sm, piv, W, dl[1]->as<GemmDesc>(), info.as<int>(), stream);
-        // invM = [S, -nu S D; -nu D S, D + nu^2 D S D]
-        if (fgemm(4 * l + 0, dl[2]->as<GemmDesc>(), nodes, i, i, i, gws) ||
-            fgemm(4 * l + 1, dl[2]->as<GemmDesc>() + nodes, 2 * nodes, i, i, i, gws))
+        const MergeDescs& o = fdoff[l];
+        const GemmDesc* fd = fdesc.as<GemmDesc>();
+        // a = zeta I - T1_II, c = zeta I - T2_II
+        launch_form_iface(Am, Tc, tstride, 2, 0, b, bn, i, nodes, zeta, make_double2(0, 0), nullptr, stream);
+        launch_form_iface(Cm, Tc, tstride, 2, 1, b, bn, i, nodes, zeta, make_double2(0, 0), nullptr, stream);
+        // Wi = (nu^2 I - a c)^-1
+        if (fgemm(5 * l + 0, fd + o.w, nodes, i, i, i, gws)) return 1;
+        launch_add_diag(Wm, ii, i, i, nodes, nu2, stream);
+        launch_inverse(m.ip, Wm, piv, GW, fd + o.gj, info.as<int>(), stream);
+        // invM = [-c Wi, I / nu - (1 / nu) (-c Wi) a; nu Wi, -Wi a]
+        if (fgemm(5 * l + 1, fd + o.mb, 2 * nodes, i, i, i, gws) || fgemm(5 * l + 2, fd + o.m12, nodes, i, i, i, gws))
             return 1;
-        launch_copy_blocks(IM, 4 * ii, 2 * i, Sm, ii, i, i, i, nodes, make_double2(1, 0), stream);
-        launch_copy_blocks(IM + (size_t)i * 2 * i, 4 * ii, 2 * i, SD, ii, i, i, i, nodes, make_double2(-nu.x, -nu.y),
-                           stream);
+        launch_add_diag(IM + (size_t)i * 2 * i, 4 * ii, 2 * i, i, nodes, rnu, stream);
+        launch_copy_blocks(IM + i, 4 * ii, 2 * i, Wm, ii, i, i, i, nodes, nu, stream);
         // Y = invM diag(T1_IN, T2_IN); parent T = diag(T_NN) + diag(T_NI) Y
-        if (fgemm(4 * l + 2, dl[3]->as<GemmDesc>(), 2 * nodes, 2 * i, bn, i, gws)) return 1;
-        if ((l > 0 || !opts.skip_root_rtr) && fgemm(4 * l + 3, dl[4]->as<GemmDesc>(), 4 * nodes, bn, bn, i, gws))
+        if (fgemm(5 * l + 3, fd + o.y, 2 * nodes, 2 * i, bn, i, gws)) return 1;
+        if ((l > 0 || !opts.skip_root_rtr) && fgemm(5 * l + 4, fd + o.t, 4 * nodes, bn, bn, i, gws))
             return 1;
         // the solve data of this merge: inv(M), and the children's T_NI / T_IN,
         // before the next level reuses the buffers they come from
@@ -710,8 +730,8 @@ int Tree::factor_device(const double2* k) {
     if (!keep_scratch) {
         fscratch.release(); invM_t.release(); tmap[0].release(); tmap[1].release();
         if (!keep64) sinv.release();
-        for (DevBuf* b : fdesc) delete b;
-        fdesc.clear();
+        fdesc.release(); fcmap.release();
+        fdoff.clear();
     }
     factored = true;
     have64 = keep64;

@@ -186,8 +186,13 @@ struct Tree {
     DevBuf tmap[2];
     DevBuf invM_t;                // FP64 inv(M) of one level (mixed trees without FP64 solve data)
     DevBuf fscratch;              // merge scratch of the largest level
-    std::vector<int> ftile;       // per level, 4 merge products: GEMM tile chosen by the first factor's timing
-    std::vector<DevBuf*> fdesc;   // per level: the merge's product descriptors
+    std::vector<int> ftile;       // per level, 5 merge products: GEMM tile chosen by the first factor's timing
+    // the merge's product descriptors of every level in one array (fdesc), the
+    // Gauss-Jordan column maps in another (fcmap); per level the offsets of
+    // each product's descriptors (elements)
+    struct MergeDescs { size_t gj = 0, w = 0, mb = 0, m12 = 0, y = 0, t = 0, cm = 0; };
+    DevBuf fdesc, fcmap;
+    std::vector<MergeDescs> fdoff;
     bool have32 = false;
     bool merge_c64 = false;      // some merge products of the factorization ran in complex64
     // maps
