@@ -450,6 +450,116 @@ __global__ void __launch_bounds__(256) gemm_skinny_kernel(GemmLaunch GL) {
     }
 }
 
+// Products of at most NC columns without the matrix cores: one lane per
+// (problem, K slice, output row), lanes of a wave on consecutive rows, so each
+// load of A is one coalesced column segment and nothing is computed for the
+// 15 of 16 MFMA columns a one-column product leaves empty.  A
+// v_mfma_*_16x16x4 accumulates its four k terms as four fused multiply-adds
+// in k order, so the chains below -- k ascending from the slice start in
+// groups of four, the last group padded with zero terms, the three real
+// products of Gauss's form -- give the bits of every other tile; the K split,
+// the workspace and the epilogue are those of gemm_skinny_kernel.
+template <typename R, int NC, int U>
+__global__ void __launch_bounds__(256) gemv_kernel(GemmLaunch GL) {
+    using V = c_t<R>;
+    static_assert(U % 4 == 0, "k steps come in groups of four");
+    const GemmDesc* descs = GL.g[0].descs;
+    V* ws = (V*)GL.g[0].ws;
+    int M = GL.g[0].M, N = GL.g[0].N, K = GL.g[0].K, nprob = GL.g[0].nprob, kslices = GL.g[0].ks,
+        kchunk = GL.g[0].kchunk, wg = blockIdx.x;
+#pragma unroll
+    for (int q = 1; q < GEMM_MAX_GROUPS; ++q)
+        if (q < GL.n && (int)blockIdx.x >= GL.g[q].wg0) {
+            descs = GL.g[q].descs; ws = (V*)GL.g[q].ws;
+            M = GL.g[q].M; N = GL.g[q].N; K = GL.g[q].K; nprob = GL.g[q].nprob;
+            kslices = GL.g[q].ks; kchunk = GL.g[q].kchunk; wg = blockIdx.x - GL.g[q].wg0;
+        }
+    const long long chain = (long long)wg * 256 + threadIdx.x;
+    if (chain >= (long long)nprob * kslices * M) return;
+    const int row = (int)(chain % M);
+    const long long ps = chain / M;
+    const int ks = (int)(ps % kslices), p = (int)(ps / kslices);
+    const GemmDesc& D = descs[p];
+    const V* const Ar = (const V*)D.A + row;
+    const V* const B = (const V*)D.B;
+    const V* const B2 = (const V*)D.B2;
+    const int* const bmap = D.bmap;
+    const int lda = D.lda, ldb = D.ldb, ksplit = D.ksplit;
+    const int kbeg = ks * kchunk, kend = min(K, kbeg + kchunk);
+    const int kend4 = kbeg + ((kend - kbeg + 3) & ~3);   // end of the last group of four
+
+    R p1[NC], p2[NC], p3[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) { p1[c] = R(0); p2[c] = R(0); p3[c] = R(0); }
+    // two register stages of U k-steps: the loads of the next stage are in
+    // flight while the current one is summed
+    V a0[U], b0[U][NC], a1[U], b1[U][NC];
+    auto fetch = [&](int k0, V (&a)[U], V (&b)[U][NC]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u;
+            const bool ok = k < kend;
+            const int kc = ok ? k : kend - 1;
+            a[u] = gld(Ar + (size_t)kc * lda);
+            const int bo = bmap ? gldi(bmap + kc) : 0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const size_t boff = (size_t)(c < N ? c : N - 1) * ldb;
+                b[u][c] = gld(bmap ? B + boff + bo : (kc < ksplit ? B + boff + kc : B2 + boff + (kc - ksplit)));
+            }
+            if (!ok) {
+                a[u] = cz<R>();
+#pragma unroll
+                for (int c = 0; c < NC; ++c) b[u][c] = cz<R>();
+            }
+        }
+    };
+    auto step = [&](int k0, V (&a)[U], V (&b)[U][NC]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (k0 + u >= kend4) break;
+            const R as = a[u].x + a[u].y;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                p1[c] = fma(a[u].x, b[u][c].x, p1[c]);
+                p2[c] = fma(a[u].y, b[u][c].y, p2[c]);
+                p3[c] = fma(as, b[u][c].x + b[u][c].y, p3[c]);
+            }
+        }
+    };
+    if (kbeg < kend4) fetch(kbeg, a0, b0);
+    for (int k0 = kbeg; k0 < kend4; k0 += 2 * U) {
+        if (k0 + U < kend4) fetch(k0 + U, a1, b1);
+        step(k0, a0, b0);
+        if (k0 + U >= kend4) break;
+        if (k0 + 2 * U < kend4) fetch(k0 + 2 * U, a0, b0);
+        step(k0 + U, a1, b1);
+    }
+
+    if (kslices > 1) {
+        V* W = ws + ((size_t)p * kslices + ks) * (size_t)M * N;
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            if (c < N) W[(size_t)row + (size_t)c * M] = mk<R>(p1[c] - p2[c], (p3[c] - p1[c]) - p2[c]);
+        return;
+    }
+    V* const Out = (V*)D.Out;
+    const V* const Din = (const V*)D.Din;
+    const V alpha = mk<R>((R)D.ar, (R)D.ai), beta = mk<R>((R)D.br, (R)D.bi);
+    const int orow = D.rmap ? gldi(D.rmap + row) : row;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        if (c >= N) break;
+        const int ocol = D.cmap ? gldi(D.cmap + c) : c;
+        V val = cmul(alpha, mk<R>(p1[c] - p2[c], (p3[c] - p1[c]) - p2[c]));
+        if (Din) {
+            const size_t di = D.dmap ? (size_t)orow + (size_t)ocol * D.ldd : (size_t)row + (size_t)c * D.ldd;
+            val = cfma(beta, gld(Din + di), val);
+        }
+        gst(Out + ((size_t)orow + (size_t)ocol * D.ldo), val);
+    }
+}
+
 // Sum of the K slices in slice order, then the epilogue of gemm_kernel.
 // (The slices are summed in precision R, the epilogue runs in S.)
 template <typename S, typename R>
@@ -520,10 +630,13 @@ void launch_gemm_groups(const GemmGroup* grp, int ngroups, hipStream_t st, int t
     // tile, so the tile shape (by rule from M and N, or chosen by the caller's
     // autotuning) never changes the results.
     if (tile < 0 || tile >= GEMM_TILES) tile = gemm_tile_rule(tile_M, tile_N);
-    // the skinny kernel runs the products of the mixed-precision merge (S != R) as 64x16
+    // the skinny kernel runs the products of the mixed-precision merge (S != R) as
+    // 64x16, and so does the matrix-core-free kernel those and products of more
+    // than GEMM_GEMV_MAX_N columns
     if (tile == GEMM_TILE_SKINNY && !std::is_same<S, R>::value) tile = GEMM_TILE_64x16;
-    static const int TBM[GEMM_TILES] = {64, 32, 16, 64, 32, 64, 128, 64},
-                     TBN[GEMM_TILES] = {64, 64, 64, 16, 32, 32, 16, 16};
+    if (tile == GEMM_TILE_GEMV && (!std::is_same<S, R>::value || tile_N > GEMM_GEMV_MAX_N)) tile = GEMM_TILE_64x16;
+    static const int TBM[GEMM_TILES] = {64, 32, 16, 64, 32, 64, 128, 64, 1},
+                     TBN[GEMM_TILES] = {64, 64, 64, 16, 32, 32, 16, 16, GEMM_GEMV_MAX_N};
     const int bm = TBM[tile], bn = TBN[tile];
     int wg = 0;
     for (int q = 0; q < ngroups; ++q) {
@@ -537,13 +650,22 @@ void launch_gemm_groups(const GemmGroup* grp, int ngroups, hipStream_t st, int t
         e.descs = g.descs; e.ws = g.ws;
         e.M = g.M; e.N = g.N; e.K = g.K;
         e.tilesM = (g.M + bm - 1) / bm; e.tilesN = (g.N + bn - 1) / bn;
-        e.ks = ks; e.kchunk = kchunk; e.wg0 = wg;
-        wg += g.nprob * e.tilesM * e.tilesN * ks;
+        e.ks = ks; e.kchunk = kchunk; e.wg0 = wg; e.nprob = g.nprob;
+        if (tile == GEMM_TILE_GEMV)   // one lane per (problem, slice, row)
+            wg += (int)(((long long)g.nprob * ks * g.M + 255) / 256);
+        else
+            wg += g.nprob * e.tilesM * e.tilesN * ks;
     }
     auto run = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(wg), dim3(256), 0, st, GL); };
     switch (tile) {
         case GEMM_TILE_SKINNY:
             if constexpr (std::is_same<S, R>::value) run(gemm_skinny_kernel<R, 8>);
+            break;
+        case GEMM_TILE_GEMV:
+            if constexpr (std::is_same<S, R>::value) {
+                if (tile_N == 1) run(gemv_kernel<R, 1, 8>);
+                else run(gemv_kernel<R, GEMM_GEMV_MAX_N, 4>);
+            }
             break;
         case GEMM_TILE_64x16: run(gemm_kernel<S, R, 64, 16>); break;
         case GEMM_TILE_16x64: run(gemm_kernel<S, R, 16, 64>); break;
@@ -557,7 +679,7 @@ void launch_gemm_groups(const GemmGroup* grp, int ngroups, hipStream_t st, int t
         const GemmLaunch::Grp& e = GL.g[q];
         if (e.ks <= 1) continue;
         const size_t mn = (size_t)e.M * e.N;
-        const int nprob = ((q + 1 < GL.n ? GL.g[q + 1].wg0 : wg) - e.wg0) / (e.tilesM * e.tilesN * e.ks);
+        const int nprob = e.nprob;
         hipLaunchKernelGGL((gemm_reduce_kernel<S, R>), dim3((unsigned)((mn + 255) / 256), nprob), dim3(256), 0, st,
                            e.descs, e.M, e.N, e.ks, (const c_t<R>*)e.ws);
     }

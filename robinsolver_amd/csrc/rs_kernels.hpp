@@ -31,9 +31,15 @@ struct GemmDesc {
 // K-split form (NULL: no split).  tile: one of GEMM_TILE_* (output tile rows x
 // columns), or -1 for the shape rule gemm_tile_rule; results do not depend on it.
 // GEMM_TILE_SKINNY: 64 x 16 tiles whose waves read A straight from memory
-// (products of few columns).
+// (products of few columns).  GEMM_TILE_GEMV: products of at most
+// GEMM_GEMV_MAX_N columns on the vector ALUs, one lane per output row and K
+// slice (other products take the 64 x 16 tiles under this choice).
 enum { GEMM_TILE_64x64 = 0, GEMM_TILE_32x64 = 1, GEMM_TILE_16x64 = 2, GEMM_TILE_64x16 = 3, GEMM_TILE_32x32 = 4,
-       GEMM_TILE_64x32 = 5, GEMM_TILE_128x16 = 6, GEMM_TILE_SKINNY = 7, GEMM_TILES = 8 };
+       GEMM_TILE_64x32 = 5, GEMM_TILE_128x16 = 6, GEMM_TILE_SKINNY = 7, GEMM_TILE_GEMV = 8, GEMM_TILES = 9 };
+constexpr int GEMM_GEMV_MAX_N = 4;
+// whether a tile choice is a kernel of its own for an N-column product (the
+// tuners skip the ones that would repeat another)
+inline bool gemm_tile_distinct(int tile, int N) { return tile != GEMM_TILE_GEMV || N <= GEMM_GEMV_MAX_N; }
 int gemm_tile_rule(int M, int N);
 // ks_target: the K split aims at this many (row tile x slice) workgroups per
 // launch; it never depends on N.
@@ -62,7 +68,7 @@ struct GemmLaunch {
     struct Grp {
         const GemmDesc* descs;
         void* ws;
-        int M, N, K, tilesM, tilesN, ks, kchunk, wg0;
+        int M, N, K, tilesM, tilesN, ks, kchunk, wg0, nprob;
     } g[GEMM_MAX_GROUPS];
     int n;
 };

@@ -668,6 +668,7 @@ int Tree::factor_device(const double2* k) {
         if (tune) {
             float best = 0.f;
             for (int t = 0; t < GEMM_TILES; ++t) {
+                if (!gemm_tile_distinct(t, N)) continue;
                 (void)hipEventRecord(te0, stream);
                 gemm(site / 5, d, np, M, N, K, gws, t);
                 (void)hipEventRecord(te1, stream);
@@ -970,42 +971,59 @@ int Tree::prepare_tiles(int cap) {
 }
 
 template <typename R> int Tree::tune_work(SolveWork<R>& w) {
-    // Each product is run on the workspace's own buffers; every one of them
-    // writes a buffer it does not read, so repeating it is harmless.
+    // The products are timed where they run: in the order of a pass (up_g and
+    // up_h per level from the leaves up, then dn_t and dn_g per level from the
+    // root down), every product of a sequence with one tile choice, events
+    // between the launches.  Each product then meets the caches as in a solve:
+    // its factor operand comes from HBM (the other levels' data went through
+    // the caches since its last use) and the vectors it reads were just
+    // written by the product before it.  (Timed alone and repeated, a product
+    // finds its operands in the caches, which favours tiles that keep few
+    // loads in flight.)  Two sequences per choice, the second timed; per
+    // product the fastest choice is kept.  The products run on the workspace's
+    // own buffers, whose contents every solve overwrites before reading them.
     const Plan& P = plan;
     const int L = P.L, N = w.cap;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    int rc = 0;
-    for (int l = 0; l < L && !rc; ++l) {
+    struct Site { int kind, l; const GemmDesc* d; int np, M, K; };
+    std::vector<Site> seq;
+    for (int l = L - 1; l >= 0; --l) {
         const Level& V = P.lv[l];
-        struct Site { int kind; const GemmDesc* d; int np, M, K; };
-        const Site sites[4] = {{0, w.d_up_g[l], V.nodes, 2 * V.isz, 2 * V.isz},
-                               {1, w.d_up_h[l], 2 * V.nodes, V.bn, V.isz},
-                               {2, w.d_dn_t[l], 2 * V.nodes, V.isz, V.bn},
-                               {3, w.d_dn_g[l], V.nodes, 2 * V.isz, 2 * V.isz}};
-        for (const Site& st_ : sites) {
-            if (st_.kind == 1 && l == 0) continue;   // no parent: the root level has no up_h
-            float best = 0.f;
-            int bt = -1;
-            for (int t = 0; t < GEMM_TILES; ++t) {
-                const int kt = solve_ks_target(st_.np, st_.M, st_.K);
-                launch_gemm<R>(st_.d, st_.np, st_.M, N, st_.K, stream, w.ws, t, kt);
-                (void)hipEventRecord(e0, stream);
-                for (int r = 0; r < 3; ++r) launch_gemm<R>(st_.d, st_.np, st_.M, N, st_.K, stream, w.ws, t, kt);
-                (void)hipEventRecord(e1, stream);
-                if (hipEventSynchronize(e1) != hipSuccess) { rc = fail("Error: GEMM autotuning failed."); break; }
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (bt < 0 || ms < best) { best = ms; bt = t; }
+        seq.push_back({0, l, w.d_up_g[l], V.nodes, 2 * V.isz, 2 * V.isz});
+        if (l > 0) seq.push_back({1, l, w.d_up_h[l], 2 * V.nodes, V.bn, V.isz});
+    }
+    for (int l = 0; l < L; ++l) {
+        const Level& V = P.lv[l];
+        seq.push_back({2, l, w.d_dn_t[l], 2 * V.nodes, V.isz, V.bn});
+        seq.push_back({3, l, w.d_dn_g[l], V.nodes, 2 * V.isz, 2 * V.isz});
+    }
+    const size_t ns = seq.size();
+    std::vector<hipEvent_t> ev(ns + 1, nullptr);
+    int rc = 0;
+    for (hipEvent_t& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) { e = nullptr; rc = fail("Error: hipEventCreate failed."); }
+    std::vector<float> best(ns, 0.f);
+    std::vector<int> bt(ns, -1);
+    for (int t = 0; t < GEMM_TILES && !rc; ++t) {
+        if (!gemm_tile_distinct(t, N)) continue;
+        for (int rep = 0; rep < 2; ++rep) {
+            for (size_t j = 0; j < ns; ++j) {
+                const Site& s = seq[j];
+                if (rep) (void)hipEventRecord(ev[j], stream);
+                launch_gemm<R>(s.d, s.np, s.M, N, s.K, stream, w.ws, t, solve_ks_target(s.np, s.M, s.K));
             }
-            if (rc) break;
-            w.tile[st_.kind][l] = bt;
+            if (rep) (void)hipEventRecord(ev[ns], stream);
+        }
+        if (hipEventSynchronize(ev[ns]) != hipSuccess) { rc = fail("Error: GEMM autotuning failed."); break; }
+        for (size_t j = 0; j < ns; ++j) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[j], ev[j + 1]);
+            if (bt[j] < 0 || ms < best[j]) { best[j] = ms; bt[j] = t; }
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (!rc)
+        for (size_t j = 0; j < ns; ++j) w.tile[seq[j].kind][seq[j].l] = bt[j];
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
     return rc;
 }
 
