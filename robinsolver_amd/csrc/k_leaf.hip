@@ -414,6 +414,170 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
     }
 }
 
+// Few right-hand sides (at most CW per workgroup, FP64, 2D layers of at most
+// 16 points): one wave per leaf.  The block layer recursion of
+// leaf_apply_kernel is latency-bound with one column -- a 256-thread
+// workgroup per leaf waits for each layer's block (4 KB) before its 16 dot
+// products -- so here each lane keeps its share of the next PF layers' blocks
+// in flight in registers: lane (w, q) = (lane % 16, lane / 16) holds row w of
+// the block at columns w2 = q, q + 4, ... (and the columns past the last
+// multiple of four for q = 0), which is the partial sum a_q of layer_dot.  The
+// four partial sums meet in LDS and are added in layer_dot's order, and every
+// other step is leaf_apply_kernel's own arithmetic, so a column gets the same
+// bits from either kernel.  The forward sweep's blocks 0..K-1 and the
+// backward sweep's K-2..0 form one stream of 2K - 1 steps.
+template <int CW>
+__global__ void __launch_bounds__(64) leaf_wave_kernel(LeafGeom G, LeafCoef C, LeafIO<double> io) {
+    using V = double2;
+    constexpr int PF = 4;   // steps of blocks in flight
+    extern __shared__ unsigned char smraw[];
+    const int s = G.s, K = G.K, ss = s * s, ni = G.n_int, nb = G.nb;
+    V* Y = (V*)smraw;                // [ni][CW]
+    V* Vt = Y + (size_t)ni * CW;     // [s][CW]
+    V* Pp = Vt + 16 * CW;            // [4][16][CW] partial sums
+    const int leaf = blockIdx.x, c0 = blockIdx.y * CW, lane = threadIdx.x;
+    const int ncol = min(CW, io.ncols - c0);
+    const int parity = leaf & 1;
+    const V* sl = io.sinv + (size_t)leaf * K * ss;
+    const int w = lane & 15, q = lane >> 4;
+    const int s4 = s & ~3;
+    const int nsteps = 2 * K - 1;
+
+    auto gval = [&](int b, int c) -> V {
+        const int col = c0 + c;
+        if (io.mode == LEAF_DOWN)
+            return io.gin[(size_t)io.gsrc[(size_t)leaf * nb + local_to_node(G, parity, b)] + (size_t)col * io.gld];
+        return cz<double>();
+    };
+    // step j's block: the forward sweep's j, then the backward sweep's 2K - 2 - j
+    auto blk = [&](int j) { return j < K ? j : 2 * K - 2 - j; };
+    V br[PF][4], bt[PF][3];
+    auto load = [&](int j, V (&r)[4], V (&t)[3]) {
+        const V* Bk = sl + (size_t)blk(min(j, nsteps - 1)) * ss;
+        const int wl = w < s ? w : 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int w2 = q + 4 * u;
+            r[u] = gld(Bk + wl + (size_t)(w2 < s4 ? w2 : 0) * s);
+            if (w2 >= s4) r[u] = cz<double>();
+        }
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int w2 = s4 + u;
+            const bool ok = q == 0 && w2 < s;
+            t[u] = gld(Bk + wl + (size_t)(ok ? w2 : 0) * s);
+            if (!ok) t[u] = cz<double>();
+        }
+    };
+#pragma unroll
+    for (int p = 0; p < PF; ++p) load(p, br[p], bt[p]);
+
+    // 1. right-hand side r = f - A_IB A_BB^-1 g (leaf_apply_kernel step 1)
+    for (int e = lane; e < ni * CW; e += 64) {
+        const int qp = e / CW, c = e % CW;
+        V v = cz<double>();
+        if (c < ncol) {
+            v = load_f<V>(io, c0 + c, leaf, ni, qp);
+            if (io.mode != LEAF_UP) {
+                int g[3];
+                interior_coords(G, qp, g);
+                for (int j = 0; j < G.d; ++j) {
+                    const AxisCoef& a = C.ax[j];
+                    double2 wgt = cscale(cinv(a.dd), -a.cib);
+                    V wr = ccast<V>(wgt);
+                    if (g[j] == 1) v = cfma(wr, gval(face_point(G, 2 * j + 1, g), c), v);
+                    if (g[j] == G.n[j]) v = cfma(wr, gval(face_point(G, 2 * j + 2, g), c), v);
+                }
+            }
+        }
+        Y[e] = v;
+    }
+    __syncthreads();
+    // the first step's input: layer 0 of r
+    for (int e = lane; e < s * CW; e += 64) Vt[e] = Y[e];
+    __syncthreads();
+
+    // 2./3. the 2K - 1 block steps
+    auto step = [&](int j, V (&r)[4], V (&t)[3]) {
+        // partial sums a_q of row w (layer_dot's chains)
+        if (w < s) {
+#pragma unroll
+            for (int c = 0; c < CW; ++c) {
+                if (c >= ncol) break;
+                V a = cz<double>();
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (q + 4 * u < s4) a = cfma(r[u], Vt[(q + 4 * u) * CW + c], a);
+                if (q == 0) {
+#pragma unroll
+                    for (int u = 0; u < 3; ++u)
+                        if (s4 + u < s) a = cfma(t[u], Vt[(s4 + u) * CW + c], a);
+                }
+                Pp[(q * 16 + w) * CW + c] = a;
+            }
+        }
+        __syncthreads();
+        // the layer's result, and the next step's input
+        const int kb = blk(j);
+        for (int e = lane; e < s * CW; e += 64) {
+            const int wr = e / CW, c = e % CW;
+            const V a0 = Pp[(0 * 16 + wr) * CW + c], a1 = Pp[(1 * 16 + wr) * CW + c];
+            const V a2 = Pp[(2 * 16 + wr) * CW + c], a3 = Pp[(3 * 16 + wr) * CW + c];
+            const V res = cadd(cadd(a0, a1), cadd(a2, a3));
+            V* yk = Y + (size_t)kb * s * CW + e;
+            const V y = j < K ? res : csub(*yk, res);
+            *yk = y;
+            if (j + 1 < K) {          // forward: r_{k+1} - lambda_{k+1} y_k
+                const V lam = slow_lambda(G, C, j + 1);
+                Vt[e] = csub(Y[(size_t)(j + 1) * s * CW + e], cmul(lam, y));
+            } else if (j + 1 < nsteps) {   // backward: mu_k x_{k+1}, k = blk(j + 1) = kb - 1
+                const V mu = slow_mu(G, C, kb - 1);
+                Vt[e] = cmul(mu, y);
+            }
+        }
+        __syncthreads();
+    };
+    for (int j0 = 0; j0 < nsteps; j0 += PF) {
+#pragma unroll
+        for (int p = 0; p < PF; ++p) {
+            const int j = j0 + p;
+            if (j < nsteps) {
+                step(j, br[p], bt[p]);
+                if (j + PF < nsteps) load(j + PF, br[p], bt[p]);
+            }
+        }
+    }
+
+    // 4. boundary values and outputs (leaf_apply_kernel step 4)
+    const double2 sma = make_double2(0.0, -2.0 * G.kb);  // sigma - alpha
+    for (int e = lane; e < nb * CW; e += 64) {
+        const int b = e / CW, c = e % CW;
+        if (c >= ncol) continue;
+        int f, j, q1, stp;
+        boundary_info(G, b, f, j, q1, stp);
+        const AxisCoef& a = C.ax[j];
+        V gb = gval(b, c);
+        V x1 = Y[(size_t)q1 * CW + c], x2 = Y[(size_t)(q1 + stp) * CW + c];
+        V num = csub(csub(gb, cscale(x1, a.a1)), cscale(x2, a.a2));
+        V uB = cmul(num, ccast<V>(cinv(a.dd)));
+        V hb = cfma(ccast<V>(sma), uB, gb);
+        const int col = c0 + c;
+        if (io.mode == LEAF_UP) {
+            if (io.jump) hb = csub(hb, ccast<V>(io.jump[((size_t)col * io.n_leaves + leaf) * nb + b]));
+            io.hout[((size_t)col * io.n_leaves + leaf) * nb + local_to_node(G, parity, b)] = hb;
+        } else {
+            store_u(io, col, leaf, G.m, b, uB);
+        }
+    }
+    if (io.mode == LEAF_DOWN) {
+        for (int e = lane; e < ni * CW; e += 64) {
+            const int qp = e / CW, c = e % CW;
+            if (c >= ncol) continue;
+            store_u(io, c0 + c, leaf, G.m, nb + qp, Y[e]);
+        }
+    }
+}
+
 // FP64 residual of the global FD system at the current leaf solutions u:
 //   interior rows       rf = f - (Delta_h + k) u                     (fd_module.f90:140-181)
 //   physical rows       rg = g - (alpha + D_n) u                     (fd_module.f90:187-216)
@@ -1314,6 +1478,18 @@ void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit
             else if (G.K == 6) RS_LEAF3_MFMA(6);
             else RS_LEAF3_MFMA(4);
 #undef RS_LEAF3_MFMA
+            return;
+        }
+    }
+    if constexpr (std::is_same<R, double>::value) {
+        // few columns of a 2D tree: one wave per leaf (same bits)
+        if (G.d == 2 && G.s <= 16 && ncols <= 4 && (mode == LEAF_UP || mode == LEAF_DOWN)) {
+            const int cw = ncols == 1 ? 1 : 4;
+            io.CH = cw;
+            const size_t lds = ((size_t)G.n_int * cw + 16 * cw + 64 * cw) * sizeof(double2);
+            dim3 grid(n_leaves, (ncols + cw - 1) / cw);
+            if (cw == 1) hipLaunchKernelGGL(leaf_wave_kernel<1>, grid, dim3(64), lds, st, G, C, io);
+            else hipLaunchKernelGGL(leaf_wave_kernel<4>, grid, dim3(64), lds, st, G, C, io);
             return;
         }
     }
