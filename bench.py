@@ -458,7 +458,19 @@ def main():
     f_time_64 = t64.timings()[0]
     t64.factor_device(k_dev.data_ptr())
     f_time_64 = min(f_time_64, t64.timings()[0])
-    log("factor s: fp64 %.4f, mixed %.4f" % (f_time_64, f_time_mixed))
+    # a refactorization in place (new potential, same tree): the tree keeps its
+    # factorization working memory (rs_opts.retain = RS_RETAIN_SCRATCH, as the
+    # server's resident trees do), so nothing is allocated or freed
+    t64r = rsa.GpuTree(d, args.levels, unit, 1.0 / args.grid, kb_opt, precision=rsa.RS_FP64,
+                       retain=rsa.RS_RETAIN_SCRATCH)
+    t64r.factor_device(k_dev.data_ptr())
+    refactor_64 = []
+    for _ in range(3):
+        t64r.factor_device(k_dev.data_ptr())
+        refactor_64.append(t64r.timings()[0])
+    refactor_64 = min(refactor_64)
+    del t64r
+    log("factor s: fp64 %.4f, mixed %.4f; fp64 refactor in place %.4f" % (f_time_64, f_time_mixed, refactor_64))
 
     def solve(tree, nrhs, fp, gp, up, prec):
         tree.solve_device(nrhs, fp, gp, up, precision=prec)
@@ -649,6 +661,7 @@ def main():
                    "dim": d, "grid": args.grid, "levels": args.levels, "kb": kb_phys, "nrhs": args.nrhs,
                    "seed": args.seed},
         "factor_s_fp64": f_time_64,
+        "refactor_in_place_s_fp64": refactor_64,
         "factor_s_mixed": f_time_mixed,
         "fp64_r1_solves_per_s": r1_rate,
         "fp64_r1_roofline": r1_roof,

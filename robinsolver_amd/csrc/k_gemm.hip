@@ -85,8 +85,8 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmLaunch GL) {
     using V = c_t<R>;
     using VS = c_t<S>;
     using A4 = typename acc_t<R>::T;
-    // four waves: 2x2 for 64x64, 32x64, 32x32 and 64x32 tiles, 4x1 for 64x16 and
-    // 128x16, 1x4 for 16x64
+    // four waves: 2x2 for 128x64, 64x128, 64x64, 32x64, 32x32 and 64x32 tiles,
+    // 4x1 for 64x16 and 128x16, 1x4 for 16x64
     constexpr int WAVES_N = (BN == 16) ? 1 : (BM == 16 ? 4 : 2);
     constexpr int WAVES_M = 4 / WAVES_N;
     constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
@@ -635,8 +635,9 @@ void launch_gemm_groups(const GemmGroup* grp, int ngroups, hipStream_t st, int t
     // than GEMM_GEMV_MAX_N columns
     if (tile == GEMM_TILE_SKINNY && !std::is_same<S, R>::value) tile = GEMM_TILE_64x16;
     if (tile == GEMM_TILE_GEMV && (!std::is_same<S, R>::value || tile_N > GEMM_GEMV_MAX_N)) tile = GEMM_TILE_64x16;
-    static const int TBM[GEMM_TILES] = {64, 32, 16, 64, 32, 64, 128, 64, 1},
-                     TBN[GEMM_TILES] = {64, 64, 64, 16, 32, 32, 16, 16, GEMM_GEMV_MAX_N};
+    if ((tile == GEMM_TILE_128x64 || tile == GEMM_TILE_64x128) && !std::is_same<R, float>::value) tile = GEMM_TILE_64x64;
+    static const int TBM[GEMM_TILES] = {64, 32, 16, 64, 32, 64, 128, 64, 1, 128, 64},
+                     TBN[GEMM_TILES] = {64, 64, 64, 16, 32, 32, 16, 16, GEMM_GEMV_MAX_N, 64, 128};
     const int bm = TBM[tile], bn = TBN[tile];
     int wg = 0;
     for (int q = 0; q < ngroups; ++q) {
@@ -673,6 +674,12 @@ void launch_gemm_groups(const GemmGroup* grp, int ngroups, hipStream_t st, int t
         case GEMM_TILE_32x32: run(gemm_kernel<S, R, 32, 32>); break;
         case GEMM_TILE_64x32: run(gemm_kernel<S, R, 64, 32>); break;
         case GEMM_TILE_128x16: run(gemm_kernel<S, R, 128, 16>); break;
+        case GEMM_TILE_128x64:
+            if constexpr (std::is_same<R, float>::value) run(gemm_kernel<S, R, 128, 64>);
+            break;
+        case GEMM_TILE_64x128:
+            if constexpr (std::is_same<R, float>::value) run(gemm_kernel<S, R, 64, 128>);
+            break;
         default: run(gemm_kernel<S, R, 64, 64>); break;
     }
     for (int q = 0; q < GL.n; ++q) {
@@ -709,8 +716,9 @@ void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStr
     launch_gemm_sr<R, R>(d_descs, nprob, M, N, K, st, ws, tile, ks_target);
 }
 
-void launch_gemm_c64(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile) {
-    launch_gemm_sr<double, float>(d_descs, nprob, M, N, K, st, ws, tile, 256);
+void launch_gemm_c64(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws, int tile,
+                     int ks_target) {
+    launch_gemm_sr<double, float>(d_descs, nprob, M, N, K, st, ws, tile, ks_target);
 }
 
 template void launch_gemm<double>(const GemmDesc*, int, int, int, int, hipStream_t, void*, int, int);

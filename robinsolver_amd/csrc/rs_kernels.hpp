@@ -34,12 +34,20 @@ struct GemmDesc {
 // (products of few columns).  GEMM_TILE_GEMV: products of at most
 // GEMM_GEMV_MAX_N columns on the vector ALUs, one lane per output row and K
 // slice (other products take the 64 x 16 tiles under this choice).
+// GEMM_TILE_128x64 / 64x128: complex64 products only (complex128 ones take
+// the 64 x 64 tiles under these choices).
 enum { GEMM_TILE_64x64 = 0, GEMM_TILE_32x64 = 1, GEMM_TILE_16x64 = 2, GEMM_TILE_64x16 = 3, GEMM_TILE_32x32 = 4,
-       GEMM_TILE_64x32 = 5, GEMM_TILE_128x16 = 6, GEMM_TILE_SKINNY = 7, GEMM_TILE_GEMV = 8, GEMM_TILES = 9 };
+       GEMM_TILE_64x32 = 5, GEMM_TILE_128x16 = 6, GEMM_TILE_SKINNY = 7, GEMM_TILE_GEMV = 8, GEMM_TILE_128x64 = 9,
+       GEMM_TILE_64x128 = 10, GEMM_TILES = 11 };
 constexpr int GEMM_GEMV_MAX_N = 4;
-// whether a tile choice is a kernel of its own for an N-column product (the
-// tuners skip the ones that would repeat another)
-inline bool gemm_tile_distinct(int tile, int N) { return tile != GEMM_TILE_GEMV || N <= GEMM_GEMV_MAX_N; }
+// whether a tile choice is a kernel of its own for an N-column product with
+// products in FP64 (fp64) or FP32 (the tuners skip the ones that would repeat
+// another)
+inline bool gemm_tile_distinct(int tile, int N, bool fp64) {
+    if (tile == GEMM_TILE_GEMV) return N <= GEMM_GEMV_MAX_N;
+    if (tile == GEMM_TILE_128x64 || tile == GEMM_TILE_64x128) return !fp64 && N > 16;
+    return true;
+}
 int gemm_tile_rule(int M, int N);
 // ks_target: the K split aims at this many (row tile x slice) workgroups per
 // launch; it never depends on N.
@@ -50,7 +58,7 @@ void launch_gemm(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStr
 // products and FP32 accumulation (the merge of the mixed-precision
 // factorization).  ws as for launch_gemm<float>.
 void launch_gemm_c64(const GemmDesc* d_descs, int nprob, int M, int N, int K, hipStream_t st, void* ws = nullptr,
-                     int tile = -1);
+                     int tile = -1, int ks_target = 256);
 // Several groups of problems of different shapes in one launch (problems of
 // several trees at one level): group q's problems are products of its own M, N
 // and K with its own K split (gemm_kslices of its own nprob, M, K and
@@ -82,6 +90,14 @@ size_t gemm_ws_elems(int nprob, int M, int N, int K, int ks_target = 256);
 // give.  The factor's products keep the default.
 inline int solve_ks_target(int nprob, int M, int K) {
     return (K >= 4096 || (K >= 2048 && (long long)nprob * M >= 4096)) ? 1024 : 256;
+}
+// K-split target of the factorization's merge products: the column tiles
+// count toward filling the chip too (a merge product has one fixed N, so
+// nothing asks for the same bits at every N; wide products then need no
+// split, no partial sums and no reduction).
+inline int merge_ks_target(int N) {
+    const int tn = (N + 63) / 64;
+    return tn >= 128 ? 2 : 256 / tn;
 }
 
 // leaf stage

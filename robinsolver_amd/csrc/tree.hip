@@ -381,10 +381,11 @@ static MergeLayout merge_layout(const Level& V, size_t lds_limit) {
     m.W = take(nodes * ii * 16);
     m.Y = take(nodes * 2 * i * 2 * bn * 16);
     size_t wse = 0;
-    wse = std::max(wse, gemm_ws_elems((int)nodes, (int)i, (int)i, (int)i));
-    wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, (int)i, (int)i, (int)i));
-    wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, 2 * (int)i, (int)bn, (int)i));
-    wse = std::max(wse, gemm_ws_elems(4 * (int)nodes, (int)bn, (int)bn, (int)i));
+    const int ni = (int)i, nbn = (int)bn;
+    wse = std::max(wse, gemm_ws_elems((int)nodes, ni, ni, ni, merge_ks_target(ni)));
+    wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, ni, ni, ni, merge_ks_target(ni)));
+    wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, 2 * ni, nbn, ni, merge_ks_target(nbn)));
+    wse = std::max(wse, gemm_ws_elems(4 * (int)nodes, nbn, nbn, ni, merge_ks_target(nbn)));
     m.ws = take(wse * 16);
     m.ip.build((int)i, (int)nodes, ii, lds_limit);
     m.GW = take(m.ip.w_elems() * 16);
@@ -659,16 +660,16 @@ int Tree::factor_device(const double2* k) {
     merge_c64 = c64_from < L;
     auto gemm = [&](int lvl, const GemmDesc* d, int np, int M, int N, int K, void* gws, int t) {
         if (lvl >= c64_from)
-            launch_gemm_c64(d, np, M, N, K, stream, gws, t);
+            launch_gemm_c64(d, np, M, N, K, stream, gws, t, merge_ks_target(N));
         else
-            launch_gemm<double>(d, np, M, N, K, stream, gws, t);
+            launch_gemm<double>(d, np, M, N, K, stream, gws, t, merge_ks_target(N));
     };
     auto fgemm = [&](int site, const GemmDesc* d, int np, int M, int N, int K, void* gws) -> int {
         int tile = site < (int)ftile.size() ? ftile[site] : -1;
         if (tune) {
             float best = 0.f;
             for (int t = 0; t < GEMM_TILES; ++t) {
-                if (!gemm_tile_distinct(t, N)) continue;
+                if (!gemm_tile_distinct(t, N, site / 5 < c64_from)) continue;
                 (void)hipEventRecord(te0, stream);
                 gemm(site / 5, d, np, M, N, K, gws, t);
                 (void)hipEventRecord(te1, stream);
@@ -1004,7 +1005,7 @@ template <typename R> int Tree::tune_work(SolveWork<R>& w) {
     std::vector<float> best(ns, 0.f);
     std::vector<int> bt(ns, -1);
     for (int t = 0; t < GEMM_TILES && !rc; ++t) {
-        if (!gemm_tile_distinct(t, N)) continue;
+        if (!gemm_tile_distinct(t, N, std::is_same<R, double>::value)) continue;
         for (int rep = 0; rep < 2; ++rep) {
             for (size_t j = 0; j < ns; ++j) {
                 const Site& s = seq[j];
