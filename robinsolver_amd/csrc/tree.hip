@@ -371,7 +371,9 @@ struct MergeLayout {
     InvPlan ip;
 };
 
-static MergeLayout merge_layout(const Level& V, size_t lds_limit) {
+// with_rtr: the level builds its parents' maps (every level but the root, and
+// the root too unless its map is skipped); Y exists only for them.
+static MergeLayout merge_layout(const Level& V, size_t lds_limit, bool with_rtr) {
     MergeLayout m;
     const size_t nodes = V.nodes, i = V.isz, bn = V.bn, ii = i * i;
     size_t off = 0;
@@ -379,13 +381,15 @@ static MergeLayout merge_layout(const Level& V, size_t lds_limit) {
     m.A = take(nodes * ii * 16);
     m.C = take(nodes * ii * 16);
     m.W = take(nodes * ii * 16);
-    m.Y = take(nodes * 2 * i * 2 * bn * 16);
+    m.Y = take(with_rtr ? nodes * 2 * i * 2 * bn * 16 : 0);
     size_t wse = 0;
     const int ni = (int)i, nbn = (int)bn;
     wse = std::max(wse, gemm_ws_elems((int)nodes, ni, ni, ni, merge_ks_target(ni)));
     wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, ni, ni, ni, merge_ks_target(ni)));
-    wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, 2 * ni, nbn, ni, merge_ks_target(nbn)));
-    wse = std::max(wse, gemm_ws_elems(4 * (int)nodes, nbn, nbn, ni, merge_ks_target(nbn)));
+    if (with_rtr) {
+        wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, 2 * ni, nbn, ni, merge_ks_target(nbn)));
+        wse = std::max(wse, gemm_ws_elems(4 * (int)nodes, nbn, nbn, ni, merge_ks_target(nbn)));
+    }
     m.ws = take(wse * 16);
     m.ip.build((int)i, (int)nodes, ii, lds_limit);
     m.GW = take(m.ip.w_elems() * 16);
@@ -424,10 +428,10 @@ static FactorSizes factor_sizes(const Plan& P, const LeafGeom& G, const rs_opts&
         else z.invm_t = std::max(z.invm_t, im);
         if (mixed) z.invm32[l] = im / 2;
         if (l > 0 || !o.skip_root_rtr) z.Tfull[l] = nodes * (size_t)V.P * V.P * 16;
-        const MergeLayout m = merge_layout(V, lds);
+        const MergeLayout m = merge_layout(V, lds, l > 0 || !o.skip_root_rtr);
         z.arena = std::max(z.arena, m.total);
         if (!m.ip.small) z.fdesc += (long long)m.ip.panels() * nodes * gd + (long long)m.ip.panels() * (i - m.ip.nb) * 4;
-        z.fdesc += 4 * nodes * gd + 2 * nodes * gd + ((l > 0 || !o.skip_root_rtr) ? 4 * nodes * gd : 0);
+        z.fdesc += 4 * nodes * gd + ((l > 0 || !o.skip_root_rtr) ? 6 * nodes * gd : 0);
     }
     if (!z.arena) z.arena = 256;
     for (int l = 0; l <= L; ++l) {
@@ -455,7 +459,7 @@ int Tree::factor_device(const double2* k) {
     const bool keep_maps = (opts.retain & RS_RETAIN_MAPS) != 0;
     const bool keep_scratch = (opts.retain & RS_RETAIN_SCRATCH) != 0;
     for (int l = L - 1; l >= 0; --l)
-        if (!merge_layout(P.lv[l], lds_limit).ip.fits())
+        if (!merge_layout(P.lv[l], lds_limit, true).ip.fits())
             return fail("Error: interface of " + std::to_string(P.lv[l].isz) +
                         " points is larger than the interface inverses support (LDS panel).");
     const FactorSizes z = factor_sizes(P, G, opts, lds_limit);
@@ -499,7 +503,7 @@ int Tree::factor_device(const double2* k) {
         if (need(tmap[q], keep_maps ? 0 : z.tmap[q])) return 1;
     if (solve_ptrs() != before) clear_slots();
     std::vector<MergeLayout> ml(L);
-    for (int l = 0; l < L; ++l) ml[l] = merge_layout(P.lv[l], lds_limit);
+    for (int l = 0; l < L; ++l) ml[l] = merge_layout(P.lv[l], lds_limit, l > 0 || !opts.skip_root_rtr);
     char* ar = fscratch.as<char>();
     // full node maps of level l, and the FP64 inv(M) of level l
     auto Tfull = [&](int l) -> double2* { return keep_maps ? T[l]->as<double2>() : tmap[l & 1].as<double2>(); };
@@ -593,7 +597,7 @@ int Tree::factor_device(const double2* k) {
                 v.push_back(g);
             }
             o.y = v.size();
-            for (int n = 0; n < nodes; ++n)  // Y = invM diag(T1_IN, T2_IN)
+            for (int n = 0; n < nodes && (l > 0 || !opts.skip_root_rtr); ++n)  // Y = invM diag(T1_IN, T2_IN)
                 for (int c = 0; c < 2; ++c) {
                     GemmDesc g = desc0();
                     g.A = IM + n * 4 * ii + (size_t)c * i * 2 * i; g.lda = 2 * i;
@@ -711,8 +715,9 @@ int Tree::factor_device(const double2* k) {
         launch_add_diag(IM + (size_t)i * 2 * i, 4 * ii, 2 * i, i, nodes, rnu, stream);
         launch_copy_blocks(IM + i, 4 * ii, 2 * i, Wm, ii, i, i, i, nodes, nu, stream);
         // Y = invM diag(T1_IN, T2_IN); parent T = diag(T_NN) + diag(T_NI) Y
-        if (fgemm(5 * l + 3, fd + o.y, 2 * nodes, 2 * i, bn, i, gws)) return 1;
-        if ((l > 0 || !opts.skip_root_rtr) && fgemm(5 * l + 4, fd + o.t, 4 * nodes, bn, bn, i, gws))
+        // (neither at a root whose map is skipped: nothing reads them)
+        if ((l > 0 || !opts.skip_root_rtr) && (fgemm(5 * l + 3, fd + o.y, 2 * nodes, 2 * i, bn, i, gws) ||
+                                               fgemm(5 * l + 4, fd + o.t, 4 * nodes, bn, bn, i, gws)))
             return 1;
         // the solve data of this merge: inv(M), and the children's T_NI / T_IN,
         // before the next level reuses the buffers they come from
