@@ -171,6 +171,25 @@ int rs_solve_device_noalloc(rs_tree *t, int nrhs, const void *f_dev, const void 
  * (rs_opts.n_refine; no reference counterpart). */
 int rs_set_refine(rs_tree *t, int n_refine);
 
+/* Factored-tree persistence (no reference counterpart: the reference keeps no
+ * factorization beyond its process -- write_variable is a stub,
+ * robin_output_module.f90:157-221, robin_tree_memsize is empty,
+ * robin_tree_module.f90:243-250 -- so every run re-factors).  rs_tree_save writes
+ * a factored tree's plan parameters (the arguments of rs_plan) and its retained
+ * solve data (the potential, the leaf block inverses, inv(M) of every merge
+ * level and the children's T_NI / T_IN, in each precision the tree holds) to a
+ * file.  rs_tree_load reads such a file into a handle: it plans the handle from
+ * the file (dropping what the handle held), takes the saved tree's precision
+ * (rs_opts.precision; the other options stay the handle's), uploads the data and
+ * leaves the tree factored, so that its solves give the saved tree's bits; its
+ * f_time is the load's wall time.  A file that is not a saved tree or of another
+ * format version is refused before the handle changes; one whose sections do
+ * not match its plan, or that ends early, is refused with a message and leaves
+ * the handle planned but not factored.  Full node maps (RS_RETAIN_MAPS) and the
+ * factorization's working memory are not saved. */
+int rs_tree_save(rs_tree *t, const char *path);
+int rs_tree_load(rs_tree *t, const char *path);
+
 /* T%f_time and T%s_time (robin_tree_module.f90:63-64, time_module.f90:26-36):
  * wall seconds of the last factor and solve. */
 int rs_timings(const rs_tree *t, double *f_time, double *s_time);

@@ -15,6 +15,7 @@ here falls back to a CPU path: without the library or a GPU, calls raise.
 """
 import ctypes
 import os
+import struct
 import time
 
 import numpy as np
@@ -79,6 +80,8 @@ def lib():
         L.rs_leaf_grid.argtypes = [vp, ip, dp]
         L.rs_points_device.argtypes = [vp, vp, vp, vp]
         L.rs_factor_device.argtypes = [vp, vp]
+        L.rs_tree_save.argtypes = [vp, ctypes.c_char_p]
+        L.rs_tree_load.argtypes = [vp, ctypes.c_char_p]
         from . import serve as _serve
         _serve.bind(L)
         _lib = L
@@ -91,6 +94,26 @@ def _dp(a):
 
 def _c128(a, shape):
     return np.ascontiguousarray(np.asarray(a, dtype=np.complex128).reshape(shape))
+
+
+_SAVED_HEADER = struct.Struct("<8sI11i16d")   # rs_tree_save's header (persist.cpp)
+
+
+def read_saved_header(path):
+    """The plan parameters and flags of a file written by rs_tree_save."""
+    with open(path, "rb") as fh:
+        raw = fh.read(_SAVED_HEADER.size)
+    if len(raw) < _SAVED_HEADER.size:
+        raise RobinError("Error: %s is not a saved tree." % path)
+    v = _SAVED_HEADER.unpack(raw)
+    if v[0] != b"RSTREE\x00\x01":
+        raise RobinError("Error: %s is not a saved tree." % path)
+    d, L, prec, h64, h32, mc64, nl, m, ni, nb = v[2:12]
+    dbl = v[13:]
+    return {"version": v[1], "d": d, "L": L, "precision": prec, "have64": h64, "have32": h32,
+            "merge_c64": mc64, "n_leaves": nl, "m": m, "n_int": ni, "nb": nb, "domain": list(dbl[0:6]),
+            "h_tgt": dbl[6], "kb": dbl[7],
+            "bndry_params": [complex(dbl[8 + 2 * q], dbl[9 + 2 * q]) for q in range(4)]}
 
 
 class GpuTree:
@@ -196,6 +219,23 @@ class GpuTree:
         f, s = ctypes.c_double(), ctypes.c_double()
         lib().rs_timings(self.h, ctypes.byref(f), ctypes.byref(s))
         return f.value, s.value
+
+    def save(self, path):
+        """Write the factored tree to a file (rs_tree_save)."""
+        self._check(lib().rs_tree_save(self.h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path, **opts):
+        """A tree read from a file written by save(): planned from the file's
+        parameters and factored, with the saved precision (rs_tree_load).  The
+        keyword options are the constructor's other than the geometry and the
+        precision."""
+        hdr = read_saved_header(path)
+        bp = np.array(hdr["bndry_params"]).reshape(2, 2, order="F")
+        t = cls(hdr["d"], hdr["L"], np.array(hdr["domain"]).reshape(3, 2), hdr["h_tgt"], hdr["kb"],
+                bndry_params=bp, precision=hdr["precision"], **opts)
+        t._check(lib().rs_tree_load(t.h, os.fsencode(path)))
+        return t
 
     def residual_norms(self, nrhs, f_ptr, g_ptr, u_ptr):
         """||r|| / ||(f, g)|| of the global FD system for each of nrhs device-resident solutions."""

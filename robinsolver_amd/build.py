@@ -12,7 +12,7 @@ OUT = os.path.join(HERE, "librobinsolver.so")
 OBJ = os.path.join(HERE, "_build")
 
 SOURCES = ["k_leaf.hip", "k_gemm.hip", "k_inv.hip", "k_misc.hip", "tree.hip", "plan.cpp", "capi.cpp",
-           "server.cpp"]
+           "server.cpp", "persist.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -160,6 +160,16 @@ int rs_set_refine(rs_tree* t, int n_refine) {
     return 0;
 }
 
+int rs_tree_save(rs_tree* t, const char* path) {
+    if (!t) { g_err = "Error: null argument."; return 1; }
+    return t->t.save(path);
+}
+
+int rs_tree_load(rs_tree* t, const char* path) {
+    if (!t) { g_err = "Error: null argument."; return 1; }
+    return t->t.load(path);
+}
+
 int rs_timings(const rs_tree* t, double* f_time, double* s_time) {
     if (!t) return 1;
     if (f_time) *f_time = t->t.f_time;

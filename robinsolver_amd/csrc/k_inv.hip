@@ -3,12 +3,14 @@
 //
 // Reference: D = (zeta I - T2_II)^-1 by zgetrf + zgetri and the LU of
 // S^-1 = zeta I - T1_II - nu^2 D by zgetrf (robin_tree_module.f90:440-467;
-// factorization_module.f90:49-141).  Here both D and S are formed explicitly
-// (Gauss-Jordan, the pivot being the first entry of maximal |re|+|im| in the
-// column, as zgetrf chooses it; the register panels compare the candidates on
-// the top 52 bits of that value, so they can choose differently only between
-// candidates equal to 12 digits), so that the merge and the solve passes are
-// plain matrix products on the matrix cores.
+// factorization_module.f90:49-141).  The merge here inverts one matrix per
+// node, W = nu^2 I - (zeta I - T1_II)(zeta I - T2_II), from which it forms
+// inv(M) by products; the inverse is explicit (Gauss-Jordan, the pivot being
+// the first entry of maximal |re|+|im| in the column, as zgetrf chooses it;
+// the register panels compare the candidates on the top 52 bits of that
+// value, so they can choose differently only between candidates equal to 12
+// digits), so that the merge and the solve passes are plain matrix products
+// on the matrix cores.
 //
 // Interfaces below 64 points use one workgroup per matrix with the whole matrix
 // in LDS.  Larger ones use a blocked right-looking Gauss-Jordan: an n x nb

@@ -246,6 +246,9 @@ struct Tree {
     int factor(const double* k_int_host);
     int factor_device(const double2* k_int_dev);
     int make_c64();
+    // a factored tree to a file and back (persist.cpp)
+    int save(const char* path);
+    int load(const char* path);
     template <typename R> int ensure_work(SolveWork<R>& w, int nrhs, bool refine = false);
     // the arena layout of a workspace of cap columns (esz 8: complex64 passes,
     // 16: complex128), with refinement buffers and/or column tables
