@@ -80,7 +80,8 @@ enum {
 };
 
 /* Defaults: precision RS_FP64, skip_root_rtr 1, n_refine 2, verbose 0, no cap,
- * autotune 1, graphs 1, merge_fp64_levels -1, retain 0. */
+ * autotune 1, graphs 1, merge_fp64_levels 0 (an RS_MIXED tree runs every merge
+ * product in complex64), retain 0. */
 void rs_opts_default(rs_opts *o);
 
 /* Create / destroy a tree handle (type robin_tree, robin_tree_module.f90:55-77;

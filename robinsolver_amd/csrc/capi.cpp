@@ -52,7 +52,7 @@ void rs_opts_default(rs_opts* o) {
     o->hbm_budget = 0;
     o->autotune = 1;
     o->graphs = 1;
-    o->merge_fp64_levels = -1;
+    o->merge_fp64_levels = 0;
 }
 
 int rs_tree_create(rs_tree** t, const rs_opts* o) {

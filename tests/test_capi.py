@@ -73,5 +73,5 @@ def test_opts_defaults_and_layout():
     o = rsa.RsOpts()
     rsa.lib().rs_opts_default(ctypes.byref(o))
     assert (o.precision, o.skip_root_rtr, o.n_refine, o.verbose, o.hbm_budget, o.autotune, o.graphs,
-            o.merge_fp64_levels) == (0, 1, 2, 0, 0, 1, 1, -1)
+            o.merge_fp64_levels) == (0, 1, 2, 0, 0, 1, 1, 0)
     assert ctypes.sizeof(rsa.RsOpts) == 40

@@ -211,7 +211,7 @@ def test_gemm_tile_shapes_give_identical_bits(prec, M, N, K, scatter, gpu):
     C0 = _rc(rng, ldc, N)
     rmap = np.ascontiguousarray(rng.permutation(ldc)[:M].astype(np.int32)) if scatter else None
     outs = []
-    for tile in (-1, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10):
+    for tile in range(-1, 11):
         C = np.asfortranarray(C0.copy())
         a = np.asfortranarray(A)
         b = np.asfortranarray(B)

@@ -99,10 +99,11 @@ def test_complex64_merges_and_precision_rules(fp64_levels, gpu):
     """merge_fp64_levels = k >= 0: the merges below the top k levels run in
     complex64, so the RtR maps differ from an FP64 tree's by complex64 rounding
     (not more); such a tree refuses FP64 solves and its refined mixed solves
-    reach the mixed-path bound.  The default mixed tree factors in FP64: its
-    RtR maps are bitwise an FP64 tree's; it keeps complex64 solve data only and
-    refuses FP64 solves, unless it keeps the FP64 data too (RS_RETAIN_FP64),
-    and then its FP64 solves are bitwise an FP64 tree's."""
+    reach the mixed-path bound.  A mixed tree that factors in FP64
+    (merge_fp64_levels -1): its RtR maps are bitwise an FP64 tree's; it keeps
+    complex64 solve data only and refuses FP64 solves, unless it keeps the FP64
+    data too (RS_RETAIN_FP64), and then its FP64 solves are bitwise an FP64
+    tree's."""
     print("case: complex64 merges, fp64 levels", fp64_levels, flush=True)
     kb, n, L = 12.0, 64, 4
     o = OracleTree(2, L, UNIT, 1.0 / n, -kb)
@@ -110,8 +111,8 @@ def test_complex64_merges_and_precision_rules(fp64_levels, gpu):
     tm = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=2, merge_fp64_levels=fp64_levels,
                      retain=maps)
     tdef = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=1,
-                       retain=maps | rsa.RS_RETAIN_FP64)
-    tplain = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=1)
+                       retain=maps | rsa.RS_RETAIN_FP64, merge_fp64_levels=-1)
+    tplain = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_MIXED, n_refine=1, merge_fp64_levels=-1)
     td = rsa.GpuTree(2, L, UNIT, 1.0 / n, -kb, precision=rsa.RS_FP64, retain=maps)
     pts, _ = o.all_points()
     k = _k(pts[:, o.nb:], kb)

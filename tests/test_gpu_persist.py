@@ -41,7 +41,9 @@ def _rhs(t, nrhs, seed):
 ])
 def test_loaded_tree_solves_with_the_saved_bits(d, n, L, kb, prec, retain, tmp_path, gpu):
     _case("persist %dD %d^%d L=%d prec=%d retain=%d" % (d, n, d, L, prec, retain))
-    t = rsa.GpuTree(d, L, UNIT, 1.0 / n, kb, precision=prec, retain=retain)
+    # (FP64 solves on a mixed tree need its factorization in FP64: merge_fp64_levels -1)
+    lv = -1 if retain & rsa.RS_RETAIN_FP64 else 0
+    t = rsa.GpuTree(d, L, UNIT, 1.0 / n, kb, precision=prec, retain=retain, merge_fp64_levels=lv)
     pts, _ = t.all_points()
     t.factor(_k(pts[:, t.nb:], kb))
     f, g = _rhs(t, 3, 7)
@@ -53,14 +55,14 @@ def test_loaded_tree_solves_with_the_saved_bits(d, n, L, kb, prec, retain, tmp_p
     assert (hdr["d"], hdr["L"], hdr["precision"]) == (d, L, prec)
     assert hdr["have32"] == (prec == rsa.RS_MIXED)
     assert hdr["have64"] == (prec == rsa.RS_FP64 or bool(retain & rsa.RS_RETAIN_FP64))
-    t2 = rsa.GpuTree.load(path)
+    t2 = rsa.GpuTree.load(path, merge_fp64_levels=lv)
     assert (t2.n_leaves, t2.m, t2.n_int, t2.nb) == (t.n_leaves, t.m, t.n_int, t.nb)
     for p in precs:
         assert np.array_equal(t2.solve(f, g, precision=p), ref[p]), p
     # a refactor of the loaded tree (new potential) matches a fresh factorization
     k2 = _k(pts[:, t.nb:], kb, shift=0.5)
     t2.factor(k2)
-    t3 = rsa.GpuTree(d, L, UNIT, 1.0 / n, kb, precision=prec, retain=retain)
+    t3 = rsa.GpuTree(d, L, UNIT, 1.0 / n, kb, precision=prec, retain=retain, merge_fp64_levels=lv)
     t3.factor(k2)
     assert np.array_equal(t2.solve(f, g), t3.solve(f, g))
 
