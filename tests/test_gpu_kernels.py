@@ -198,7 +198,8 @@ def test_inverse_with_tied_pivots(n, gpu):
 @pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED, rsa.RS_MIXED_MERGE])
 @pytest.mark.parametrize("M,N,K,scatter", [(48, 256, 16, True), (100, 70, 777, False), (200, 33, 1000, False),
                                            (16, 64, 32, True), (2048, 1, 2048, False), (130, 5, 77, True),
-                                           (64, 16, 4096, True)])
+                                           (64, 16, 4096, True), (64, 1, 4096, True), (37, 3, 1001, False),
+                                           (300, 4, 50, True)])
 def test_gemm_tile_shapes_give_identical_bits(prec, M, N, K, scatter, gpu):
     """The autotuner may pick any tile shape: every shape must give the same bits."""
     print("case: gemm tiles", prec, M, N, K, scatter, flush=True)

@@ -114,6 +114,9 @@ def test_batch_composition_is_bitwise_invariant(gpu):
     for r in (0, 17, 36):
         u1 = t.solve(f[r], g[r])
         assert np.array_equal(u1, u_all[r]), r
+    # three columns: the few-column leaf and product kernels (one wave per leaf,
+    # matrix-core-free products) against the wide ones
+    assert np.array_equal(t.solve(f[5:8], g[5:8]), u_all[5:8])
 
 
 def test_repeated_solves_and_refactor(gpu):
