@@ -885,9 +885,14 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
         for (int k = 0; k < K; ++k) Bl[k * 256 + tid] = ok ? t[k] : make_float2(0.f, 0.f);
     }
     // boundary data of the wave's 16 columns, read once with coalesced loads:
-    // DOWN: incoming g (node order); UP: the transmission residual (local order)
+    // DOWN: incoming g (node order); UP: the transmission residual (local order);
+    // FACTOR_T: column c is the unit vector of node-order boundary row c
     {
-        if (MODE == LEAF_DOWN || io.jump) {
+        if (MODE == LEAF_FACTOR_T) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (bok) bd[c * BS + bi] = (bi == wcol0 + c) ? make_float2(1.f, 0.f) : make_float2(0.f, 0.f);
+        } else if (MODE == LEAF_DOWN || io.jump) {
             // stored in node order (g already is; the residual is in local order)
             float2 t[16];
             const size_t gs = (MODE == LEAF_DOWN) ? (size_t)io.gsrc[(size_t)leaf * nb + bi] : 0;
@@ -947,7 +952,7 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     // raw[i] = f at row rr of layer k, column cc(i) (coalesced mapping)
     auto fetch = [&](int k, FT (&raw)[4]) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) raw[i] = gld(fcp[i] + (k * s + rrc));
+        for (int i = 0; i < 4; ++i) raw[i] = (MODE == LEAF_FACTOR_T) ? FT{} : gld(fcp[i] + (k * s + rrc));
     };
     // the interior data runs PF layers ahead of the sweep (memory latency is
     // several layer steps long)
@@ -972,13 +977,14 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
         // the x face of the lane's group at row 0 / s-1 of every layer (one LDS
         // read), the y faces at every row of the first and last layer.  Columns
         // beyond the batch hold zeros here and are not stored anyway.
-        const float2 gx = (MODE == LEAF_DOWN) ? gval(fnx, k) : make_float2(0.f, 0.f);
+        constexpr bool GIN = MODE == LEAF_DOWN || MODE == LEAF_FACTOR_T;
+        const float2 gx = GIN ? gval(fnx, k) : make_float2(0.f, 0.f);
         float2 v[4];
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) {
             const int row = 4 * grp + vv;
             float2 r = scr[row * LT + ccol];
-            if (MODE == LEAF_DOWN) {
+            if (GIN) {
                 if (vv == 0) r = xf1 ? cfma(cbx, gx, r) : r;
                 if (vv == ((s - 1) & 3)) r = xf2 ? cfma(cbx, gx, r) : r;
                 if (k == 0 || k == K - 1) {
@@ -1060,10 +1066,11 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
     const float2 iddx = ccast<float2>(cinv(ax0.dd)), iddy = ccast<float2>(cinv(ax1.dd));
     auto emit = [&](int fn, int t, float2 x1, float2 x2, float a1, float a2, float2 idd) {
         float2* slot = bd + ccol * BS + fn + t;
-        const float2 gb = (MODE == LEAF_DOWN) ? *slot : make_float2(0.f, 0.f);
+        const float2 gb = (MODE == LEAF_UP) ? make_float2(0.f, 0.f) : *slot;
         float2 num = csub(csub(gb, cscale(x1, a1)), cscale(x2, a2));
         float2 uB = cmul(num, idd);
         if (MODE == LEAF_UP) *slot = csub(cfma(sma, uB, gb), *slot);
+        else if (MODE == LEAF_FACTOR_T) *slot = cfma(sma, uB, gb);
         else *slot = uB;
     };
     // one emit per layer serves both x faces (a lane handles at most one)
@@ -1100,6 +1107,9 @@ __global__ void __launch_bounds__(256) leaf_mfma_kernel(LeafGeom G, LeafCoef C, 
             if (!bok || col >= io.ncols) continue;
             if (MODE == LEAF_UP) {
                 io.hout[((size_t)col * nl + leaf) * nb + bi] = bd[c * BS + bi];
+            } else if (MODE == LEAF_FACTOR_T) {
+                // column col of the leaf's map, node-order rows (complex128)
+                io.T[((size_t)leaf * nb + col) * nb + bi] = ccast<double2>(bd[c * BS + bi]);
             } else {
                 const float2 x = bd[c * BS + bnode];
                 put_col(col, (OUT != OUT_U32) ? io.u64[col] + (size_t)leaf * m : nullptr, bi, x, base[j]);
@@ -1413,9 +1423,15 @@ static bool mfma_leaf_ok(const LeafGeom& G, const LeafArgs<float>& a) {
     const bool form = a.mode == LEAF_UP ? (a.f64 || a.f32)
                     : a.mode == LEAF_DOWN ? ((a.f64 && !a.ubase32 && !a.uacc) ||
                                              (a.f32 && a.u64 && (a.ubase32 || a.uacc)))
-                    : false;
+                    : (a.mode == LEAF_FACTOR_T && a.ncols == G.nb);
     return G.d == 2 && form && G.s == G.n[0] && G.s == G.K && G.nb == 4 * G.K && G.n_int == G.K * G.K &&
            (G.K == 12 || G.K == 14 || G.K == 16);
+}
+
+bool leaf_mfma_factor_ok(const LeafGeom& G) {
+    LeafArgs<float> a;
+    a.mode = LEAF_FACTOR_T; a.ncols = G.nb;
+    return mfma_leaf_ok(G, a);
 }
 
 template <typename R>
@@ -1437,7 +1453,9 @@ void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit
 #define RS_LEAF_MFMA(KK)                                                                                        \
     do {                                                                                                         \
         auto L = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, G, C, io, nchunks); };     \
-        if (mode == LEAF_UP) {                                                                                   \
+        if (mode == LEAF_FACTOR_T) {                                                                             \
+            L(leaf_mfma_kernel<KK, LEAF_FACTOR_T, float2, OUT_U32>);                                              \
+        } else if (mode == LEAF_UP) {                                                                            \
             if (io.f32) L(leaf_mfma_kernel<KK, LEAF_UP, float2, OUT_U32>);                                       \
             else L(leaf_mfma_kernel<KK, LEAF_UP, double2, OUT_U32>);                                             \
         } else if (io.f32) {                                                                                     \

@@ -127,6 +127,9 @@ template <typename R> struct LeafArgs {
 };
 template <typename R>
 void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit, hipStream_t st);
+// whether the leaf maps (LEAF_FACTOR_T, complex64) of this geometry run on the
+// matrix-core leaf kernel (square 2D leaves of 12, 14 or 16 points per axis)
+bool leaf_mfma_factor_ok(const LeafGeom& G);
 // u is complex128 (u64, column table) or complex64 (u32); f and g are column
 // tables; rf, rg and rj are written in complex64.
 void launch_residual(const LeafGeom& G, double2* const* u64, const float2* u32, const double2* kint,

@@ -636,13 +636,23 @@ int Tree::factor_device(const double2* k) {
     HIPCHK(hipMemcpyAsync(kint.p, k, (size_t)nl * P.n_int * sizeof(double2), hipMemcpyDeviceToDevice, stream));
     HIPCHK(hipMemsetAsync(info.p, 0, sizeof(int), stream));
     launch_leaf_blocks(G, kint.as<double2>(), sinv.as<double2>(), info.as<int>(), nl, stream);
-    {
+    if (mixed) launch_to_c64(sinv32.as<float2>(), sinv.as<double2>(), z.sinv64 / 16, stream);
+    // The leaf maps feed the bottom merge.  When that merge runs in complex64
+    // (a mixed tree with complex64 merges down to the leaves) the maps come from
+    // the complex64 block inverses on the matrix cores (one workgroup of 64
+    // columns per leaf); otherwise they are FP64.
+    const int c64_from = (mixed && opts.merge_fp64_levels >= 0) ? opts.merge_fp64_levels : L;
+    if (L > 0 && c64_from <= L - 1 && leaf_mfma_factor_ok(G)) {
+        LeafArgs<float> a;
+        a.mode = LEAF_FACTOR_T; a.ncols = P.nb; a.n_leaves = nl;
+        a.sinv = sinv32.as<float2>(); a.T = Tfull(L);
+        launch_leaf_apply<float>(G, a, lds_limit, stream);
+    } else {
         LeafArgs<double> a;
         a.mode = LEAF_FACTOR_T; a.ncols = P.nb; a.n_leaves = nl;
         a.sinv = sinv.as<double2>(); a.T = Tfull(L);
         launch_leaf_apply<double>(G, a, lds_limit, stream);
     }
-    if (mixed) launch_to_c64(sinv32.as<float2>(), sinv.as<double2>(), z.sinv64 / 16, stream);
     HIPCHK(hipGetLastError());
     // merges, bottom-up; every level of one launch sequence on the tree's stream.
     // The merge products are idempotent (each writes a buffer it does not read;
@@ -660,7 +670,6 @@ int Tree::factor_device(const double2* k) {
     // products of the levels below the top k run in complex64 on the matrix
     // cores (FP32 accumulation, complex128 operands and results); the interface
     // inverses around them stay FP64 with pivoting.
-    const int c64_from = (mixed && opts.merge_fp64_levels >= 0) ? opts.merge_fp64_levels : L;
     merge_c64 = c64_from < L;
     auto gemm = [&](int lvl, const GemmDesc* d, int np, int M, int N, int K, void* gws, int t) {
         if (lvl >= c64_from)
