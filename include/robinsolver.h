@@ -256,11 +256,18 @@ int rs_check_gemm(int precision, int M, int N, int K, const double *A, int lda, 
                   const double *alpha, const double *beta);
 int rs_check_inverse(int n, int batch, double *mats);
 /* rs_check_gemm with the output tile shape forced (0: 64x64, 1: 32x64, 2: 16x64,
- * 3: 64x16, 4: 32x32, 5: 64x32, 6: 128x16, -1: by shape); a test hook for the
+ * 3: 64x16, 4: 32x32, 5: 64x32, 6: 128x16, 7: skinny 64x16, 8: matrix-core-free
+ * GEMV, 9: 128x64, 10: 64x128, 11..15: 64x64, 128x64, 64x128, 64x32, 128x128
+ * with the operands copied straight into LDS, -1: by shape); a test hook for the
  * autotuner's premise that the tile never changes the results. */
 int rs_check_gemm_tile(int precision, int M, int N, int K, const double *A, int lda, const double *B,
                        const double *B2, int ldb, int ksplit, double *C, int ldc, const int *rmap,
                        const double *alpha, const double *beta, int tile);
+/* rs_check_gemm_tile with B's row k read at row bmap[k] of B (ldb rows; no B2),
+ * as the down pass reads a node's incoming rows where the level above wrote
+ * them. */
+int rs_check_gemm_bmap(int precision, int M, int N, int K, const double *A, int lda, const double *B, int ldb,
+                       const int *bmap, double *C, int ldc, const double *alpha, const double *beta, int tile);
 
 /* Kernel timing (no reference counterpart; the reference's only timer is
  * time_module).  While enabled, every solve-pass launch is bracketed by HIP

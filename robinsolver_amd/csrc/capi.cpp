@@ -259,16 +259,17 @@ std::vector<V> to_prec(const double* p, size_t n) {
 // S: precision of the operands and result, R: of the products
 template <typename S, typename R>
 int check_gemm_t(int tile, int M, int N, int K, const double* A, int lda, const double* B, const double* B2, int ldb,
-                 int ksplit, double* C, int ldc, const int* rmap, const double* alpha, const double* beta) {
+                 int ksplit, double* C, int ldc, const int* rmap, const double* alpha, const double* beta,
+                 const int* bmap = nullptr) {
     using V = rs::c_t<S>;
     size_t na = (size_t)lda * K, nb = (size_t)ldb * N, nc = (size_t)ldc * N;
     std::vector<V> hA = to_prec<V>(A, na), hB = to_prec<V>(B, nb), hC = to_prec<V>(C, nc);
     std::vector<V> hB2 = B2 ? to_prec<V>(B2, nb) : std::vector<V>();
     V *dA = nullptr, *dB = nullptr, *dB2 = nullptr, *dC = nullptr, *dD = nullptr;
-    int* dmap = nullptr;
+    int *dmap = nullptr, *dbmap = nullptr;
     rs::GemmDesc* dd = nullptr;
     auto cleanup = [&]() {
-        for (void* p : {(void*)dA, (void*)dB, (void*)dB2, (void*)dC, (void*)dD, (void*)dmap, (void*)dd})
+        for (void* p : {(void*)dA, (void*)dB, (void*)dB2, (void*)dC, (void*)dD, (void*)dmap, (void*)dbmap, (void*)dd})
             if (p) (void)hipFree(p);
     };
     bool ok = hipMalloc(&dA, na * sizeof(V)) == hipSuccess && hipMalloc(&dB, nb * sizeof(V)) == hipSuccess &&
@@ -276,6 +277,7 @@ int check_gemm_t(int tile, int M, int N, int K, const double* A, int lda, const 
               hipMalloc(&dd, sizeof(rs::GemmDesc)) == hipSuccess;
     if (ok && B2) ok = hipMalloc(&dB2, nb * sizeof(V)) == hipSuccess;
     if (ok && rmap) ok = hipMalloc(&dmap, (size_t)M * sizeof(int)) == hipSuccess;
+    if (ok && bmap) ok = hipMalloc(&dbmap, (size_t)K * sizeof(int)) == hipSuccess;
     if (!ok) { cleanup(); g_err = "Error: hipMalloc failed in rs_check_gemm."; return 1; }
     (void)hipMemcpy(dA, hA.data(), na * sizeof(V), hipMemcpyHostToDevice);
     (void)hipMemcpy(dB, hB.data(), nb * sizeof(V), hipMemcpyHostToDevice);
@@ -283,10 +285,12 @@ int check_gemm_t(int tile, int M, int N, int K, const double* A, int lda, const 
     (void)hipMemcpy(dC, hC.data(), nc * sizeof(V), hipMemcpyHostToDevice);
     (void)hipMemcpy(dD, hC.data(), nc * sizeof(V), hipMemcpyHostToDevice);
     if (rmap) (void)hipMemcpy(dmap, rmap, (size_t)M * sizeof(int), hipMemcpyHostToDevice);
+    if (bmap) (void)hipMemcpy(dbmap, bmap, (size_t)K * sizeof(int), hipMemcpyHostToDevice);
     rs::GemmDesc d;
     std::memset(&d, 0, sizeof d);
     d.A = dA; d.lda = lda; d.B = dB; d.B2 = dB2; d.ldb = ldb; d.ksplit = B2 ? ksplit : (1 << 30);
     d.Out = dC; d.ldo = ldc; d.rmap = dmap; d.dmap = rmap ? 1 : 0;
+    d.bmap = dbmap;
     d.Din = dD; d.ldd = ldc;
     d.ar = alpha[0]; d.ai = alpha[1]; d.br = beta[0]; d.bi = beta[1];
     (void)hipMemcpy(dd, &d, sizeof d, hipMemcpyHostToDevice);
@@ -320,6 +324,24 @@ int rs_check_gemm_tile(int precision, int M, int N, int K, const double* A, int 
     if (precision == RS_MIXED_MERGE)
         return check_gemm_t<double, float>(tile, M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta);
     return check_gemm_t<double, double>(tile, M, N, K, A, lda, B, B2, ldb, ksplit, C, ldc, rmap, alpha, beta);
+}
+
+int rs_check_gemm_bmap(int precision, int M, int N, int K, const double* A, int lda, const double* B, int ldb,
+                       const int* bmap, double* C, int ldc, const double* alpha, const double* beta, int tile) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_err = "Error: no HIP device."; return 1; }
+    if (!bmap) { g_err = "Error: no row map."; return 1; }
+    for (int k = 0; k < K; ++k)
+        if (bmap[k] < 0 || bmap[k] >= ldb) { g_err = "Error: row map entry out of range."; return 1; }
+    const int nok = 1 << 30;
+    if (precision == RS_MIXED)
+        return check_gemm_t<float, float>(tile, M, N, K, A, lda, B, nullptr, ldb, nok, C, ldc, nullptr, alpha, beta,
+                                          bmap);
+    if (precision == RS_MIXED_MERGE)
+        return check_gemm_t<double, float>(tile, M, N, K, A, lda, B, nullptr, ldb, nok, C, ldc, nullptr, alpha, beta,
+                                           bmap);
+    return check_gemm_t<double, double>(tile, M, N, K, A, lda, B, nullptr, ldb, nok, C, ldc, nullptr, alpha, beta,
+                                        bmap);
 }
 
 int rs_check_gemm(int precision, int M, int N, int K, const double* A, int lda, const double* B, const double* B2,

@@ -56,6 +56,109 @@ template <> __device__ inline int out_row<double>(int lane, int v) { return (lan
 constexpr int BMX = 64;   // row-tile height assumed by the K-split rule (whatever the tile)
 constexpr int KALIGN = 32; // K-split slices start at multiples of this (whatever the chunk depth)
 
+// Epilogue through LDS: the accumulators (MFMA layout: 16 columns per wave
+// instruction) go to a column-major tile Ct [BN][BM + 1], and each thread then
+// handles consecutive rows of one column, so stores (and the Din reads) are
+// contiguous runs of a column.
+template <typename R, int BM, int TM, int TN>
+__device__ __forceinline__ void gemm_acc_to_lds(c_t<R>* Ct, const typename acc_t<R>::T (&p1)[TM][TN],
+                                       const typename acc_t<R>::T (&p2)[TM][TN],
+                                       const typename acc_t<R>::T (&p3)[TM][TN], int wm, int wn, int lane) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = wm + i * 16 + out_row<R>(lane, v);
+                const int col = wn + j * 16 + (lane & 15);
+                Ct[col * (BM + 1) + row] = mk<R>(p1[i][j][v] - p2[i][j][v], (p3[i][j][v] - p1[i][j][v]) - p2[i][j][v]);
+            }
+}
+
+// The tile from Ct to the K-slice workspace, or through the problem's maps and
+// alpha / beta * Din to its output.
+template <typename S, typename R, int BM, int BN, int NT = 256>
+__device__ __forceinline__ void gemm_tile_out(const c_t<R>* Ct, const GemmDesc& D, c_t<R>* ws, int p, int ks, int kslices,
+                                     int M, int N, int m0, int n0, int tid) {
+    using V = c_t<R>;
+    using VS = c_t<S>;
+    constexpr int NE = BM * BN / NT;
+    if (kslices > 1) {
+        V* W = ws + ((size_t)p * kslices + ks) * (size_t)M * N;
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            const int e = tid + NT * q;
+            const int r = e % BM, c = e / BM;
+            const int row = m0 + r, col = n0 + c;
+            if (row < M && col < N) W[(size_t)row + (size_t)col * M] = Ct[c * (BM + 1) + r];
+        }
+        return;
+    }
+    VS* const Out = (VS*)D.Out;
+    const VS* const Din = (const VS*)D.Din;
+    const int* const rmap = D.rmap;
+    const int* const cmap = D.cmap;
+    const int ldd = D.ldd, ldo = D.ldo;
+    const VS alpha = mk<S>((S)D.ar, (S)D.ai), beta = mk<S>((S)D.br, (S)D.bi);
+    // In groups of up to eight elements: all index-map and Din loads of a group
+    // are issued back to back (branches only around whole loops), then the
+    // results are formed and stored.  (Whole-tile groups would hold too many
+    // registers for the largest tiles.)
+    constexpr int NG = NE < 8 ? NE : 8;
+    static_assert(NE % NG == 0, "epilogue group size");
+    const bool dmap = D.dmap != 0;
+#pragma unroll
+    for (int g0 = 0; g0 < NE; g0 += NG) {
+        int rowc[NG], colc[NG], orow[NG], ocol[NG];
+        bool ok[NG];
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            const int e = tid + NT * (g0 + q);
+            const int row = m0 + e % BM, col = n0 + e / BM;
+            ok[q] = row < M && col < N;
+            rowc[q] = row < M ? row : M - 1;
+            colc[q] = col < N ? col : N - 1;
+        }
+        if (rmap) {
+#pragma unroll
+            for (int q = 0; q < NG; ++q) orow[q] = gldi(rmap + rowc[q]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < NG; ++q) orow[q] = rowc[q];
+        }
+        if (cmap) {
+#pragma unroll
+            for (int q = 0; q < NG; ++q) ocol[q] = gldi(cmap + colc[q]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < NG; ++q) ocol[q] = colc[q];
+        }
+        VS val[NG];
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            const int e = tid + NT * (g0 + q);
+            const V c = Ct[(e / BM) * (BM + 1) + e % BM];
+            val[q] = cmul(alpha, mk<S>((S)c.x, (S)c.y));
+        }
+        if (Din) {
+            VS din[NG];
+            if (dmap) {
+#pragma unroll
+                for (int q = 0; q < NG; ++q) din[q] = gld(Din + ((size_t)orow[q] + (size_t)ocol[q] * ldd));
+            } else {
+#pragma unroll
+                for (int q = 0; q < NG; ++q) din[q] = gld(Din + ((size_t)rowc[q] + (size_t)colc[q] * ldd));
+            }
+#pragma unroll
+            for (int q = 0; q < NG; ++q) val[q] = cfma(beta, din[q], val[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < NG; ++q)
+            if (ok[q]) gst(Out + ((size_t)orow[q] + (size_t)ocol[q] * ldo), val[q]);
+    }
+}
+
 // One workgroup computes a 64 x BN tile over the K range of its slice.  The
 // next 64 x 32 / 32 x BN operand tiles are fetched into registers while the
 // current ones, staged in LDS, feed the matrix cores.  With kslices > 1 the raw
@@ -233,97 +336,249 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmLaunch GL) {
         }
     }
 
-    // Epilogue through LDS: the accumulators (MFMA layout: 16 columns per wave
-    // instruction) go to a column-major tile, and each thread then handles
-    // consecutive rows of one column, so stores (and the Din reads) are
-    // contiguous runs of a column.  The K loop ended with a barrier, so the
-    // operand tiles are free.
-    V* Ct = smem;   // [BN][BM + 1]
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = wm + i * 16 + out_row<R>(lane, v);
-                const int col = wn + j * 16 + (lane & 15);
-                Ct[col * (BM + 1) + row] = mk<R>(p1[i][j][v] - p2[i][j][v], (p3[i][j][v] - p1[i][j][v]) - p2[i][j][v]);
-            }
+    // the K loop ended with a barrier, so the operand tiles are free
+    gemm_acc_to_lds<R, BM, TM, TN>(smem, p1, p2, p3, wm, wn, lane);
     __syncthreads();
-    constexpr int NE = BM * BN / 256;
-    if (kslices > 1) {
-        V* W = ws + ((size_t)p * kslices + ks) * (size_t)M * N;
+    gemm_tile_out<S, R, BM, BN>(smem, D, ws, p, ks, kslices, M, N, m0, n0, tid);
+}
+
+// 16 zero bytes: the source of staged operand elements beyond the K range
+__device__ __align__(16) float4 g_zero16 = {0.f, 0.f, 0.f, 0.f};
+
+// s_waitcnt vmcnt(n) alone (expcnt and lgkmcnt not waited for)
+template <int n> __device__ inline void wait_vm() {
+    static_assert(n >= 0 && n < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Copies from global memory straight into LDS: each lane's 16 (4) bytes at g
+// land at LDS byte address lds + 16 (4) x lane; lds is the same in every lane.
+// Written as asm (with the LDS base in M0), so that the compiler keeps no
+// count of them: the kernel waits for them itself (wait_vm) before the
+// barrier after which they are read.
+__device__ inline void lds_copy16(const void* g, unsigned lds) {
+    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds) : "memory", "m0");
+}
+__device__ inline void lds_copy4(const void* g, unsigned lds) {
+    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off" ::"v"(g), "s"(lds) : "memory", "m0");
+}
+// workgroup barrier after this wave's LDS reads (lgkmcnt(0)), without the
+// release fence's wait for outstanding global loads
+__device__ inline void bare_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4));
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// gemm_kernel with the operands copied from memory straight into LDS
+// (global_load_lds_dwordx4: each lane's 16 bytes land at the wave's LDS base
+// plus 16 x lane, no registers, no conversion) through NS chunk stages, so
+// that the loads of NS - 1 chunks are in flight while one is multiplied; one
+// barrier per chunk.  The products, their order over K (16-deep chunks from
+// the slice start, k ascending in steps of four, Gauss's three real
+// products) and the epilogue are those of gemm_kernel, so the results are the
+// same bits.  S = R only (the operands are staged as stored).
+//
+// Stage layout in 16-byte units (EPU complex elements each): A as [k][BM/EPU]
+// with unit index i/EPU xor swzA(k), B as [n][BK/EPU] (k contiguous, as B is
+// stored) with unit index (k/EPU) xor swzB(n); the MFMA operand reads (16 lanes
+// on 16 consecutive rows or columns at one k) then fall in distinct banks.
+// Each lane loads the unit whose position in the stage is its own, so the
+// swizzle is applied to the global addresses.  Elements beyond the K range
+// come from g_zero16; rows beyond M and columns beyond N from in-range data
+// (their products land in output rows and columns that are not stored).
+// Operands that cannot be moved in 16-byte units (a row-mapped B, odd leading
+// dimensions or offsets) are moved in 4-byte pieces into the same layout.
+template <typename R, int BM, int BN, int NS, int NT = 256, int WAVES_N = 2>
+__global__ void __launch_bounds__(NT) gemm_dma_kernel(GemmLaunch GL) {
+    using V = c_t<R>;
+    using A4 = typename acc_t<R>::T;
+    constexpr int BK = 16;
+    constexpr int EPU = 16 / (int)sizeof(V);
+    constexpr int NW = NT / 64;
+    constexpr int WAVES_M = NW / WAVES_N;
+    constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
+    constexpr int TM = WM / 16, TN = WN / 16;
+    constexpr int AROW = BM / EPU, BCOL = BK / EPU;   // units per A k-row, per B column
+    constexpr int AU = BK * AROW, BU = BN * BCOL;      // units per stage
+    constexpr int NUA = AU / NT, NUB = BU / NT;        // 16-byte loads per thread and chunk
+    static_assert(AU % NT == 0 && BU % NT == 0 && AROW >= 16 && TM >= 1 && TN >= 1, "tile shape");
+    constexpr int STAGE = (AU + BU) * EPU;             // elements per stage
+    constexpr int SMEM = NS * STAGE > BN * (BM + 1) ? NS * STAGE : BN * (BM + 1);
+    __shared__ __align__(16) V smem[SMEM];
+
+    const GemmDesc* descs = GL.g[0].descs;
+    V* ws = (V*)GL.g[0].ws;
+    int M = GL.g[0].M, N = GL.g[0].N, K = GL.g[0].K, tilesM = GL.g[0].tilesM, tilesN = GL.g[0].tilesN,
+        kslices = GL.g[0].ks, kchunk = GL.g[0].kchunk, wg = blockIdx.x;
 #pragma unroll
-        for (int q = 0; q < NE; ++q) {
-            const int e = tid + 256 * q;
-            const int r = e % BM, c = e / BM;
-            const int row = m0 + r, col = n0 + c;
-            if (row < M && col < N) W[(size_t)row + (size_t)col * M] = Ct[c * (BM + 1) + r];
+    for (int q = 1; q < GEMM_MAX_GROUPS; ++q)
+        if (q < GL.n && (int)blockIdx.x >= GL.g[q].wg0) {
+            descs = GL.g[q].descs; ws = (V*)GL.g[q].ws;
+            M = GL.g[q].M; N = GL.g[q].N; K = GL.g[q].K; tilesM = GL.g[q].tilesM; tilesN = GL.g[q].tilesN;
+            kslices = GL.g[q].ks; kchunk = GL.g[q].kchunk; wg = blockIdx.x - GL.g[q].wg0;
         }
-        return;
+    const int tiles = tilesM * tilesN;
+    const int per = tiles * kslices;
+    const int p = wg / per;
+    const int rem = wg - p * per;
+    const int ks = rem / tiles;
+    const int t = rem - ks * tiles;
+    const int tm = t % tilesM, tn = t / tilesM;
+    const GemmDesc& D = descs[p];
+    const V* const A = (const V*)D.A;
+    const V* const B = (const V*)D.B;
+    const V* const B2 = (const V*)D.B2;
+    const int* const bmap = D.bmap;
+    const int lda = D.lda, ldb = D.ldb, ksplit = D.ksplit;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int kbeg = ks * kchunk, kend = min(K, kbeg + kchunk);
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = (wave % WAVES_M) * WM, wn = (wave / WAVES_M) * WN;
+    const unsigned lds0 = (unsigned)(size_t)(__attribute__((address_space(3))) V*)smem;
+
+    A4 p1[TM][TN], p2[TM][TN], p3[TM][TN];
+    for (int i = 0; i < TM; ++i)
+        for (int j = 0; j < TN; ++j)
+            for (int v = 0; v < 4; ++v) { p1[i][j][v] = R(0); p2[i][j][v] = R(0); p3[i][j][v] = R(0); }
+
+    auto swzA = [](int k) { return EPU == 2 ? (k & 1) << 3 : 0; };
+    auto swzB = [](int n) { return EPU == 2 ? (n >> 1) & 7 : n & 15; };
+    // 16-byte units need both operands' elements 16-byte aligned in pairs
+    const bool fast = !bmap && ((((size_t)A | (size_t)B | (size_t)(B2 ? B2 : B)) & 15) == 0) &&
+                      (EPU == 1 || (((lda | ldb | ksplit | M | K | kbeg) & 1) == 0));
+    const char* zero = (const char*)&g_zero16;
+    // 16-byte copies: each thread's source addresses advance by one chunk per
+    // issue (chunks are issued in order); an element beyond the K range comes
+    // from g_zero16
+    const char* pa[NUA];
+    int ka[NUA];
+#pragma unroll
+    for (int q = 0; q < NUA; ++q) {
+        const int u = (q * NW + wave) * 64 + lane;
+        const int k = u / AROW, ip = (u % AROW) ^ swzA(k);
+        const int row = min(m0 + ip * EPU, M - EPU);
+        ka[q] = kbeg + k;
+        pa[q] = (const char*)(A + (size_t)row + (size_t)ka[q] * lda);
     }
-    VS* const Out = (VS*)D.Out;
-    const VS* const Din = (const VS*)D.Din;
-    const int* const rmap = D.rmap;
-    const int* const cmap = D.cmap;
-    const int ldd = D.ldd, ldo = D.ldo;
-    const VS alpha = mk<S>((S)D.ar, (S)D.ai), beta = mk<S>((S)D.br, (S)D.bi);
-    // In groups of up to eight elements: all index-map and Din loads of a group
-    // are issued back to back (branches only around whole loops), then the
-    // results are formed and stored.  (Whole-tile groups would hold too many
-    // registers for the largest tiles.)
-    constexpr int NG = NE < 8 ? NE : 8;
-    static_assert(NE % NG == 0, "epilogue group size");
-    const bool dmap = D.dmap != 0;
+    const size_t astep = (size_t)BK * lda * sizeof(V);
+    const char* pb1[NUB];
+    const char* pb2[NUB];
+    int kb[NUB];
 #pragma unroll
-    for (int g0 = 0; g0 < NE; g0 += NG) {
-        int rowc[NG], colc[NG], orow[NG], ocol[NG];
-        bool ok[NG];
+    for (int q = 0; q < NUB; ++q) {
+        const int u = (q * NW + wave) * 64 + lane;
+        const int n = u / BCOL, kp = (u % BCOL) ^ swzB(n);
+        const size_t col = (size_t)min(n0 + n, N - 1) * ldb;
+        kb[q] = kbeg + kp * EPU;
+        pb1[q] = (const char*)(B + col);
+        pb2[q] = (const char*)((B2 ? B2 : B) + col) - (size_t)ksplit * sizeof(V);
+    }
+    auto issue16 = [&](int k0, int st) {
+        const unsigned sA = lds0 + st * STAGE * (int)sizeof(V);
+        const unsigned sB = sA + AU * 16;
 #pragma unroll
-        for (int q = 0; q < NG; ++q) {
-            const int e = tid + 256 * (g0 + q);
-            const int row = m0 + e % BM, col = n0 + e / BM;
-            ok[q] = row < M && col < N;
-            rowc[q] = row < M ? row : M - 1;
-            colc[q] = col < N ? col : N - 1;
+        for (int q = 0; q < NUA; ++q) {
+            lds_copy16(ka[q] < kend ? pa[q] : zero, sA + (q * NW + wave) * 1024);
+            ka[q] += BK;
+            pa[q] += astep;
         }
-        if (rmap) {
 #pragma unroll
-            for (int q = 0; q < NG; ++q) orow[q] = gldi(rmap + rowc[q]);
-        } else {
-#pragma unroll
-            for (int q = 0; q < NG; ++q) orow[q] = rowc[q];
+        for (int q = 0; q < NUB; ++q) {
+            const int gk = kb[q];
+            const char* b = (gk < ksplit ? pb1[q] : pb2[q]) + (size_t)gk * sizeof(V);
+            lds_copy16(gk < kend ? b : zero, sB + (q * NW + wave) * 1024);
+            kb[q] += BK;
         }
-        if (cmap) {
+    };
+    // the same layout in 4-byte pieces (4 per unit), element by element
+    auto issue4 = [&](int k0, int st) {
+        const unsigned sA = lds0 + st * STAGE * (int)sizeof(V);
+        const unsigned sB = sA + AU * 16;
 #pragma unroll
-            for (int q = 0; q < NG; ++q) ocol[q] = gldi(cmap + colc[q]);
-        } else {
-#pragma unroll
-            for (int q = 0; q < NG; ++q) ocol[q] = colc[q];
+        for (int q = 0; q < 4 * NUA; ++q) {
+            const int d = (q * NW + wave) * 64 + lane;
+            const int u = d >> 2, w = d & 3;
+            const int k = u / AROW, ip = (u % AROW) ^ swzA(k);
+            const int e = w / (4 / EPU), c = w % (4 / EPU);
+            const int row = min(m0 + ip * EPU + e, M - 1), gk = k0 + k;
+            const char* src = gk < kend ? (const char*)(A + (size_t)row + (size_t)gk * lda) + 4 * c : zero;
+            lds_copy4(src, sA + (q * NW + wave) * 256);
         }
-        VS val[NG];
 #pragma unroll
-        for (int q = 0; q < NG; ++q) {
-            const int e = tid + 256 * (g0 + q);
-            const V c = Ct[(e / BM) * (BM + 1) + e % BM];
-            val[q] = cmul(alpha, mk<S>((S)c.x, (S)c.y));
+        for (int q = 0; q < 4 * NUB; ++q) {
+            const int d = (q * NW + wave) * 64 + lane;
+            const int u = d >> 2, w = d & 3;
+            const int n = u / BCOL, kp = (u % BCOL) ^ swzB(n);
+            const int e = w / (4 / EPU), c = w % (4 / EPU);
+            const int col = min(n0 + n, N - 1), gk = k0 + kp * EPU + e;
+            const char* src = zero;
+            if (gk < kend) {
+                const V* b = bmap ? B + gldi(bmap + gk) : (gk < ksplit ? B + (size_t)gk : B2 + (size_t)(gk - ksplit));
+                src = (const char*)(b + (size_t)col * ldb) + 4 * c;
+            }
+            lds_copy4(src, sB + (q * NW + wave) * 256);
         }
-        if (Din) {
-            VS din[NG];
-            if (dmap) {
+    };
+    auto compute = [&](int st) {
+        const V* sA = smem + st * STAGE;
+        const V* sB = sA + AU * EPU;
 #pragma unroll
-                for (int q = 0; q < NG; ++q) din[q] = gld(Din + ((size_t)orow[q] + (size_t)ocol[q] * ldd));
-            } else {
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int k = kk + (lane >> 4);
+            V af[TM], bf[TN];
+            R as[TM], bs[TN];
 #pragma unroll
-                for (int q = 0; q < NG; ++q) din[q] = gld(Din + ((size_t)rowc[q] + (size_t)colc[q] * ldd));
+            for (int i = 0; i < TM; ++i) {
+                const int r = wm + i * 16 + (lane & 15);
+                af[i] = sA[(k * AROW + ((r / EPU) ^ swzA(k))) * EPU + r % EPU];
+                as[i] = af[i].x + af[i].y;
             }
 #pragma unroll
-            for (int q = 0; q < NG; ++q) val[q] = cfma(beta, din[q], val[q]);
-        }
+            for (int j = 0; j < TN; ++j) {
+                const int n = wn + j * 16 + (lane & 15);
+                bf[j] = sB[(n * BCOL + ((k / EPU) ^ swzB(n))) * EPU + k % EPU];
+                bs[j] = bf[j].x + bf[j].y;
+            }
 #pragma unroll
-        for (int q = 0; q < NG; ++q)
-            if (ok[q]) gst(Out + ((size_t)orow[q] + (size_t)ocol[q] * ldo), val[q]);
-    }
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    p1[i][j] = mfma(af[i].x, bf[j].x, p1[i][j]);
+                    p2[i][j] = mfma(af[i].y, bf[j].y, p2[i][j]);
+                    p3[i][j] = mfma(as[i], bs[j], p3[i][j]);
+                }
+        }
+    };
+    const int nch = kbeg < kend ? (kend - kbeg + BK - 1) / BK : 0;
+    // chunk c is multiplied once this thread's loads of it are done (those of
+    // the chunks after it may still be in flight) and every wave has passed
+    // the barrier; the stage refilled after that barrier is the one multiplied
+    // in the previous iteration
+    auto run = [&](auto issue, auto per_chunk) {
+        constexpr int PC = decltype(per_chunk)::value;
+#pragma unroll
+        for (int c = 0; c < NS - 1; ++c)
+            if (c < nch) issue(kbeg + c * BK, c);
+        for (int c = 0; c < nch; ++c) {
+            const int ahead = min(NS - 2, nch - 1 - c);
+            if (ahead >= 2) wait_vm<(2 * PC < 63 ? 2 * PC : 63)>();
+            else if (ahead == 1) wait_vm<PC>();
+            else wait_vm<0>();
+            bare_barrier();
+            if (c + NS - 1 < nch) issue(kbeg + (c + NS - 1) * BK, (c + NS - 1) % NS);
+            compute(c % NS);
+        }
+    };
+    static_assert(NS >= 2 && NS <= 4, "stages");
+    if (fast) run(issue16, std::integral_constant<int, NUA + NUB>());
+    else run(issue4, std::integral_constant<int, 4 * (NUA + NUB)>());
+    wait_vm<0>();
+    __syncthreads();
+    gemm_acc_to_lds<R, BM, TM, TN>(smem, p1, p2, p3, wm, wn, lane);
+    __syncthreads();
+    gemm_tile_out<R, R, BM, BN, NT>(smem, D, ws, p, ks, kslices, M, N, m0, n0, tid);
 }
 
 // Products of at most 16 columns per tile (one-RHS and few-RHS solve passes),
@@ -636,8 +891,13 @@ void launch_gemm_groups(const GemmGroup* grp, int ngroups, hipStream_t st, int t
     if (tile == GEMM_TILE_SKINNY && !std::is_same<S, R>::value) tile = GEMM_TILE_64x16;
     if (tile == GEMM_TILE_GEMV && (!std::is_same<S, R>::value || tile_N > GEMM_GEMV_MAX_N)) tile = GEMM_TILE_64x16;
     if ((tile == GEMM_TILE_128x64 || tile == GEMM_TILE_64x128) && !std::is_same<R, float>::value) tile = GEMM_TILE_64x64;
-    static const int TBM[GEMM_TILES] = {64, 32, 16, 64, 32, 64, 128, 64, 1, 128, 64},
-                     TBN[GEMM_TILES] = {64, 64, 64, 16, 32, 32, 16, 16, GEMM_GEMV_MAX_N, 64, 128};
+    // the direct-to-LDS tiles: products with S = R only; complex128 ones without the
+    // 64 x 128 and 128 x 128 tiles (64 x 64 under these choices)
+    if (tile >= GEMM_TILE_D64x64 && !std::is_same<S, R>::value) tile = GEMM_TILE_64x64;
+    if ((tile == GEMM_TILE_D128x128 || tile == GEMM_TILE_D64x128) && !std::is_same<R, float>::value)
+        tile = GEMM_TILE_D64x64;
+    static const int TBM[GEMM_TILES] = {64, 32, 16, 64, 32, 64, 128, 64, 1, 128, 64, 64, 128, 64, 64, 128},
+                     TBN[GEMM_TILES] = {64, 64, 64, 16, 32, 32, 16, 16, GEMM_GEMV_MAX_N, 64, 128, 64, 64, 128, 32, 128};
     const int bm = TBM[tile], bn = TBN[tile];
     int wg = 0;
     for (int q = 0; q < ngroups; ++q) {
@@ -657,7 +917,7 @@ void launch_gemm_groups(const GemmGroup* grp, int ngroups, hipStream_t st, int t
         else
             wg += g.nprob * e.tilesM * e.tilesN * ks;
     }
-    auto run = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(wg), dim3(256), 0, st, GL); };
+    auto run = [&](auto kern, int nt = 256) { hipLaunchKernelGGL(kern, dim3(wg), dim3(nt), 0, st, GL); };
     switch (tile) {
         case GEMM_TILE_SKINNY:
             if constexpr (std::is_same<S, R>::value) run(gemm_skinny_kernel<R, 8>);
@@ -679,6 +939,25 @@ void launch_gemm_groups(const GemmGroup* grp, int ngroups, hipStream_t st, int t
             break;
         case GEMM_TILE_64x128:
             if constexpr (std::is_same<R, float>::value) run(gemm_kernel<S, R, 64, 128>);
+            break;
+        // (stage counts and wave layouts from timings of the solve-pass shapes)
+        case GEMM_TILE_D64x64:
+            if constexpr (std::is_same<S, R>::value) run(gemm_dma_kernel<R, 64, 64, std::is_same<R, float>::value ? 3 : 2>);
+            break;
+        case GEMM_TILE_D64x32:
+            if constexpr (std::is_same<S, R>::value) run(gemm_dma_kernel<R, 64, 32, std::is_same<R, float>::value ? 4 : 2>);
+            break;
+        case GEMM_TILE_D128x64:   // eight waves of 32 x 32
+            if constexpr (std::is_same<S, R>::value)
+                run(gemm_dma_kernel<R, 128, 64, std::is_same<R, float>::value ? 3 : 2, 512, 2>, 512);
+            break;
+        case GEMM_TILE_D64x128:   // eight waves of 32 x 32
+            if constexpr (std::is_same<R, float>::value && std::is_same<S, R>::value)
+                run(gemm_dma_kernel<R, 64, 128, 3, 512, 4>, 512);
+            break;
+        case GEMM_TILE_D128x128:   // eight waves of 64 x 64
+            if constexpr (std::is_same<R, float>::value && std::is_same<S, R>::value)
+                run(gemm_dma_kernel<R, 128, 128, 2, 512, 2>, 512);
             break;
         default: run(gemm_kernel<S, R, 64, 64>); break;
     }

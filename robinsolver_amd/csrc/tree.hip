@@ -682,7 +682,7 @@ int Tree::factor_device(const double2* k) {
         if (tune) {
             float best = 0.f;
             for (int t = 0; t < GEMM_TILES; ++t) {
-                if (!gemm_tile_distinct(t, N, site / 5 < c64_from)) continue;
+                if (!gemm_tile_distinct(t, N, site / 5 < c64_from, site / 5 < c64_from)) continue;
                 (void)hipEventRecord(te0, stream);
                 gemm(site / 5, d, np, M, N, K, gws, t);
                 (void)hipEventRecord(te1, stream);

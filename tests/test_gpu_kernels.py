@@ -10,6 +10,8 @@ import robinsolver_amd as rsa
 
 pytestmark = pytest.mark.gpu
 
+N_TILES = 16   # GEMM tile choices of rs_check_gemm_tile (0 .. 15)
+
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
@@ -212,7 +214,7 @@ def test_gemm_tile_shapes_give_identical_bits(prec, M, N, K, scatter, gpu):
     C0 = _rc(rng, ldc, N)
     rmap = np.ascontiguousarray(rng.permutation(ldc)[:M].astype(np.int32)) if scatter else None
     outs = []
-    for tile in range(-1, 11):
+    for tile in range(-1, N_TILES):
         C = np.asfortranarray(C0.copy())
         a = np.asfortranarray(A)
         b = np.asfortranarray(B)
@@ -227,6 +229,74 @@ def test_gemm_tile_shapes_give_identical_bits(prec, M, N, K, scatter, gpu):
         outs.append(C.copy())
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
+
+
+@pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED, rsa.RS_MIXED_MERGE])
+@pytest.mark.parametrize("M,N,K,ksplit", [(128, 64, 96, 32), (96, 40, 80, 33), (64, 256, 512, 256)])
+def test_gemm_tiles_with_split_b(prec, M, N, K, ksplit, gpu):
+    """B's rows k >= ksplit from a second array (the two children's interface
+    vectors): every tile gives the same bits, and the product is A [B; B2]
+    (an odd ksplit moves the operands in 4-byte pieces on the direct-to-LDS tiles)."""
+    print("case: gemm tiles split B", prec, M, N, K, ksplit, flush=True)
+    L = _lib()
+    L.rs_check_gemm_tile.argtypes = L.rs_check_gemm.argtypes + [ctypes.c_int]
+    rng = np.random.default_rng(7 * M + K)
+    A = np.asfortranarray(_rc(rng, M, K))
+    B1 = np.asfortranarray(_rc(rng, K, N))
+    B2 = np.asfortranarray(_rc(rng, K, N))
+    C0 = _rc(rng, M, N)
+    alpha = np.array([1.0, 0.5])
+    beta = np.array([0.25, -1.0])
+    outs = []
+    for tile in range(-1, N_TILES):
+        C = np.asfortranarray(C0.copy())
+        rc = L.rs_check_gemm_tile(prec, M, N, K, _dp(A.reshape(-1, order="F").view(np.float64)), M,
+                                  _dp(B1.reshape(-1, order="F").view(np.float64)),
+                                  _dp(B2.reshape(-1, order="F").view(np.float64)), K, ksplit,
+                                  _dp(C.reshape(-1, order="F").view(np.float64)), M, None, _dp(alpha), _dp(beta),
+                                  tile)
+        assert rc == 0
+        outs.append(C.copy())
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    Bfull = np.vstack([B1[:ksplit], B2[:K - ksplit]])
+    ref = (1.0 + 0.5j) * (A @ Bfull) + (0.25 - 1.0j) * C0
+    tol = 1e-12 if prec == rsa.RS_FP64 else 1e-4
+    assert np.linalg.norm(outs[0] - ref) / np.linalg.norm(ref) < tol
+
+
+@pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED, rsa.RS_MIXED_MERGE])
+@pytest.mark.parametrize("M,N,K,ldb", [(64, 64, 48, 100), (130, 70, 96, 96), (256, 256, 128, 300)])
+def test_gemm_tiles_with_row_mapped_b(prec, M, N, K, ldb, gpu):
+    """B's row k read at row bmap[k] (the down pass's incoming rows where the
+    level above wrote them): every tile gives the same bits and the product
+    A B[bmap]."""
+    print("case: gemm tiles row-mapped B", prec, M, N, K, ldb, flush=True)
+    L = _lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.rs_check_gemm_bmap.argtypes = [ctypes.c_int] * 4 + [dp, ctypes.c_int, dp, ctypes.c_int, ip, dp, ctypes.c_int,
+                                                          dp, dp, ctypes.c_int]
+    rng = np.random.default_rng(11 * M + K)
+    A = np.asfortranarray(_rc(rng, M, K))
+    B = np.asfortranarray(_rc(rng, ldb, N))
+    bmap = np.ascontiguousarray(rng.permutation(ldb)[:K].astype(np.int32))
+    C0 = _rc(rng, M, N)
+    alpha = np.array([1.0, 0.5])
+    beta = np.array([0.25, -1.0])
+    outs = []
+    for tile in range(-1, N_TILES):
+        C = np.asfortranarray(C0.copy())
+        rc = L.rs_check_gemm_bmap(prec, M, N, K, _dp(A.reshape(-1, order="F").view(np.float64)), M,
+                                  _dp(B.reshape(-1, order="F").view(np.float64)), ldb, bmap.ctypes.data_as(ip),
+                                  _dp(C.reshape(-1, order="F").view(np.float64)), M, _dp(alpha), _dp(beta), tile)
+        assert rc == 0
+        outs.append(C.copy())
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    ref = (1.0 + 0.5j) * (A @ B[bmap]) + (0.25 - 1.0j) * C0
+    tol = 1e-12 if prec == rsa.RS_FP64 else 1e-4
+    assert np.linalg.norm(outs[0] - ref) / np.linalg.norm(ref) < tol
 
 
 def test_autotuned_solve_matches_untuned_bits(gpu):
