@@ -291,6 +291,11 @@ int check_gemm_t(int tile, int M, int N, int K, const double* A, int lda, const 
     d.A = dA; d.lda = lda; d.B = dB; d.B2 = dB2; d.ldb = ldb; d.ksplit = B2 ? ksplit : (1 << 30);
     d.Out = dC; d.ldo = ldc; d.rmap = dmap; d.dmap = rmap ? 1 : 0;
     d.bmap = dbmap;
+    if (bmap) {   // the row map's pair property, as the tree sets it for its down-pass maps
+        bool pairs = K % 2 == 0;
+        for (int k = 0; pairs && k + 1 < K; k += 2) pairs = bmap[k] % 2 == 0 && bmap[k + 1] == bmap[k] + 1;
+        d.bpair = pairs ? 1 : 0;
+    }
     d.Din = dD; d.ldd = ldc;
     d.ar = alpha[0]; d.ai = alpha[1]; d.br = beta[0]; d.bi = beta[1];
     (void)hipMemcpy(dd, &d, sizeof d, hipMemcpyHostToDevice);

@@ -394,7 +394,7 @@ template <typename R, int BM, int BN, int NS, int NT = 256, int WAVES_N = 2>
 __global__ void __launch_bounds__(NT) gemm_dma_kernel(GemmLaunch GL) {
     using V = c_t<R>;
     using A4 = typename acc_t<R>::T;
-    constexpr int BK = 16;
+    constexpr int BK = 16;   // (32-deep chunks and B in padded columns, moved in 4-byte pieces, measured slower)
     constexpr int EPU = 16 / (int)sizeof(V);
     constexpr int NW = NT / 64;
     constexpr int WAVES_M = NW / WAVES_N;
@@ -407,6 +407,8 @@ __global__ void __launch_bounds__(NT) gemm_dma_kernel(GemmLaunch GL) {
     constexpr int STAGE = (AU + BU) * EPU;             // elements per stage
     constexpr int SMEM = NS * STAGE > BN * (BM + 1) ? NS * STAGE : BN * (BM + 1);
     __shared__ __align__(16) V smem[SMEM];
+    constexpr int PBL = 512;   // pairs of a row-mapped B's K range held in LDS
+    __shared__ int pbl[PBL];
 
     const GemmDesc* descs = GL.g[0].descs;
     V* ws = (V*)GL.g[0].ws;
@@ -445,9 +447,16 @@ __global__ void __launch_bounds__(NT) gemm_dma_kernel(GemmLaunch GL) {
 
     auto swzA = [](int k) { return EPU == 2 ? (k & 1) << 3 : 0; };
     auto swzB = [](int n) { return EPU == 2 ? (n >> 1) & 7 : n & 15; };
-    // 16-byte units need both operands' elements 16-byte aligned in pairs
-    const bool fast = !bmap && ((((size_t)A | (size_t)B | (size_t)(B2 ? B2 : B)) & 15) == 0) &&
-                      (EPU == 1 || (((lda | ldb | ksplit | M | K | kbeg) & 1) == 0));
+    // 16-byte units need both operands' elements 16-byte aligned in pairs; a
+    // row-mapped B in 16-byte units needs its rows in contiguous even pairs
+    // (D.bpair) and its map over the slice's K range in LDS
+    const bool mapped = bmap && (EPU == 1 || D.bpair) && (kend - kbeg) <= EPU * PBL;
+    const bool fast = (!bmap || mapped) && ((((size_t)A | (size_t)B | (size_t)(B2 ? B2 : B)) & 15) == 0) &&
+                      (EPU == 1 || (((lda | ldb | (bmap ? 0 : ksplit) | M | K | kbeg) & 1) == 0));
+    if (fast && mapped) {
+        for (int j = tid; j < (kend - kbeg + EPU - 1) / EPU; j += NT) pbl[j] = gldi(bmap + kbeg + EPU * j);
+        __syncthreads();
+    }
     const char* zero = (const char*)&g_zero16;
     // 16-byte copies: each thread's source addresses advance by one chunk per
     // issue (chunks are issued in order); an element beyond the K range comes
@@ -484,12 +493,22 @@ __global__ void __launch_bounds__(NT) gemm_dma_kernel(GemmLaunch GL) {
             ka[q] += BK;
             pa[q] += astep;
         }
+        if (mapped) {
 #pragma unroll
-        for (int q = 0; q < NUB; ++q) {
-            const int gk = kb[q];
-            const char* b = (gk < ksplit ? pb1[q] : pb2[q]) + (size_t)gk * sizeof(V);
-            lds_copy16(gk < kend ? b : zero, sB + (q * NW + wave) * 1024);
-            kb[q] += BK;
+            for (int q = 0; q < NUB; ++q) {
+                const int gk = kb[q];
+                const int row = pbl[min((gk - kbeg) / EPU, PBL - 1)];
+                lds_copy16(gk < kend ? pb1[q] + (size_t)row * sizeof(V) : zero, sB + (q * NW + wave) * 1024);
+                kb[q] += BK;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NUB; ++q) {
+                const int gk = kb[q];
+                const char* b = (gk < ksplit ? pb1[q] : pb2[q]) + (size_t)gk * sizeof(V);
+                lds_copy16(gk < kend ? b : zero, sB + (q * NW + wave) * 1024);
+                kb[q] += BK;
+            }
         }
     };
     // the same layout in 4-byte pieces (4 per unit), element by element

@@ -24,6 +24,8 @@ struct GemmDesc {
     const int* bmap;     // B row k at B + bmap[k] (elements) instead of B + k (no B2 then)
     int lda, ldb, ldo, ldd;
     int ksplit, dmap;
+    int bpair;           // bmap[2j + 1] = bmap[2j] + 1 and bmap[2j] even for every j
+    int reserved;
     double ar, ai, br, bi;
 };
 

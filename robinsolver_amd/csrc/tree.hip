@@ -128,7 +128,7 @@ __host__ __device__ inline void solve_descs(const SolveLevel& p, int n, GemmDesc
         // B: the child's non-interface incoming rows, read where they were written
         GemmDesc t = z;
         t.A = Tcn + bn * i * e; t.lda = p.i;
-        t.B = p.U; t.bmap = p.dsrc + (size_t)(2 * n + c) * bn; t.ldb = p.ldg;
+        t.B = p.U; t.bmap = p.dsrc + (size_t)(2 * n + c) * bn; t.ldb = p.ldg; t.bpair = p.dpair;
         t.Out = p.Zl + ((size_t)n * 2 * i + (size_t)c * i) * e; t.ldo = p.ldz;
         dt[c] = t;
     }
@@ -326,6 +326,15 @@ int Tree::build_sources() {
         S.swap(Sn);
     }
     src_leaf.swap(S);
+    // rows read in 16-byte pairs by the direct-to-LDS GEMM tiles (with bn even,
+    // the pairs of every node's map start at even offsets of the table)
+    src_dn_pairs.assign(L, 0);
+    for (int l = 0; l < L; ++l) {
+        const std::vector<int>& dn = src_dn[l];
+        bool ok = P.lv[l].bn % 2 == 0;
+        for (size_t j = 0; ok && j + 1 < dn.size(); j += 2) ok = dn[j] % 2 == 0 && dn[j + 1] == dn[j] + 1;
+        src_dn_pairs[l] = ok ? 1 : 0;
+    }
     return 0;
 }
 
@@ -843,6 +852,7 @@ template <typename R> SolveLevel Tree::solve_level(const SolveWork<R>& w, int l)
     p.U = (char*)w.gF[0];
     p.prow = d_prow[l]->as<int>();
     p.dsrc = d_src_dn[l]->as<int>();
+    p.dpair = src_dn_pairs[l];
     p.nodes = Vl.nodes; p.i = Vl.isz; p.b = Vl.bc; p.bn = Vl.bn; p.Pp = Vl.P;
     p.ldc = C.nodes * Vl.bc; p.ldg = iface_len; p.ldz = Vl.nodes * 2 * Vl.isz; p.ldp = Vl.nodes * Vl.P;
     p.lvl = l;

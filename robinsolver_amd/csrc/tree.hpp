@@ -74,6 +74,7 @@ struct SolveLevel {
     char* U;            // every level's interface data and the root boundary data (column stride ldg)
     const int *prow, *dsrc;   // up-pass row scatter; the children's non-interface rows in U
     int nodes, i, b, bn, Pp, ldc, ldg, ldz, ldp, lvl, esz;
+    int dpair;          // dsrc pairs contiguous and even (GemmDesc::bpair)
 };
 
 enum ProfCat { P_LEAF_UP = 0, P_LEAF_DOWN, P_GEMM_UP_G, P_GEMM_UP_H, P_GEMM_DN_T, P_GEMM_DN_G, P_GATHER,
@@ -207,6 +208,7 @@ struct Tree {
     int iface_len = 0;
     std::vector<int> iface_off;
     std::vector<std::vector<int>> src_dn;
+    std::vector<char> src_dn_pairs;   // per level: src_dn[l] in contiguous, even-aligned pairs
     std::vector<int> src_leaf;
     std::vector<DevBuf*> d_src_dn;
     DevBuf d_src_leaf;

@@ -266,12 +266,17 @@ def test_gemm_tiles_with_split_b(prec, M, N, K, ksplit, gpu):
 
 
 @pytest.mark.parametrize("prec", [rsa.RS_FP64, rsa.RS_MIXED, rsa.RS_MIXED_MERGE])
-@pytest.mark.parametrize("M,N,K,ldb", [(64, 64, 48, 100), (130, 70, 96, 96), (256, 256, 128, 300)])
-def test_gemm_tiles_with_row_mapped_b(prec, M, N, K, ldb, gpu):
+@pytest.mark.parametrize("M,N,K,ldb,pairs", [(64, 64, 48, 100, False), (130, 70, 96, 96, False),
+                                             (256, 256, 128, 300, False), (128, 256, 192, 400, True),
+                                             (64, 96, 1100, 1200, True)])
+def test_gemm_tiles_with_row_mapped_b(prec, M, N, K, ldb, pairs, gpu):
     """B's row k read at row bmap[k] (the down pass's incoming rows where the
     level above wrote them): every tile gives the same bits and the product
-    A B[bmap]."""
-    print("case: gemm tiles row-mapped B", prec, M, N, K, ldb, flush=True)
+    A B[bmap].  With pairs, the map sends rows 2j, 2j + 1 to consecutive rows
+    starting at an even one (as the tree's down-pass maps do), which the
+    direct-to-LDS tiles read in 16-byte units; K = 1100 exceeds the map range
+    they hold in LDS."""
+    print("case: gemm tiles row-mapped B", prec, M, N, K, ldb, pairs, flush=True)
     L = _lib()
     dp = ctypes.POINTER(ctypes.c_double)
     ip = ctypes.POINTER(ctypes.c_int)
@@ -280,7 +285,11 @@ def test_gemm_tiles_with_row_mapped_b(prec, M, N, K, ldb, gpu):
     rng = np.random.default_rng(11 * M + K)
     A = np.asfortranarray(_rc(rng, M, K))
     B = np.asfortranarray(_rc(rng, ldb, N))
-    bmap = np.ascontiguousarray(rng.permutation(ldb)[:K].astype(np.int32))
+    if pairs:
+        bmap = np.ascontiguousarray((2 * rng.permutation(ldb // 2)[:K // 2, None] + np.arange(2)).reshape(-1)
+                                    .astype(np.int32))
+    else:
+        bmap = np.ascontiguousarray(rng.permutation(ldb)[:K].astype(np.int32))
     C0 = _rc(rng, M, N)
     alpha = np.array([1.0, 0.5])
     beta = np.array([0.25, -1.0])
