@@ -381,25 +381,14 @@ int rs_check_inverse(int n, int batch, double* mats) {
     if (e == hipSuccess && !ip.small) {
         e = hipMalloc(&W, ip.w_elems() * 16 + 16);
         if (e == hipSuccess) e = hipMalloc(&piv, (size_t)n * batch * sizeof(int));
-        cmaps.resize((size_t)ip.panels() * (n - ip.nb));
-        for (int pi = 0; pi < ip.panels(); ++pi) {
-            int j0 = pi * ip.nb;
-            for (int c2 = 0; c2 < n - ip.nb; ++c2) cmaps[(size_t)pi * (n - ip.nb) + c2] = c2 < j0 ? c2 : c2 + ip.nb;
-        }
+        cmaps.resize(ip.cmap_elems());
         if (e == hipSuccess) e = hipMalloc(&cm, cmaps.size() * sizeof(int) + 4);
-        if (e == hipSuccess) e = hipMemcpy(cm, cmaps.data(), cmaps.size() * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess) {
-            for (int pi = 0; pi < ip.panels(); ++pi)
-                for (int b = 0; b < batch; ++b) {
-                    rs::GemmDesc g;
-                    std::memset(&g, 0, sizeof g);
-                    double2* Mb = dm + (size_t)b * nn;
-                    g.A = Mb + (size_t)pi * ip.nb * n; g.lda = n;
-                    g.B = W + (size_t)b * ip.nb * (n - ip.nb); g.ldb = ip.nb;
-                    g.Out = Mb; g.ldo = n; g.cmap = cm + (size_t)pi * (n - ip.nb);
-                    g.Din = Mb; g.ldd = n; g.dmap = 1; g.ar = 1.0; g.br = 1.0; g.ksplit = 1 << 30;
-                    descs.push_back(g);
-                }
+            descs.resize(ip.desc_count());
+            ip.make_descs(dm, W, cm, descs.data(), cmaps.data());
+            e = hipMemcpy(cm, cmaps.data(), cmaps.size() * sizeof(int), hipMemcpyHostToDevice);
+        }
+        if (e == hipSuccess) {
             e = hipMalloc(&dd, descs.size() * sizeof(rs::GemmDesc));
             if (e == hipSuccess)
                 e = hipMemcpy(dd, descs.data(), descs.size() * sizeof(rs::GemmDesc), hipMemcpyHostToDevice);

@@ -20,6 +20,7 @@
 // as a batched GEMM.  After the last panel the interchanges are undone as
 // column interchanges.
 #include <algorithm>
+#include <cstring>
 #include <utility>
 
 #include "rs_common.hpp"
@@ -277,16 +278,19 @@ __global__ void __launch_bounds__(LB) gj_panel_reg_kernel(double2* __restrict__ 
     if (bad && tid == 0) atomicOr(info, 2);
 }
 
-// Apply the panel's row interchanges to the columns outside the panel, move the
-// pivot rows of those columns to W (nb x (n-nb), compact) and zero them.
-// A group of nb lanes per column (nb <= 32 divides 256): the interchanges
-// touch the contiguous rows j0 .. j0+nb-1 and the rows piv[j], so lane i
-// reads the rows whose values end at j0 + i and at piv[j0 + i] (from the
-// composed permutation, built once per workgroup in LDS), moves the first to
-// W, zeroes row j0 + i and stores the second.  Every load of a column comes
-// before its stores in program order (the stores wait for the loaded values).
+// Apply the row interchanges of pivots j0 .. j0+np-1 (in order) to the columns
+// of [cb, ce) outside [j0, j0+np) (a subrange of it), move those columns'
+// rows j0 .. j0+np-1 to W (np x (ce - cb - np), compact) and zero them.
+// A group of np lanes per column (np <= 64 divides 256: a column's lanes are
+// in one wave): the interchanges touch the contiguous rows j0 .. j0+np-1 and
+// the rows piv[j], so lane i reads the rows whose values end at j0 + i and at
+// piv[j0 + i] (from the composed permutation, built once per workgroup in
+// LDS), moves the first to W, zeroes row j0 + i and stores the second.  Every
+// load of a column comes before its stores in program order (the stores wait
+// for the loaded values).
 __global__ void __launch_bounds__(256) gj_swap_extract_grp_kernel(double2* __restrict__ mats, int n, size_t stride,
-                                                                  int j0, int nb, const int* __restrict__ pivs,
+                                                                  int j0, int nb, int cb, int ce,
+                                                                  const int* __restrict__ pivs,
                                                                   double2* __restrict__ W, size_t wstride) {
     extern __shared__ int rowat[];   // [n]: row whose value ends at each (touched) position
     const int tid = threadIdx.x;
@@ -305,11 +309,11 @@ __global__ void __launch_bounds__(256) gj_swap_extract_grp_kernel(double2* __res
     __syncthreads();
     const int g = tid / nb, i = tid - g * nb;
     const int c2 = blockIdx.x * (256 / nb) + g;
-    if (c2 >= n - nb) return;
-    const int c = c2 < j0 ? c2 : c2 + nb;
+    if (c2 >= ce - cb - nb) return;
+    const int c = cb + c2 < j0 ? cb + c2 : cb + c2 + nb;
     double2* col = mats + (size_t)blockIdx.y * stride + (size_t)c * n;
     const int j = j0 + i, p = piv[j];
-    const bool outside = p >= j0 + nb;   // (a pivot row inside the panel's rows is some lane's j)
+    const bool outside = p >= j0 + nb;   // (a pivot row inside the pivots' rows is some lane's j)
     const double2 v1 = col[rowat[j]];
     const double2 v2 = outside ? col[rowat[p]] : make_double2(0.0, 0.0);
     W[(size_t)blockIdx.y * wstride + (size_t)c2 * nb + i] = v1;
@@ -318,12 +322,13 @@ __global__ void __launch_bounds__(256) gj_swap_extract_grp_kernel(double2* __res
 }
 
 // (one thread per column, sequential interchanges: interfaces too large for the
-// LDS position table of the kernel above)
-__global__ void gj_swap_extract_kernel(double2* __restrict__ mats, int n, size_t stride, int j0, int nb,
-                                       const int* __restrict__ pivs, double2* __restrict__ W, size_t wstride) {
+// LDS position table of the kernel above, or pivot counts it does not take)
+__global__ void gj_swap_extract_kernel(double2* __restrict__ mats, int n, size_t stride, int j0, int nb, int cb,
+                                       int ce, const int* __restrict__ pivs, double2* __restrict__ W,
+                                       size_t wstride) {
     const int c2 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c2 >= n - nb) return;
-    const int c = c2 < j0 ? c2 : c2 + nb;
+    if (c2 >= ce - cb - nb) return;
+    const int c = cb + c2 < j0 ? cb + c2 : cb + c2 + nb;
     double2* col = mats + (size_t)blockIdx.y * stride + (size_t)c * n;
     const int* piv = pivs + (size_t)blockIdx.y * n;
     double2* w = W + (size_t)blockIdx.y * wstride + (size_t)c2 * nb;
@@ -379,11 +384,81 @@ void InvPlan::build(int n_, int batch_, size_t stride_, size_t lds_limit_) {
     // 252 / 246 us with the LDS kernel)
     small = n < 64 && inv_small_lds(n) + 512 <= lds_limit;
     nb = small ? n : gj_panel_width(n, lds_limit);
+    // Above 1024 points the register panels are 4 or 8 columns wide, and a
+    // rank-nb update per panel would stream the whole matrix n / nb times:
+    // the panels are grouped into blocks of OB columns instead, each panel's
+    // update reaching only the rest of its block, and the columns outside a
+    // block get one rank-OB update from the block's composed transform.
+    ob = (!small && gj_reg_rows(n) >= 2 && nb < 16) ? 64 : 0;
 }
 
 bool InvPlan::fits() const {
     if (small || gj_reg_rows(n)) return true;   // register panels: rows up to 4096 (12-bit row in the pivot key)
     return nb >= 1 && ((size_t)n * nb + n) * sizeof(double2) + 1024 <= lds_limit;
+}
+
+size_t InvPlan::w_elems() const {
+    if (small) return 0;
+    if (!ob) return (size_t)nb * (n - nb) * batch;
+    size_t w = 0;
+    for (int J = 0; J < n; J += ob) {
+        const int bw = std::min(ob, n - J);
+        w = std::max(w, std::max((size_t)nb * (bw - nb), (size_t)bw * (n - bw)));
+    }
+    return w * batch;
+}
+
+// The update products of launch_inverse in launch order, each a batch of
+// descriptors: unblocked, one rank-nb update per panel; blocked, per block its
+// panels' updates inside the block, then the rank-bw update of the rest.
+template <typename F> static void inv_steps(const InvPlan& ip, F&& f) {
+    const int n = ip.n, nb = ip.nb;
+    if (!ip.ob) {
+        for (int j0 = 0; j0 < n; j0 += nb) f(j0, nb, 0, n, true);
+        return;
+    }
+    for (int J = 0; J < n; J += ip.ob) {
+        const int bw = std::min(ip.ob, n - J);
+        for (int p = J; p < J + bw; p += nb) f(p, nb, J, J + bw, true);
+        f(J, bw, 0, n, false);
+    }
+}
+
+size_t InvPlan::desc_count() const {
+    if (small) return 0;
+    size_t c = 0;
+    inv_steps(*this, [&](int, int np, int cb, int ce, bool) { if (ce - cb - np > 0) c += batch; });
+    return c;
+}
+
+size_t InvPlan::cmap_elems() const {
+    if (small) return 0;
+    size_t c = 0;
+    inv_steps(*this, [&](int, int np, int cb, int ce, bool) { c += ce - cb - np; });
+    return c;
+}
+
+void InvPlan::make_descs(double2* mats, double2* W, const int* cmap_dev, GemmDesc* d, int* cmaps) const {
+    if (small) return;
+    size_t nd = 0, nc = 0;
+    inv_steps(*this, [&](int j0, int np, int cb, int ce, bool) {
+        const int m = ce - cb - np;
+        for (int c2 = 0; c2 < m; ++c2) cmaps[nc + c2] = cb + c2 < j0 ? cb + c2 : cb + c2 + np;
+        if (m > 0)
+            for (int b = 0; b < batch; ++b) {
+                GemmDesc g;
+                std::memset(&g, 0, sizeof g);
+                double2* Mb = mats + (size_t)b * stride;
+                g.A = Mb + (size_t)j0 * n; g.lda = n;
+                g.B = W + (size_t)b * (w_elems() / batch); g.ldb = np;
+                g.Out = Mb; g.ldo = n;
+                g.cmap = cmap_dev + nc;
+                g.Din = Mb; g.ldd = n; g.dmap = 1;
+                g.ar = 1.0; g.br = 1.0; g.ksplit = 1 << 30;
+                d[nd++] = g;
+            }
+        nc += m;
+    });
 }
 
 void launch_inverse(const InvPlan& ip, double2* mats, int* pivs, double2* W, const GemmDesc* d_descs,
@@ -396,39 +471,48 @@ void launch_inverse(const InvPlan& ip, double2* mats, int* pivs, double2* W, con
     }
     const int n = ip.n, nb = ip.nb;
     const size_t plds = ((size_t)n * nb + n) * sizeof(double2);
-    const size_t wstride = (size_t)nb * (n - nb);
+    const size_t wstride = ip.w_elems() / ip.batch;
     const int rpt = gj_reg_rows(n);
     const int nt = rpt ? std::min(1024, ((n + rpt - 1) / rpt + 63) / 64 * 64) : 256;
-    for (int j0 = 0, pi = 0; j0 < n; j0 += nb, ++pi) {
-        if (rpt == 1)
-            hipLaunchKernelGGL((gj_panel_reg_kernel<16, 1>), dim3(ip.batch), dim3(nt), 0, st, mats, n, ip.stride, j0,
-                               nb, pivs, info);
-        else if (rpt == 2 && nb == 16)
-            hipLaunchKernelGGL((gj_panel_reg_kernel<16, 2, 512>), dim3(ip.batch), dim3(nt), 0, st, mats, n, ip.stride,
-                               j0, nb, pivs, info);
-        else if (rpt == 2)
-            hipLaunchKernelGGL((gj_panel_reg_kernel<8, 2>), dim3(ip.batch), dim3(nt), 0, st, mats, n, ip.stride, j0,
-                               nb, pivs, info);
-        else if (rpt == 4)
-            hipLaunchKernelGGL((gj_panel_reg_kernel<4, 4>), dim3(ip.batch), dim3(nt), 0, st, mats, n, ip.stride, j0,
-                               nb, pivs, info);
-        else
-            hipLaunchKernelGGL(gj_panel_kernel, dim3(ip.batch), dim3(256), plds, st, mats, n, ip.stride, j0, nb,
-                               pivs, info);
-        if ((size_t)n * sizeof(int) <= 32768 && nb <= 32 && (nb & (nb - 1)) == 0) {
-            const int cpb = 256 / nb;
-            hipLaunchKernelGGL(gj_swap_extract_grp_kernel, dim3((n - nb + cpb - 1) / cpb, ip.batch), dim3(256),
-                               (size_t)n * sizeof(int), st, mats, n, ip.stride, j0, nb, pivs, W, wstride);
-        } else {
-            hipLaunchKernelGGL(gj_swap_extract_kernel, dim3((n - nb + 255) / 256, ip.batch), dim3(256), 0, st, mats,
-                               n, ip.stride, j0, nb, pivs, W, wstride);
+    size_t nd = 0;
+    inv_steps(ip, [&](int j0, int np, int cb, int ce, bool panel) {
+        if (panel) {
+            if (rpt == 1)
+                hipLaunchKernelGGL((gj_panel_reg_kernel<16, 1>), dim3(ip.batch), dim3(nt), 0, st, mats, n, ip.stride,
+                                   j0, nb, pivs, info);
+            else if (rpt == 2 && nb == 16)
+                hipLaunchKernelGGL((gj_panel_reg_kernel<16, 2, 512>), dim3(ip.batch), dim3(nt), 0, st, mats, n,
+                                   ip.stride, j0, nb, pivs, info);
+            else if (rpt == 2)
+                hipLaunchKernelGGL((gj_panel_reg_kernel<8, 2>), dim3(ip.batch), dim3(nt), 0, st, mats, n, ip.stride,
+                                   j0, nb, pivs, info);
+            else if (rpt == 4)
+                hipLaunchKernelGGL((gj_panel_reg_kernel<4, 4>), dim3(ip.batch), dim3(nt), 0, st, mats, n, ip.stride,
+                                   j0, nb, pivs, info);
+            else
+                hipLaunchKernelGGL(gj_panel_kernel, dim3(ip.batch), dim3(256), plds, st, mats, n, ip.stride, j0, nb,
+                                   pivs, info);
         }
-        // the rank-nb update streams the trailing columns once per panel (at 1024^2
-        // the top levels' launches read and write 16.5 MB each in 8.5 us, 3.9 TB/s);
-        // 32x32 tiles (four times the workgroups of 64x64) measured fastest
-        // (factor at 1024^2: 64x64 38.4 ms, 32x64 38.0, 64x32 38.0, 16x64 38.0, 32x32 37.8)
-        launch_gemm<double>(d_descs + (size_t)pi * ip.batch, ip.batch, n, n - nb, nb, st, nullptr, GEMM_TILE_32x32);
-    }
+        const int m = ce - cb - np;
+        if (m <= 0) return;
+        if ((size_t)n * sizeof(int) <= 32768 && np <= 64 && (np & (np - 1)) == 0) {
+            const int cpb = 256 / np;
+            hipLaunchKernelGGL(gj_swap_extract_grp_kernel, dim3((m + cpb - 1) / cpb, ip.batch), dim3(256),
+                               (size_t)n * sizeof(int), st, mats, n, ip.stride, j0, np, cb, ce, pivs, W, wstride);
+        } else {
+            hipLaunchKernelGGL(gj_swap_extract_kernel, dim3((m + 255) / 256, ip.batch), dim3(256), 0, st, mats, n,
+                               ip.stride, j0, np, cb, ce, pivs, W, wstride);
+        }
+        // a panel's rank-nb update (of the rest of the matrix, or of the rest of
+        // its block) streams its columns once (at 1024^2 the top levels' launches
+        // read and write 16.5 MB each in 8.5 us, 3.9 TB/s); 32x32 tiles (four
+        // times the workgroups of 64x64) measured fastest (factor at 1024^2:
+        // 64x64 38.4 ms, 32x64 38.0, 64x32 38.0, 16x64 38.0, 32x32 37.8); a
+        // block's rank-64 update is a matrix-core product
+        launch_gemm<double>(d_descs + nd, ip.batch, n, m, np, st, nullptr,
+                            np >= 32 ? GEMM_TILE_D64x64 : GEMM_TILE_32x32);
+        nd += ip.batch;
+    });
     hipLaunchKernelGGL(gj_unpermute_kernel, dim3((n + 255) / 256, ip.batch), dim3(256), 0, st, mats, n, ip.stride,
                        pivs);
 }

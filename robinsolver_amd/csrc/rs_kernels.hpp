@@ -154,14 +154,19 @@ void launch_fd_norms(const LeafGeom& G, const float2* rf, const float2* rg, cons
 // batched inverse
 struct InvPlan {
     int n = 0, batch = 0, nb = 0;
+    int ob = 0;   // columns per block of panels (0: every panel updates the whole matrix)
     size_t stride = 0, lds_limit = 0;
     bool small = true;
     void build(int n, int batch, size_t stride, size_t lds_limit);
     // false when no kernel of launch_inverse takes an n-point matrix (above ~5k
     // points the LDS panel of even one column exceeds the LDS)
     bool fits() const;
-    int panels() const { return small ? 0 : n / nb; }
-    size_t w_elems() const { return small ? 0 : (size_t)nb * (n - nb) * batch; }
+    size_t w_elems() const;      // complex elements of launch_inverse's W (all matrices)
+    size_t desc_count() const;   // GEMM descriptors of launch_inverse
+    size_t cmap_elems() const;   // column-map entries they point to
+    // the descriptors (d, desc_count()) and column maps (cmaps, cmap_elems(), to
+    // be copied to cmap_dev) of launch_inverse on mats with the W buffer W
+    void make_descs(double2* mats, double2* W, const int* cmap_dev, GemmDesc* d, int* cmaps) const;
 };
 size_t inv_small_lds(int n);
 void launch_inverse(const InvPlan& ip, double2* mats, int* pivs, double2* W, const GemmDesc* d_descs, int* info,

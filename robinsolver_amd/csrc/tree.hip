@@ -439,7 +439,7 @@ static FactorSizes factor_sizes(const Plan& P, const LeafGeom& G, const rs_opts&
         if (l > 0 || !o.skip_root_rtr) z.Tfull[l] = nodes * (size_t)V.P * V.P * 16;
         const MergeLayout m = merge_layout(V, lds, l > 0 || !o.skip_root_rtr);
         z.arena = std::max(z.arena, m.total);
-        if (!m.ip.small) z.fdesc += (long long)m.ip.panels() * nodes * gd + (long long)m.ip.panels() * (i - m.ip.nb) * 4;
+        if (!m.ip.small) z.fdesc += (long long)m.ip.desc_count() * gd + (long long)m.ip.cmap_elems() * 4;
         z.fdesc += 4 * nodes * gd + ((l > 0 || !o.skip_root_rtr) ? 6 * nodes * gd : 0);
     }
     if (!z.arena) z.arena = 256;
@@ -531,7 +531,7 @@ int Tree::factor_device(const double2* k) {
         for (int l = L - 1; l >= 0; --l)
             if (!ml[l].ip.small) {
                 fdoff[l].cm = ncm;
-                ncm += (size_t)ml[l].ip.panels() * (P.lv[l].isz - ml[l].ip.nb);
+                ncm += ml[l].ip.cmap_elems();
             }
         if (need(fcmap, ncm * sizeof(int))) return 1;
         std::vector<int> colmaps(ncm);
@@ -549,26 +549,10 @@ int Tree::factor_device(const double2* k) {
             double2* Y = (double2*)(ar + m.Y);
             double2* GW = (double2*)(ar + m.GW);
             double2* IM = IMf(l);
-            if (!m.ip.small) {  // Gauss-Jordan rank-nb updates of W, one per panel
-                const int n = i, nb_ = m.ip.nb;
-                const int* cm = fcmap.as<int>() + o.cm;
-                for (int pi = 0; pi < m.ip.panels(); ++pi) {
-                    const int j0 = pi * nb_;
-                    for (int c2 = 0; c2 < n - nb_; ++c2)
-                        colmaps[o.cm + (size_t)pi * (n - nb_) + c2] = c2 < j0 ? c2 : c2 + nb_;
-                }
+            if (!m.ip.small) {  // the Gauss-Jordan updates of W
                 o.gj = v.size();
-                for (int pi = 0; pi < m.ip.panels(); ++pi)
-                    for (int bt = 0; bt < nodes; ++bt) {
-                        GemmDesc g = desc0();
-                        double2* Mb = Wm + (size_t)bt * ii;
-                        g.A = Mb + (size_t)pi * nb_ * n; g.lda = n;
-                        g.B = GW + (size_t)bt * nb_ * (n - nb_); g.ldb = nb_;
-                        g.Out = Mb; g.ldo = n;
-                        g.cmap = cm + (size_t)pi * (n - nb_);
-                        g.Din = Mb; g.ldd = n; g.dmap = 1; g.br = 1.0;
-                        v.push_back(g);
-                    }
+                v.resize(v.size() + m.ip.desc_count());
+                m.ip.make_descs(Wm, GW, fcmap.as<int>() + o.cm, v.data() + o.gj, colmaps.data() + o.cm);
             }
             o.w = v.size();
             for (int n = 0; n < nodes; ++n) {  // W = -a c (nu^2 on the diagonal added after)

@@ -125,11 +125,13 @@ def test_gemm_split_and_scatter(prec, N, gpu):
 
 
 @pytest.mark.parametrize("n,batch", [(16, 5), (48, 3), (64, 2), (128, 2), (200, 1), (202, 2), (512, 2), (1004, 1),
-                                     (1024, 2), (1536, 1), (2048, 1), (3072, 1), (4096, 1), (4100, 1)])
+                                     (1024, 2), (1100, 2), (1536, 1), (2048, 1), (3072, 1), (4096, 1), (4100, 1)])
 def test_inverse_matches_numpy(n, batch, gpu):
     """Up to 4096 points the panels live in registers (4096: the largest row the
-    pivot key's 12-bit row field holds); 4100 takes the LDS panel kernel with
-    one-column panels and the one-thread-per-column interchanges."""
+    pivot key's 12-bit row field holds); above 1024 points the panels are grouped
+    in blocks of 64 columns (1100: 4-column panels, a last block of 12 columns);
+    4100 takes the LDS panel kernel with one-column panels and the
+    one-thread-per-column interchanges."""
     print("case: inverse", n, batch, flush=True)
     rng = np.random.default_rng(n)
     mats = _rc(rng, batch, n, n) + 0.1 * np.eye(n)
