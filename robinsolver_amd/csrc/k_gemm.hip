@@ -378,7 +378,8 @@ __device__ inline void bare_barrier() {
 // barrier per chunk.  The products, their order over K (16-deep chunks from
 // the slice start, k ascending in steps of four, Gauss's three real
 // products) and the epilogue are those of gemm_kernel, so the results are the
-// same bits.  S = R only (the operands are staged as stored).
+// same bits.  The operands are staged as stored (precision S) and rounded to
+// the precision of the products R as they are read from LDS.
 //
 // Stage layout in 16-byte units (EPU complex elements each): A as [k][BM/EPU]
 // with unit index i/EPU xor swzA(k), B as [n][BK/EPU] (k contiguous, as B is
@@ -390,9 +391,10 @@ __device__ inline void bare_barrier() {
 // (their products land in output rows and columns that are not stored).
 // Operands that cannot be moved in 16-byte units (a row-mapped B, odd leading
 // dimensions or offsets) are moved in 4-byte pieces into the same layout.
-template <typename R, int BM, int BN, int NS, int NT = 256, int WAVES_N = 2>
+template <typename S, typename R, int BM, int BN, int NS, int NT = 256, int WAVES_N = 2>
 __global__ void __launch_bounds__(NT) gemm_dma_kernel(GemmLaunch GL) {
-    using V = c_t<R>;
+    using V = c_t<S>;    // operands as stored and staged
+    using VR = c_t<R>;   // operands of the products
     using A4 = typename acc_t<R>::T;
     constexpr int BK = 16;   // (32-deep chunks and B in padded columns, moved in 4-byte pieces, measured slower)
     constexpr int EPU = 16 / (int)sizeof(V);
@@ -405,19 +407,20 @@ __global__ void __launch_bounds__(NT) gemm_dma_kernel(GemmLaunch GL) {
     constexpr int NUA = AU / NT, NUB = BU / NT;        // 16-byte loads per thread and chunk
     static_assert(AU % NT == 0 && BU % NT == 0 && AROW >= 16 && TM >= 1 && TN >= 1, "tile shape");
     constexpr int STAGE = (AU + BU) * EPU;             // elements per stage
-    constexpr int SMEM = NS * STAGE > BN * (BM + 1) ? NS * STAGE : BN * (BM + 1);
+    constexpr int CT = (BN * (BM + 1) * (int)sizeof(VR) + (int)sizeof(V) - 1) / (int)sizeof(V);   // output tile
+    constexpr int SMEM = NS * STAGE > CT ? NS * STAGE : CT;
     __shared__ __align__(16) V smem[SMEM];
     constexpr int PBL = 512;   // pairs of a row-mapped B's K range held in LDS
     __shared__ int pbl[PBL];
 
     const GemmDesc* descs = GL.g[0].descs;
-    V* ws = (V*)GL.g[0].ws;
+    VR* ws = (VR*)GL.g[0].ws;
     int M = GL.g[0].M, N = GL.g[0].N, K = GL.g[0].K, tilesM = GL.g[0].tilesM, tilesN = GL.g[0].tilesN,
         kslices = GL.g[0].ks, kchunk = GL.g[0].kchunk, wg = blockIdx.x;
 #pragma unroll
     for (int q = 1; q < GEMM_MAX_GROUPS; ++q)
         if (q < GL.n && (int)blockIdx.x >= GL.g[q].wg0) {
-            descs = GL.g[q].descs; ws = (V*)GL.g[q].ws;
+            descs = GL.g[q].descs; ws = (VR*)GL.g[q].ws;
             M = GL.g[q].M; N = GL.g[q].N; K = GL.g[q].K; tilesM = GL.g[q].tilesM; tilesN = GL.g[q].tilesN;
             kslices = GL.g[q].ks; kchunk = GL.g[q].kchunk; wg = blockIdx.x - GL.g[q].wg0;
         }
@@ -546,18 +549,20 @@ __global__ void __launch_bounds__(NT) gemm_dma_kernel(GemmLaunch GL) {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             const int k = kk + (lane >> 4);
-            V af[TM], bf[TN];
+            VR af[TM], bf[TN];
             R as[TM], bs[TN];
 #pragma unroll
             for (int i = 0; i < TM; ++i) {
                 const int r = wm + i * 16 + (lane & 15);
-                af[i] = sA[(k * AROW + ((r / EPU) ^ swzA(k))) * EPU + r % EPU];
+                const V x = sA[(k * AROW + ((r / EPU) ^ swzA(k))) * EPU + r % EPU];
+                af[i] = mk<R>((R)x.x, (R)x.y);
                 as[i] = af[i].x + af[i].y;
             }
 #pragma unroll
             for (int j = 0; j < TN; ++j) {
                 const int n = wn + j * 16 + (lane & 15);
-                bf[j] = sB[(n * BCOL + ((k / EPU) ^ swzB(n))) * EPU + k % EPU];
+                const V x = sB[(n * BCOL + ((k / EPU) ^ swzB(n))) * EPU + k % EPU];
+                bf[j] = mk<R>((R)x.x, (R)x.y);
                 bs[j] = bf[j].x + bf[j].y;
             }
 #pragma unroll
@@ -595,9 +600,9 @@ __global__ void __launch_bounds__(NT) gemm_dma_kernel(GemmLaunch GL) {
     else run(issue4, std::integral_constant<int, 4 * (NUA + NUB)>());
     wait_vm<0>();
     __syncthreads();
-    gemm_acc_to_lds<R, BM, TM, TN>(smem, p1, p2, p3, wm, wn, lane);
+    gemm_acc_to_lds<R, BM, TM, TN>((VR*)smem, p1, p2, p3, wm, wn, lane);
     __syncthreads();
-    gemm_tile_out<R, R, BM, BN, NT>(smem, D, ws, p, ks, kslices, M, N, m0, n0, tid);
+    gemm_tile_out<S, R, BM, BN, NT>((const VR*)smem, D, ws, p, ks, kslices, M, N, m0, n0, tid);
 }
 
 // Products of at most 16 columns per tile (one-RHS and few-RHS solve passes),
@@ -910,10 +915,10 @@ void launch_gemm_groups(const GemmGroup* grp, int ngroups, hipStream_t st, int t
     if (tile == GEMM_TILE_SKINNY && !std::is_same<S, R>::value) tile = GEMM_TILE_64x16;
     if (tile == GEMM_TILE_GEMV && (!std::is_same<S, R>::value || tile_N > GEMM_GEMV_MAX_N)) tile = GEMM_TILE_64x16;
     if ((tile == GEMM_TILE_128x64 || tile == GEMM_TILE_64x128) && !std::is_same<R, float>::value) tile = GEMM_TILE_64x64;
-    // the direct-to-LDS tiles: products with S = R only; complex128 ones without the
-    // 64 x 128 and 128 x 128 tiles (64 x 64 under these choices)
-    if (tile >= GEMM_TILE_D64x64 && !std::is_same<S, R>::value) tile = GEMM_TILE_64x64;
-    if ((tile == GEMM_TILE_D128x128 || tile == GEMM_TILE_D64x128) && !std::is_same<R, float>::value)
+    // the direct-to-LDS tiles: complex128 operands (complex128 products or the
+    // mixed-precision merge) without the 64 x 128 and 128 x 128 tiles (64 x 64
+    // under these choices)
+    if ((tile == GEMM_TILE_D128x128 || tile == GEMM_TILE_D64x128) && !std::is_same<S, float>::value)
         tile = GEMM_TILE_D64x64;
     static const int TBM[GEMM_TILES] = {64, 32, 16, 64, 32, 64, 128, 64, 1, 128, 64, 64, 128, 64, 64, 128},
                      TBN[GEMM_TILES] = {64, 64, 64, 16, 32, 32, 16, 16, GEMM_GEMV_MAX_N, 64, 128, 64, 64, 128, 32, 128};
@@ -961,22 +966,19 @@ void launch_gemm_groups(const GemmGroup* grp, int ngroups, hipStream_t st, int t
             break;
         // (stage counts and wave layouts from timings of the solve-pass shapes)
         case GEMM_TILE_D64x64:
-            if constexpr (std::is_same<S, R>::value) run(gemm_dma_kernel<R, 64, 64, std::is_same<R, float>::value ? 3 : 2>);
+            run(gemm_dma_kernel<S, R, 64, 64, std::is_same<S, float>::value ? 3 : 2>);
             break;
         case GEMM_TILE_D64x32:
-            if constexpr (std::is_same<S, R>::value) run(gemm_dma_kernel<R, 64, 32, std::is_same<R, float>::value ? 4 : 2>);
+            run(gemm_dma_kernel<S, R, 64, 32, std::is_same<S, float>::value ? 4 : 2>);
             break;
         case GEMM_TILE_D128x64:   // eight waves of 32 x 32
-            if constexpr (std::is_same<S, R>::value)
-                run(gemm_dma_kernel<R, 128, 64, std::is_same<R, float>::value ? 3 : 2, 512, 2>, 512);
+            run(gemm_dma_kernel<S, R, 128, 64, std::is_same<S, float>::value ? 3 : 2, 512, 2>, 512);
             break;
         case GEMM_TILE_D64x128:   // eight waves of 32 x 32
-            if constexpr (std::is_same<R, float>::value && std::is_same<S, R>::value)
-                run(gemm_dma_kernel<R, 64, 128, 3, 512, 4>, 512);
+            if constexpr (std::is_same<S, float>::value) run(gemm_dma_kernel<S, R, 64, 128, 3, 512, 4>, 512);
             break;
         case GEMM_TILE_D128x128:   // eight waves of 64 x 64
-            if constexpr (std::is_same<R, float>::value && std::is_same<S, R>::value)
-                run(gemm_dma_kernel<R, 128, 128, 2, 512, 2>, 512);
+            if constexpr (std::is_same<S, float>::value) run(gemm_dma_kernel<S, R, 128, 128, 2, 512, 2>, 512);
             break;
         default: run(gemm_kernel<S, R, 64, 64>); break;
     }

@@ -38,9 +38,8 @@ struct GemmDesc {
 // slice (other products take the 64 x 16 tiles under this choice).
 // GEMM_TILE_128x64 / 64x128: complex64 products only (complex128 ones take
 // the 64 x 64 tiles under these choices).  GEMM_TILE_D*: the operands move
-// from memory straight into LDS stages (products with operands and results in
-// the precision of the products; the mixed-precision merge takes 64 x 64 under
-// these choices, complex128 products D64x64 under D64x128 and D128x128).
+// from memory straight into LDS stages in the precision they are stored in
+// (complex128 operands take D64x64 under D64x128 and D128x128).
 enum { GEMM_TILE_64x64 = 0, GEMM_TILE_32x64 = 1, GEMM_TILE_16x64 = 2, GEMM_TILE_64x16 = 3, GEMM_TILE_32x32 = 4,
        GEMM_TILE_64x32 = 5, GEMM_TILE_128x16 = 6, GEMM_TILE_SKINNY = 7, GEMM_TILE_GEMV = 8, GEMM_TILE_128x64 = 9,
        GEMM_TILE_64x128 = 10, GEMM_TILE_D64x64 = 11, GEMM_TILE_D128x64 = 12, GEMM_TILE_D64x128 = 13,
@@ -54,7 +53,7 @@ inline bool gemm_tile_distinct(int tile, int N, bool fp64, bool same = true) {
     if (tile == GEMM_TILE_GEMV) return N <= GEMM_GEMV_MAX_N;
     if (tile == GEMM_TILE_128x64 || tile == GEMM_TILE_64x128) return !fp64 && N > 16;
     if (tile == GEMM_TILE_D64x128 || tile == GEMM_TILE_D128x128) return !fp64 && same && N > 16;
-    if (tile == GEMM_TILE_D64x64 || tile == GEMM_TILE_D64x32 || tile == GEMM_TILE_D128x64) return same && N > 16;
+    if (tile == GEMM_TILE_D64x64 || tile == GEMM_TILE_D64x32 || tile == GEMM_TILE_D128x64) return N > 16;
     return true;
 }
 int gemm_tile_rule(int M, int N);
