@@ -380,9 +380,56 @@ struct MergeLayout {
     InvPlan ip;
 };
 
+// When the root's map is skipped, the root merge reads of each of its children's
+// maps only the rows and columns on the root interface (T_II for a and c, T_IN
+// and T_NI kept for the solve); the rest (T_NN, 44-56% of the map at 1024^2 and
+// 64^3) is never read.  Level 1 then forms its nodes' maps in two kinds of
+// blocks: a child's rows that land on the root interface (one run [a, a + m) of
+// its bn rows) against all columns, and the child's other rows, in runs of m,
+// against the columns on the root interface.  Returns m, with a per level-1
+// node (the same for both children), or 0 where the rows are not one run or the
+// runs do not tile bn.
+static int root_child_split(const Plan& P, std::vector<int>& a) {
+    if (P.L < 2) return 0;
+    const Level& V0 = P.lv[0];
+    const Level& V1 = P.lv[1];
+    const int b = V0.bc, bn = V1.bn;
+    std::vector<char> isI((size_t)2 * b, 0);
+    for (int q = 0; q < 2 * V0.isz; ++q) isI[V0.imap[q]] = 1;
+    int m = 0;
+    a.assign(V1.nodes, 0);
+    for (int n = 0; n < V1.nodes; ++n) {
+        const int par = n & 1;
+        for (int c = 0; c < 2; ++c) {
+            int first = -1, cnt = 0, runs = 0;
+            bool prev = false;
+            for (int r = 0; r < bn; ++r) {
+                const bool in = isI[(size_t)n * b + V1.prow[(size_t)(par * 2 + c) * bn + r]] != 0;
+                if (in) { ++cnt; if (first < 0) first = r; if (!prev) ++runs; }
+                prev = in;
+            }
+            if (runs != 1 || (m && cnt != m) || (c == 1 && first != a[n])) return 0;
+            m = cnt;
+            a[n] = first;
+        }
+        if (bn % m || a[n] % m) return 0;
+    }
+    return m;
+}
+
+// The split applies at level 1 of a tree whose root map is skipped and whose
+// full maps are not kept; tn2: the problems of the second kind.
+static int root_child_split(const Plan& P, const rs_opts& o, std::vector<int>& a, int& tn2) {
+    tn2 = 0;
+    if (!o.skip_root_rtr || (o.retain & RS_RETAIN_MAPS)) return 0;
+    const int m = root_child_split(P, a);
+    if (m) tn2 = 4 * P.lv[1].nodes * (P.lv[1].bn / m - 1);
+    return m;
+}
+
 // with_rtr: the level builds its parents' maps (every level but the root, and
 // the root too unless its map is skipped); Y exists only for them.
-static MergeLayout merge_layout(const Level& V, size_t lds_limit, bool with_rtr) {
+static MergeLayout merge_layout(const Level& V, size_t lds_limit, bool with_rtr, int tsplit = 0, int tn2 = 0) {
     MergeLayout m;
     const size_t nodes = V.nodes, i = V.isz, bn = V.bn, ii = i * i;
     size_t off = 0;
@@ -398,6 +445,10 @@ static MergeLayout merge_layout(const Level& V, size_t lds_limit, bool with_rtr)
     if (with_rtr) {
         wse = std::max(wse, gemm_ws_elems(2 * (int)nodes, 2 * ni, nbn, ni, merge_ks_target(nbn)));
         wse = std::max(wse, gemm_ws_elems(4 * (int)nodes, nbn, nbn, ni, merge_ks_target(nbn)));
+        if (tsplit) {
+            wse = std::max(wse, gemm_ws_elems(4 * (int)nodes, tsplit, nbn, ni, merge_ks_target(nbn)));
+            wse = std::max(wse, gemm_ws_elems(tn2, tsplit, tsplit, ni, merge_ks_target(tsplit)));
+        }
     }
     m.ws = take(wse * 16);
     m.ip.build((int)i, (int)nodes, ii, lds_limit);
@@ -426,6 +477,9 @@ static FactorSizes factor_sizes(const Plan& P, const LeafGeom& G, const rs_opts&
     z.invm64.assign(L, 0); z.invm32.assign(L, 0);
     z.Tfull[L] = nl * (size_t)P.nb * P.nb * 16;
     const long long gd = (long long)sizeof(GemmDesc);
+    std::vector<int> ta;
+    int tn2 = 0;
+    const int tm = root_child_split(P, o, ta, tn2);
     for (int l = L - 1; l >= 0; --l) {
         const Level& V = P.lv[l];
         const size_t nodes = V.nodes, i = V.isz, bn = V.bn;
@@ -437,10 +491,11 @@ static FactorSizes factor_sizes(const Plan& P, const LeafGeom& G, const rs_opts&
         else z.invm_t = std::max(z.invm_t, im);
         if (mixed) z.invm32[l] = im / 2;
         if (l > 0 || !o.skip_root_rtr) z.Tfull[l] = nodes * (size_t)V.P * V.P * 16;
-        const MergeLayout m = merge_layout(V, lds, l > 0 || !o.skip_root_rtr);
+        const MergeLayout m = merge_layout(V, lds, l > 0 || !o.skip_root_rtr, l == 1 ? tm : 0, l == 1 ? tn2 : 0);
         z.arena = std::max(z.arena, m.total);
         if (!m.ip.small) z.fdesc += (long long)m.ip.desc_count() * gd + (long long)m.ip.cmap_elems() * 4;
         z.fdesc += 4 * nodes * gd + ((l > 0 || !o.skip_root_rtr) ? 6 * nodes * gd : 0);
+        if (l == 1 && tm) z.fdesc += (long long)tn2 * gd;
     }
     if (!z.arena) z.arena = 256;
     for (int l = 0; l <= L; ++l) {
@@ -512,7 +567,11 @@ int Tree::factor_device(const double2* k) {
         if (need(tmap[q], keep_maps ? 0 : z.tmap[q])) return 1;
     if (solve_ptrs() != before) clear_slots();
     std::vector<MergeLayout> ml(L);
-    for (int l = 0; l < L; ++l) ml[l] = merge_layout(P.lv[l], lds_limit, l > 0 || !opts.skip_root_rtr);
+    std::vector<int> ta;
+    int tn2 = 0;
+    const int tm = root_child_split(P, opts, ta, tn2);
+    for (int l = 0; l < L; ++l)
+        ml[l] = merge_layout(P.lv[l], lds_limit, l > 0 || !opts.skip_root_rtr, l == 1 ? tm : 0, l == 1 ? tn2 : 0);
     char* ar = fscratch.as<char>();
     // full node maps of level l, and the FP64 inv(M) of level l
     auto Tfull = [&](int l) -> double2* { return keep_maps ? T[l]->as<double2>() : tmap[l & 1].as<double2>(); };
@@ -599,7 +658,46 @@ int Tree::factor_device(const double2* k) {
                     v.push_back(g);
                 }
             o.t = v.size();
-            if (l > 0 || !opts.skip_root_rtr) {  // parent RtR, scattered into the parent's node order
+            o.tm = o.tn2 = 0;
+            if (l == 1 && tm) {  // parent RtR where the root reads it (rows or columns on its interface)
+                const int* prow = d_prow[l]->as<int>();
+                o.tm = tm;
+                for (int n = 0; n < nodes; ++n)
+                    for (int c = 0; c < 2; ++c)
+                        for (int c2 = 0; c2 < 2; ++c2) {   // the rows on the root interface, all columns
+                            GemmDesc g = desc0();
+                            const double2* Tcn = Tc + (size_t)(2 * n + c) * tstride;
+                            const int a = ta[n], par = n & 1;
+                            g.A = Tcn + (size_t)bn * b + a; g.lda = b;
+                            g.B = Y + (size_t)n * 2 * i * 2 * bn + (size_t)c * i + (size_t)c2 * bn * 2 * i;
+                            g.ldb = 2 * i;
+                            g.Out = Tfull(l) + (size_t)n * Pp * Pp; g.ldo = Pp;
+                            g.rmap = prow + (size_t)(par * 2 + c) * bn + a;
+                            g.cmap = prow + (size_t)(par * 2 + c2) * bn;
+                            if (c == c2) { g.Din = Tcn + a; g.ldd = b; g.br = 1.0; }
+                            v.push_back(g);
+                        }
+                o.t2 = v.size();
+                for (int n = 0; n < nodes; ++n)
+                    for (int c = 0; c < 2; ++c)
+                        for (int c2 = 0; c2 < 2; ++c2)
+                            for (int r0 = 0; r0 < bn; r0 += tm) {   // the other rows, the columns on the root interface
+                                const int a = ta[n], par = n & 1;
+                                if (r0 == a) continue;
+                                GemmDesc g = desc0();
+                                const double2* Tcn = Tc + (size_t)(2 * n + c) * tstride;
+                                g.A = Tcn + (size_t)bn * b + r0; g.lda = b;
+                                g.B = Y + (size_t)n * 2 * i * 2 * bn + (size_t)c * i + (size_t)c2 * bn * 2 * i +
+                                      (size_t)a * 2 * i;
+                                g.ldb = 2 * i;
+                                g.Out = Tfull(l) + (size_t)n * Pp * Pp; g.ldo = Pp;
+                                g.rmap = prow + (size_t)(par * 2 + c) * bn + r0;
+                                g.cmap = prow + (size_t)(par * 2 + c2) * bn + a;
+                                if (c == c2) { g.Din = Tcn + r0 + (size_t)a * b; g.ldd = b; g.br = 1.0; }
+                                v.push_back(g);
+                            }
+                o.tn2 = (int)(v.size() - o.t2);
+            } else if (l > 0 || !opts.skip_root_rtr) {  // parent RtR, scattered into the parent's node order
                 const int* prow = d_prow[l]->as<int>();
                 for (int n = 0; n < nodes; ++n) {
                     const int par = (l == 0) ? 0 : (n & 1);
@@ -652,8 +750,8 @@ int Tree::factor_device(const double2* k) {
     // the diagonal terms of W and invM_12 are added by kernels of their own),
     // so the first factorization of a plan times their tile shapes and keeps the
     // fastest; later factorizations reuse the choice (results do not depend on it).
-    const bool tune = opts.autotune && (int)ftile.size() != 5 * L;
-    if (tune) ftile.assign(5 * L, -1);
+    const bool tune = opts.autotune && (int)ftile.size() != 6 * L;
+    if (tune) ftile.assign(6 * L, -1);
     hipEvent_t te0 = nullptr, te1 = nullptr;
     if (tune) {
         HIPCHK(hipEventCreate(&te0));
@@ -675,9 +773,9 @@ int Tree::factor_device(const double2* k) {
         if (tune) {
             float best = 0.f;
             for (int t = 0; t < GEMM_TILES; ++t) {
-                if (!gemm_tile_distinct(t, N, site / 5 < c64_from, site / 5 < c64_from)) continue;
+                if (!gemm_tile_distinct(t, N, site / 6 < c64_from, site / 6 < c64_from)) continue;
                 (void)hipEventRecord(te0, stream);
-                gemm(site / 5, d, np, M, N, K, gws, t);
+                gemm(site / 6, d, np, M, N, K, gws, t);
                 (void)hipEventRecord(te1, stream);
                 if (hipEventSynchronize(te1) != hipSuccess) return fail("Error: GEMM autotuning failed.");
                 float ms = 0.f;
@@ -686,7 +784,7 @@ int Tree::factor_device(const double2* k) {
             }
             ftile[site] = tile;
         }
-        gemm(site / 5, d, np, M, N, K, gws, tile);
+        gemm(site / 6, d, np, M, N, K, gws, tile);
         return 0;
     };
     for (int l = L - 1; l >= 0; --l) {
@@ -708,19 +806,24 @@ int Tree::factor_device(const double2* k) {
         launch_form_iface(Am, Tc, tstride, 2, 0, b, bn, i, nodes, zeta, make_double2(0, 0), nullptr, stream);
         launch_form_iface(Cm, Tc, tstride, 2, 1, b, bn, i, nodes, zeta, make_double2(0, 0), nullptr, stream);
         // Wi = (nu^2 I - a c)^-1
-        if (fgemm(5 * l + 0, fd + o.w, nodes, i, i, i, gws)) return 1;
+        if (fgemm(6 * l + 0, fd + o.w, nodes, i, i, i, gws)) return 1;
         launch_add_diag(Wm, ii, i, i, nodes, nu2, stream);
         launch_inverse(m.ip, Wm, piv, GW, fd + o.gj, info.as<int>(), stream);
         // invM = [-c Wi, I / nu - (1 / nu) (-c Wi) a; nu Wi, -Wi a]
-        if (fgemm(5 * l + 1, fd + o.mb, 2 * nodes, i, i, i, gws) || fgemm(5 * l + 2, fd + o.m12, nodes, i, i, i, gws))
+        if (fgemm(6 * l + 1, fd + o.mb, 2 * nodes, i, i, i, gws) || fgemm(6 * l + 2, fd + o.m12, nodes, i, i, i, gws))
             return 1;
         launch_add_diag(IM + (size_t)i * 2 * i, 4 * ii, 2 * i, i, nodes, rnu, stream);
         launch_copy_blocks(IM + i, 4 * ii, 2 * i, Wm, ii, i, i, i, nodes, nu, stream);
         // Y = invM diag(T1_IN, T2_IN); parent T = diag(T_NN) + diag(T_NI) Y
         // (neither at a root whose map is skipped: nothing reads them)
-        if ((l > 0 || !opts.skip_root_rtr) && (fgemm(5 * l + 3, fd + o.y, 2 * nodes, 2 * i, bn, i, gws) ||
-                                               fgemm(5 * l + 4, fd + o.t, 4 * nodes, bn, bn, i, gws)))
+        if ((l > 0 || !opts.skip_root_rtr) && fgemm(6 * l + 3, fd + o.y, 2 * nodes, 2 * i, bn, i, gws)) return 1;
+        if (o.tm) {
+            if (fgemm(6 * l + 4, fd + o.t, 4 * nodes, o.tm, bn, i, gws) ||
+                fgemm(6 * l + 5, fd + o.t2, o.tn2, o.tm, o.tm, i, gws))
+                return 1;
+        } else if ((l > 0 || !opts.skip_root_rtr) && fgemm(6 * l + 4, fd + o.t, 4 * nodes, bn, bn, i, gws)) {
             return 1;
+        }
         // the solve data of this merge: inv(M), and the children's T_NI / T_IN,
         // before the next level reuses the buffers they come from
         if (mixed) launch_to_c64(invM32[l]->as<float2>(), IM, (size_t)nodes * 4 * ii, stream);

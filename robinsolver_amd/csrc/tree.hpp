@@ -187,11 +187,14 @@ struct Tree {
     DevBuf tmap[2];
     DevBuf invM_t;                // FP64 inv(M) of one level (mixed trees without FP64 solve data)
     DevBuf fscratch;              // merge scratch of the largest level
-    std::vector<int> ftile;       // per level, 5 merge products: GEMM tile chosen by the first factor's timing
+    std::vector<int> ftile;       // per level, 6 merge product sites: GEMM tile chosen by the first factor's timing
     // the merge's product descriptors of every level in one array (fdesc), the
     // Gauss-Jordan column maps in another (fcmap); per level the offsets of
     // each product's descriptors (elements)
-    struct MergeDescs { size_t gj = 0, w = 0, mb = 0, m12 = 0, y = 0, t = 0, cm = 0; };
+    // tm > 0: the level's parent maps are formed only where the root reads them
+    // (rows or columns on the root interface, tm of a child's rows; tn2 blocks of
+    // the rest, from t2)
+    struct MergeDescs { size_t gj = 0, w = 0, mb = 0, m12 = 0, y = 0, t = 0, cm = 0, t2 = 0; int tm = 0, tn2 = 0; };
     DevBuf fdesc, fcmap;
     std::vector<MergeDescs> fdoff;
     bool have32 = false;

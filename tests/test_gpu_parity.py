@@ -65,17 +65,25 @@ def _pair(d, n, L, kb, bp, skip_root=True):
 
 @pytest.mark.parametrize("d,n,L,kb,bp", CASES)
 def test_fp64_solve_matches_oracle(d, n, L, kb, bp, gpu):
+    """Both the tree that keeps its full maps and a tree with the default
+    retention, whose level 1 forms its maps only where the root merge reads
+    them (the root interface's rows and columns)."""
     _case(("fp64_solve", d, n, L, kb, bp is not None))
     o, t, pts, face = _pair(d, n, L, kb, bp)
+    t2 = rsa.GpuTree(d, L, UNIT, 1.0 / n, kb, bndry_params=bp)
+    t2.factor(_k(pts[:, o.nb:], kb))
     rng = _rng(11 + n)
     nrhs = 3
     f = _rand_c(rng, (nrhs, o.n_leaves, o.n_int))
     g = _rand_c(rng, (nrhs, o.n_leaves, o.nb))
     u = t.solve(f, g)
+    u2 = t2.solve(f, g)
     for r in range(nrhs):
         u_ref = o.solve(f[r], g[r])
         rel = _rel(u[r], u_ref)
         assert rel <= TOL64, (r, rel)
+        rel2 = _rel(u2[r], u_ref)
+        assert rel2 <= TOL64, (r, rel2)
 
 
 @pytest.mark.parametrize("d,n,L,kb,bp", [CASES[0], CASES[2], CASES[4], CASES[6], CASES[8], CASES[9]])
