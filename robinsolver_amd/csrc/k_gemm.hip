@@ -353,14 +353,18 @@ template <int n> __device__ inline void wait_vm() {
 
 // Copies from global memory straight into LDS: each lane's 16 (4) bytes at g
 // land at LDS byte address lds + 16 (4) x lane; lds is the same in every lane.
-// Written as asm (with the LDS base in M0), so that the compiler keeps no
-// count of them: the kernel waits for them itself (wait_vm) before the
-// barrier after which they are read.
+// Written as asm (with the LDS base in M0, one wait state between setting M0
+// and the copy, as the compiler leaves for its own LDS copies), so that the
+// compiler keeps no count of them: the kernel waits for them itself (wait_vm)
+// before the barrier after which they are read.  No other instruction of
+// these kernels uses M0.
 __device__ inline void lds_copy16(const void* g, unsigned lds) {
-    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds) : "memory", "m0");
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds)
+                 : "memory", "m0");
 }
 __device__ inline void lds_copy4(const void* g, unsigned lds) {
-    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off" ::"v"(g), "s"(lds) : "memory", "m0");
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off" ::"v"(g), "s"(lds)
+                 : "memory", "m0");
 }
 // workgroup barrier after this wave's LDS reads (lgkmcnt(0)), without the
 // release fence's wait for outstanding global loads
