@@ -240,11 +240,17 @@ template <typename T_> int Tree::upload(DevBuf& b, const std::vector<T_>& v) {
     return 0;
 }
 
+void Tree::drop_factor_graph() {
+    if (fgraph) (void)hipGraphExecDestroy(fgraph);
+    fgraph = nullptr;
+}
+
 void Tree::free_factor() {
     auto drop = [&](std::vector<DevBuf*>& v) {
         for (DevBuf* b : v) delete b;
         v.clear();
     };
+    drop_factor_graph();
     drop(T); drop(Tn); drop(Tn32); drop(invM); drop(invM32);
     ftile.clear();
     fdoff.clear();
@@ -726,25 +732,7 @@ int Tree::factor_device(const double2* k) {
     if (join_default_in(stream)) return 1;
     HIPCHK(hipMemcpyAsync(kint.p, k, (size_t)nl * P.n_int * sizeof(double2), hipMemcpyDeviceToDevice, stream));
     HIPCHK(hipMemsetAsync(info.p, 0, sizeof(int), stream));
-    launch_leaf_blocks(G, kint.as<double2>(), sinv.as<double2>(), info.as<int>(), nl, stream);
-    if (mixed) launch_to_c64(sinv32.as<float2>(), sinv.as<double2>(), z.sinv64 / 16, stream);
-    // The leaf maps feed the bottom merge.  When that merge runs in complex64
-    // (a mixed tree with complex64 merges down to the leaves) the maps come from
-    // the complex64 block inverses on the matrix cores (one workgroup of 64
-    // columns per leaf); otherwise they are FP64.
     const int c64_from = (mixed && opts.merge_fp64_levels >= 0) ? opts.merge_fp64_levels : L;
-    if (L > 0 && c64_from <= L - 1 && leaf_mfma_factor_ok(G)) {
-        LeafArgs<float> a;
-        a.mode = LEAF_FACTOR_T; a.ncols = P.nb; a.n_leaves = nl;
-        a.sinv = sinv32.as<float2>(); a.T = Tfull(L);
-        launch_leaf_apply<float>(G, a, lds_limit, stream);
-    } else {
-        LeafArgs<double> a;
-        a.mode = LEAF_FACTOR_T; a.ncols = P.nb; a.n_leaves = nl;
-        a.sinv = sinv.as<double2>(); a.T = Tfull(L);
-        launch_leaf_apply<double>(G, a, lds_limit, stream);
-    }
-    HIPCHK(hipGetLastError());
     // merges, bottom-up; every level of one launch sequence on the tree's stream.
     // The merge products are idempotent (each writes a buffer it does not read;
     // the diagonal terms of W and invM_12 are added by kernels of their own),
@@ -787,49 +775,100 @@ int Tree::factor_device(const double2* k) {
         gemm(site / 6, d, np, M, N, K, gws, tile);
         return 0;
     };
-    for (int l = L - 1; l >= 0; --l) {
-        const Level& V = P.lv[l];
-        const MergeLayout& m = ml[l];
-        const int nodes = V.nodes, i = V.isz, b = V.bc, bn = V.bn;
-        const size_t ii = (size_t)i * i, tstride = (size_t)b * b;
-        const double2* Tc = Tfull(l + 1);
-        double2* Am = (double2*)(ar + m.A);
-        double2* Cm = (double2*)(ar + m.C);
-        double2* Wm = (double2*)(ar + m.W);
-        double2* GW = (double2*)(ar + m.GW);
-        int* piv = (int*)(ar + m.piv);
-        void* gws = ar + m.ws;
-        double2* IM = IMf(l);
-        const MergeDescs& o = fdoff[l];
-        const GemmDesc* fd = fdesc.as<GemmDesc>();
-        // a = zeta I - T1_II, c = zeta I - T2_II
-        launch_form_iface(Am, Tc, tstride, 2, 0, b, bn, i, nodes, zeta, make_double2(0, 0), nullptr, stream);
-        launch_form_iface(Cm, Tc, tstride, 2, 1, b, bn, i, nodes, zeta, make_double2(0, 0), nullptr, stream);
-        // Wi = (nu^2 I - a c)^-1
-        if (fgemm(6 * l + 0, fd + o.w, nodes, i, i, i, gws)) return 1;
-        launch_add_diag(Wm, ii, i, i, nodes, nu2, stream);
-        launch_inverse(m.ip, Wm, piv, GW, fd + o.gj, info.as<int>(), stream);
-        // invM = [-c Wi, I / nu - (1 / nu) (-c Wi) a; nu Wi, -Wi a]
-        if (fgemm(6 * l + 1, fd + o.mb, 2 * nodes, i, i, i, gws) || fgemm(6 * l + 2, fd + o.m12, nodes, i, i, i, gws))
-            return 1;
-        launch_add_diag(IM + (size_t)i * 2 * i, 4 * ii, 2 * i, i, nodes, rnu, stream);
-        launch_copy_blocks(IM + i, 4 * ii, 2 * i, Wm, ii, i, i, i, nodes, nu, stream);
-        // Y = invM diag(T1_IN, T2_IN); parent T = diag(T_NN) + diag(T_NI) Y
-        // (neither at a root whose map is skipped: nothing reads them)
-        if ((l > 0 || !opts.skip_root_rtr) && fgemm(6 * l + 3, fd + o.y, 2 * nodes, 2 * i, bn, i, gws)) return 1;
-        if (o.tm) {
-            if (fgemm(6 * l + 4, fd + o.t, 4 * nodes, o.tm, bn, i, gws) ||
-                fgemm(6 * l + 5, fd + o.t2, o.tn2, o.tm, o.tm, i, gws))
-                return 1;
-        } else if ((l > 0 || !opts.skip_root_rtr) && fgemm(6 * l + 4, fd + o.t, 4 * nodes, bn, bn, i, gws)) {
-            return 1;
+    // The launches from the leaves to the last merge, as one sequence.  A refactor
+    // in place (RS_RETAIN_SCRATCH, nothing reallocated, the tiles tuned) replays
+    // it from a graph captured on the first such refactor: about 650 launches,
+    // most of them short (the Gauss-Jordan panels), whose gaps the graph removes.
+    // The graph reads k through kint and clears nothing (info is cleared above).
+    auto launch_seq = [&]() -> int {
+        launch_leaf_blocks(G, kint.as<double2>(), sinv.as<double2>(), info.as<int>(), nl, stream);
+        if (mixed) launch_to_c64(sinv32.as<float2>(), sinv.as<double2>(), z.sinv64 / 16, stream);
+        // The leaf maps feed the bottom merge.  When that merge runs in complex64
+        // (a mixed tree with complex64 merges down to the leaves) the maps come from
+        // the complex64 block inverses on the matrix cores (one workgroup of 64
+        // columns per leaf); otherwise they are FP64.
+        if (L > 0 && c64_from <= L - 1 && leaf_mfma_factor_ok(G)) {
+            LeafArgs<float> a;
+            a.mode = LEAF_FACTOR_T; a.ncols = P.nb; a.n_leaves = nl;
+            a.sinv = sinv32.as<float2>(); a.T = Tfull(L);
+            launch_leaf_apply<float>(G, a, lds_limit, stream);
+        } else {
+            LeafArgs<double> a;
+            a.mode = LEAF_FACTOR_T; a.ncols = P.nb; a.n_leaves = nl;
+            a.sinv = sinv.as<double2>(); a.T = Tfull(L);
+            launch_leaf_apply<double>(G, a, lds_limit, stream);
         }
-        // the solve data of this merge: inv(M), and the children's T_NI / T_IN,
-        // before the next level reuses the buffers they come from
-        if (mixed) launch_to_c64(invM32[l]->as<float2>(), IM, (size_t)nodes * 4 * ii, stream);
-        launch_extract_ni_in(Tc, tstride, b, bn, i, 2 * nodes, keep64 ? Tn[l + 1]->as<double2>() : nullptr,
-                             mixed ? Tn32[l + 1]->as<float2>() : nullptr, stream);
         HIPCHK(hipGetLastError());
+        for (int l = L - 1; l >= 0; --l) {
+            const Level& V = P.lv[l];
+            const MergeLayout& m = ml[l];
+            const int nodes = V.nodes, i = V.isz, b = V.bc, bn = V.bn;
+            const size_t ii = (size_t)i * i, tstride = (size_t)b * b;
+            const double2* Tc = Tfull(l + 1);
+            double2* Am = (double2*)(ar + m.A);
+            double2* Cm = (double2*)(ar + m.C);
+            double2* Wm = (double2*)(ar + m.W);
+            double2* GW = (double2*)(ar + m.GW);
+            int* piv = (int*)(ar + m.piv);
+            void* gws = ar + m.ws;
+            double2* IM = IMf(l);
+            const MergeDescs& o = fdoff[l];
+            const GemmDesc* fd = fdesc.as<GemmDesc>();
+            // a = zeta I - T1_II, c = zeta I - T2_II
+            launch_form_iface(Am, Tc, tstride, 2, 0, b, bn, i, nodes, zeta, make_double2(0, 0), nullptr, stream);
+            launch_form_iface(Cm, Tc, tstride, 2, 1, b, bn, i, nodes, zeta, make_double2(0, 0), nullptr, stream);
+            // Wi = (nu^2 I - a c)^-1
+            if (fgemm(6 * l + 0, fd + o.w, nodes, i, i, i, gws)) return 1;
+            launch_add_diag(Wm, ii, i, i, nodes, nu2, stream);
+            launch_inverse(m.ip, Wm, piv, GW, fd + o.gj, info.as<int>(), stream);
+            // invM = [-c Wi, I / nu - (1 / nu) (-c Wi) a; nu Wi, -Wi a]
+            if (fgemm(6 * l + 1, fd + o.mb, 2 * nodes, i, i, i, gws) || fgemm(6 * l + 2, fd + o.m12, nodes, i, i, i, gws))
+                return 1;
+            launch_add_diag(IM + (size_t)i * 2 * i, 4 * ii, 2 * i, i, nodes, rnu, stream);
+            launch_copy_blocks(IM + i, 4 * ii, 2 * i, Wm, ii, i, i, i, nodes, nu, stream);
+            // Y = invM diag(T1_IN, T2_IN); parent T = diag(T_NN) + diag(T_NI) Y
+            // (neither at a root whose map is skipped: nothing reads them)
+            if ((l > 0 || !opts.skip_root_rtr) && fgemm(6 * l + 3, fd + o.y, 2 * nodes, 2 * i, bn, i, gws)) return 1;
+            if (o.tm) {
+                if (fgemm(6 * l + 4, fd + o.t, 4 * nodes, o.tm, bn, i, gws) ||
+                    fgemm(6 * l + 5, fd + o.t2, o.tn2, o.tm, o.tm, i, gws))
+                    return 1;
+            } else if ((l > 0 || !opts.skip_root_rtr) && fgemm(6 * l + 4, fd + o.t, 4 * nodes, bn, bn, i, gws)) {
+                return 1;
+            }
+            // the solve data of this merge: inv(M), and the children's T_NI / T_IN,
+            // before the next level reuses the buffers they come from
+            if (mixed) launch_to_c64(invM32[l]->as<float2>(), IM, (size_t)nodes * 4 * ii, stream);
+            launch_extract_ni_in(Tc, tstride, b, bn, i, 2 * nodes, keep64 ? Tn[l + 1]->as<double2>() : nullptr,
+                                 mixed ? Tn32[l + 1]->as<float2>() : nullptr, stream);
+            HIPCHK(hipGetLastError());
+        }
+        return 0;
+    };
+    const FactorKey key = {mixed, keep64, keep_maps, opts.skip_root_rtr != 0, c64_from};
+    const bool use_graph = opts.graphs && keep_scratch && !moved && !tune;
+    if (!use_graph || !(fkey == key)) drop_factor_graph();
+    if (!use_graph) {
+        if (launch_seq()) return 1;
+    } else {
+        if (!fgraph) {
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            const int rc = launch_seq();
+            hipGraph_t gr = nullptr;
+            const hipError_t e = hipStreamEndCapture(stream, &gr);
+            if (rc || e != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return rc ? 1 : fail(std::string("HIP error: stream capture: ") + hipGetErrorString(e));
+            }
+            const hipError_t e2 = hipGraphInstantiate(&fgraph, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e2 != hipSuccess) {
+                fgraph = nullptr;
+                return fail(std::string("HIP error: graph instantiation: ") + hipGetErrorString(e2));
+            }
+            fkey = key;
+        }
+        HIPCHK(hipGraphLaunch(fgraph, stream));
     }
     if (te0) (void)hipEventDestroy(te0);
     if (te1) (void)hipEventDestroy(te1);
@@ -840,6 +879,7 @@ int Tree::factor_device(const double2* k) {
     if (hinfo & 2) return fail("Error: singular interface matrix in the merge.");
     // the working memory goes unless it is to be kept for refactors
     if (!keep_scratch) {
+        drop_factor_graph();
         fscratch.release(); invM_t.release(); tmap[0].release(); tmap[1].release();
         if (!keep64) sinv.release();
         fdesc.release(); fcmap.release();

@@ -188,6 +188,18 @@ struct Tree {
     DevBuf invM_t;                // FP64 inv(M) of one level (mixed trees without FP64 solve data)
     DevBuf fscratch;              // merge scratch of the largest level
     std::vector<int> ftile;       // per level, 6 merge product sites: GEMM tile chosen by the first factor's timing
+    // the launch sequence of an in-place refactor, captured, and the options it was captured under
+    struct FactorKey {
+        bool mixed = false, keep64 = false, keep_maps = false, skip_root = false;
+        int c64_from = 0;
+        bool operator==(const FactorKey& o) const {
+            return mixed == o.mixed && keep64 == o.keep64 && keep_maps == o.keep_maps && skip_root == o.skip_root &&
+                   c64_from == o.c64_from;
+        }
+    };
+    hipGraphExec_t fgraph = nullptr;
+    FactorKey fkey;
+    void drop_factor_graph();
     // the merge's product descriptors of every level in one array (fdesc), the
     // Gauss-Jordan column maps in another (fcmap); per level the offsets of
     // each product's descriptors (elements)
