@@ -15,9 +15,11 @@
 // (|Ar|+|Ai|)(|Br|+|Bi|) in place of |A||B|, at most twice as large.  A 256-thread workgroup computes a 64 x BN
 // output tile from 64 x 16 and 16 x BN operand tiles staged in LDS; BN = 64 for
 // wide products (merge), BN = 16 for the few-RHS solve passes, which then stay
-// HBM-bound rather than MFMA-bound.  The K reduction of every output column is
-// a fixed sequence that does not depend on N or on the other columns, so a
-// right-hand side gets the same bits alone or batched with others.
+// HBM-bound rather than MFMA-bound; gemm_dma_kernel does the same with its
+// operands copied from memory straight into LDS stages (up to 128 x 128 tiles,
+// 256 or 512 threads).  The K reduction of every output column is a fixed
+// sequence that does not depend on N, on the other columns or on the tile, so
+// a right-hand side gets the same bits alone or batched with others.
 //
 // Each problem of a batch may scatter its output rows/columns through index
 // maps (the face permutation between a child's and its parent's boundary
@@ -393,8 +395,12 @@ __device__ inline void bare_barrier() {
 // swizzle is applied to the global addresses.  Elements beyond the K range
 // come from g_zero16; rows beyond M and columns beyond N from in-range data
 // (their products land in output rows and columns that are not stored).
-// Operands that cannot be moved in 16-byte units (a row-mapped B, odd leading
-// dimensions or offsets) are moved in 4-byte pieces into the same layout.
+// A row-mapped B (the down pass's incoming rows) moves in 16-byte units too when
+// the map sends every row pair to two contiguous rows starting at an even one
+// (GemmDesc::bpair; complex128 units are single rows): the map over the slice's
+// K range is first copied into LDS.  Operands that cannot move in 16-byte units
+// (other row maps, odd leading dimensions or offsets) move in 4-byte pieces
+// into the same layout.
 template <typename S, typename R, int BM, int BN, int NS, int NT = 256, int WAVES_N = 2>
 __global__ void __launch_bounds__(NT) gemm_dma_kernel(GemmLaunch GL) {
     using V = c_t<S>;    // operands as stored and staged
@@ -414,7 +420,7 @@ __global__ void __launch_bounds__(NT) gemm_dma_kernel(GemmLaunch GL) {
     constexpr int CT = (BN * (BM + 1) * (int)sizeof(VR) + (int)sizeof(V) - 1) / (int)sizeof(V);   // output tile
     constexpr int SMEM = NS * STAGE > CT ? NS * STAGE : CT;
     __shared__ __align__(16) V smem[SMEM];
-    constexpr int PBL = 512;   // pairs of a row-mapped B's K range held in LDS
+    constexpr int PBL = 512;   // map entries of a row-mapped B (one per 16-byte unit) held in LDS
     __shared__ int pbl[PBL];
 
     const GemmDesc* descs = GL.g[0].descs;
