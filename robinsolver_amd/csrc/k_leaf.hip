@@ -425,8 +425,9 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
 // four partial sums meet in LDS and are added in layer_dot's order, and every
 // other step is leaf_apply_kernel's own arithmetic, so a column gets the same
 // bits from either kernel.  The forward sweep's blocks 0..K-1 and the
-// backward sweep's K-2..0 form one stream of 2K - 1 steps.
-template <int CW>
+// backward sweep's K-2..0 form one stream of 2K - 1 steps.  TAIL: s is not a
+// multiple of four (the lanes q = 0 also hold the columns past the last one).
+template <int CW, bool TAIL>
 __global__ void __launch_bounds__(64) leaf_wave_kernel(LeafGeom G, LeafCoef C, LeafIO<double> io) {
     using V = double2;
     constexpr int PF = 4;   // steps of blocks in flight
@@ -451,6 +452,10 @@ __global__ void __launch_bounds__(64) leaf_wave_kernel(LeafGeom G, LeafCoef C, L
     };
     // step j's block: the forward sweep's j, then the backward sweep's 2K - 2 - j
     auto blk = [&](int j) { return j < K ? j : 2 * K - 2 - j; };
+    // An entry outside the block (w2 past the last multiple of four, or a
+    // column past s) loads column 0 and is never read: step() skips it by the
+    // same conditions.  Zeroing it here instead would make every load wait at
+    // once for its data and leave no block in flight.
     V br[PF][4], bt[PF][3];
     auto load = [&](int j, V (&r)[4], V (&t)[3]) {
         const V* Bk = sl + (size_t)blk(min(j, nsteps - 1)) * ss;
@@ -459,14 +464,14 @@ __global__ void __launch_bounds__(64) leaf_wave_kernel(LeafGeom G, LeafCoef C, L
         for (int u = 0; u < 4; ++u) {
             const int w2 = q + 4 * u;
             r[u] = gld(Bk + wl + (size_t)(w2 < s4 ? w2 : 0) * s);
-            if (w2 >= s4) r[u] = cz<double>();
         }
+        if constexpr (TAIL) {
 #pragma unroll
-        for (int u = 0; u < 3; ++u) {
-            const int w2 = s4 + u;
-            const bool ok = q == 0 && w2 < s;
-            t[u] = gld(Bk + wl + (size_t)(ok ? w2 : 0) * s);
-            if (!ok) t[u] = cz<double>();
+            for (int u = 0; u < 3; ++u) {
+                const int w2 = s4 + u;
+                const bool ok = q == 0 && w2 < s;
+                t[u] = gld(Bk + wl + (size_t)(ok ? w2 : 0) * s);
+            }
         }
     };
 #pragma unroll
@@ -508,7 +513,7 @@ __global__ void __launch_bounds__(64) leaf_wave_kernel(LeafGeom G, LeafCoef C, L
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
                     if (q + 4 * u < s4) a = cfma(r[u], Vt[(q + 4 * u) * CW + c], a);
-                if (q == 0) {
+                if (TAIL && q == 0) {
 #pragma unroll
                     for (int u = 0; u < 3; ++u)
                         if (s4 + u < s) a = cfma(t[u], Vt[(s4 + u) * CW + c], a);
@@ -541,10 +546,8 @@ __global__ void __launch_bounds__(64) leaf_wave_kernel(LeafGeom G, LeafCoef C, L
 #pragma unroll
         for (int p = 0; p < PF; ++p) {
             const int j = j0 + p;
-            if (j < nsteps) {
-                step(j, br[p], bt[p]);
-                if (j + PF < nsteps) load(j + PF, br[p], bt[p]);
-            }
+            if (j < nsteps) step(j, br[p], bt[p]);
+            load(j + PF, br[p], bt[p]);   // past the last step: the last block again (never read)
         }
     }
 
@@ -1506,8 +1509,11 @@ void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit
             io.CH = cw;
             const size_t lds = ((size_t)G.n_int * cw + 16 * cw + 64 * cw) * sizeof(double2);
             dim3 grid(n_leaves, (ncols + cw - 1) / cw);
-            if (cw == 1) hipLaunchKernelGGL(leaf_wave_kernel<1>, grid, dim3(64), lds, st, G, C, io);
-            else hipLaunchKernelGGL(leaf_wave_kernel<4>, grid, dim3(64), lds, st, G, C, io);
+            const bool tail = G.s % 4 != 0;
+            if (cw == 1 && tail) hipLaunchKernelGGL((leaf_wave_kernel<1, true>), grid, dim3(64), lds, st, G, C, io);
+            else if (cw == 1) hipLaunchKernelGGL((leaf_wave_kernel<1, false>), grid, dim3(64), lds, st, G, C, io);
+            else if (tail) hipLaunchKernelGGL((leaf_wave_kernel<4, true>), grid, dim3(64), lds, st, G, C, io);
+            else hipLaunchKernelGGL((leaf_wave_kernel<4, false>), grid, dim3(64), lds, st, G, C, io);
             return;
         }
     }
