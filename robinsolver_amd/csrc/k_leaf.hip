@@ -307,7 +307,21 @@ __device__ inline V layer_dot(const V* __restrict__ B, const V* __restrict__ Vt,
     return cadd(cadd(a0, a1), cadd(a2, a3));
 }
 
-template <typename R>
+// A workgroup barrier that waits for the wave's LDS accesses only: the
+// global loads a thread has in flight into its registers stay in flight
+// (__syncthreads also waits for those).
+__device__ inline void lds_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4));   // lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// EPT > 0: each thread holds up to EPT elements of the next layer's block in
+// registers, loaded while the current layer is computed (s^2 <= EPT x 256);
+// the layer barriers then wait for LDS accesses only.  EPT = 0: each layer's
+// block is read when the layer starts.  The arithmetic is the same.
+template <typename R, int EPT>
 __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C, LeafIO<R> io) {
     using V = c_t<R>;
     extern __shared__ unsigned char smraw[];
@@ -353,6 +367,50 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
     }
     __syncthreads();
 
+    if constexpr (EPT > 0) {
+        // the forward sweep's blocks 0..K-1 and the backward sweep's K-2..0 as
+        // one stream of 2K - 1 layer steps (the last step loads its own block
+        // again; every load is unconditional, with the index clamped)
+        V pre[EPT];
+        int pidx[EPT];
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) pidx[i] = min(tid + i * 256, ss - 1);
+        const int nsteps = 2 * K - 1;
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) pre[i] = gld(sl + pidx[i]);
+        for (int j = 0; j < nsteps; ++j) {
+            const int k = j < K ? j : 2 * K - 2 - j;
+#pragma unroll
+            for (int i = 0; i < EPT; ++i)
+                if (tid + i * 256 < ss) B[tid + i * 256] = pre[i];
+            const int j1 = min(j + 1, nsteps - 1);
+            const V* src = sl + (size_t)(j1 < K ? j1 : 2 * K - 2 - j1) * ss;
+#pragma unroll
+            for (int i = 0; i < EPT; ++i) pre[i] = gld(src + pidx[i]);
+            if (j < K) {   // 2. forward: y_k = Shat_k^-1 (r_k - lambda_k y_{k-1})
+                V lam = ccast<V>(slow_lambda(G, C, k));
+                for (int e = tid; e < s * CH; e += nt) {
+                    V v = Y[(size_t)k * s * CH + e];
+                    if (k > 0) v = csub(v, cmul(lam, Y[(size_t)(k - 1) * s * CH + e]));
+                    Vt[e] = v;
+                }
+                lds_barrier();
+                for (int e = tid; e < s * CH; e += nt) {
+                    int w = e / CH, c = e % CH;
+                    Y[(size_t)k * s * CH + e] = layer_dot(B, Vt, s, CH, w, c);
+                }
+            } else {       // 3. backward: x_k = y_k - mu_k Shat_k^-1 x_{k+1}
+                V mu = ccast<V>(slow_mu(G, C, k));
+                for (int e = tid; e < s * CH; e += nt) Vt[e] = cmul(mu, Y[(size_t)(k + 1) * s * CH + e]);
+                lds_barrier();
+                for (int e = tid; e < s * CH; e += nt) {
+                    int w = e / CH, c = e % CH;
+                    Y[(size_t)k * s * CH + e] = csub(Y[(size_t)k * s * CH + e], layer_dot(B, Vt, s, CH, w, c));
+                }
+            }
+            lds_barrier();
+        }
+    } else {
     // 2. forward sweep: y_k = Shat_k^-1 (r_k - lambda_k y_{k-1})
     for (int k = 0; k < K; ++k) {
         for (int e = tid; e < ss; e += nt) B[e] = sl[(size_t)k * ss + e];
@@ -380,6 +438,7 @@ __global__ void __launch_bounds__(256) leaf_apply_kernel(LeafGeom G, LeafCoef C,
             Y[(size_t)k * s * CH + e] = csub(Y[(size_t)k * s * CH + e], layer_dot(B, Vt, s, CH, w, c));
         }
         __syncthreads();
+    }
     }
     // 4. boundary values and outputs
     const double2 sma = make_double2(0.0, -2.0 * G.kb);  // sigma - alpha
@@ -1519,7 +1578,11 @@ void launch_leaf_apply(const LeafGeom& G, const LeafArgs<R>& a, size_t lds_limit
     }
     size_t lds = ((size_t)G.n_int * io.CH + (size_t)G.s * G.s + (size_t)G.s * io.CH) * sizeof(c_t<R>);
     dim3 grid(n_leaves, (ncols + io.CH - 1) / io.CH);
-    hipLaunchKernelGGL(leaf_apply_kernel<R>, grid, dim3(256), lds, st, G, C, io);
+    const int ss = G.s * G.s;
+    if (ss <= 256) hipLaunchKernelGGL((leaf_apply_kernel<R, 1>), grid, dim3(256), lds, st, G, C, io);
+    else if (ss <= 1024) hipLaunchKernelGGL((leaf_apply_kernel<R, 4>), grid, dim3(256), lds, st, G, C, io);
+    else if (ss <= 4096) hipLaunchKernelGGL((leaf_apply_kernel<R, 16>), grid, dim3(256), lds, st, G, C, io);
+    else hipLaunchKernelGGL((leaf_apply_kernel<R, 0>), grid, dim3(256), lds, st, G, C, io);
 }
 
 template int leaf_apply_chunk<double>(const LeafGeom&, int, size_t);
