@@ -301,7 +301,12 @@ int check_gemm_t(int tile, int M, int N, int K, const double* A, int lda, const 
     (void)hipMemcpy(dd, &d, sizeof d, hipMemcpyHostToDevice);
     void* ws = nullptr;
     size_t wse = rs::gemm_ws_elems(1, M, N, K);
-    if (wse) (void)hipMalloc(&ws, wse * sizeof(rs::c_t<R>));
+    // (without its workspace the product would run unsplit: other bits than the tree's)
+    if (wse && hipMalloc(&ws, wse * sizeof(rs::c_t<R>)) != hipSuccess) {
+        cleanup();
+        g_err = "Error: hipMalloc failed in rs_check_gemm.";
+        return 1;
+    }
     if (sizeof(S) != sizeof(R))
         rs::launch_gemm_c64(dd, 1, M, N, K, nullptr, ws, tile);
     else
