@@ -78,12 +78,18 @@ def test_fp64_solve_matches_oracle(d, n, L, kb, bp, gpu):
     g = _rand_c(rng, (nrhs, o.n_leaves, o.nb))
     u = t.solve(f, g)
     u2 = t2.solve(f, g)
+    # one column alone (the one-RHS leaf passes: layers of 6 points at 24^2, L = 3)
+    u1 = t.solve(f[1:2], g[1:2])
     for r in range(nrhs):
         u_ref = o.solve(f[r], g[r])
         rel = _rel(u[r], u_ref)
         assert rel <= TOL64, (r, rel)
         rel2 = _rel(u2[r], u_ref)
         assert rel2 <= TOL64, (r, rel2)
+        if r == 1:
+            assert _rel(u1[0], u_ref) <= TOL64
+            if d == 2:
+                assert np.array_equal(u1[0], u[1])
 
 
 @pytest.mark.parametrize("d,n,L,kb,bp", [CASES[0], CASES[2], CASES[4], CASES[6], CASES[8], CASES[9]])
